@@ -1,0 +1,389 @@
+"""gknextengine_amd - MI355X (gfx950) path-tracing core with a C ABI.
+
+This package is a thin ctypes binding of ``include/gknx_rt.h`` (``libgknx_rt.so``,
+built in-tree by ``gknextengine_amd.build``). It exists for tests and benchmarks;
+engines bind the C ABI directly. Loading fails loudly when the
+library has not been built: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "_lib" / "libgknx_rt.so"
+
+# ---------------------------------------------------------------- constants (gknx_rt.h)
+GK_OK = 0
+GK_ERR_ARG, GK_ERR_HIP, GK_ERR_OOM, GK_ERR_STATE, GK_ERR_LIMIT, GK_ERR_NO_DEVICE, GK_ERR_IO = -1, -2, -3, -4, -5, -6, -7
+
+MAT_LAMBERTIAN, MAT_METALLIC, MAT_DIELECTRIC, MAT_ISOTROPIC, MAT_DIFFUSE_LIGHT, MAT_MIXTURE = range(6)
+ESTIMATOR_REF_V2, ESTIMATOR_PT_MIS = 0, 1
+DENOISER_NONE, DENOISER_BILATERAL, DENOISER_UNET = 0, 1, 2
+FRAME_NO_MATERIAL_SORT, FRAME_SKIP_POST = 1, 2
+TRACE_CLOSEST, TRACE_ANY = 0, 1
+
+TARGETS = [
+    "SINGLE_DIFFUSE", "SINGLE_SPECULAR", "ALBEDO", "NORMAL", "OBJECTID_0", "OBJECTID_1",
+    "MOTIONVECTOR", "MOTIONMOMENT", "PREV_DEPTH", "ACCUM_DIFFUSE", "ACCUM_SPECULAR",
+    "ACCUM_ALBEDO", "DENOISED", "MINIGBUFFER", "PRIMARY_HIT", "PRIMARY_INST", "RADIANCE",
+]
+RT = {name: i for i, name in enumerate(TARGETS)}
+# numpy dtype and channel count of each target
+TARGET_FORMAT = {
+    "SINGLE_DIFFUSE": (np.float16, 4), "SINGLE_SPECULAR": (np.float16, 4), "ALBEDO": (np.float16, 4),
+    "NORMAL": (np.float16, 4), "OBJECTID_0": (np.uint32, 1), "OBJECTID_1": (np.uint32, 1),
+    "MOTIONVECTOR": (np.float32, 2), "MOTIONMOMENT": (np.uint16, 1), "PREV_DEPTH": (np.float32, 1),
+    "ACCUM_DIFFUSE": (np.float16, 4), "ACCUM_SPECULAR": (np.float16, 4), "ACCUM_ALBEDO": (np.float16, 4),
+    "DENOISED": (np.float16, 4), "MINIGBUFFER": (np.uint32, 1), "PRIMARY_HIT": (np.float32, 4),
+    "PRIMARY_INST": (np.uint32, 1), "RADIANCE": (np.float32, 4),
+}
+
+
+# ---------------------------------------------------------------- structures
+class Mesh(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("normals", C.c_void_p), ("texcoords", C.c_void_p),
+                ("indices", C.c_void_p), ("tri_material", C.c_void_p),
+                ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32)]
+
+
+class Material(C.Structure):
+    _fields_ = [("diffuse", C.c_float * 4), ("diffuse_texture_id", C.c_int32), ("mra_texture_id", C.c_int32),
+                ("normal_texture_id", C.c_int32), ("fuzziness", C.c_float), ("refraction_index", C.c_float),
+                ("material_model", C.c_uint32), ("metalness", C.c_float), ("refraction_index2", C.c_float),
+                ("normal_texture_scale", C.c_float), ("reserved", C.c_float * 3)]
+
+
+class Light(C.Structure):
+    _fields_ = [("p0", C.c_float * 4), ("p1", C.c_float * 4), ("p3", C.c_float * 4),
+                ("normal_area", C.c_float * 4), ("light_mat_idx", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
+class Instance(C.Structure):
+    _fields_ = [("world", C.c_float * 16), ("prev_world", C.c_float * 16), ("instance_id", C.c_uint32),
+                ("mesh_id", C.c_uint32), ("visible", C.c_uint32), ("nort", C.c_uint32),
+                ("mat_id", C.c_uint32 * 16), ("mat_offset", C.c_uint32), ("use_remap", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
+
+
+class FrameParams(C.Structure):
+    _fields_ = [(n, C.c_float * 16) for n in (
+        "model_view", "projection", "model_view_inverse", "projection_inverse", "view_projection",
+        "prev_view_projection", "view_projection_unjit", "prev_view_projection_unjit")] + [
+        ("sun_direction", C.c_float * 4), ("sun_color", C.c_float * 4), ("sky_color", C.c_float * 4)] + [
+        (n, C.c_float) for n in ("aperture", "focus_distance", "paper_white_nit", "bf_sigma", "bf_sigma_lum",
+                                 "bf_sigma_normal", "exit_probability", "hit_normal_offset")] + [
+        (n, C.c_uint32) for n in ("total_frames", "global_seed", "samples", "bounces", "max_bounces",
+                                  "temporal_frames", "bf_size", "estimator", "denoiser", "has_sky", "has_sun",
+                                  "taa", "progressive", "hdr", "flags", "reserved")]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3),
+                ("fov_y_degrees", C.c_float), ("aperture", C.c_float), ("focus_distance", C.c_float)]
+
+
+class DeviceDesc(C.Structure):
+    _fields_ = [("device", C.c_int32), ("width", C.c_uint32), ("height", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class BvhStats(C.Structure):
+    _fields_ = [("blas_count", C.c_uint32), ("blas_nodes", C.c_uint32), ("blas_leaves", C.c_uint32),
+                ("tlas_nodes", C.c_uint32), ("blas_bytes", C.c_uint64), ("tlas_bytes", C.c_uint64),
+                ("sah_cost", C.c_double), ("build_ms", C.c_double), ("tlas_ms", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("rays_traced", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("path_vertices", C.c_uint64)] + [
+        (n, C.c_double) for n in ("ms_tlas", "ms_trace", "ms_shade", "ms_reproject", "ms_denoise", "ms_frame")]
+
+
+assert C.sizeof(Material) == 64 and C.sizeof(Instance) == 224 and C.sizeof(FrameParams) == 656
+assert C.sizeof(Light) == 80 and C.sizeof(Mesh) == 48
+
+# every entry point declared in include/gknx_rt.h: (name, restype, argtypes)
+_P = C.c_void_p
+_U = C.c_uint32
+SIGNATURES = [
+    ("gk_get_abi_version", C.c_int, []),
+    ("gk_status_string", C.c_char_p, [C.c_int]),
+    ("gk_create", C.c_int, [C.POINTER(DeviceDesc), C.POINTER(_P)]),
+    ("gk_destroy", C.c_int, [_P]),
+    ("gk_resize", C.c_int, [_P, _U, _U]),
+    ("gk_scene_upload", C.c_int, [_P, C.POINTER(Mesh), _U, C.POINTER(Material), _U, C.POINTER(Light), _U,
+                                  C.POINTER(Instance), _U]),
+    ("gk_materials_update", C.c_int, [_P, C.POINTER(Material), _U, _U]),
+    ("gk_instances_update", C.c_int, [_P, _P, _U, _U]),
+    ("gk_instances_update_device", C.c_int, [_P, _P, _U, _U]),
+    ("gk_get_bvh_stats", C.c_int, [_P, C.POINTER(BvhStats)]),
+    ("gk_frame", C.c_int, [_P, C.POINTER(FrameParams)]),
+    ("gk_render", C.c_int, [_P]),
+    ("gk_sync", C.c_int, [_P]),
+    ("gk_reset_accumulation", C.c_int, [_P]),
+    ("gk_read_target", C.c_int, [_P, C.c_int, _P, C.c_size_t]),
+    ("gk_target_info", C.c_int, [_P, C.c_int, C.POINTER(C.c_size_t), C.POINTER(_P)]),
+    ("gk_get_stats", C.c_int, [_P, C.POINTER(Stats)]),
+    ("gk_trace", C.c_int, [_P, _P, _U, _P, _P, _U]),
+    ("gk_make_frame_params", C.c_int, [C.POINTER(CameraDesc), _U, _U, _U, _U, C.POINTER(FrameParams),
+                                       C.POINTER(FrameParams)]),
+    ("gk_procgen_cornell", C.c_int, [_P, _P, _P, _P, C.POINTER(_U), C.POINTER(_U)]),
+    ("gk_procgen_cornell_materials", C.c_int, [C.POINTER(Material), C.POINTER(_U), C.POINTER(Light),
+                                               C.POINTER(_U)]),
+    ("gk_procgen_city", C.c_int, [_U, _U, _U, _U, C.c_uint64, _P, _P, _P, _P, C.POINTER(_U), C.POINTER(_U),
+                                  C.POINTER(Material), C.POINTER(_U)]),
+    ("gk_procgen_brick", C.c_int, [_P, _P, _P, _P, C.POINTER(_U), C.POINTER(_U)]),
+    ("gk_adjoint", C.c_int, [_P, _P, _P]),
+    ("gk_render_radiance", C.c_int, [_P, _P]),
+    ("gk_unet_create", C.c_int, [_P, _U, C.c_uint64, C.POINTER(_P)]),
+    ("gk_unet_destroy", C.c_int, [_P]),
+    ("gk_unet_num_params", C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    ("gk_unet_params", C.c_int, [_P, C.POINTER(_P)]),
+    ("gk_unet_save", C.c_int, [_P, C.c_char_p]),
+    ("gk_unet_load", C.c_int, [_P, C.c_char_p]),
+    ("gk_unet_train_step", C.c_int, [_P, _P, _P, _U, _U, _U, C.c_float, C.POINTER(C.c_float)]),
+    ("gk_unet_infer", C.c_int, [_P, _P, _P, _U, _U]),
+]
+
+_lib = None
+
+
+class GkError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what} failed: {status} ({lib().gk_status_string(status).decode()})")
+        self.status = status
+
+
+def lib() -> C.CDLL:
+    """Load libgknx_rt.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m gknextengine_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str) -> None:
+    if status != GK_OK:
+        raise GkError(status, what)
+
+
+def ptr(a: np.ndarray | None) -> int | None:
+    return None if a is None else a.ctypes.data
+
+
+# ---------------------------------------------------------------- host helpers
+def make_frame_params(eye, target, width, height, fov=40.0, up=(0, 1, 0), frame=0, taa=False,
+                      prev: FrameParams | None = None) -> FrameParams:
+    cam = CameraDesc()
+    cam.eye[:] = eye
+    cam.target[:] = target
+    cam.up[:] = up
+    cam.fov_y_degrees = fov
+    cam.aperture = 0.0
+    cam.focus_distance = 10.0
+    out = FrameParams()
+    check(lib().gk_make_frame_params(C.byref(cam), width, height, frame, int(taa),
+                                     C.byref(prev) if prev is not None else None, C.byref(out)),
+          "gk_make_frame_params")
+    return out
+
+
+class SceneData:
+    """Host-side arrays of one scene (meshes, materials, instances)."""
+
+    def __init__(self):
+        self.meshes: list[dict] = []
+        self.materials: list[Material] = []
+        self.instances: list[Instance] = []
+
+    def add_mesh(self, positions, indices, normals=None, tri_material=None) -> int:
+        m = {"positions": np.ascontiguousarray(positions, np.float32).reshape(-1, 3),
+             "indices": np.ascontiguousarray(indices, np.uint32).reshape(-1, 3),
+             "normals": None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3),
+             "tri_material": None if tri_material is None else np.ascontiguousarray(tri_material, np.uint32)}
+        self.meshes.append(m)
+        return len(self.meshes) - 1
+
+    def add_instance(self, mesh_id: int, world=None, prev_world=None, instance_id=None, mat_offset=0,
+                     visible=True) -> int:
+        inst = Instance()
+        w = np.eye(4, dtype=np.float32) if world is None else np.asarray(world, np.float32)
+        pw = w if prev_world is None else np.asarray(prev_world, np.float32)
+        inst.world[:] = w.T.reshape(-1).tolist()       # column-major
+        inst.prev_world[:] = pw.T.reshape(-1).tolist()
+        inst.instance_id = len(self.instances) if instance_id is None else instance_id
+        inst.mesh_id = mesh_id
+        inst.visible = 1 if visible else 0
+        inst.nort = 0
+        inst.mat_offset = mat_offset
+        inst.use_remap = 0
+        self.instances.append(inst)
+        return len(self.instances) - 1
+
+    def c_meshes(self):
+        arr = (Mesh * len(self.meshes))()
+        for i, m in enumerate(self.meshes):
+            arr[i].positions = ptr(m["positions"])
+            arr[i].normals = ptr(m["normals"])
+            arr[i].texcoords = None
+            arr[i].indices = ptr(m["indices"])
+            arr[i].tri_material = ptr(m["tri_material"])
+            arr[i].num_vertices = m["positions"].shape[0]
+            arr[i].num_triangles = m["indices"].shape[0]
+        return arr
+
+    def c_materials(self):
+        arr = (Material * len(self.materials))()
+        for i, m in enumerate(self.materials):
+            arr[i] = m
+        return arr
+
+    def c_instances(self):
+        arr = (Instance * len(self.instances))()
+        for i, m in enumerate(self.instances):
+            arr[i] = m
+        return arr
+
+
+def procgen_cornell() -> SceneData:
+    L = lib()
+    nv, nt = C.c_uint32(0), C.c_uint32(0)
+    check(L.gk_procgen_cornell(None, None, None, None, C.byref(nv), C.byref(nt)), "gk_procgen_cornell")
+    pos = np.zeros((nv.value, 3), np.float32)
+    nrm = np.zeros((nv.value, 3), np.float32)
+    idx = np.zeros((nt.value, 3), np.uint32)
+    mat = np.zeros(nt.value, np.uint32)
+    check(L.gk_procgen_cornell(ptr(pos), ptr(nrm), ptr(idx), ptr(mat), C.byref(nv), C.byref(nt)),
+          "gk_procgen_cornell")
+    nm = C.c_uint32(8)
+    mats = (Material * 8)()
+    check(L.gk_procgen_cornell_materials(mats, C.byref(nm), None, None), "gk_procgen_cornell_materials")
+    s = SceneData()
+    s.add_mesh(pos, idx, nrm, mat)
+    s.materials = [mats[i] for i in range(nm.value)]
+    s.add_instance(0)
+    return s
+
+
+def procgen_city(grid=64, subdiv=4, ground_res=256, num_lights=64, seed=2024) -> SceneData:
+    L = lib()
+    nv, nt, nm = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
+    check(L.gk_procgen_city(grid, subdiv, ground_res, num_lights, seed, None, None, None, None,
+                            C.byref(nv), C.byref(nt), None, C.byref(nm)), "gk_procgen_city")
+    pos = np.zeros((nv.value, 3), np.float32)
+    nrm = np.zeros((nv.value, 3), np.float32)
+    idx = np.zeros((nt.value, 3), np.uint32)
+    mat = np.zeros(nt.value, np.uint32)
+    mats = (Material * nm.value)()
+    check(L.gk_procgen_city(grid, subdiv, ground_res, num_lights, seed, ptr(pos), ptr(nrm), ptr(idx), ptr(mat),
+                            C.byref(nv), C.byref(nt), mats, C.byref(nm)), "gk_procgen_city")
+    s = SceneData()
+    s.add_mesh(pos, idx, nrm, mat)
+    s.materials = [mats[i] for i in range(nm.value)]
+    s.add_instance(0)
+    return s
+
+
+def procgen_brick():
+    L = lib()
+    nv, nt = C.c_uint32(0), C.c_uint32(0)
+    check(L.gk_procgen_brick(None, None, None, None, C.byref(nv), C.byref(nt)), "gk_procgen_brick")
+    pos = np.zeros((nv.value, 3), np.float32)
+    nrm = np.zeros((nv.value, 3), np.float32)
+    idx = np.zeros((nt.value, 3), np.uint32)
+    mat = np.zeros(nt.value, np.uint32)
+    check(L.gk_procgen_brick(ptr(pos), ptr(nrm), ptr(idx), ptr(mat), C.byref(nv), C.byref(nt)), "gk_procgen_brick")
+    return pos, nrm, idx, mat
+
+
+# ---------------------------------------------------------------- device context
+class Context:
+    """Owns a gk_ctx (one HIP stream, all device state)."""
+
+    def __init__(self, width: int, height: int, device: int = 0):
+        d = DeviceDesc(device, width, height, 0)
+        h = C.c_void_p()
+        check(lib().gk_create(C.byref(d), C.byref(h)), "gk_create")
+        self.h = h
+        self.width, self.height = width, height
+        self._keep = None
+
+    def close(self):
+        if self.h:
+            lib().gk_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, scene: SceneData):
+        meshes, mats, insts = scene.c_meshes(), scene.c_materials(), scene.c_instances()
+        self._keep = (scene, meshes, mats, insts)
+        check(lib().gk_scene_upload(self.h, meshes, len(scene.meshes), mats, len(scene.materials), None, 0,
+                                    insts, len(scene.instances)), "gk_scene_upload")
+
+    def update_instances(self, worlds: np.ndarray, first: int = 0):
+        w = np.ascontiguousarray(np.transpose(np.asarray(worlds, np.float32), (0, 2, 1)))  # column-major
+        check(lib().gk_instances_update(self.h, ptr(w), first, w.shape[0]), "gk_instances_update")
+
+    def update_materials(self, materials, first=0):
+        arr = (Material * len(materials))(*materials)
+        check(lib().gk_materials_update(self.h, arr, first, len(materials)), "gk_materials_update")
+
+    def frame(self, params: FrameParams):
+        check(lib().gk_frame(self.h, C.byref(params)), "gk_frame")
+
+    def render(self):
+        check(lib().gk_render(self.h), "gk_render")
+
+    def sync(self):
+        check(lib().gk_sync(self.h), "gk_sync")
+
+    def reset(self):
+        check(lib().gk_reset_accumulation(self.h), "gk_reset_accumulation")
+
+    def read(self, name: str) -> np.ndarray:
+        dt, ch = TARGET_FORMAT[name]
+        shape = (self.height, self.width, ch) if ch > 1 else (self.height, self.width)
+        out = np.zeros(shape, dt)
+        check(lib().gk_read_target(self.h, RT[name], ptr(out), out.nbytes), f"gk_read_target({name})")
+        return out
+
+    def target_ptr(self, name: str) -> int:
+        n, p = C.c_size_t(0), C.c_void_p()
+        check(lib().gk_target_info(self.h, RT[name], C.byref(n), C.byref(p)), "gk_target_info")
+        return p.value
+
+    def bvh_stats(self) -> BvhStats:
+        s = BvhStats()
+        check(lib().gk_get_bvh_stats(self.h, C.byref(s)), "gk_get_bvh_stats")
+        return s
+
+    def stats(self) -> Stats:
+        s = Stats()
+        check(lib().gk_get_stats(self.h, C.byref(s)), "gk_get_stats")
+        return s
+
+    def trace(self, d_rays: int, n: int, d_hits: int, d_inst: int | None = None, any_hit: bool = False):
+        """Device-pointer ray cast (gk_ray in, gk_hit out); enqueue only."""
+        check(lib().gk_trace(self.h, d_rays, n, d_hits, d_inst, TRACE_ANY if any_hit else TRACE_CLOSEST),
+              "gk_trace")

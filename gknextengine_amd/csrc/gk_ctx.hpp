@@ -1,0 +1,206 @@
+// gk_ctx.hpp - host-side context of the path-tracing core (one per gk_create).
+//
+// The context owns every device allocation and one HIP stream; the layout of the
+// device data is described next to each member.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/gknx_rt.h"
+#include "gk_math.hpp"
+
+namespace gk {
+
+// ------------------------------------------------------------------ device buffer
+template <class T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t count = 0;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = cap = 0;
+    }
+    // (re)allocate for n elements; keeps the allocation when it is large enough
+    hipError_t resize(size_t n) {
+        if (n <= cap) { count = n; return hipSuccess; }
+        release();
+        if (n == 0) return hipSuccess;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T));
+        if (e != hipSuccess) { ptr = nullptr; return e; }
+        count = cap = n;
+        return hipSuccess;
+    }
+    size_t bytes() const { return count * sizeof(T); }
+};
+
+// ------------------------------------------------------------------ scene records
+struct MeshInfo {        // per mesh
+    uint32_t tri_base;   // first triangle of the mesh in tri attribute arrays (original order)
+    uint32_t tri_count;
+    uint32_t root;       // root node (absolute index into blas_nodes)
+    uint32_t depth;      // wide-tree depth
+    float bmin[3];
+    float bmax[3];
+};
+
+struct InstDev {         // per instance, 256 bytes
+    float world[16];     // column-major
+    float inv[16];       // row-major world->object (tinybvh InvertTransform layout)
+    float combined[16];  // column-major prev * inverse(world), identity when static
+    uint32_t mesh;
+    uint32_t instance_id;
+    uint32_t active;     // visible && !nort
+    uint32_t use_remap;
+    uint32_t mat_offset;
+    uint32_t tri_base;   // = meshes[mesh].tri_base
+    uint32_t root;       // = meshes[mesh].root
+    uint32_t pad;
+    uint32_t mat_id[16];
+};
+static_assert(sizeof(InstDev) == 288, "InstDev size");
+
+struct LightTri {        // emissive triangle in world space, 64 bytes
+    float4 p0;           // w = area
+    float4 e1;           // w = luminance-weighted selection pdf
+    float4 e2;           // w = inst (bits)
+    float4 le;           // w = prim (bits)
+};
+
+// ------------------------------------------------------------------ build scratch
+struct BuildScratch {
+    DevBuf<uint64_t> keys, keys_alt;
+    DevBuf<uint32_t> vals, vals_alt;
+    DevBuf<uint8_t> sort_tmp;
+    DevBuf<float4> prim_lo, prim_hi;      // primitive boxes (w unused)
+    DevBuf<uint32_t> bparent;             // binary node parents
+    DevBuf<uint2> bchild;                 // binary internal children (leaf: bit 31)
+    DevBuf<uint2> brange;                 // [first, last] of each internal node
+    DevBuf<float4> bnode_lo, bnode_hi;    // internal node boxes (w = cost / prim count)
+    DevBuf<uint32_t> bflag;               // bottom-up arrival counters
+    DevBuf<uint32_t> queue_a, queue_b;    // top-down collapse work queues (pairs)
+    DevBuf<uint32_t> counters;            // [0] wide node count [1] queue size [2] next queue size
+    DevBuf<float> bounds;                 // scene centroid bounds (6 floats as ordered ints)
+    DevBuf<uint32_t> node_depth;
+};
+
+// ------------------------------------------------------------------ context
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t width = 0, height = 0;
+    bool scene_ready = false;
+    bool frame_ready = false;
+    bool tlas_dirty = false;
+    bool blas_only = false;   // one identity instance: trace the BLAS directly
+
+    // scene (host copies)
+    std::vector<gk_material> h_materials;
+    std::vector<MeshInfo> h_meshes;
+    std::vector<gk_instance> h_instances;
+    uint32_t num_tris = 0;
+
+    // scene (device)
+    DevBuf<gk_material> materials;
+    DevBuf<TriRec> tris;            // all meshes, leaf order (absolute record index)
+    DevBuf<float4> tri_nrm;         // 3 per triangle, original order: n0, n1, n2 (object space)
+    DevBuf<uint32_t> tri_mat;       // local material slot per triangle, original order
+    DevBuf<Bvh4Node> blas_nodes;
+    DevBuf<MeshInfo> meshes;
+    DevBuf<InstDev> inst;
+    DevBuf<float> inst_world_in;    // staging for instance updates (16 floats each)
+    DevBuf<float4> inst_lo, inst_hi;  // world-space instance boxes
+    DevBuf<Bvh4Node> tlas_nodes;
+    DevBuf<uint32_t> tlas_prims;    // TLAS leaf -> instance index
+    DevBuf<uint32_t> tlas_refit_order;
+    std::vector<uint32_t> tlas_level_offsets;
+    uint32_t tlas_node_count = 0;
+    uint32_t tlas_depth = 0;
+    DevBuf<LightTri> lights;
+    uint32_t num_lights = 0;
+    DevBuf<float> light_cdf;        // num_lights + 1
+    DevBuf<float4> light_obj;       // 3 per light: object-space v0, e1, e2
+    BuildScratch scratch;
+    gk_bvh_stats bvh_stats{};
+
+    // frame
+    gk_frame_params params{};
+    gk_frame_params prev_params{};
+    uint32_t frames_rendered = 0;
+
+    // render targets (indexed by gk_target)
+    void* target_ptr[GK_RT_COUNT] = {};
+    size_t target_bytes[GK_RT_COUNT] = {};
+    std::vector<void*> target_allocs;
+
+    // wavefront buffers
+    struct Wavefront {
+        uint32_t capacity = 0;
+        DevBuf<float4> ray_o[2], ray_d[2];    // queue rays (o,tmin) (d,tmax)
+        DevBuf<uint32_t> ray_path[2];         // queue entry -> path index
+        DevBuf<float4> hit;                   // per queue entry: t,u,v,prim
+        DevBuf<uint32_t> hit_inst;
+        DevBuf<uint32_t> sorted_idx;          // material-sorted permutation of queue entries
+        DevBuf<float4> shadow_o, shadow_d;    // shadow queue
+        DevBuf<float4> shadow_c;              // contribution (rgb) + target (w)
+        DevBuf<uint32_t> shadow_path;
+        DevBuf<uint32_t> counters;            // [0..1] queue sizes [2] shadow size [3..] sort
+        DevBuf<uint32_t> stats;               // ray counters (u64 pairs)
+        // path state (SoA)
+        DevBuf<float4> s_thr;                 // throughput rgb, last bsdf pdf
+        DevBuf<float4> s_ld;                  // primary-diffuse-lobe radiance, w = flags
+        DevBuf<float4> s_ls;                  // other radiance
+        DevBuf<uint4> s_rng;
+        DevBuf<uint4> s_misc;                 // pixel, depth/flags, sample counters, lobe
+        // REF_V2 extra state
+        DevBuf<float4> s_pos, s_nrm;          // current vertex (w: material bits / misc)
+        DevBuf<float4> s_dir;                 // current ray direction
+        DevBuf<float4> s_ppos, s_pnrm;        // primary vertex
+        DevBuf<ushort4> hist_d, hist_s, hist_a;  // previous accumulation targets (RGBA16F)
+        DevBuf<uint32_t> spill;               // traversal stack overflow
+        uint32_t spill_threads = 0;
+    } wf;
+
+    // timing
+    hipEvent_t ev[16] = {};
+    gk_stats stats{};
+};
+
+// ------------------------------------------------------------------ helpers
+#define GK_HIP(call)                                            \
+    do {                                                        \
+        hipError_t e__ = (call);                                \
+        if (e__ != hipSuccess) {                                \
+            gk::set_error(#call, e__);                          \
+            return e__ == hipErrorOutOfMemory ? GK_ERR_OOM : GK_ERR_HIP; \
+        }                                                       \
+    } while (0)
+
+void set_error(const char* what, hipError_t e);
+
+// bvh_build.hip
+int build_blas(Ctx& c, const gk_mesh* meshes, uint32_t num_meshes);
+int build_tlas(Ctx& c);
+int update_instances_device(Ctx& c, uint32_t first, uint32_t count);  // from inst_world_in
+int refit_tlas(Ctx& c);
+int update_lights(Ctx& c);  // world-space emissive triangles from the instance matrices
+
+// trace.hip
+int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev,
+                 uint32_t max_count, float4* hit, uint32_t* hit_inst, bool any_hit,
+                 uint32_t* occluded_flags);
+int launch_trace_user(Ctx& c, const gk_ray* rays, uint32_t n, gk_hit* hits, uint32_t* inst, uint32_t flags);
+int ensure_spill(Ctx& c, uint32_t threads);
+
+}  // namespace gk
+
+struct gk_ctx {
+    gk::Ctx c;
+};

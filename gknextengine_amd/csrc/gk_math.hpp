@@ -1,0 +1,131 @@
+// gk_math.hpp - device math shared by the HIP kernels of the path-tracing core.
+//
+// All of the library is compiled with -ffp-contract=off: a multiply-add is fused only
+// where fmaf() is written out. The ray/triangle test, the camera ray and the samplers
+// therefore produce the same bits on the GPU as the IEEE-754 single-precision
+// arithmetic they are specified with.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gk {
+
+struct f3 {
+    float x, y, z;
+};
+__host__ __device__ inline f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__host__ __device__ inline f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+__host__ __device__ inline f3 operator*(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__host__ __device__ inline f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline f3 cross(f3 a, f3 b) {
+    return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__host__ __device__ inline float length(f3 a) { return sqrtf(dot(a, a)); }
+// v * (1/sqrt(dot(v,v))), IEEE sqrt and divide (not rsqrt)
+__host__ __device__ inline f3 normalize(f3 a) {
+    const float l = sqrtf(dot(a, a));
+    const float rl = l == 0.f ? 0.f : 1.0f / l;
+    return a * rl;
+}
+__host__ __device__ inline f3 fmin3(f3 a, f3 b) {
+    return f3{a.x < b.x ? a.x : b.x, a.y < b.y ? a.y : b.y, a.z < b.z ? a.z : b.z};
+}
+__host__ __device__ inline f3 fmax3(f3 a, f3 b) {
+    return f3{a.x > b.x ? a.x : b.x, a.y > b.y ? a.y : b.y, a.z > b.z ? a.z : b.z};
+}
+__device__ inline f3 ld3(const float4& v) { return f3{v.x, v.y, v.z}; }
+
+// column-major 4x4 times (x,y,z,w): ((m0*x + m1*y) + m2*z) + m3*w
+__host__ __device__ inline void mul44v(const float* m, float x, float y, float z, float w, float* o) {
+    for (int r = 0; r < 4; ++r) {
+        float a = m[r] * x;
+        a = a + m[4 + r] * y;
+        a = a + m[8 + r] * z;
+        a = a + m[12 + r] * w;
+        o[r] = a;
+    }
+}
+
+// ---------------------------------------------------------------- RNG
+// pcg4d hash (Jarzynski & Olano 2020), the per-pixel generator of Const_Func.slang:227-258.
+struct rng4 {
+    uint32_t x, y, z, w;
+};
+__device__ inline void pcg4d(rng4& v) {
+    v.x = v.x * 1664525u + 1013904223u;
+    v.y = v.y * 1664525u + 1013904223u;
+    v.z = v.z * 1664525u + 1013904223u;
+    v.w = v.w * 1664525u + 1013904223u;
+    v.x += v.y * v.w; v.y += v.z * v.x; v.z += v.x * v.y; v.w += v.y * v.z;
+    v.x ^= v.x >> 16u; v.y ^= v.y >> 16u; v.z ^= v.z >> 16u; v.w ^= v.w >> 16u;
+    v.x += v.y * v.w; v.y += v.z * v.x; v.z += v.x * v.y; v.w += v.y * v.z;
+}
+__device__ inline float u2f(uint32_t x) { return __uint_as_float(0x3f800000u | (x >> 9)) - 1.0f; }
+__device__ inline float rnd1(rng4& s) { pcg4d(s); return u2f(s.x); }
+__device__ inline void rnd2(rng4& s, float& a, float& b) { pcg4d(s); a = u2f(s.x); b = u2f(s.y); }
+
+// sin/cos of 2*pi*u for u in [0,1): quarter-turn reduction and the Taylor series of
+// sin (degree 9) and cos (degree 10) on |x| <= pi/4, evaluated by Horner's rule.
+template <class T>
+__device__ inline void sincos_2pi(T u, T& s, T& c) {
+    const T q4 = u * T(4);
+    int q = int(q4);
+    if (q > 3) q = 3;
+    const T r = q4 - T(q);
+    const T x = (r - T(0.5)) * T(1.5707963267948966);
+    const T x2 = x * x;
+    T sp = T(1.0 / 362880.0);
+    sp = sp * x2 - T(1.0 / 5040.0);
+    sp = sp * x2 + T(1.0 / 120.0);
+    sp = sp * x2 - T(1.0 / 6.0);
+    sp = sp * x2 + T(1);
+    const T sx = sp * x;
+    T cp = T(-1.0 / 3628800.0);
+    cp = cp * x2 + T(1.0 / 40320.0);
+    cp = cp * x2 - T(1.0 / 720.0);
+    cp = cp * x2 + T(1.0 / 24.0);
+    cp = cp * x2 - T(0.5);
+    cp = cp * x2 + T(1);
+    const T k = T(0.70710678118654752);
+    const T sq = (sx + cp) * k;
+    const T cq = (cp - sx) * k;
+    switch (q) {
+        case 0: s = sq; c = cq; break;
+        case 1: s = cq; c = -sq; break;
+        case 2: s = -sq; c = -cq; break;
+        default: s = -cq; c = sq; break;
+    }
+}
+
+// ---------------------------------------------------------------- acceleration structures
+// 4-wide BVH node, one 128-byte cache line. Child i is
+//   internal: meta[i] == 0, child[i] = node index
+//   leaf:     meta[i] = primitive count (> 0), child[i] = first primitive record
+//   empty:    lo > hi (never hit)
+struct alignas(128) Bvh4Node {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    uint4 child;
+    uint4 meta;
+};
+static_assert(sizeof(Bvh4Node) == 128, "node size");
+
+// Triangle record, 48 bytes: v0 (w = primitive index within its mesh), e1 = v1 - v0,
+// e2 = v2 - v0 (w of e1 = local material slot).
+struct TriRec {
+    float4 v0;
+    float4 e1;
+    float4 e2;
+};
+
+constexpr uint32_t kStackMark = 0xffffffffu;  // return from an instance to the TLAS
+constexpr uint32_t kInstFlag = 0x80000000u;   // stack entry: enter instance (low bits)
+constexpr float kFar = 1e30f;
+
+__host__ __device__ inline float safercp(float x) {
+    return x > 1e-12f ? (1.0f / x) : (x < -1e-12f ? (1.0f / x) : kFar);
+}
+
+}  // namespace gk

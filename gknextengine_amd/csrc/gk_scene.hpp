@@ -1,0 +1,148 @@
+// gk_scene.hpp - device view of the scene and per-hit reconstruction shared by the
+// wavefront kernels (wavefront.hip, adjoint.hip).
+#pragma once
+#include <hip/hip_fp16.h>
+
+#include "gk_ctx.hpp"
+
+namespace gk {
+
+struct DScene {
+    const Bvh4Node* blas;
+    const TriRec* tris;
+    const Bvh4Node* tlas;
+    const uint32_t* tlas_prims;
+    const InstDev* inst;
+    const float4* tri_nrm;
+    const uint32_t* tri_mat;
+    const gk_material* mats;
+    const LightTri* lights;
+    const float* light_cdf;
+    uint32_t nmat;
+    uint32_t nlights;
+    uint32_t blas_root;
+    uint32_t two_level;
+};
+
+inline DScene make_dscene(const Ctx& c) {
+    DScene s;
+    s.blas = c.blas_nodes.ptr;
+    s.tris = c.tris.ptr;
+    s.tlas = c.tlas_nodes.ptr;
+    s.tlas_prims = c.tlas_prims.ptr;
+    s.inst = c.inst.ptr;
+    s.tri_nrm = c.tri_nrm.ptr;
+    s.tri_mat = c.tri_mat.ptr;
+    s.mats = c.materials.ptr;
+    s.lights = c.lights.ptr;
+    s.light_cdf = c.light_cdf.ptr;
+    s.nmat = uint32_t(c.h_materials.size());
+    s.nlights = c.num_lights;
+    s.blas_root = c.h_meshes.empty() ? 0u : c.h_meshes[0].root;
+    s.two_level = c.blas_only ? 0u : 1u;
+    return s;
+}
+
+// FetchMaterialId (Const_Func.slang:370-424) or offset form, clamped as GetMaterial
+// (BasicTypes.slang:376-379)
+__device__ inline uint32_t material_of(const DScene& s, uint32_t inst, uint32_t prim) {
+    const InstDev& I = s.inst[inst];
+    const uint32_t local = s.tri_mat[I.tri_base + prim];
+    const uint32_t m = I.use_remap ? I.mat_id[local & 15u] : I.mat_offset + local;
+    return m > s.nmat - 1u ? s.nmat - 1u : m;
+}
+
+// row-major world->object inverse applied as the inverse transpose (normals)
+__device__ inline f3 inv_transpose(const float* W, f3 v) {
+    f3 o;
+    o.x = W[0] * v.x + W[4] * v.y + W[8] * v.z;
+    o.y = W[1] * v.x + W[5] * v.y + W[9] * v.z;
+    o.z = W[2] * v.x + W[6] * v.y + W[10] * v.z;
+    return o;
+}
+
+__device__ inline ushort4 to_h4(float a, float b, float c, float d) {
+    return make_ushort4(__half_as_ushort(__float2half_rn(a)), __half_as_ushort(__float2half_rn(b)),
+                        __half_as_ushort(__float2half_rn(c)), __half_as_ushort(__float2half_rn(d)));
+}
+__device__ inline float4 from_h4(ushort4 v) {
+    return make_float4(__half2float(__ushort_as_half(v.x)), __half2float(__ushort_as_half(v.y)),
+                       __half2float(__ushort_as_half(v.z)), __half2float(__ushort_as_half(v.w)));
+}
+
+// ---------------------------------------------------------------- single-ray any-hit
+// Occlusion query for rare inline uses (REF_V2 sun cone); private stack.
+__device__ inline bool occluded_single(const DScene& s, f3 O, f3 D, float tmin, float tmax) {
+    uint32_t stack[96];
+    int sp = 0;
+    f3 wO = O, wD = D;
+    f3 rD = f3{safercp(D.x), safercp(D.y), safercp(D.z)};
+    f3 wrD = rD;
+    uint32_t level = s.two_level ? 0u : 1u;
+    uint32_t node = s.two_level ? 0u : s.blas_root;
+    for (;;) {
+        if (node == 0xfffffffeu) {
+            if (sp == 0) return false;
+            const uint32_t e = stack[--sp];
+            if (e == kStackMark) { O = wO; D = wD; rD = wrD; level = 0; continue; }
+            if (e & kInstFlag) {
+                const InstDev& I = s.inst[e & ~kInstFlag];
+                const float* M = I.inv;
+                f3 r = f3{M[0] * wO.x + M[1] * wO.y + M[2] * wO.z + M[3], M[4] * wO.x + M[5] * wO.y + M[6] * wO.z + M[7],
+                          M[8] * wO.x + M[9] * wO.y + M[10] * wO.z + M[11]};
+                const float w = M[12] * wO.x + M[13] * wO.y + M[14] * wO.z + M[15];
+                O = w == 1.0f ? r : r * (1.f / w);
+                D = f3{M[0] * wD.x + M[1] * wD.y + M[2] * wD.z, M[4] * wD.x + M[5] * wD.y + M[6] * wD.z,
+                       M[8] * wD.x + M[9] * wD.y + M[10] * wD.z};
+                rD = f3{safercp(D.x), safercp(D.y), safercp(D.z)};
+                if (sp < 96) stack[sp++] = kStackMark;
+                level = 1;
+                node = I.root;
+                continue;
+            }
+            node = e;
+            continue;
+        }
+        const Bvh4Node n = (level == 0) ? s.tlas[node] : s.blas[node];
+        const float lx[4] = {n.lox.x, n.lox.y, n.lox.z, n.lox.w}, hx[4] = {n.hix.x, n.hix.y, n.hix.z, n.hix.w};
+        const float ly[4] = {n.loy.x, n.loy.y, n.loy.z, n.loy.w}, hy[4] = {n.hiy.x, n.hiy.y, n.hiy.z, n.hiy.w};
+        const float lz[4] = {n.loz.x, n.loz.y, n.loz.z, n.loz.w}, hz[4] = {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w};
+        const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+        const uint32_t me[4] = {n.meta.x, n.meta.y, n.meta.z, n.meta.w};
+        node = 0xfffffffeu;
+        for (int i = 0; i < 4; ++i) {
+            float a = (lx[i] - O.x) * rD.x, b = (hx[i] - O.x) * rD.x;
+            float lo = fminf(a, b), hi = fmaxf(a, b);
+            a = (ly[i] - O.y) * rD.y; b = (hy[i] - O.y) * rD.y;
+            lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
+            a = (lz[i] - O.z) * rD.z; b = (hz[i] - O.z) * rD.z;
+            lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
+            lo = lo > 0.f ? lo * 0.99999904632568359375f : lo * 1.00000095367431640625f;
+            hi = hi > 0.f ? hi * 1.00000095367431640625f : hi * 0.99999904632568359375f;
+            if (!(hi >= lo && hi >= tmin && lo <= tmax)) continue;
+            if (me[i] == 0u) { if (sp < 96) stack[sp++] = ch[i]; continue; }
+            if (level == 0) {
+                for (uint32_t k = 0; k < me[i]; ++k) if (sp < 96) stack[sp++] = kInstFlag | s.tlas_prims[ch[i] + k];
+                continue;
+            }
+            for (uint32_t k = 0; k < me[i]; ++k) {
+                const TriRec tr = s.tris[ch[i] + k];
+                const f3 v0 = ld3(tr.v0), e1 = ld3(tr.e1), e2 = ld3(tr.e2);
+                const f3 h = cross(D, e2);
+                const float aa = dot(e1, h);
+                if (fabsf(aa) < 0.0000001f) continue;
+                const float f = 1 / aa;
+                const f3 sv = O - v0;
+                const float u = f * dot(sv, h);
+                if (u < 0 || u > 1) continue;
+                const f3 q = cross(sv, e1);
+                const float v = f * dot(D, q);
+                if (v < 0 || u + v > 1) continue;
+                const float t = f * dot(e2, q);
+                if (t > tmin && t < tmax) return true;
+            }
+        }
+    }
+}
+
+}  // namespace gk

@@ -1,0 +1,289 @@
+// post.hip - temporal reprojection, joint bilateral filter + tonemap, history copy.
+//
+//   k_reproject   Process.ReProject.comp.slang:71-223 (8x8 groups, one thread per pixel)
+//   k_jbf         Process.DenoiseJBF.comp.slang:75-181: an 18x18 LDS tile of
+//                 {diffuse, spec, normal, object id}, 6x6 taps at stride 2 without the
+//                 centre, weights exp(-d^2/2s^2) * exp(-dlum^2/2sL^2); GT tonemap
+//                 (Const_Func.slang:84-126) of Total * PaperWhiteNit / 40000
+//   history copy  the "copy pass" accum -> prev of PathTracingRenderer.cpp:160-216
+// Storage images read out of bounds return zero, as robust Vulkan image loads do.
+#include <hip/hip_runtime.h>
+
+#include "gk_scene.hpp"
+
+namespace gk {
+
+int unet_denoise_frame(Ctx& c);  // unet.hip
+
+namespace {
+
+struct PostK {
+    uint32_t W, H, progressive, temporal, bf_size, hdr;
+    float bf_sigma, bf_sigma_lum, paper_white;
+};
+
+__device__ inline bool inb(int x, int y, const PostK& K) { return x >= 0 && y >= 0 && x < int(K.W) && y < int(K.H); }
+__device__ inline float4 ld_h4(const ushort4* p, int x, int y, const PostK& K) {
+    return inb(x, y, K) ? from_h4(p[size_t(y) * K.W + x]) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+__device__ inline uint32_t ld_u32(const uint32_t* p, int x, int y, const PostK& K) {
+    return inb(x, y, K) ? p[size_t(y) * K.W + x] : 0u;
+}
+__device__ inline float lerpf(float a, float b, float t) { return a * (1.f - t) + b * t; }
+
+struct PostT {
+    const ushort4 *src_d, *src_s, *src_a, *normal;
+    const ushort4 *prev_d, *prev_s, *prev_a;
+    ushort4 *acc_d, *acc_s, *acc_a;
+    const uint32_t *obj0, *obj1;
+    const float2* motion;
+    const uint16_t* moment;
+    ushort4* out;
+};
+
+// calculateWeight, Process.ReProject.comp.slang:32-47
+__device__ inline float calc_weight(float center_dist, bool same, bool center, f3 n, f3 cn) {
+    if (center) return 0.4f;
+    if (!same) return 0.f;
+    const float nd = fminf(fmaxf(dot(n, cn), 0.f), 1.f);
+    const float th = 0.98f;
+    if (nd < th) return 0.f;
+    const float nw = (nd - th) / (1.0f - th);
+    return nw * 2.0f / (center_dist * 1.5f + 4.0f);
+}
+
+// FilterHistoryColor, Process.ReProject.comp.slang:49-68
+__device__ inline f3 filter_history(const ushort4* prev, float sx, float sy, int px, int py, uint32_t cur, uint32_t p0,
+                                    uint32_t p1, uint32_t p2, uint32_t p3, float4 curr, const PostK& K) {
+    const f3 c = ld3(curr);
+    const f3 h0 = cur == p0 ? ld3(ld_h4(prev, px, py, K)) : c;
+    const f3 h1 = cur == p1 ? ld3(ld_h4(prev, px + 1, py, K)) : c;
+    const f3 h2 = cur == p2 ? ld3(ld_h4(prev, px, py + 1, K)) : c;
+    const f3 h3 = cur == p3 ? ld3(ld_h4(prev, px + 1, py + 1, K)) : c;
+    f3 a = f3{lerpf(h0.x, h1.x, sx), lerpf(h0.y, h1.y, sx), lerpf(h0.z, h1.z, sx)};
+    f3 b = f3{lerpf(h2.x, h3.x, sx), lerpf(h2.y, h3.y, sx), lerpf(h2.z, h3.z, sx)};
+    f3 h = f3{lerpf(a.x, b.x, sy), lerpf(a.y, b.y, sy), lerpf(a.z, b.z, sy)};
+    h.x = fminf(fmaxf(h.x, 0.f), 1600.f);
+    h.y = fminf(fmaxf(h.y, 0.f), 1600.f);
+    h.z = fminf(fmaxf(h.z, 0.f), 1600.f);
+    return h;
+}
+
+__global__ __launch_bounds__(64) void k_reproject(PostK K, PostT T) {
+    const int x = int(blockIdx.x * 8 + threadIdx.x), y = int(blockIdx.y * 8 + threadIdx.y);
+    if (x >= int(K.W) || y >= int(K.H)) return;
+    const size_t i = size_t(y) * K.W + x;
+    const float4 cd = from_h4(T.src_d[i]), cs = from_h4(T.src_s[i]), ca = from_h4(T.src_a[i]);
+    if (K.progressive) {
+        const float t = 1.0f / 512.0f;
+        const float4 pd = from_h4(T.prev_d[i]), ps = from_h4(T.prev_s[i]), pa = from_h4(T.prev_a[i]);
+        T.acc_d[i] = to_h4(lerpf(pd.x, cd.x, t), lerpf(pd.y, cd.y, t), lerpf(pd.z, cd.z, t), lerpf(pd.w, cd.w, t));
+        T.acc_s[i] = to_h4(lerpf(ps.x, cs.x, t), lerpf(ps.y, cs.y, t), lerpf(ps.z, cs.z, t), lerpf(ps.w, cs.w, t));
+        T.acc_a[i] = to_h4(lerpf(pa.x, ca.x, t), lerpf(pa.y, ca.y, t), lerpf(pa.z, ca.z, t), lerpf(pa.w, ca.w, t));
+        return;
+    }
+    const float2 mv = T.motion[i];
+    const float fx = float(x) + mv.x, fy = float(y) + mv.y;
+    const int ppx = int(floorf(fx)), ppy = int(floorf(fy));
+    const bool inside = ppx < int(K.W) && ppy < int(K.H) && ppx >= -1 && ppy >= -1;
+    const uint32_t cur = T.obj0[i];
+    uint32_t p0 = ld_u32(T.obj1, ppx, ppy, K);
+    const bool miss = cur == 65535u;
+    const uint32_t moment = T.moment[i];
+    if (miss || !inside || moment > 0u) {
+        T.acc_d[i] = T.src_d[i];
+        T.acc_s[i] = T.src_s[i];
+        T.acc_a[i] = T.src_a[i];
+        return;
+    }
+    const f3 cn = ld3(from_h4(T.normal[i]));
+    float4 sd = make_float4(0.f, 0.f, 0.f, 0.f), ss = make_float4(0.f, 0.f, 0.f, 0.f);
+    float tw = 0.f, motion_in_range = 0.f;
+    for (int oy = -2; oy <= 2; ++oy)
+        for (int ox = -2; ox <= 2; ++ox) {
+            const int sx = x + ox, sy = y + oy;
+            const uint32_t pid = ld_u32(T.obj0, sx, sy, K);
+            const f3 wn = ld3(ld_h4(T.normal, sx, sy, K));
+            const bool center = ox == 0 && oy == 0;
+            const float dist = sqrtf(float(ox * ox + oy * oy));
+            const float w = calc_weight(dist, pid == cur, center, wn, cn);
+            const float4 a = ld_h4(T.src_d, sx, sy, K), b = ld_h4(T.src_s, sx, sy, K);
+            sd.x += a.x * w; sd.y += a.y * w; sd.z += a.z * w; sd.w += a.w * w;
+            ss.x += b.x * w; ss.y += b.y * w; ss.z += b.z * w; ss.w += b.w * w;
+            tw += w;
+            const int mx = x + ox * 2, my = y + oy * 2;
+            if (inb(mx, my, K)) {
+                const float2 m2 = T.motion[size_t(my) * K.W + mx];
+                motion_in_range += sqrtf(m2.x * m2.x + m2.y * m2.y);
+            }
+        }
+    sd.x /= tw; sd.y /= tw; sd.z /= tw; sd.w /= tw;
+    ss.x /= tw; ss.y /= tw; ss.z /= tw; ss.w /= tw;
+    uint32_t p1 = ld_u32(T.obj1, ppx + 1, ppy, K);
+    uint32_t p2 = ld_u32(T.obj1, ppx, ppy + 1, K);
+    uint32_t p3 = ld_u32(T.obj1, ppx + 1, ppy + 1, K);
+    if (fabsf(motion_in_range) < 0.1f) { p0 = cur; p1 = cur; p2 = cur; p3 = cur; }
+    const float subx = fx - floorf(fx), suby = fy - floorf(fy);
+    const f3 hd = filter_history(T.prev_d, subx, suby, ppx, ppy, cur, p0, p1, p2, p3, sd, K);
+    const f3 hs = filter_history(T.prev_s, subx, suby, ppx, ppy, cur, p0, p1, p2, p3, ss, K);
+    const f3 ha = filter_history(T.prev_a, subx, suby, ppx, ppy, cur, p0, p1, p2, p3, ca, K);
+    float keep = 1.0f / float(K.temporal > 1u ? K.temporal : 1u);
+    keep = fminf(fmaxf(keep, 0.f), 1.f);
+    // .rgb written, alpha keeps the accumulation target's previous value
+    const float4 od = from_h4(T.acc_d[i]), os = from_h4(T.acc_s[i]), oa = from_h4(T.acc_a[i]);
+    T.acc_d[i] = to_h4(lerpf(hd.x, cd.x, keep), lerpf(hd.y, cd.y, keep), lerpf(hd.z, cd.z, keep), od.w);
+    T.acc_s[i] = to_h4(lerpf(hs.x, cs.x, keep), lerpf(hs.y, cs.y, keep), lerpf(hs.z, cs.z, keep), os.w);
+    T.acc_a[i] = to_h4(lerpf(ha.x, ca.x, keep), lerpf(ha.y, ca.y, keep), lerpf(ha.z, ca.z, keep), oa.w);
+}
+
+// W_f / H_f / GranTurismoTonemapper, Const_Func.slang:84-121
+__device__ inline float w_f(float x, float e0, float e1) {
+    if (x <= e0) return 0.f;
+    if (x >= e1) return 1.f;
+    const float a = (x - e0) / (e1 - e0);
+    return a * a * (3.f - 2.f * a);
+}
+__device__ inline float h_f(float x, float e0, float e1) {
+    if (x <= e0) return 0.f;
+    if (x >= e1) return 1.f;
+    return (x - e0) / (e1 - e0);
+}
+__device__ inline float gt_tonemap(float x) {
+    const float P = 1.f, a = 0.7f, m = 0.22f, l = 0.4f, c = 1.33f, b = 0.f;
+    const float l0 = (P - m) * l / a;
+    const float L_x = m + a * (x - m);
+    const float T_x = m * powf(x / m, c) + b;
+    const float S0 = m + l0;
+    const float S1 = m + a * l0;
+    const float C2 = a * P / (P - S1);
+    const float S_x = P - (P - S1) * powf(2.71828f, -(C2 * (x - S0) / P));
+    const float w0 = 1.f - w_f(x, 0.f, m);
+    const float w2 = h_f(x, m + l0, m + l0);
+    const float w1 = 1.f - w0 - w2;
+    return T_x * w0 + L_x * w1 + S_x * w2;
+}
+
+constexpr int kR = 5, kTile = 8 + 2 * kR;
+
+__global__ __launch_bounds__(64) void k_jbf(PostK K, PostT T) {
+    __shared__ float3 sd[kTile][kTile];
+    __shared__ float3 ssp[kTile][kTile];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int gx = blockIdx.x * 8, gy = blockIdx.y * 8;
+    const int x = gx + tx, y = gy + ty;
+    f3 total;
+    if (K.bf_size > 0u) {
+        for (int idx = ty * 8 + tx; idx < kTile * kTile; idx += 64) {
+            const int row = idx / kTile, col = idx % kTile;
+            const int cx = gx + col - kR, cy = gy + row - kR;
+            const float4 d = ld_h4(T.acc_d, cx, cy, K), s = ld_h4(T.acc_s, cx, cy, K);
+            sd[row][col] = make_float3(d.x, d.y, d.z);
+            ssp[row][col] = make_float3(s.x, s.y, s.z);
+        }
+        __syncthreads();
+        if (x >= int(K.W) || y >= int(K.H)) return;
+        const float3 c3 = sd[ty + kR][tx + kR];
+        const f3 cc = f3{c3.x + 0.001f, c3.y + 0.001f, c3.z + 0.001f};
+        const float3 s3 = ssp[ty + kR][tx + kR];
+        const f3 cspec = f3{s3.x + 0.001f, s3.y + 0.001f, s3.z + 0.001f};
+        const f3 lc = f3{0.212671f, 0.715160f, 0.072169f};
+        const float clum = dot(cc, lc);
+        const float sigma = K.bf_sigma, sigma_l = K.bf_sigma_lum * 100.0f;
+        f3 acc = f3{0.f, 0.f, 0.f};
+        float wsum = 0.f;
+        for (int i = -kR; i <= kR; i += 2)
+            for (int j = -kR; j <= kR; j += 2) {
+                const float3 q = sd[ty + kR + i][tx + kR + j];
+                const f3 ci = f3{q.x + 0.001f, q.y + 0.001f, q.z + 0.001f};
+                const float lumi = dot(ci, lc);
+                float dist = float(i * i + j * j) / float(kR * kR);
+                dist = fminf(fmaxf(dist, 0.f), 1.f);
+                const float dl = (clum - lumi) * (clum - lumi);
+                const float Fi = expf(-dist * dist / (2.0f * sigma * sigma));
+                const float Li = expf(-dl * dl / (2.0f * sigma_l * sigma_l));
+                acc = acc + ci * (Fi * Li);
+                wsum += Fi * Li;
+            }
+        total = acc * (1.f / wsum);
+        const f3 alb = ld3(from_h4(T.acc_a[size_t(y) * K.W + x]));
+        total = total * alb + cspec;
+    } else {
+        if (x >= int(K.W) || y >= int(K.H)) return;
+        const size_t i = size_t(y) * K.W + x;
+        total = ld3(from_h4(T.acc_d[i])) * ld3(from_h4(T.acc_a[i])) + ld3(from_h4(T.acc_s[i]));
+    }
+    const float sc = K.paper_white / 40000.0f;
+    T.out[size_t(y) * K.W + x] = to_h4(gt_tonemap(total.x * sc), gt_tonemap(total.y * sc), gt_tonemap(total.z * sc), 1.f);
+}
+
+__global__ void k_copy_history(const ushort4* a, const ushort4* b, const ushort4* c, ushort4* pa, ushort4* pb,
+                               ushort4* pc, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pa[i] = a[i];
+    pb[i] = b[i];
+    pc[i] = c[i];
+}
+
+}  // namespace
+
+int post_process(Ctx& c) {
+    const uint32_t n = c.width * c.height;
+    // history images (RT_SINGLE_PREV_*): last frame's accumulation targets
+    auto& w = c.wf;
+    if (w.hist_d.count < n) {
+        GK_HIP(w.hist_d.resize(n));
+        GK_HIP(w.hist_s.resize(n));
+        GK_HIP(w.hist_a.resize(n));
+        GK_HIP(hipMemsetAsync(w.hist_d.ptr, 0, n * 8, c.stream));
+        GK_HIP(hipMemsetAsync(w.hist_s.ptr, 0, n * 8, c.stream));
+        GK_HIP(hipMemsetAsync(w.hist_a.ptr, 0, n * 8, c.stream));
+    }
+    ushort4* prev_d = w.hist_d.ptr;
+    ushort4* prev_s = w.hist_s.ptr;
+    ushort4* prev_a = w.hist_a.ptr;
+    PostK K;
+    K.W = c.width;
+    K.H = c.height;
+    K.progressive = c.params.progressive;
+    K.temporal = c.params.temporal_frames;
+    K.bf_size = c.params.denoiser == GK_DENOISER_BILATERAL ? c.params.bf_size : 0u;
+    K.hdr = c.params.hdr;
+    K.bf_sigma = c.params.bf_sigma;
+    K.bf_sigma_lum = c.params.bf_sigma_lum;
+    K.paper_white = c.params.paper_white_nit;
+    PostT T;
+    T.src_d = static_cast<const ushort4*>(c.target_ptr[GK_RT_SINGLE_DIFFUSE]);
+    T.src_s = static_cast<const ushort4*>(c.target_ptr[GK_RT_SINGLE_SPECULAR]);
+    T.src_a = static_cast<const ushort4*>(c.target_ptr[GK_RT_ALBEDO]);
+    T.normal = static_cast<const ushort4*>(c.target_ptr[GK_RT_NORMAL]);
+    T.prev_d = prev_d;
+    T.prev_s = prev_s;
+    T.prev_a = prev_a;
+    T.acc_d = static_cast<ushort4*>(c.target_ptr[GK_RT_ACCUM_DIFFUSE]);
+    T.acc_s = static_cast<ushort4*>(c.target_ptr[GK_RT_ACCUM_SPECULAR]);
+    T.acc_a = static_cast<ushort4*>(c.target_ptr[GK_RT_ACCUM_ALBEDO]);
+    T.obj0 = static_cast<const uint32_t*>(c.target_ptr[GK_RT_OBJECTID_0]);
+    T.obj1 = static_cast<const uint32_t*>(c.target_ptr[GK_RT_OBJECTID_1]);
+    T.motion = static_cast<const float2*>(c.target_ptr[GK_RT_MOTIONVECTOR]);
+    T.moment = static_cast<const uint16_t*>(c.target_ptr[GK_RT_MOTIONMOMENT]);
+    T.out = static_cast<ushort4*>(c.target_ptr[GK_RT_DENOISED]);
+    const dim3 grid((c.width + 7) / 8, (c.height + 7) / 8), blk(8, 8);
+    hipEventRecord(c.ev[4], c.stream);
+    k_reproject<<<grid, blk, 0, c.stream>>>(K, T);
+    GK_HIP(hipGetLastError());
+    hipEventRecord(c.ev[5], c.stream);
+    if (c.params.denoiser == GK_DENOISER_UNET) {
+        const int s = unet_denoise_frame(c);
+        if (s != GK_OK) return s;
+    } else {
+        k_jbf<<<grid, blk, 0, c.stream>>>(K, T);
+        GK_HIP(hipGetLastError());
+    }
+    hipEventRecord(c.ev[6], c.stream);
+    k_copy_history<<<(n + 255) / 256, 256, 0, c.stream>>>(T.acc_d, T.acc_s, T.acc_a, prev_d, prev_s, prev_a, n);
+    GK_HIP(hipGetLastError());
+    return GK_OK;
+}
+
+}  // namespace gk

@@ -1,0 +1,380 @@
+// trace.hip - software ray traversal for gfx950 (no RT cores on CDNA4).
+//
+// Replaces the hardware RayQuery of FHardwareRayTracerV2::TraceRay / TraceOcclusion
+// (Shading.slang:735-827) and the CPU ray cast BVH::Intersect / IntersectTLAS /
+// IsOccluded (tiny_bvh.h:2245-2397).
+//
+// Design
+//   * 4-wide BVH, one 128-byte node per visit (gk_math.hpp Bvh4Node), 48-byte triangle
+//     records stored in leaf order.
+//   * Persistent waves with dynamic ray fetch: lanes whose ray has finished are refilled
+//     together -- one ballot, one atomicAdd per wave, ranks from mbcnt -- so a wave keeps
+//     all 64 lanes busy until the queue drains (wave-level compaction).
+//   * Traversal stack: a 16-entry ring per lane in LDS; when a push would overwrite the
+//     oldest entry that entry is spilled to a per-thread global array (coalesced
+//     [entry][thread] layout) and read back when the ring runs empty.
+//   * Two-level scenes: TLAS leaves push instance entries; entering an instance pushes a
+//     return mark and continues in the instance's BLAS with the ray transformed by the
+//     same inverse-matrix arithmetic tinybvh uses (tiny_bvh.h:396-409,2309-2316).
+//   * Triangle test: Moeller-Trumbore with exactly tinybvh's operations and rejects
+//     (tiny_bvh.h:6815-6844). Closest hit keeps the smallest t, ties broken towards the
+//     lower (instance, primitive), so results do not depend on traversal order. Box
+//     tests are widened by 2^-20 relative so they never cull an accepted triangle.
+#include <hip/hip_runtime.h>
+
+#include "gk_ctx.hpp"
+
+namespace gk {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRing = 16;            // LDS stack entries per lane
+constexpr int kSpill = 64;           // global spill entries per thread
+constexpr int kBlocksPerCU = 4;
+
+struct Lane {
+    f3 O, D, rD;
+    f3 wO, wD, wrD;                  // world-space ray (two-level)
+    float tmin, tmax;
+    float bt, bu, bv;
+    uint32_t bprim, binst;
+    uint32_t node;                   // next node to visit, or kNone
+    uint32_t cur_inst;
+    uint32_t level;                  // 0 = TLAS, 1 = BLAS
+    uint32_t head, count, gsp;       // ring head, entries in ring, spilled entries
+    uint32_t ray;
+    bool active, found, overflow;
+};
+
+constexpr uint32_t kNone = 0xfffffffeu;
+
+__device__ inline void push(Lane& L, uint32_t (*ring)[kBlock], uint32_t* spill, uint32_t gtid,
+                            uint32_t stride, uint32_t v) {
+    const uint32_t lane = threadIdx.x;
+    if (L.count == kRing) {
+        // ring full: the slot at head holds the oldest entry; move it to global memory
+        if (L.gsp < kSpill) spill[size_t(L.gsp) * stride + gtid] = ring[L.head][lane];
+        else L.overflow = true;
+        L.gsp++;
+        ring[L.head][lane] = v;
+        L.head = (L.head + 1) & (kRing - 1);
+    } else {
+        ring[L.head][lane] = v;
+        L.head = (L.head + 1) & (kRing - 1);
+        L.count++;
+    }
+}
+
+__device__ inline bool pop(Lane& L, uint32_t (*ring)[kBlock], const uint32_t* spill, uint32_t gtid,
+                           uint32_t stride, uint32_t& v) {
+    if (L.count > 0) {
+        L.head = (L.head - 1) & (kRing - 1);
+        L.count--;
+        v = ring[L.head][threadIdx.x];
+        return true;
+    }
+    if (L.gsp > 0) {
+        L.gsp--;
+        v = L.gsp < kSpill ? spill[size_t(L.gsp) * stride + gtid] : 0u;
+        return true;
+    }
+    return false;
+}
+
+// tinybvh_transform_point / _vector with a row-major 4x4 (tiny_bvh.h:396-409)
+__device__ inline f3 tpoint(f3 v, const float* M) {
+    const f3 r = f3{M[0] * v.x + M[1] * v.y + M[2] * v.z + M[3],
+                    M[4] * v.x + M[5] * v.y + M[6] * v.z + M[7],
+                    M[8] * v.x + M[9] * v.y + M[10] * v.z + M[11]};
+    const float w = M[12] * v.x + M[13] * v.y + M[14] * v.z + M[15];
+    if (w == 1.0f) return r;
+    return r * (1.f / w);
+}
+__device__ inline f3 tvector(f3 v, const float* M) {
+    return f3{M[0] * v.x + M[1] * v.y + M[2] * v.z, M[4] * v.x + M[5] * v.y + M[6] * v.z,
+              M[8] * v.x + M[9] * v.y + M[10] * v.z};
+}
+
+// widen an interval end by 2^-20 relative (also well defined for +-inf)
+__device__ inline float widen_hi(float x) { return x > 0.f ? x * 1.00000095367431640625f : x * 0.99999904632568359375f; }
+__device__ inline float widen_lo(float x) { return x > 0.f ? x * 0.99999904632568359375f : x * 1.00000095367431640625f; }
+
+// slab test of the 4 children; returns entry distances (or +inf when missed)
+__device__ inline void slab4(const Bvh4Node& n, const Lane& L, float limit, float tl[4]) {
+    const float lx[4] = {n.lox.x, n.lox.y, n.lox.z, n.lox.w};
+    const float hx[4] = {n.hix.x, n.hix.y, n.hix.z, n.hix.w};
+    const float ly[4] = {n.loy.x, n.loy.y, n.loy.z, n.loy.w};
+    const float hy[4] = {n.hiy.x, n.hiy.y, n.hiy.z, n.hiy.w};
+    const float lz[4] = {n.loz.x, n.loz.y, n.loz.z, n.loz.w};
+    const float hz[4] = {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float a = (lx[i] - L.O.x) * L.rD.x, b = (hx[i] - L.O.x) * L.rD.x;
+        float lo = fminf(a, b), hi = fmaxf(a, b);
+        a = (ly[i] - L.O.y) * L.rD.y; b = (hy[i] - L.O.y) * L.rD.y;
+        lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
+        a = (lz[i] - L.O.z) * L.rD.z; b = (hz[i] - L.O.z) * L.rD.z;
+        lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
+        lo = widen_lo(lo);
+        hi = widen_hi(hi);
+        tl[i] = (hi >= lo && hi >= L.tmin && lo <= limit) ? lo : __builtin_inff();
+    }
+}
+
+// Moeller-Trumbore, tinybvh IntersectTri operation order (tiny_bvh.h:6818-6833)
+__device__ inline bool mt(const TriRec& tr, f3 O, f3 D, float& t, float& u, float& v) {
+    const f3 v0 = ld3(tr.v0), e1 = ld3(tr.e1), e2 = ld3(tr.e2);
+    const f3 h = cross(D, e2);
+    const float a = dot(e1, h);
+    if (fabsf(a) < 0.0000001f) return false;
+    const float f = 1 / a;
+    const f3 s = O - v0;
+    u = f * dot(s, h);
+    if (u < 0 || u > 1) return false;
+    const f3 q = cross(s, e1);
+    v = f * dot(D, q);
+    if (v < 0 || u + v > 1) return false;
+    t = f * dot(e2, q);
+    return true;
+}
+
+template <bool ANY, bool TWO>
+__global__ __launch_bounds__(kBlock) void k_trace(
+    const Bvh4Node* __restrict__ blas, const TriRec* __restrict__ tris,
+    const Bvh4Node* __restrict__ tlas, const uint32_t* __restrict__ tlas_prims,
+    const InstDev* __restrict__ inst, uint32_t blas_root,
+    const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
+    const uint32_t* __restrict__ count_dev, uint32_t max_count, uint32_t* __restrict__ next_ray,
+    float4* __restrict__ hit_out, uint32_t* __restrict__ inst_out, uint32_t* __restrict__ spill,
+    uint32_t* __restrict__ overflow_flag) {
+    __shared__ uint32_t ring[kRing][kBlock];
+    const uint32_t stride = gridDim.x * kBlock;
+    const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t count = max_count;
+    if (count_dev) { const uint32_t c = *count_dev; count = c < max_count ? c : max_count; }
+
+    Lane L;
+    L.active = false;
+    L.overflow = false;
+    bool exhausted = false;
+    for (;;) {
+        // ---- refill idle lanes: one atomic per wave, ranks from the ballot
+        if (!exhausted) {
+            const uint64_t need = __ballot(!L.active);
+            const uint32_t n_need = __popcll(need);
+            if (n_need >= 16 || n_need == 64) {
+                uint32_t base = 0;
+                if (__lane_id() == __ffsll((unsigned long long)need) - 1) base = atomicAdd(next_ray, n_need);
+                base = __shfl(base, __ffsll((unsigned long long)need) - 1);
+                if (base >= count) exhausted = true;
+                if (!L.active) {
+                    const uint32_t rank = __popcll(need & ((1ull << __lane_id()) - 1ull));
+                    const uint32_t r = base + rank;
+                    if (r < count) {
+                        const float4 o = ray_o[r], d = ray_d[r];
+                        L.ray = r;
+                        L.O = ld3(o);
+                        L.D = ld3(d);
+                        L.tmin = o.w;
+                        L.tmax = d.w;
+                        L.rD = f3{safercp(L.D.x), safercp(L.D.y), safercp(L.D.z)};
+                        L.wO = L.O; L.wD = L.D; L.wrD = L.rD;
+                        L.bt = L.tmax; L.bu = 0.f; L.bv = 0.f;
+                        L.bprim = 0xffffffffu; L.binst = 0xffffffffu;
+                        L.found = false;
+                        L.head = 0; L.count = 0; L.gsp = 0;
+                        L.level = TWO ? 0u : 1u;
+                        L.cur_inst = 0;
+                        L.node = TWO ? 0u : blas_root;
+                        L.active = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(L.active) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (!L.active) continue;
+
+        // ---- one traversal step
+        if (L.node == kNone) {
+            uint32_t e;
+            bool done = false;
+            for (;;) {
+                if (!pop(L, ring, spill, gtid, stride, e)) { done = true; break; }
+                if (TWO && e == kStackMark) {  // leave the instance
+                    L.O = L.wO; L.D = L.wD; L.rD = L.wrD;
+                    L.level = 0;
+                    continue;
+                }
+                if (TWO && (e & kInstFlag)) {  // enter an instance
+                    const uint32_t ii = e & ~kInstFlag;
+                    const InstDev& I = inst[ii];
+                    L.O = tpoint(L.wO, I.inv);
+                    L.D = tvector(L.wD, I.inv);
+                    L.rD = f3{safercp(L.D.x), safercp(L.D.y), safercp(L.D.z)};
+                    L.cur_inst = ii;
+                    L.level = 1;
+                    push(L, ring, spill, gtid, stride, kStackMark);
+                    L.node = I.root;
+                    break;
+                }
+                L.node = e;
+                break;
+            }
+            if (done) {
+                // ---- ray finished
+                if (ANY) {
+                    hit_out[L.ray] = make_float4(0.f, 0.f, 0.f, __uint_as_float(L.found ? 1u : 0u));
+                } else {
+                    hit_out[L.ray] = L.found ? make_float4(L.bt, L.bu, L.bv, __uint_as_float(L.bprim))
+                                             : make_float4(__builtin_inff(), 0.f, 0.f, __uint_as_float(0xffffffffu));
+                    if (inst_out) inst_out[L.ray] = L.found ? L.binst : 0xffffffffu;
+                }
+                if (L.overflow) atomicOr(overflow_flag, 1u);
+                L.active = false;
+                continue;
+            }
+        }
+
+        const bool tl_level = TWO && L.level == 0;
+        const Bvh4Node& N = tl_level ? tlas[L.node] : blas[L.node];
+        const Bvh4Node n = N;
+        float tl[4];
+        slab4(n, L, L.bt, tl);
+        const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+        const uint32_t mt4[4] = {n.meta.x, n.meta.y, n.meta.z, n.meta.w};
+        bool terminate = false;
+        // leaves first
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!(tl[i] < __builtin_inff()) || mt4[i] == 0u) continue;
+            if (tl_level) {
+                for (uint32_t k = 0; k < mt4[i]; ++k)
+                    push(L, ring, spill, gtid, stride, kInstFlag | tlas_prims[ch[i] + k]);
+                continue;
+            }
+            for (uint32_t k = 0; k < mt4[i]; ++k) {
+                const TriRec tr = tris[ch[i] + k];
+                float t, u, v;
+                if (!mt(tr, L.O, L.D, t, u, v)) continue;
+                if (!(t > L.tmin && t < L.tmax)) continue;
+                const uint32_t prim = __float_as_uint(tr.v0.w);
+                if (ANY) { L.found = true; terminate = true; break; }
+                const uint32_t ci = TWO ? L.cur_inst : 0u;
+                if (!L.found || t < L.bt ||
+                    (t == L.bt && (ci < L.binst || (ci == L.binst && prim < L.bprim)))) {
+                    L.bt = t; L.bu = u; L.bv = v; L.bprim = prim; L.binst = ci; L.found = true;
+                }
+            }
+            if (ANY && terminate) break;
+        }
+        if (ANY && terminate) {
+            L.count = 0; L.gsp = 0; L.node = kNone;
+            continue;
+        }
+        // internal children, near to far
+        float kt[4];
+        uint32_t kc[4];
+        int nk = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (mt4[i] == 0u && tl[i] <= L.bt) { kt[nk] = tl[i]; kc[nk] = ch[i]; nk++; }
+        }
+        // insertion sort (<= 4 entries)
+        for (int i = 1; i < nk; ++i) {
+            const float t = kt[i];
+            const uint32_t c = kc[i];
+            int j = i - 1;
+            while (j >= 0 && kt[j] > t) { kt[j + 1] = kt[j]; kc[j + 1] = kc[j]; --j; }
+            kt[j + 1] = t;
+            kc[j + 1] = c;
+        }
+        if (nk == 0) {
+            L.node = kNone;
+        } else {
+            for (int i = nk - 1; i >= 1; --i) push(L, ring, spill, gtid, stride, kc[i]);
+            L.node = kc[0];
+        }
+    }
+}
+
+__global__ void k_reset_counter(uint32_t* p) { *p = 0u; }
+
+// user gk_ray/gk_hit records <-> queue layout
+__global__ void k_unpack_rays(const gk_ray* __restrict__ r, uint32_t n, float4* o, float4* d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const gk_ray x = r[i];
+    o[i] = make_float4(x.origin[0], x.origin[1], x.origin[2], x.tmin);
+    d[i] = make_float4(x.dir[0], x.dir[1], x.dir[2], x.tmax);
+}
+
+uint32_t persistent_blocks(int device) {
+    static int cus = 0;
+    if (!cus) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, device) == hipSuccess) cus = p.multiProcessorCount;
+        if (cus <= 0) cus = 256;
+    }
+    return uint32_t(cus) * kBlocksPerCU;
+}
+
+}  // namespace
+
+int ensure_spill(Ctx& c, uint32_t threads) {
+    const size_t need = size_t(threads) * kSpill;
+    if (c.wf.spill.count < need + 2) GK_HIP(c.wf.spill.resize(need + 2));
+    c.wf.spill_threads = threads;
+    return GK_OK;
+}
+
+// Trace a queue of rays. count_dev (device) caps the number of rays when non-null.
+// hit: float4 (t,u,v,prim) per ray (any-hit: w = 1 when occluded).
+int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev,
+                 uint32_t max_count, float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t*) {
+    if (max_count == 0) return GK_OK;
+    const uint32_t blocks = persistent_blocks(c.device);
+    const uint32_t threads = blocks * kBlock;
+    int st = ensure_spill(c, threads);
+    if (st != GK_OK) return st;
+    uint32_t* next = c.wf.spill.ptr + size_t(threads) * kSpill;  // 2 words after the spill area
+    uint32_t* ovf = next + 1;
+    k_reset_counter<<<1, 1, 0, c.stream>>>(next);
+    const bool two = !c.blas_only;
+    const uint32_t root = c.h_meshes.empty() ? 0u : c.h_meshes[0].root;
+    const Bvh4Node* tl = c.tlas_nodes.ptr;
+    if (any_hit) {
+        if (two)
+            k_trace<true, true><<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, tl, c.tlas_prims.ptr,
+                c.inst.ptr, root, ray_o, ray_d, count_dev, max_count, next, hit, hit_inst, c.wf.spill.ptr, ovf);
+        else
+            k_trace<true, false><<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, tl, c.tlas_prims.ptr,
+                c.inst.ptr, root, ray_o, ray_d, count_dev, max_count, next, hit, hit_inst, c.wf.spill.ptr, ovf);
+    } else {
+        if (two)
+            k_trace<false, true><<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, tl, c.tlas_prims.ptr,
+                c.inst.ptr, root, ray_o, ray_d, count_dev, max_count, next, hit, hit_inst, c.wf.spill.ptr, ovf);
+        else
+            k_trace<false, false><<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, tl, c.tlas_prims.ptr,
+                c.inst.ptr, root, ray_o, ray_d, count_dev, max_count, next, hit, hit_inst, c.wf.spill.ptr, ovf);
+    }
+    GK_HIP(hipGetLastError());
+    return GK_OK;
+}
+
+int launch_trace_user(Ctx& c, const gk_ray* rays, uint32_t n, gk_hit* hits, uint32_t* inst, uint32_t flags) {
+    if (n == 0) return GK_OK;
+    // staging in the wavefront queue 0 (grown on demand)
+    auto& w = c.wf;
+    if (w.ray_o[0].count < n) { GK_HIP(w.ray_o[0].resize(n)); GK_HIP(w.ray_d[0].resize(n)); }
+    k_unpack_rays<<<(n + 255) / 256, 256, 0, c.stream>>>(rays, n, w.ray_o[0].ptr, w.ray_d[0].ptr);
+    GK_HIP(hipGetLastError());
+    static_assert(sizeof(gk_hit) == sizeof(float4), "gk_hit is a float4");
+    return launch_trace(c, w.ray_o[0].ptr, w.ray_d[0].ptr, nullptr, n, reinterpret_cast<float4*>(hits), inst,
+                        (flags & GK_TRACE_ANY) != 0, nullptr);
+}
+
+}  // namespace gk

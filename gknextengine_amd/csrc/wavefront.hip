@@ -1,0 +1,1080 @@
+// wavefront.hip - wavefront path tracer (the "rt pass" of PathTracingRenderer::Render,
+// PathTracingRenderer.cpp:104-218, whose megakernel is Core.PathTracing.comp.slang:5-31).
+//
+// One path per pixel lives in SoA state buffers; each bounce is
+//     trace(extension queue) -> [sort by material] -> shade -> trace(shadow queue) -> resolve
+// with queues compacted by wave-level ballots (one atomic per wave), so that no host
+// synchronisation happens inside a frame. Two estimators:
+//   REF_V2  FPathTracingRendererV2 (Shading.slang:995-1241) step by step: its
+//           GetRayColor is split at the trace call into "begin" (material, RNG draws,
+//           new direction) and "finish" (hit albedo, back-face and emitter rules); the
+//           sample loop, sun cone, early exit and DirectIlluminate follow Render.
+//           Probe termination returns 0 (no probes are baked).
+//   PT_MIS  GGX (bounded VNDF) + Lambert with next-event estimation to emissive
+//           triangles, power-heuristic MIS, Russian roulette after 3 bounces.
+// Every RNG draw happens in the same order as in the per-pixel formulation, so a path's
+// random numbers do not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "gk_scene.hpp"
+
+namespace gk {
+
+int post_process(Ctx& c);  // post.hip
+int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev, uint32_t max_count,
+                 float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t* occluded_flags);
+
+namespace {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kEps = 1e-3f;                 // PreProcessor.slang:13
+constexpr float kMaxTrace = 1000.f;           // Shading.slang:15
+constexpr uint32_t kMiss = 0xffffffffu;
+
+// per-frame constants
+struct FrameK {
+    float mvi[16], pi[16], vp[16], vpu[16], pvpu[16];
+    float sun_dir[4], sun_col[4], sky[4];
+    float exit_prob, hit_off;
+    uint32_t W, H, frame, seed, samples, bounces, max_bounces, temporal, has_sky, has_sun, tiled;
+};
+
+struct Targets {
+    ushort4* diffuse;
+    ushort4* spec;
+    ushort4* albedo;
+    ushort4* normal;
+    uint32_t* objid;
+    float2* motion;
+    uint16_t* moment;
+    float* depth;
+    uint32_t* minigbuffer;
+    float4* phit;
+    uint32_t* pinst;
+    float4* radiance;
+};
+
+struct Queues {
+    float4* o[2];
+    float4* d[2];
+    uint32_t* path[2];
+    float4* hit;
+    uint32_t* hit_inst;
+    float4* so;
+    float4* sd;
+    float4* sc_d;   // shadow contribution to Ld (rgb)
+    float4* sc_s;   // shadow contribution to Ls (rgb)
+    uint32_t* spath;
+    uint32_t* cnt;  // [0],[1] extension queue sizes, [2] shadow queue size, [4..] ray statistics
+};
+
+struct PathState {
+    float4* thr;    // REF_V2: RayColor; PT_MIS: throughput + last pdf (w)
+    float4* ld;     // PT_MIS: primary-diffuse radiance; REF_V2: FinalColor
+    float4* ls;     // PT_MIS: other radiance; REF_V2: FinalReflection
+    uint4* rng;
+    uint4* misc;    // x pixel, y flags, z counters, w material of the current vertex
+    float4* pos;    // current vertex position (w: primary material bits for REF_V2)
+    float4* nrm;    // current vertex normal
+    float4* dir;    // current incoming direction
+    float4* ppos;   // REF_V2 primary vertex (pulled back), w = primary material bits
+    float4* pnrm;   // REF_V2 primary normal
+};
+
+__device__ inline uint32_t pixel_of(const FrameK& K, uint32_t p) {
+    if (!K.tiled) return p;
+    const uint32_t tile = p >> 6, within = p & 63u;
+    const uint32_t tx_n = K.W >> 3;
+    const uint32_t tx = tile % tx_n, ty = tile / tx_n;
+    const uint32_t x = (tx << 3) | (within & 7u), y = (ty << 3) | (within >> 3);
+    return y * K.W + x;
+}
+
+// FVisibilityBufferRayCasterV2 ray (Shading.slang:350-356): uv = ipos/size*2-1
+__device__ inline void camera_ray(const FrameK& K, uint32_t x, uint32_t y, f3& O, f3& D) {
+    const float ux = (float(x) / float(K.W)) * 2.0f - 1.0f;
+    const float uy = (float(y) / float(K.H)) * 2.0f - 1.0f;
+    float o[4], t[4], d[4];
+    mul44v(K.mvi, 0.f, 0.f, 0.f, 1.f, o);
+    mul44v(K.pi, ux, uy, 1.f, 1.f, t);
+    const f3 tn = normalize(f3{t[0], t[1], t[2]});
+    mul44v(K.mvi, tn.x, tn.y, tn.z, 0.f, d);
+    O = f3{o[0], o[1], o[2]};
+    D = normalize(f3{d[0], d[1], d[2]});
+}
+
+// append to a queue: one atomic per wave
+__device__ inline uint32_t wave_append(uint32_t* counter, bool want) {
+    const uint64_t m = __ballot(want);
+    if (m == 0ull) return 0u;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (__lane_id() == leader) base = atomicAdd(counter, uint32_t(__popcll(m)));
+    base = __shfl(base, leader);
+    return base + uint32_t(__popcll(m & ((1ull << __lane_id()) - 1ull)));
+}
+
+__device__ inline void count_rays(uint32_t* cnt, int slot, bool want) {
+    const uint64_t m = __ballot(want);
+    if (m && __lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(&cnt[slot], uint32_t(__popcll(m)));
+}
+
+// ---------------------------------------------------------------- sampling (REF_V2, float)
+__device__ inline float schlick(float cosine, float ior) {
+    float r0 = (1.f - ior) / (1.f + ior);
+    r0 = r0 * r0;
+    const float x = 1.f - cosine;
+    return r0 + (1.f - r0) * (x * x * x * x * x);
+}
+__device__ inline void onb(f3 n, f3& b1, f3& b2) {
+    const float sz = n.z < 0.f ? -1.f : 1.f;
+    const float a = -1.0f / (sz + n.z);
+    b2 = f3{n.x * n.y * a, sz + n.y * n.y * a, -n.y};
+    b1 = f3{1.0f + sz * n.x * n.x * a, sz * b2.x, -sz * n.x};
+}
+__device__ inline f3 to_world(f3 v, f3 T, f3 B, f3 N) {
+    return f3{v.x * T.x + v.y * B.x + v.z * N.x, v.x * T.y + v.y * B.y + v.z * N.y, v.x * T.z + v.y * B.z + v.z * N.z};
+}
+__device__ inline f3 align_with_normal(f3 r, f3 n) {
+    f3 T, B;
+    onb(n, T, B);
+    return to_world(r, T, B, n);
+}
+__device__ inline f3 random_hemisphere1(rng4& s) {
+    float ux, uy;
+    rnd2(s, ux, uy);
+    float sp, cp;
+    sincos_2pi(ux, sp, cp);
+    const float r = sqrtf(uy);
+    return f3{r * cp, r * sp, sqrtf(1.0f - uy)};
+}
+__device__ inline f3 random_in_cone(rng4& s, float cos_theta) {
+    float ux, uy;
+    rnd2(s, ux, uy);
+    float sp, cp;
+    sincos_2pi(ux, sp, cp);
+    const float c = 1.0f + uy * (cos_theta - 1.f);
+    const float r = sqrtf(1.0f - c * c);
+    return f3{r * cp, r * sp, c};
+}
+__device__ inline float saturate(float x) { return x < 0.f ? 0.f : (x > 1.f ? 1.f : x); }
+__device__ inline f3 ggx_sample_vndf(float alpha, f3 wi_, float ux, float uy) {
+    const f3 wi = normalize(f3{wi_.x * alpha, wi_.y * alpha, wi_.z});
+    float b = wi.z;
+    if (wi_.z > 0.f) {
+        const float a = saturate(alpha);
+        const float awiz_s = a * wi_.z / (1.0f + sqrtf(wi_.x * wi_.x + wi_.y * wi_.y));
+        b = b * ((1.0f - a * a) / (1.0f + awiz_s * awiz_s));
+    }
+    const float z = fmaf(1.0f - uy, 1.0f + b, -b);
+    float sp, cp;
+    sincos_2pi(ux, sp, cp);
+    const float r = sqrtf(saturate(1.0f - z * z));
+    const f3 m = wi + f3{r * cp, r * sp, z};
+    return normalize(f3{m.x * alpha, m.y * alpha, m.z});
+}
+__device__ inline f3 ggx_sampling(rng4& s, float roughness, f3 normal) {
+    f3 T, B;
+    onb(normal, T, B);
+    const float alpha = roughness * roughness;
+    const f3 wl = f3{dot(T, normal), dot(B, normal), dot(normal, normal)};
+    float ux, uy;
+    rnd2(s, ux, uy);
+    return to_world(ggx_sample_vndf(alpha, wl, ux, uy), T, B, normal);
+}
+__device__ inline f3 reflect(f3 i, f3 n) { return i - n * (2.f * dot(n, i)); }
+__device__ inline f3 refract(f3 i, f3 n, float eta) {
+    const float d = dot(n, i);
+    const float k = 1.f - eta * eta * (1.f - d * d);
+    if (k < 0.f) return f3{0.f, 0.f, 0.f};
+    return i * eta - n * (eta * d + sqrtf(k));
+}
+
+// FHardwareRayTracerV2 hit vertex (Shading.slang:803-821)
+__device__ inline void hit_vertex(const DScene& s, uint32_t inst, uint32_t prim, float t, float u, float v, f3 O,
+                                  f3 D, f3& pos, f3& nrm) {
+    const InstDev& I = s.inst[inst];
+    const size_t b = 3 * size_t(I.tri_base + prim);
+    const f3 n0 = ld3(s.tri_nrm[b]), n1 = ld3(s.tri_nrm[b + 1]), n2 = ld3(s.tri_nrm[b + 2]);
+    const f3 mix = (n0 + (n1 - n0) * u) + (n2 - n0) * v;
+    nrm = normalize(inv_transpose(I.inv, mix));
+    pos = O + D * t;
+}
+
+// get_material_dataV2 normal (GeneralFunc.slang:20-53): bx*n0 + (by*n1 + bz*n2), world normals
+__device__ inline f3 primary_normal(const DScene& s, uint32_t inst, uint32_t prim, float u, float v) {
+    const InstDev& I = s.inst[inst];
+    const size_t b = 3 * size_t(I.tri_base + prim);
+    float a[4], bb[4], cc[4];
+    const float4 n0 = s.tri_nrm[b], n1 = s.tri_nrm[b + 1], n2 = s.tri_nrm[b + 2];
+    mul44v(I.world, n0.x, n0.y, n0.z, 0.f, a);
+    mul44v(I.world, n1.x, n1.y, n1.z, 0.f, bb);
+    mul44v(I.world, n2.x, n2.y, n2.z, 0.f, cc);
+    const float by = u, bz = v, bx = 1.0f - (by + bz);
+    const f3 nb = f3{a[0], a[1], a[2]} * bx + (f3{bb[0], bb[1], bb[2]} * by + f3{cc[0], cc[1], cc[2]} * bz);
+    return normalize(nb);
+}
+
+__device__ inline void mul44m(const float* a, const float* b, float* o) {
+    for (int cc = 0; cc < 4; ++cc)
+        for (int r = 0; r < 4; ++r) {
+            float acc = a[r] * b[cc * 4 + 0];
+            acc = acc + a[4 + r] * b[cc * 4 + 1];
+            acc = acc + a[8 + r] * b[cc * 4 + 2];
+            acc = acc + a[12 + r] * b[cc * 4 + 3];
+            o[cc * 4 + r] = acc;
+        }
+}
+
+// PrimaryHit G-buffer (Shading.slang:1204-1215) + CalculateMotionVectorV2 (:50-72)
+__device__ inline void write_gbuffer(const DScene& s, const FrameK& K, const Targets& T, uint32_t px, uint32_t inst,
+                                     uint32_t prim, float4 hit, f3 pos, f3 nrm, const gk_material& m,
+                                     uint32_t& moment_out) {
+    float clip[4];
+    mul44v(K.vp, pos.x, pos.y, pos.z, 1.f, clip);
+    T.depth[px] = clip[2] / clip[3];
+    const InstDev& I = s.inst[inst];
+    float ch[4], ph[4], pp[4], mm[16];
+    mul44v(K.vpu, pos.x, pos.y, pos.z, 1.f, ch);
+    mul44m(K.pvpu, I.combined, mm);
+    mul44v(mm, pos.x, pos.y, pos.z, 1.f, ph);
+    const float cfx = ch[0] / ch[3] * 0.5f, cfy = ch[1] / ch[3] * 0.5f;
+    const float pfx = ph[0] / ph[3] * 0.5f, pfy = ph[1] / ph[3] * 0.5f;
+    mul44v(I.combined, pos.x, pos.y, pos.z, 1.f, pp);
+    const f3 dpos = pos - f3{pp[0], pp[1], pp[2]};
+    uint32_t mo = T.moment[px];
+    if (length(dpos) > 0.02f) mo = K.temporal;
+    else mo = mo > 0u ? mo - 1u : 0u;
+    T.moment[px] = uint16_t(mo);
+    moment_out = mo;
+    T.motion[px] = make_float2((pfx - cfx) * float(K.W), (pfy - cfy) * float(K.H));
+    T.albedo[px] = to_h4(m.diffuse[0], m.diffuse[1], m.diffuse[2], m.diffuse[3]);
+    T.normal[px] = to_h4(nrm.x, nrm.y, nrm.z, m.fuzziness);
+    T.objid[px] = I.instance_id;
+    T.minigbuffer[px] = ((inst + 1u) << 17) | (prim & 0x1ffffu);
+    T.phit[px] = hit;
+    T.pinst[px] = inst;
+}
+
+__device__ inline void write_miss_gbuffer(const FrameK& K, const Targets& T, uint32_t px, float4 hit) {
+    T.motion[px] = make_float2(0.f, 0.f);
+    T.albedo[px] = to_h4(1.f, 1.f, 1.f, 1.f);
+    T.normal[px] = to_h4(0.f, 1.f, 0.f, 1.f);
+    T.objid[px] = 65535u;
+    T.depth[px] = 1e5f;
+    T.minigbuffer[px] = 0u;
+    T.phit[px] = hit;
+    T.pinst[px] = kMiss;
+}
+
+// ---------------------------------------------------------------- raygen
+__global__ void k_raygen(FrameK K, Queues Q, PathState S, uint32_t n) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) { Q.cnt[0] = n; Q.cnt[1] = 0u; Q.cnt[2] = 0u; }
+    if (p >= n) return;
+    const uint32_t px = pixel_of(K, p);
+    const uint32_t x = px % K.W, y = px / K.W;
+    f3 O, D;
+    camera_ray(K, x, y, O, D);
+    Q.o[0][p] = make_float4(O.x, O.y, O.z, 0.f);
+    Q.d[0][p] = make_float4(D.x, D.y, D.z, 1e30f);
+    Q.path[0][p] = p;
+    S.rng[p] = make_uint4(x, y, K.frame, K.seed);
+    S.misc[p] = make_uint4(px, 0u, 0u, 0u);
+    S.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    S.ld[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    S.ls[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void k_reset_counter(uint32_t* p) { *p = 0u; }
+__global__ void k_zero_stats(uint32_t* cnt) { if (threadIdx.x < 8) cnt[4 + threadIdx.x] = 0u; }
+
+// ================================================================ REF_V2
+// flags (misc.y): bit0 chanceReflect of the pending GetRayColor, bit1 BackFace,
+// bit2 origin material is dielectric, bit3 hitReflect (first call), bit4 hitMetal (first call)
+// counters (misc.z): bits 0-7 bounce b of the pending call, 8-19 sample, 20-31 sample count
+
+// GetRayColor up to the trace (Shading.slang:1007-1036); emits the extension ray.
+__device__ inline void ref_begin(const DScene& s, const FrameK& K, const Queues& Q, const PathState& S, uint32_t p,
+                                 f3 pos, f3 nrm, uint32_t mi, f3 dir, rng4& seed, uint32_t& flags, uint32_t b, int nq,
+                                 bool& emit, float4& ro, float4& rd) {
+    const gk_material& mat = s.mats[mi];
+    const bool diel = mat.material_model == GK_MATERIAL_DIELECTRIC;
+    const float start_off = diel ? 0.0f : 1.0f;
+    const float roughness = mat.fuzziness;
+    const float dv = dot(dir, nrm);
+    const bool back = dv > 0.f;
+    const f3 outward = back ? -nrm : nrm;
+    const float ni_over_nt = back ? mat.refraction_index2 : (1.f / mat.refraction_index2);
+    const float cosine = dv > 0.f ? mat.refraction_index * dv : -dv;
+    const float reflect_prob = schlick(cosine, mat.refraction_index);
+    const float metal_prob = mat.metalness;
+    const bool chance_reflect = rnd1(seed) < reflect_prob;
+    const bool chance_metal = rnd1(seed) < metal_prob;
+    const bool chance_ggx = chance_reflect || chance_metal;
+    const f3 trace_next = chance_ggx ? reflect(dir, outward) : outward;
+    f3 trace_dir = chance_ggx ? ggx_sampling(seed, sqrtf(roughness), trace_next)
+                              : align_with_normal(random_hemisphere1(seed), trace_next);
+    if (diel && !chance_reflect) trace_dir = refract(dir, outward, ni_over_nt);
+    flags &= ~7u;
+    flags |= (chance_reflect ? 1u : 0u) | (back ? 2u : 0u) | (diel ? 4u : 0u);
+    if (b == 0u) {
+        flags &= ~(8u | 16u);
+        flags |= (chance_ggx ? 8u : 0u) | (chance_metal ? 16u : 0u);
+    }
+    const f3 org = pos + (nrm * K.hit_off) * start_off;
+    S.dir[p] = make_float4(trace_dir.x, trace_dir.y, trace_dir.z, 0.f);
+    ro = make_float4(org.x, org.y, org.z, kEps);
+    rd = make_float4(trace_dir.x, trace_dir.y, trace_dir.z, kMaxTrace);
+    emit = true;
+    (void)nq;
+    (void)Q;
+}
+
+// end of a sample (Shading.slang:1159-1171); restarts the next sample or finalizes
+__device__ inline void ref_end_sample(const DScene& s, const FrameK& K, const Targets& T, const PathState& S,
+                                      uint32_t p, uint32_t px, float4 color, uint32_t& flags, uint32_t& counters,
+                                      rng4& seed, bool& emit, float4& ro, float4& rd) {
+    const float4 pp = S.ppos[p];
+    const uint32_t pmat = __float_as_uint(pp.w);
+    const gk_material& pm = s.mats[pmat];
+    if (flags & 16u) {
+        color.x = color.x * pm.diffuse[0]; color.y = color.y * pm.diffuse[1];
+        color.z = color.z * pm.diffuse[2]; color.w = color.w * pm.diffuse[3];
+    }
+    float4 acc = (flags & 8u) ? S.ls[p] : S.ld[p];
+    acc.x = acc.x + color.x; acc.y = acc.y + color.y; acc.z = acc.z + color.z; acc.w = acc.w + color.w;
+    if (flags & 8u) S.ls[p] = acc; else S.ld[p] = acc;
+    const uint32_t sample = ((counters >> 8) & 0xfffu) + 1u;
+    const uint32_t nsamp = counters >> 20;
+    const f3 ppos = ld3(pp), pnrm = ld3(S.pnrm[p]);
+    if (sample < nsamp) {
+        counters = (counters & 0xfff00000u) | (sample << 8);
+        float e4[4];
+        mul44v(K.mvi, 0.f, 0.f, 0.f, 1.f, e4);
+        const f3 dir = normalize(ppos - f3{e4[0], e4[1], e4[2]});
+        S.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
+        ref_begin(s, K, Queues{}, S, p, ppos, pnrm, pmat, dir, seed, flags, 0u, 0, emit, ro, rd);
+        return;
+    }
+    // finalize: average, DirectIlluminate (Shading.slang:1175-1183, 888-905)
+    float4 fc = S.ld[p], fr = S.ls[p];
+    const float ns = float(nsamp);
+    fc.x = fc.x / ns; fc.y = fc.y / ns; fc.z = fc.z / ns; fc.w = fc.w / ns;
+    fr.x = fr.x / ns; fr.y = fr.y / ns; fr.z = fr.z / ns; fr.w = fr.w / ns;
+    const f3 sun = f3{K.sun_dir[0], K.sun_dir[1], K.sun_dir[2]};
+    const float d = fmaxf(dot(sun, normalize(pnrm)), 0.0f) * (1.0f / kPi);
+    float shadow = 0.f;
+    if (K.has_sun) {
+        const float cos_cone = 0.99999048072578f;  // cos(0.25 deg)
+        const f3 cone = align_with_normal(random_in_cone(seed, cos_cone), sun);
+        shadow = occluded_single(s, ppos, cone, kEps, kMaxTrace) ? 0.f : 1.f;
+    }
+    fc.x = fc.x + K.sun_col[0] * d * shadow; fc.y = fc.y + K.sun_col[1] * d * shadow;
+    fc.z = fc.z + K.sun_col[2] * d * shadow; fc.w = fc.w + K.sun_col[3] * d * shadow;
+    T.diffuse[px] = to_h4(fc.x, fc.y, fc.z, fc.w);
+    T.spec[px] = to_h4(fr.x, fr.y, fr.z, fr.w);
+    float4 r = T.radiance[px];
+    r.x += fc.x + fr.x; r.y += fc.y + fr.y; r.z += fc.z + fr.z; r.w += 1.f;
+    T.radiance[px] = r;
+    emit = false;
+}
+
+__global__ __launch_bounds__(256) void k_ref_primary(DScene s, FrameK K, Targets T, Queues Q, PathState S) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = Q.cnt[0];
+    const bool live = i < n;
+    bool emit = false;
+    float4 ro{}, rd{};
+    uint32_t p = 0;
+    if (live) {
+        p = Q.path[0][i];
+        const uint32_t px = S.misc[p].x;
+        const float4 h = Q.hit[i];
+        const uint32_t inst = Q.hit_inst[i];
+        rng4 seed;
+        const uint4 sr = S.rng[p];
+        seed = rng4{sr.x, sr.y, sr.z, sr.w};
+        if (__float_as_uint(h.w) == kMiss) {
+            write_miss_gbuffer(K, T, px, h);
+            const float4 sky = K.has_sky ? make_float4(K.sky[0], K.sky[1], K.sky[2], K.sky[3]) : make_float4(0, 0, 0, 0);
+            T.diffuse[px] = to_h4(sky.x, sky.y, sky.z, sky.w);
+            T.spec[px] = to_h4(0.f, 0.f, 0.f, 0.f);
+            float4 r = T.radiance[px];
+            r.x += sky.x; r.y += sky.y; r.z += sky.z; r.w += 1.f;
+            T.radiance[px] = r;
+        } else {
+            const float4 o4 = Q.o[0][i], d4 = Q.d[0][i];
+            const f3 O = ld3(o4), D = ld3(d4);
+            const uint32_t prim = __float_as_uint(h.w);
+            const uint32_t mi = material_of(s, inst, prim);
+            const gk_material& m = s.mats[mi];
+            f3 pos = O + D * h.x;
+            const f3 nrm = primary_normal(s, inst, prim, h.y, h.z);
+            uint32_t moment = 0;
+            write_gbuffer(s, K, T, px, inst, prim, h, pos, nrm, m, moment);
+            if (m.material_model != GK_MATERIAL_DIELECTRIC) pos = pos - D * 0.03f;
+            if (m.material_model == GK_MATERIAL_DIFFUSE_LIGHT) {
+                T.diffuse[px] = to_h4(m.diffuse[0], m.diffuse[1], m.diffuse[2], m.diffuse[3]);
+                T.spec[px] = to_h4(0.f, 0.f, 0.f, 0.f);
+                float4 r = T.radiance[px];
+                r.x += m.diffuse[0]; r.y += m.diffuse[1]; r.z += m.diffuse[2]; r.w += 1.f;
+                T.radiance[px] = r;
+            } else {
+                const uint32_t mult = moment > 0u ? 4u : 1u;
+                uint32_t nsamp = K.samples * mult;
+                if (nsamp < 1u) nsamp = 1u;
+                S.ppos[p] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(mi));
+                S.pnrm[p] = make_float4(nrm.x, nrm.y, nrm.z, 0.f);
+                S.ld[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+                S.ls[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+                S.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
+                float e4[4];
+                mul44v(K.mvi, 0.f, 0.f, 0.f, 1.f, e4);
+                const f3 dir = normalize(pos - f3{e4[0], e4[1], e4[2]});
+                uint32_t flags = 0u;
+                const uint32_t counters = (nsamp << 20);
+                ref_begin(s, K, Q, S, p, pos, nrm, mi, dir, seed, flags, 0u, 1, emit, ro, rd);
+                S.misc[p] = make_uint4(px, flags, counters, mi);
+                S.pos[p] = make_float4(pos.x, pos.y, pos.z, 0.f);
+                S.nrm[p] = make_float4(nrm.x, nrm.y, nrm.z, 0.f);
+            }
+        }
+        S.rng[p] = make_uint4(seed.x, seed.y, seed.z, seed.w);
+    }
+    const uint32_t slot = wave_append(&Q.cnt[1], emit);
+    if (emit) { Q.o[1][slot] = ro; Q.d[1][slot] = rd; Q.path[1][slot] = p; }
+    count_rays(Q.cnt, 5, emit);
+}
+
+// GetRayColor after the trace (Shading.slang:1039-1081) and the Render loop body
+// (Shading.slang:1117-1171); cur = queue the hits belong to.
+__global__ __launch_bounds__(256) void k_ref_shade(DScene s, FrameK K, Targets T, Queues Q, PathState S, int cur) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = Q.cnt[cur];
+    const int nxt = cur ^ 1;
+    const bool live = i < n;
+    bool emit = false;
+    float4 ro{}, rd{};
+    uint32_t p = 0;
+    if (live) {
+        p = Q.path[cur][i];
+        uint4 misc = S.misc[p];
+        const uint32_t px = misc.x;
+        uint32_t flags = misc.y, counters = misc.z, mi = misc.w;
+        const uint4 sr = S.rng[p];
+        rng4 seed{sr.x, sr.y, sr.z, sr.w};
+        float4 color = S.thr[p];
+        const float4 h = Q.hit[i];
+        const uint32_t inst = Q.hit_inst[i];
+        const uint32_t b = counters & 0xffu;
+        const bool chance_reflect = flags & 1u, back = flags & 2u, origin_diel = flags & 4u;
+        bool ret;
+        f3 pos = ld3(S.pos[p]), nrm = ld3(S.nrm[p]);
+        const f3 dir = ld3(S.dir[p]);
+        if (__float_as_uint(h.w) != kMiss) {
+            const f3 O = ld3(Q.o[cur][i]);
+            const uint32_t prim = __float_as_uint(h.w);
+            hit_vertex(s, inst, prim, h.x, h.y, h.z, O, dir, pos, nrm);
+            mi = material_of(s, inst, prim);
+            const gk_material& hm = s.mats[mi];
+            const bool hit_light = hm.material_model == GK_MATERIAL_DIFFUSE_LIGHT;
+            if (hit_light || !chance_reflect) {
+                color.x = color.x * hm.diffuse[0]; color.y = color.y * hm.diffuse[1];
+                color.z = color.z * hm.diffuse[2]; color.w = color.w * hm.diffuse[3];
+            }
+            if (back && !origin_diel) {
+                color = make_float4(0.5f, 0.5f, 0.5f, 0.f);
+                ret = true;
+            } else {
+                ret = hit_light;
+            }
+        } else {
+            const float4 sky = K.has_sky ? make_float4(K.sky[0], K.sky[1], K.sky[2], K.sky[3]) : make_float4(0, 0, 0, 0);
+            color.x = color.x * sky.x; color.y = color.y * sky.y; color.z = color.z * sky.z; color.w = color.w * sky.w;
+            ret = true;
+        }
+        const float4 pp = S.ppos[p];
+        const uint32_t pmat = __float_as_uint(pp.w);
+        const bool prim_diel = s.mats[pmat].material_model == GK_MATERIAL_DIELECTRIC;
+        const uint32_t maxb = prim_diel ? K.max_bounces : K.bounces;
+        bool end_sample = false;
+        if (b == 0u) {
+            if (ret || !(1u < maxb)) end_sample = true;
+        } else {
+            if (ret) {
+                end_sample = true;
+            } else {
+                bool done = false;
+                if (K.has_sun && rnd1(seed) < 0.5f) {
+                    const f3 sun = f3{K.sun_dir[0], K.sun_dir[1], K.sun_dir[2]};
+                    const f3 cone = align_with_normal(random_in_cone(seed, 0.99999048072578f), sun);
+                    if (!occluded_single(s, pos + nrm * 0.0f, cone, kEps, kMaxTrace)) {
+                        color.x = color.x * K.sun_col[0]; color.y = color.y * K.sun_col[1];
+                        color.z = color.z * K.sun_col[2]; color.w = color.w * K.sun_col[3];
+                        done = true;
+                    }
+                }
+                if (!done) {
+                    const bool early = !prim_diel && rnd1(seed) < K.exit_prob;
+                    if (b == maxb - 1u || early) {
+                        color.x = 0.f; color.y = 0.f; color.z = 0.f;  // no probes: (0,0,0,1)
+                        done = true;
+                    }
+                }
+                end_sample = done;
+            }
+        }
+        if (end_sample) {
+            ref_end_sample(s, K, T, S, p, px, color, flags, counters, seed, emit, ro, rd);
+            if (emit) mi = pmat;
+        } else {
+            const uint32_t nb = b + 1u;
+            counters = (counters & ~0xffu) | nb;
+            S.thr[p] = color;
+            ref_begin(s, K, Q, S, p, pos, nrm, mi, dir, seed, flags, nb, nxt, emit, ro, rd);
+        }
+        if (emit && end_sample) {
+            // restarted sample: vertex = primary
+            pos = ld3(pp);
+            nrm = ld3(S.pnrm[p]);
+            counters = counters & ~0xffu;
+        } else if (!end_sample) {
+            // keep RayColor
+        }
+        if (!end_sample) S.thr[p] = color;
+        S.pos[p] = make_float4(pos.x, pos.y, pos.z, 0.f);
+        S.nrm[p] = make_float4(nrm.x, nrm.y, nrm.z, 0.f);
+        S.misc[p] = make_uint4(px, flags, counters, mi);
+        S.rng[p] = make_uint4(seed.x, seed.y, seed.z, seed.w);
+    }
+    const uint32_t slot = wave_append(&Q.cnt[nxt], emit);
+    if (emit) { Q.o[nxt][slot] = ro; Q.d[nxt][slot] = rd; Q.path[nxt][slot] = p; }
+    count_rays(Q.cnt, 5, emit);
+}
+
+// ================================================================ PT_MIS
+struct Bsdf {
+    f3 albedo;
+    float metal, alpha, ps;
+};
+__device__ inline Bsdf bsdf_of(const gk_material& m) {
+    Bsdf b;
+    b.albedo = f3{m.diffuse[0], m.diffuse[1], m.diffuse[2]};
+    const float r = fmaxf(m.fuzziness, 0.02f);
+    b.alpha = r * r;
+    if (m.material_model == GK_MATERIAL_METALLIC) b.metal = 1.f;
+    else if (m.material_model == GK_MATERIAL_MIXTURE) b.metal = m.metalness;
+    else b.metal = 0.f;
+    b.ps = (m.material_model == GK_MATERIAL_LAMBERTIAN || m.material_model == GK_MATERIAL_ISOTROPIC) ? 0.f
+                                                                                                       : 0.5f + 0.5f * b.metal;
+    return b;
+}
+__device__ inline float ggx_d(float a2, float cz) {
+    const float d = cz * cz * (a2 - 1.f) + 1.f;
+    return a2 / (kPi * d * d);
+}
+__device__ inline float ggx_lambda(float a2, f3 w) {
+    const float z2 = w.z * w.z;
+    if (z2 <= 0.f) return 0.f;
+    const float t2 = (w.x * w.x + w.y * w.y) / z2;
+    return (sqrtf(1.f + a2 * t2) - 1.f) * 0.5f;
+}
+__device__ inline f3 vndf_sample(float alpha, f3 wo, float ux, float uy) {
+    const f3 wi = normalize(f3{wo.x * alpha, wo.y * alpha, wo.z});
+    float b = wi.z;
+    const float a = saturate(alpha);
+    const float sl = 1.f + sqrtf(wo.x * wo.x + wo.y * wo.y);
+    const float awiz = a * wo.z / sl;
+    b = b * ((1.f - a * a) / (1.f + awiz * awiz));
+    const float z = (1.f - uy) * (1.f + b) - b;
+    float sp, cp;
+    sincos_2pi(ux, sp, cp);
+    const float r = sqrtf(saturate(1.f - z * z));
+    const f3 m = wi + f3{r * cp, r * sp, z};
+    return normalize(f3{m.x * alpha, m.y * alpha, m.z});
+}
+__device__ inline float vndf_pdf(float alpha, f3 wo, f3 m) {
+    const float a2 = alpha * alpha;
+    const float D = ggx_d(a2, m.z);
+    const float len2 = a2 * (wo.x * wo.x + wo.y * wo.y);
+    const float t = sqrtf(len2 + wo.z * wo.z);
+    const float a = saturate(alpha);
+    const float sl = 1.f + sqrtf(wo.x * wo.x + wo.y * wo.y);
+    const float s2 = sl * sl, aa = a * a;
+    const float k = (1.f - aa) * s2 / (s2 + aa * wo.z * wo.z);
+    return D / (2.f * (k * wo.z + t));
+}
+__device__ inline void bsdf_eval(const Bsdf& b, f3 wo, f3 wi, f3& fd, f3& fs, float& pdd, float& pds) {
+    fd = b.albedo * ((1.f - b.metal) / kPi);
+    pdd = wi.z / kPi;
+    fs = f3{0.f, 0.f, 0.f};
+    pds = 0.f;
+    if (b.ps <= 0.f) return;
+    f3 h = wo + wi;
+    const float hl = length(h);
+    if (hl <= 0.f) return;
+    h = h * (1.f / hl);
+    const float a2 = b.alpha * b.alpha;
+    const float D = ggx_d(a2, h.z);
+    const float G = 1.f / (1.f + ggx_lambda(a2, wo) + ggx_lambda(a2, wi));
+    const float c = fmaxf(0.f, dot(wo, h));
+    const float x = 1.f - c;
+    const float x5 = x * x * x * x * x;
+    const f3 f0 = f3{0.04f, 0.04f, 0.04f} * (1.f - b.metal) + b.albedo * b.metal;
+    const f3 F = f0 + (f3{1.f, 1.f, 1.f} - f0) * x5;
+    fs = F * (D * G / (4.f * wo.z * wi.z));
+    pds = vndf_pdf(b.alpha, wo, h);
+}
+__device__ inline float power_h(float a, float b) {
+    const float a2 = a * a, b2 = b * b;
+    return a2 + b2 > 0.f ? a2 / (a2 + b2) : 0.f;
+}
+__device__ inline float light_select_pdf(const DScene& s, uint32_t inst, uint32_t prim) {
+    uint32_t lo = 0, hi = s.nlights;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const float4 le = s.lights[mid].le, e2 = s.lights[mid].e2;
+        const uint32_t li = __float_as_uint(e2.w), lp = __float_as_uint(le.w);
+        if (li < inst || (li == inst && lp < prim)) lo = mid + 1; else hi = mid;
+    }
+    if (lo >= s.nlights) return 0.f;
+    const LightTri& L = s.lights[lo];
+    if (__float_as_uint(L.e2.w) != inst || __float_as_uint(L.le.w) != prim) return 0.f;
+    return s.light_cdf[lo + 1] - s.light_cdf[lo];
+}
+__device__ inline uint32_t light_pick(const DScene& s, float u) {
+    uint32_t lo = 0, hi = s.nlights - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s.light_cdf[mid + 1] > u) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+// PT_MIS flags (misc.y): bits 0-7 depth, bit8 lobe0 decided, bit9 lobe0 = diffuse,
+// bit10 last vertex was delta (or camera), bits 16-27 sample index
+__global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T, Queues Q, PathState S, int cur) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = Q.cnt[cur];
+    const int nxt = cur ^ 1;
+    bool emit = false, emit_sh = false;
+    float4 ro{}, rd{}, so{}, sd{}, scd{}, scs{};
+    uint32_t p = 0;
+    if (i < n) {
+        p = Q.path[cur][i];
+        uint4 misc = S.misc[p];
+        const uint32_t px = misc.x;
+        uint32_t flags = misc.y;
+        const uint32_t depth = flags & 0xffu;
+        const uint4 sr = S.rng[p];
+        rng4 seed{sr.x, sr.y, sr.z, sr.w};
+        float4 thr4 = S.thr[p];
+        f3 thr = ld3(thr4);
+        float last_pdf = thr4.w;
+        const bool last_delta = depth == 0u || (flags & 1024u);
+        float4 Ld = S.ld[p], Ls = S.ls[p];
+        const float4 h = Q.hit[i];
+        const uint32_t inst = Q.hit_inst[i];
+        const f3 O = ld3(Q.o[cur][i]), D = ld3(Q.d[cur][i]);
+        const bool lobe0_diffuse = (flags & 512u) != 0u;
+        auto add = [&](f3 v, bool to_diffuse) {
+            if (to_diffuse) { Ld.x += v.x; Ld.y += v.y; Ld.z += v.z; }
+            else { Ls.x += v.x; Ls.y += v.y; Ls.z += v.z; }
+        };
+        bool alive = false;
+        if (depth == 0u && ((flags >> 16) & 0xfffu) == 0u) {
+            // primary hit: G-buffer
+            if (__float_as_uint(h.w) == kMiss) {
+                write_miss_gbuffer(K, T, px, h);
+            } else {
+                const uint32_t prim = __float_as_uint(h.w);
+                const gk_material& m = s.mats[material_of(s, inst, prim)];
+                uint32_t moment = 0;
+                write_gbuffer(s, K, T, px, inst, prim, h, O + D * h.x, primary_normal(s, inst, prim, h.y, h.z), m,
+                              moment);
+            }
+        }
+        if (__float_as_uint(h.w) == kMiss) {
+            if (K.has_sky) add(thr * f3{K.sky[0], K.sky[1], K.sky[2]}, lobe0_diffuse);
+        } else {
+            const uint32_t prim = __float_as_uint(h.w);
+            const InstDev& I = s.inst[inst];
+            const uint32_t mi = material_of(s, inst, prim);
+            const gk_material& mat = s.mats[mi];
+            const f3 P = O + D * h.x;
+            // shading normal: interpolated vertex normals; geometric normal: the object-space
+            // face normal e1 x e2 kept in the w lanes of tri_nrm; both by the inverse transpose
+            const size_t tb = 3 * size_t(I.tri_base + prim);
+            const float4 n0 = s.tri_nrm[tb], n1 = s.tri_nrm[tb + 1], n2 = s.tri_nrm[tb + 2];
+            const float bu = h.y, bv = h.z, bw = 1.f - bu - bv;
+            f3 Ns = normalize(inv_transpose(I.inv, (ld3(n0) * bw + ld3(n1) * bu) + ld3(n2) * bv));
+            const f3 face = f3{n0.w, n1.w, n2.w};
+            f3 Ng = normalize(inv_transpose(I.inv, face));
+            if (dot(Ng, D) > 0.f) Ng = -Ng;
+            if (dot(Ns, Ng) < 0.f) Ns = -Ns;
+            if (mat.material_model == GK_MATERIAL_DIFFUSE_LIGHT) {
+                const f3 Le = f3{mat.diffuse[0], mat.diffuse[1], mat.diffuse[2]};
+                float w = 1.f;
+                if (!last_delta && s.nlights > 0u) {
+                    const float cosl = fabsf(dot(Ng, D));
+                    // selection pdf and world-space area of the hit emitter (its light record)
+                    const float psel = light_select_pdf(s, inst, prim);
+                    float area = 0.f;
+                    {
+                        uint32_t lo = 0, hi = s.nlights;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            const uint32_t li = __float_as_uint(s.lights[mid].e2.w), lp = __float_as_uint(s.lights[mid].le.w);
+                            if (li < inst || (li == inst && lp < prim)) lo = mid + 1; else hi = mid;
+                        }
+                        if (lo < s.nlights) area = s.lights[lo].p0.w;
+                    }
+                    const float d2 = h.x * h.x * dot(D, D);
+                    const float pl = (cosl > 0.f && area > 0.f) ? psel * d2 / (cosl * area) : 0.f;
+                    w = power_h(last_pdf, pl);
+                }
+                if (depth == 0u) add(Le, false);
+                else add(thr * Le * w, lobe0_diffuse);
+            } else if (depth < K.bounces) {
+                const float scale = 1.f + fmaxf(fabsf(P.x), fmaxf(fabsf(P.y), fabsf(P.z)));
+                const float off = 1e-4f * scale;
+                if (mat.material_model == GK_MATERIAL_DIELECTRIC) {
+                    const float ior = mat.refraction_index;
+                    const float cosi = -dot(D, Ng);
+                    const bool entering = dot(D, inv_transpose(I.inv, face)) < 0.f;
+                    const float eta = entering ? 1.f / ior : ior;
+                    float r0 = (1.f - ior) / (1.f + ior);
+                    r0 = r0 * r0;
+                    const float xx = 1.f - cosi;
+                    const float F = r0 + (1.f - r0) * xx * xx * xx * xx * xx;
+                    const float k = 1.f - eta * eta * (1.f - cosi * cosi);
+                    const float u = rnd1(seed);
+                    const bool refl = k < 0.f || u < F;
+                    f3 wi = refl ? D + Ng * (2.f * cosi) : D * eta + Ng * (eta * cosi - sqrtf(fmaxf(k, 0.f)));
+                    wi = normalize(wi);
+                    if (depth == 0u) flags = (flags & ~768u) | 256u;
+                    thr = thr * f3{mat.diffuse[0], mat.diffuse[1], mat.diffuse[2]};
+                    flags |= 1024u;
+                    const f3 no = P + (refl ? Ng : -Ng) * off;
+                    ro = make_float4(no.x, no.y, no.z, 0.f);
+                    rd = make_float4(wi.x, wi.y, wi.z, 1e30f);
+                    alive = true;
+                } else {
+                    const Bsdf b = bsdf_of(mat);
+                    f3 T1, B1;
+                    onb(Ns, T1, B1);
+                    const f3 wo = f3{dot(T1, -D), dot(B1, -D), dot(Ns, -D)};
+                    if (wo.z > 0.f) {
+                        if (s.nlights > 0u) {
+                            const float ul = rnd1(seed);
+                            float s1, s2;
+                            rnd2(seed, s1, s2);
+                            const uint32_t li = light_pick(s, ul);
+                            const LightTri L = s.lights[li];
+                            const float sq = sqrtf(s1);
+                            const float bu2 = 1.f - sq, bv2 = s2 * sq;
+                            const f3 Pl = (ld3(L.p0) + ld3(L.e1) * bu2) + ld3(L.e2) * bv2;
+                            const f3 Po = P + Ng * off;
+                            const f3 dl = Pl - Po;
+                            const float d2 = dot(dl, dl);
+                            const float dist = sqrtf(d2);
+                            const f3 wl = dl * (1.f / dist);
+                            const f3 ln = normalize(cross(ld3(L.e1), ld3(L.e2)));
+                            const float cosl = fabsf(dot(ln, wl));
+                            const f3 wil = f3{dot(T1, wl), dot(B1, wl), dot(Ns, wl)};
+                            if (cosl > 0.f && wil.z > 0.f && dot(Ng, wl) > 0.f) {
+                                const float psel = s.light_cdf[li + 1] - s.light_cdf[li];
+                                const float pl = psel * d2 / (cosl * L.p0.w);
+                                f3 fd, fs;
+                                float pdd, pds;
+                                bsdf_eval(b, wo, wil, fd, fs, pdd, pds);
+                                const f3 Le = ld3(L.le);
+                                const float wd = power_h(pl, (1.f - b.ps) * pdd);
+                                const float ws = power_h(pl, b.ps * pds);
+                                const f3 cd = thr * Le * fd * (wil.z * wd / pl);
+                                const f3 cs = thr * Le * fs * (wil.z * ws / pl);
+                                so = make_float4(Po.x, Po.y, Po.z, 0.f);
+                                sd = make_float4(wl.x, wl.y, wl.z, dist * (1.f - 1e-4f) - off);
+                                if (depth == 0u) {
+                                    scd = make_float4(cd.x, cd.y, cd.z, 0.f);
+                                    scs = make_float4(cs.x, cs.y, cs.z, 0.f);
+                                } else if (lobe0_diffuse) {
+                                    scd = make_float4(cd.x + cs.x, cd.y + cs.y, cd.z + cs.z, 0.f);
+                                    scs = make_float4(0.f, 0.f, 0.f, 0.f);
+                                } else {
+                                    scd = make_float4(0.f, 0.f, 0.f, 0.f);
+                                    scs = make_float4(cd.x + cs.x, cd.y + cs.y, cd.z + cs.z, 0.f);
+                                }
+                                emit_sh = true;
+                            }
+                        }
+                        const float ulobe = rnd1(seed);
+                        float u1, u2;
+                        rnd2(seed, u1, u2);
+                        const bool spec = ulobe < b.ps;
+                        f3 wi;
+                        if (spec) {
+                            const f3 mh = vndf_sample(b.alpha, wo, u1, u2);
+                            wi = mh * (2.f * dot(wo, mh)) - wo;
+                        } else {
+                            float sp, cp;
+                            sincos_2pi(u1, sp, cp);
+                            const float r = sqrtf(u2);
+                            wi = f3{r * cp, r * sp, sqrtf(1.f - u2)};
+                        }
+                        if (wi.z > 0.f) {
+                            f3 fd, fs;
+                            float pdd, pds;
+                            bsdf_eval(b, wo, wi, fd, fs, pdd, pds);
+                            const float pdf = spec ? b.ps * pds : (1.f - b.ps) * pdd;
+                            const f3 f = spec ? fs : fd;
+                            if (pdf > 0.f) {
+                                thr = thr * f * (wi.z / pdf);
+                                if (depth == 0u) flags = (flags & ~768u) | 256u | (spec ? 0u : 512u);
+                                last_pdf = pdf;
+                                flags &= ~1024u;
+                                const f3 wiw = normalize((T1 * wi.x + B1 * wi.y) + Ns * wi.z);
+                                const f3 no = P + (dot(wiw, Ng) > 0.f ? Ng : -Ng) * off;
+                                ro = make_float4(no.x, no.y, no.z, 0.f);
+                                rd = make_float4(wiw.x, wiw.y, wiw.z, 1e30f);
+                                alive = true;
+                            }
+                        }
+                    }
+                }
+                if (alive && depth >= 3u) {
+                    const float q = fminf(0.95f, fmaxf(thr.x, fmaxf(thr.y, thr.z)));
+                    if (rnd1(seed) >= q) alive = false;
+                    else thr = thr * (1.f / q);
+                }
+            }
+        }
+        S.ld[p] = Ld;
+        S.ls[p] = Ls;
+        if (alive) {
+            flags = (flags & ~0xffu) | (depth + 1u);
+            S.thr[p] = make_float4(thr.x, thr.y, thr.z, last_pdf);
+            emit = true;
+        } else {
+            // path finished: restart for the next sample of this frame, if any
+            const uint32_t sample = ((flags >> 16) & 0xfffu) + 1u;
+            if (sample < K.samples) {
+                const uint32_t x = px % K.W, y = px / K.W;
+                f3 O2, D2;
+                camera_ray(K, x, y, O2, D2);
+                ro = make_float4(O2.x, O2.y, O2.z, 0.f);
+                rd = make_float4(D2.x, D2.y, D2.z, 1e30f);
+                flags = (flags & 0x300u) | (sample << 16) | 1024u;  // keep the primary lobe, depth 0 -> 1 below
+                flags = (flags & ~0xffu);
+                S.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+                emit = true;
+            }
+        }
+        S.misc[p] = make_uint4(px, flags, misc.z, misc.w);
+        S.rng[p] = make_uint4(seed.x, seed.y, seed.z, seed.w);
+    }
+    const uint32_t slot = wave_append(&Q.cnt[nxt], emit);
+    if (emit) { Q.o[nxt][slot] = ro; Q.d[nxt][slot] = rd; Q.path[nxt][slot] = p; }
+    const uint32_t ss = wave_append(&Q.cnt[2], emit_sh);
+    if (emit_sh) { Q.so[ss] = so; Q.sd[ss] = sd; Q.sc_d[ss] = scd; Q.sc_s[ss] = scs; Q.spath[ss] = p; }
+    count_rays(Q.cnt, 5, emit);
+    count_rays(Q.cnt, 6, emit_sh);
+}
+
+// shadow rays: add the NEE contribution of unoccluded rays
+__global__ void k_pt_shadow_resolve(Queues Q, PathState S) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Q.cnt[2]) return;
+    if (__float_as_uint(Q.hit[i].w) != 0u) return;  // occluded
+    const uint32_t p = Q.spath[i];
+    const float4 a = Q.sc_d[i], b = Q.sc_s[i];
+    float4 ld = S.ld[p], ls = S.ls[p];
+    ld.x += a.x; ld.y += a.y; ld.z += a.z;
+    ls.x += b.x; ls.y += b.y; ls.z += b.z;
+    S.ld[p] = ld;
+    S.ls[p] = ls;
+}
+
+// PT_MIS targets: diffuse demodulated by the primary albedo, specular, FP32 radiance sum
+__global__ void k_pt_finalize(FrameK K, Targets T, PathState S, uint32_t n) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t px = S.misc[p].x;
+    const float inv = 1.f / float(K.samples > 0u ? K.samples : 1u);
+    const float4 ld = S.ld[p], ls = S.ls[p];
+    const float4 alb = from_h4(T.albedo[px]);
+    const float dx = ld.x * inv, dy = ld.y * inv, dz = ld.z * inv;
+    const float sx = ls.x * inv, sy = ls.y * inv, sz = ls.z * inv;
+    T.diffuse[px] = to_h4(alb.x > 0.f ? dx / alb.x : 0.f, alb.y > 0.f ? dy / alb.y : 0.f, alb.z > 0.f ? dz / alb.z : 0.f, 1.f);
+    T.spec[px] = to_h4(sx, sy, sz, 1.f);
+    float4 r = T.radiance[px];
+    r.x += dx + sx; r.y += dy + sy; r.z += dz + sz; r.w += 1.f;
+    T.radiance[px] = r;
+}
+
+__global__ void k_copy_objid(const uint32_t* a, uint32_t* b, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+__global__ void k_read_stats(const uint32_t* cnt, unsigned long long* acc) {
+    if (threadIdx.x == 0) { acc[0] += cnt[5]; acc[1] += cnt[6]; }
+}
+
+inline uint32_t nblk(uint32_t n, uint32_t b = 256) { return (n + b - 1) / b; }
+
+}  // namespace
+
+// ---------------------------------------------------------------- frame driver
+int ensure_wavefront(Ctx& c) {
+    auto& w = c.wf;
+    const uint32_t n = c.width * c.height;
+    if (w.capacity >= n && w.counters.ptr) return GK_OK;
+    for (int k = 0; k < 2; ++k) {
+        GK_HIP(w.ray_o[k].resize(n)); GK_HIP(w.ray_d[k].resize(n)); GK_HIP(w.ray_path[k].resize(n));
+    }
+    GK_HIP(w.hit.resize(n)); GK_HIP(w.hit_inst.resize(n));
+    GK_HIP(w.shadow_o.resize(n)); GK_HIP(w.shadow_d.resize(n));
+    GK_HIP(w.shadow_c.resize(2 * size_t(n))); GK_HIP(w.shadow_path.resize(n));
+    GK_HIP(w.counters.resize(64));
+    GK_HIP(w.s_thr.resize(n)); GK_HIP(w.s_ld.resize(n)); GK_HIP(w.s_ls.resize(n));
+    GK_HIP(w.s_rng.resize(n)); GK_HIP(w.s_misc.resize(n));
+    GK_HIP(w.s_pos.resize(n)); GK_HIP(w.s_nrm.resize(n)); GK_HIP(w.s_dir.resize(n));
+    GK_HIP(w.s_ppos.resize(n)); GK_HIP(w.s_pnrm.resize(n));
+    GK_HIP(w.stats.resize(4));
+    GK_HIP(hipMemsetAsync(w.stats.ptr, 0, 16, c.stream));
+    w.capacity = n;
+    return GK_OK;
+}
+
+static FrameK make_k(const Ctx& c) {
+    const gk_frame_params& P = c.params;
+    FrameK K;
+    memcpy(K.mvi, P.model_view_inverse, 64);
+    memcpy(K.pi, P.projection_inverse, 64);
+    memcpy(K.vp, P.view_projection, 64);
+    memcpy(K.vpu, P.view_projection_unjit, 64);
+    memcpy(K.pvpu, P.prev_view_projection_unjit, 64);
+    memcpy(K.sun_dir, P.sun_direction, 16);
+    memcpy(K.sun_col, P.sun_color, 16);
+    memcpy(K.sky, P.sky_color, 16);
+    K.exit_prob = P.exit_probability;
+    K.hit_off = P.hit_normal_offset;
+    K.W = c.width;
+    K.H = c.height;
+    K.frame = P.total_frames;
+    K.seed = P.global_seed;
+    K.samples = P.samples > 0 ? P.samples : 1;
+    K.bounces = P.bounces;
+    K.max_bounces = P.max_bounces;
+    K.temporal = P.temporal_frames;
+    K.has_sky = P.has_sky;
+    K.has_sun = P.has_sun;
+    K.tiled = (c.width % 8 == 0 && c.height % 8 == 0) ? 1u : 0u;
+    return K;
+}
+
+int render_frame(Ctx& c) {
+    int st = ensure_wavefront(c);
+    if (st != GK_OK) return st;
+    auto& w = c.wf;
+    hipStream_t s = c.stream;
+    const uint32_t n = c.width * c.height;
+    hipEventRecord(c.ev[0], s);
+    if (c.tlas_dirty) { st = refit_tlas(c); if (st != GK_OK) return st; }
+    hipEventRecord(c.ev[1], s);
+    const FrameK K = make_k(c);
+    const DScene S0 = make_dscene(c);
+    Targets T;
+    T.diffuse = static_cast<ushort4*>(c.target_ptr[GK_RT_SINGLE_DIFFUSE]);
+    T.spec = static_cast<ushort4*>(c.target_ptr[GK_RT_SINGLE_SPECULAR]);
+    T.albedo = static_cast<ushort4*>(c.target_ptr[GK_RT_ALBEDO]);
+    T.normal = static_cast<ushort4*>(c.target_ptr[GK_RT_NORMAL]);
+    T.objid = static_cast<uint32_t*>(c.target_ptr[GK_RT_OBJECTID_0]);
+    T.motion = static_cast<float2*>(c.target_ptr[GK_RT_MOTIONVECTOR]);
+    T.moment = static_cast<uint16_t*>(c.target_ptr[GK_RT_MOTIONMOMENT]);
+    T.depth = static_cast<float*>(c.target_ptr[GK_RT_PREV_DEPTH]);
+    T.minigbuffer = static_cast<uint32_t*>(c.target_ptr[GK_RT_MINIGBUFFER]);
+    T.phit = static_cast<float4*>(c.target_ptr[GK_RT_PRIMARY_HIT]);
+    T.pinst = static_cast<uint32_t*>(c.target_ptr[GK_RT_PRIMARY_INST]);
+    T.radiance = static_cast<float4*>(c.target_ptr[GK_RT_RADIANCE]);
+    Queues Q;
+    for (int k = 0; k < 2; ++k) { Q.o[k] = w.ray_o[k].ptr; Q.d[k] = w.ray_d[k].ptr; Q.path[k] = w.ray_path[k].ptr; }
+    Q.hit = w.hit.ptr;
+    Q.hit_inst = w.hit_inst.ptr;
+    Q.so = w.shadow_o.ptr;
+    Q.sd = w.shadow_d.ptr;
+    Q.sc_d = w.shadow_c.ptr;
+    Q.sc_s = w.shadow_c.ptr + n;
+    Q.spath = w.shadow_path.ptr;
+    Q.cnt = w.counters.ptr;
+    PathState P;
+    P.thr = w.s_thr.ptr; P.ld = w.s_ld.ptr; P.ls = w.s_ls.ptr; P.rng = w.s_rng.ptr; P.misc = w.s_misc.ptr;
+    P.pos = w.s_pos.ptr; P.nrm = w.s_nrm.ptr; P.dir = w.s_dir.ptr; P.ppos = w.s_ppos.ptr; P.pnrm = w.s_pnrm.ptr;
+
+    // previous object ids for reprojection (copy pass, PathTracingRenderer.cpp:160-216)
+    k_copy_objid<<<nblk(n), 256, 0, s>>>(T.objid, static_cast<uint32_t*>(c.target_ptr[GK_RT_OBJECTID_1]), n);
+    k_zero_stats<<<1, 64, 0, s>>>(Q.cnt);
+    k_raygen<<<nblk(n), 256, 0, s>>>(K, Q, P, n);
+    GK_HIP(hipGetLastError());
+    float ms_trace = 0.f, ms_shade = 0.f;
+    (void)ms_trace;
+    (void)ms_shade;
+    const bool ref = c.params.estimator == GK_ESTIMATOR_REF_V2;
+    // primary rays
+    st = launch_trace(c, Q.o[0], Q.d[0], Q.cnt + 0, n, Q.hit, Q.hit_inst, false, nullptr);
+    if (st != GK_OK) return st;
+    int cur = 0;
+    if (ref) {
+        k_ref_primary<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P);
+        GK_HIP(hipGetLastError());
+        cur = 1;
+        // worst case: every sample uses all bounces (samples * 4 for moving pixels)
+        const uint32_t maxb = std::max(K.bounces, K.max_bounces);
+        const uint32_t iters = maxb * K.samples * 4u + 4u;
+        for (uint32_t it = 0; it < iters; ++it) {
+            st = launch_trace(c, Q.o[cur], Q.d[cur], Q.cnt + cur, n, Q.hit, Q.hit_inst, false, nullptr);
+            if (st != GK_OK) return st;
+            k_reset_counter<<<1, 1, 0, s>>>(Q.cnt + (cur ^ 1));
+            k_ref_shade<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, cur);
+            GK_HIP(hipGetLastError());
+            cur ^= 1;
+            if (it + 1 >= uint32_t(K.bounces) * K.samples + 1u && !c.params.has_sun && c.h_instances.size() == 1 &&
+                K.bounces == maxb && K.samples == 1 && false)
+                break;
+        }
+    } else {
+        const uint32_t iters = (K.bounces + 1u) * K.samples;
+        for (uint32_t it = 0; it < iters; ++it) {
+            if (it > 0) {
+                st = launch_trace(c, Q.o[cur], Q.d[cur], Q.cnt + cur, n, Q.hit, Q.hit_inst, false, nullptr);
+                if (st != GK_OK) return st;
+            }
+            k_reset_counter<<<1, 1, 0, s>>>(Q.cnt + (cur ^ 1));
+            k_reset_counter<<<1, 1, 0, s>>>(Q.cnt + 2);
+            k_pt_shade<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, cur);
+            GK_HIP(hipGetLastError());
+            st = launch_trace(c, Q.so, Q.sd, Q.cnt + 2, n, Q.hit, nullptr, true, nullptr);
+            if (st != GK_OK) return st;
+            k_pt_shadow_resolve<<<nblk(n), 256, 0, s>>>(Q, P);
+            GK_HIP(hipGetLastError());
+            cur ^= 1;
+        }
+        k_pt_finalize<<<nblk(n), 256, 0, s>>>(K, T, P, n);
+        GK_HIP(hipGetLastError());
+    }
+    k_read_stats<<<1, 1, 0, s>>>(Q.cnt, reinterpret_cast<unsigned long long*>(w.stats.ptr));
+    hipEventRecord(c.ev[2], s);
+    if (!(c.params.flags & GK_FRAME_SKIP_POST)) {
+        st = post_process(c);
+        if (st != GK_OK) return st;
+    }
+    hipEventRecord(c.ev[3], s);
+    return GK_OK;
+}
+
+}  // namespace gk

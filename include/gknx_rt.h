@@ -1,0 +1,408 @@
+/*
+ * gknx_rt.h - C ABI of the MI355X (gfx950) path-tracing core.
+ *
+ * This is the drop-in boundary for the ray-traced renderer of gameknife/gkNextEngine.
+ * Every entry point below replaces one piece of the reference's renderer/scene
+ * interface; the replaced interface is cited (file:line, relative to the reference
+ * repository root) on each declaration.
+ *
+ * Conventions
+ *   - Plain C, no exceptions cross the ABI. Every call returns an int status
+ *     (GK_OK == 0, negative on error; gk_status_string() names it).
+ *     The reference throws std::runtime_error from Check(VkResult)
+ *     (src/Vulkan/Vulkan.hpp:39); status codes replace that.
+ *   - A gk_ctx owns all device memory (scene, acceleration structures, render
+ *     targets, wavefront queues) and one HIP stream. Calls on one context must be
+ *     serialized by the caller (the reference renders on the SDL main thread,
+ *     src/DesktopMain.cpp:21-28). Host pointers are borrowed for the duration of
+ *     the call only.
+ *   - Pointers documented as "device" are HIP device pointers (for example the
+ *     data_ptr() of a torch tensor on cuda:0); all others are host pointers.
+ *   - Matrices are 4x4 float, column-major (glm layout, m[col*4+row]), exactly as
+ *     the reference uploads UniformBufferObject / NodeProxy (BasicTypes.slang:22-95).
+ */
+#ifndef GKNX_RT_H_
+#define GKNX_RT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GKNX_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- status */
+enum gk_status {
+    GK_OK = 0,
+    GK_ERR_ARG = -1,        /* null pointer, bad size, out-of-range index     */
+    GK_ERR_HIP = -2,        /* a HIP runtime call failed                       */
+    GK_ERR_OOM = -3,        /* device allocation failed                        */
+    GK_ERR_STATE = -4,      /* call order violated (e.g. render before upload) */
+    GK_ERR_LIMIT = -5,      /* a fixed capacity is exceeded                    */
+    GK_ERR_NO_DEVICE = -6,  /* no HIP device / wrong architecture              */
+    GK_ERR_IO = -7          /* file read/write failed (denoiser weights)       */
+};
+
+/* ---------------------------------------------------------------- limits
+ * Mirrors MAX_NODES / MAX_MATERIALS of BasicTypes.slang:10-11.           */
+#define GK_MAX_INSTANCES 65535u
+#define GK_MAX_MATERIALS 8192u
+#define GK_MAX_MESHES 4096u
+
+/* Material models, BasicTypes.slang:183-188. */
+enum gk_material_model {
+    GK_MATERIAL_LAMBERTIAN = 0,
+    GK_MATERIAL_METALLIC = 1,
+    GK_MATERIAL_DIELECTRIC = 2,
+    GK_MATERIAL_ISOTROPIC = 3,
+    GK_MATERIAL_DIFFUSE_LIGHT = 4,
+    GK_MATERIAL_MIXTURE = 5
+};
+
+/* Estimators. REF_V2 restates FPathTracingRendererV2 (Shading.slang:995-1261);
+ * PT_MIS is the GGX-VNDF / next-event-estimation / MIS path tracer. */
+enum gk_estimator { GK_ESTIMATOR_REF_V2 = 0, GK_ESTIMATOR_PT_MIS = 1 };
+
+/* Post-process denoiser selection. */
+enum gk_denoiser { GK_DENOISER_NONE = 0, GK_DENOISER_BILATERAL = 1, GK_DENOISER_UNET = 2 };
+
+/* Render targets: the RT_* bindless storage images of
+ * VulkanBaseRenderer::CreateRenderImages (VulkanBaseRenderer.cpp:429-463),
+ * same formats, plus three targets that only this core has. */
+enum gk_target {
+    GK_RT_SINGLE_DIFFUSE = 0,   /* RGBA16F  this frame's diffuse radiance        */
+    GK_RT_SINGLE_SPECULAR = 1,  /* RGBA16F  this frame's specular radiance       */
+    GK_RT_ALBEDO = 2,           /* RGBA16F  primary albedo                       */
+    GK_RT_NORMAL = 3,           /* RGBA16F  primary normal (xyz) + roughness (w) */
+    GK_RT_OBJECTID_0 = 4,       /* R32UI    instance id, 65535 on miss           */
+    GK_RT_OBJECTID_1 = 5,       /* R32UI    previous frame's instance id         */
+    GK_RT_MOTIONVECTOR = 6,     /* RG32F    motion in pixels (prev - curr)       */
+    GK_RT_MOTIONMOMENT = 7,     /* R16UI    frames since the object last moved   */
+    GK_RT_PREV_DEPTH = 8,       /* R32F     ndc depth                            */
+    GK_RT_ACCUM_DIFFUSE = 9,    /* RGBA16F  reprojected/accumulated diffuse      */
+    GK_RT_ACCUM_SPECULAR = 10,  /* RGBA16F                                        */
+    GK_RT_ACCUM_ALBEDO = 11,    /* RGBA16F                                        */
+    GK_RT_DENOISED = 12,        /* RGBA16F  tonemapped output                    */
+    GK_RT_MINIGBUFFER = 13,     /* R32UI    (inst+1)<<17 | tri, visibility buffer */
+    GK_RT_PRIMARY_HIT = 14,     /* gk_hit   primary-ray hit {t,u,v,prim}         */
+    GK_RT_PRIMARY_INST = 15,    /* R32UI    primary-ray hit instance             */
+    GK_RT_RADIANCE = 16,        /* RGBA32F  running FP32 sum of diffuse+specular  */
+    GK_RT_COUNT = 17
+};
+
+/* ---------------------------------------------------------------- data types */
+typedef struct gk_ctx gk_ctx;
+
+typedef struct gk_device_desc {
+    int32_t device;     /* HIP device ordinal                          */
+    uint32_t width;     /* render extent                               */
+    uint32_t height;
+    uint32_t flags;     /* reserved, 0                                 */
+} gk_device_desc;
+
+/* One mesh (a reference Model / BLAS). FP32 positions; the reference packs FP16
+ * (Vertex.hpp:83-96) but its CPU BVH is FP32 (CPUAccelerationStructure.cpp:199-201). */
+typedef struct gk_mesh {
+    const float* positions;        /* 3 * num_vertices                            */
+    const float* normals;          /* 3 * num_vertices, or NULL (face normals)    */
+    const float* texcoords;        /* 2 * num_vertices, or NULL                   */
+    const uint32_t* indices;       /* 3 * num_triangles                           */
+    const uint32_t* tri_material;  /* num_triangles local material slots, or NULL */
+    uint32_t num_vertices;
+    uint32_t num_triangles;
+} gk_mesh;
+
+/* Material, byte-identical to Material (BasicTypes.slang:218-236). 64 bytes. */
+typedef struct gk_material {
+    float diffuse[4];               /* albedo, or radiance for DIFFUSE_LIGHT */
+    int32_t diffuse_texture_id;     /* -1: none (textures are not supported) */
+    int32_t mra_texture_id;
+    int32_t normal_texture_id;
+    float fuzziness;                /* roughness                             */
+    float refraction_index;
+    uint32_t material_model;        /* gk_material_model                     */
+    float metalness;
+    float refraction_index2;
+    float normal_texture_scale;
+    float reserved[3];
+} gk_material;
+
+/* Quad light, byte-identical to LightObject (BasicTypes.slang:171-182). */
+typedef struct gk_light {
+    float p0[4];
+    float p1[4];
+    float p3[4];
+    float normal_area[4];
+    uint32_t light_mat_idx;
+    uint32_t reserved[3];
+} gk_light;
+
+/* Instance (scene node), the fields of NodeProxy (BasicTypes.slang:86-95) that
+ * the host supplies. combinedPrevTS = prev_world * inverse(world) is derived on
+ * the device (Node::TickVelocity, Node.cpp:82-106).
+ * Material of a triangle = use_remap ? mat_id[local & 15] : mat_offset + local,
+ * local being gk_mesh.tri_material (FetchMaterialId, Const_Func.slang:370-424;
+ * the offset form lifts the reference's 16-slot limit). 224 bytes. */
+typedef struct gk_instance {
+    float world[16];
+    float prev_world[16];
+    uint32_t instance_id;   /* written to GK_RT_OBJECTID_0 */
+    uint32_t mesh_id;
+    uint32_t visible;
+    uint32_t nort;          /* 1: not ray traced (masked out of the TLAS) */
+    uint32_t mat_id[16];
+    uint32_t mat_offset;
+    uint32_t use_remap;
+    uint32_t reserved[2];
+} gk_instance;
+
+/* Per-frame constants: the subset of UniformBufferObject (BasicTypes.slang:22-84)
+ * that the path-tracing passes read, plus the estimator and denoiser switches.
+ * gk_make_frame_params() fills it the way NextEngine::GetUniformBufferObject does
+ * (Engine.cpp:736-849). 656 bytes. */
+typedef struct gk_frame_params {
+    float model_view[16];
+    float projection[16];
+    float model_view_inverse[16];
+    float projection_inverse[16];
+    float view_projection[16];
+    float prev_view_projection[16];
+    float view_projection_unjit[16];
+    float prev_view_projection_unjit[16];
+    float sun_direction[4];
+    float sun_color[4];
+    float sky_color[4];        /* constant sky radiance (the reference samples an HDRI) */
+    float aperture;
+    float focus_distance;
+    float paper_white_nit;
+    float bf_sigma;
+    float bf_sigma_lum;
+    float bf_sigma_normal;
+    float exit_probability;    /* REF_V2: 0.5 (Core.PathTracing.comp.slang:16)  */
+    float hit_normal_offset;   /* REF_V2: 0.001 (Core.PathTracing.comp.slang:18) */
+    uint32_t total_frames;     /* pcg4d seed z (Shading.slang:1232)             */
+    uint32_t global_seed;      /* pcg4d seed w                                  */
+    uint32_t samples;          /* NumberOfSamples                               */
+    uint32_t bounces;          /* NumberOfBounces                               */
+    uint32_t max_bounces;      /* MaxNumberOfBounces (dielectric primaries)     */
+    uint32_t temporal_frames;
+    uint32_t bf_size;          /* 0 disables the bilateral filter               */
+    uint32_t estimator;        /* gk_estimator                                  */
+    uint32_t denoiser;         /* gk_denoiser                                   */
+    uint32_t has_sky;
+    uint32_t has_sun;
+    uint32_t taa;
+    uint32_t progressive;
+    uint32_t hdr;
+    uint32_t flags;            /* GK_FRAME_* bits                               */
+    uint32_t reserved;
+} gk_frame_params;
+
+/* gk_frame_params.flags */
+#define GK_FRAME_NO_MATERIAL_SORT 1u   /* shade in path order instead of material order */
+#define GK_FRAME_SKIP_POST 2u          /* stop after the tracer (no reproject/denoise)  */
+
+/* Camera description for gk_make_frame_params (Assets::Camera, Scene.hpp). */
+typedef struct gk_camera_desc {
+    float eye[3];
+    float target[3];
+    float up[3];
+    float fov_y_degrees;
+    float aperture;
+    float focus_distance;
+} gk_camera_desc;
+
+/* Ray and hit records of gk_trace. Ray: 32 bytes, hit: 16 bytes.
+ * t/u/v follow tinybvh's Moeller-Trumbore (tiny_bvh.h:6815-6844): the hit point is
+ * (1-u-v)*v0 + u*v1 + v*v2; prim = triangle index within its mesh;
+ * t = +inf and prim = 0xffffffff on a miss. */
+typedef struct gk_ray {
+    float origin[3];
+    float tmin;     /* hits need t > tmin */
+    float dir[3];
+    float tmax;     /* hits need t < tmax */
+} gk_ray;
+
+typedef struct gk_hit {
+    float t;
+    float u;
+    float v;
+    uint32_t prim;
+} gk_hit;
+
+#define GK_TRACE_CLOSEST 0u
+#define GK_TRACE_ANY 1u     /* occlusion: first hit ends the search (t/u/v unspecified) */
+
+/* Acceleration-structure statistics. */
+typedef struct gk_bvh_stats {
+    uint32_t blas_count;
+    uint32_t blas_nodes;       /* total wide nodes over all meshes           */
+    uint32_t blas_leaves;
+    uint32_t tlas_nodes;
+    uint64_t blas_bytes;       /* device bytes of nodes + triangle records   */
+    uint64_t tlas_bytes;
+    double sah_cost;           /* BVH2-equivalent SAH cost of mesh 0 (C_TRAV=C_INT=1) */
+    double build_ms;           /* last BLAS build, device time               */
+    double tlas_ms;            /* last TLAS build/refit, device time         */
+} gk_bvh_stats;
+
+/* Per-frame statistics (the reference's SCOPED_GPU_TIMER passes). */
+typedef struct gk_stats {
+    uint64_t rays_traced;      /* primary + extension + shadow rays           */
+    uint64_t extension_rays;
+    uint64_t shadow_rays;
+    uint64_t path_vertices;    /* shaded hits                                  */
+    double ms_tlas;
+    double ms_trace;           /* all trace launches                           */
+    double ms_shade;           /* raygen + shade + queue compaction/sort       */
+    double ms_reproject;
+    double ms_denoise;
+    double ms_frame;           /* gk_render, end to end                        */
+} gk_stats;
+
+/* ---------------------------------------------------------------- lifecycle */
+
+int gk_get_abi_version(void);
+const char* gk_status_string(int status);
+
+/* Replaces LogicRendererBase::OnDeviceSet + CreateSwapChain
+ * (VulkanBaseRenderer.hpp:213-243; PathTracingRenderer.cpp:66-90). */
+int gk_create(const gk_device_desc* desc, gk_ctx** out_ctx);
+/* Replaces DeleteSwapChain + device teardown. */
+int gk_destroy(gk_ctx* ctx);
+/* Replaces VulkanBaseRenderer::CreateRenderImages on a swap-chain resize
+ * (VulkanBaseRenderer.cpp:429-463). Clears accumulation history. */
+int gk_resize(gk_ctx* ctx, uint32_t width, uint32_t height);
+
+/* ---------------------------------------------------------------- scene */
+
+/* Replaces Scene::Reload + RebuildMeshBuffer + OnPostLoadScene's
+ * CreateAccelerationStructures (Scene.cpp:92-289; RayTraceBaseRenderer.cpp:106-124,
+ * 276-386): uploads geometry, builds one BLAS per mesh on the device and the TLAS. */
+int gk_scene_upload(gk_ctx* ctx,
+                    const gk_mesh* meshes, uint32_t num_meshes,
+                    const gk_material* materials, uint32_t num_materials,
+                    const gk_light* lights, uint32_t num_lights,
+                    const gk_instance* instances, uint32_t num_instances);
+
+/* Replaces Scene::UpdateAllMaterials (Scene.cpp:393-408). */
+int gk_materials_update(gk_ctx* ctx, const gk_material* materials, uint32_t first, uint32_t count);
+
+/* Replaces Scene::UpdateNodesGpuDriven + RayTraceBaseRenderer::AfterUpdateScene /
+ * PreRender TLAS update (Scene.cpp:443-492; RayTraceBaseRenderer.cpp:165-218).
+ * world16: count column-major 4x4 matrices (host). The previous world matrix of
+ * each updated instance becomes its prev_world. The TLAS is refit on the device
+ * during the next gk_render. */
+int gk_instances_update(gk_ctx* ctx, const float* world16, uint32_t first, uint32_t count);
+
+/* Same as gk_instances_update with device-resident matrices (count*16 floats). */
+int gk_instances_update_device(gk_ctx* ctx, const float* d_world16, uint32_t first, uint32_t count);
+
+int gk_get_bvh_stats(gk_ctx* ctx, gk_bvh_stats* out);
+
+/* ---------------------------------------------------------------- frame */
+
+/* Replaces VulkanBaseRenderer::UpdateUniformBuffer (GetUniformBufferObject,
+ * Engine.cpp:736-849): sets the constants of the next gk_render. */
+int gk_frame(gk_ctx* ctx, const gk_frame_params* params);
+
+/* Replaces PathTracingRenderer::Render (PathTracingRenderer.cpp:104-218) together
+ * with the PreRender TLAS update: TLAS refit -> wavefront trace/shade ->
+ * reproject -> denoise -> copy accum to history. Enqueues on the context stream. */
+int gk_render(gk_ctx* ctx);
+
+/* Waits for the context stream (the reference's fence wait, VulkanBaseRenderer.cpp:832). */
+int gk_sync(gk_ctx* ctx);
+
+/* Clears temporal history and the FP32 radiance sum (progressive restart). */
+int gk_reset_accumulation(gk_ctx* ctx);
+
+/* Copies a render target to host memory. bytes must equal gk_target_bytes(). */
+int gk_read_target(gk_ctx* ctx, int target, void* host, size_t bytes);
+/* Size in bytes and device address of a render target (for zero-copy readers). */
+int gk_target_info(gk_ctx* ctx, int target, size_t* bytes, void** device_ptr);
+
+int gk_get_stats(gk_ctx* ctx, gk_stats* out);
+
+/* ---------------------------------------------------------------- ray casting */
+
+/* Batched closest-hit / any-hit query against the current scene; replaces
+ * NextEngine::RayCastGPU -> FCPUAccelerationStructure::RayCastInCPU
+ * (Engine.cpp:684-690; CPUAccelerationStructure.cpp:286-313) and the inline
+ * RayQuery of FHardwareRayTracerV2 (Shading.slang:735-827).
+ * d_rays, d_hits: device arrays of n records; d_inst: device array of n instance
+ * indices, or NULL. Enqueued on the context stream; call gk_sync before reading.
+ * Ties in t are broken towards the lower (instance, prim), so the result does
+ * not depend on the traversal order. */
+int gk_trace(gk_ctx* ctx, const gk_ray* d_rays, uint32_t n, gk_hit* d_hits,
+             uint32_t* d_inst, uint32_t flags);
+
+/* ---------------------------------------------------------------- host helpers
+ * No device needed. */
+
+/* Fills camera matrices and defaults like GetUniformBufferObject
+ * (Engine.cpp:736-849): perspective(fov, w/h, 0.2, 2000) with Y flipped, Halton
+ * jitter when taa != 0, inverses, and PrevViewProjection from prev (or the current
+ * view-projection when prev is NULL). */
+int gk_make_frame_params(const gk_camera_desc* camera, uint32_t width, uint32_t height,
+                         uint32_t frame_index, uint32_t taa,
+                         const gk_frame_params* prev, gk_frame_params* out);
+
+/* Procedural scenes. Each generator has a sizing call (pointers NULL) that
+ * reports counts, then a fill call. Vertices are per-face (flat normals).
+ *   cornell:  unit-cube Cornell box, 5 walls + 2 rotated boxes + ceiling light.
+ *   city:     grid x grid boxes, lognormal(1, 0.6) heights, faces subdivided
+ *             subdiv x subdiv, ground plane, num_lights emissive roof quads.
+ *   brick:    one 24-triangle brick mesh (box with two studs).
+ * Material arrays are written by gk_procgen_*_materials. */
+int gk_procgen_cornell(float* positions, float* normals, uint32_t* indices, uint32_t* tri_material,
+                       uint32_t* num_vertices, uint32_t* num_triangles);
+int gk_procgen_cornell_materials(gk_material* materials, uint32_t* num_materials,
+                                 gk_light* lights, uint32_t* num_lights);
+int gk_procgen_city(uint32_t grid, uint32_t subdiv, uint32_t ground_res, uint32_t num_lights,
+                    uint64_t seed,
+                    float* positions, float* normals, uint32_t* indices, uint32_t* tri_material,
+                    uint32_t* num_vertices, uint32_t* num_triangles,
+                    gk_material* materials, uint32_t* num_materials);
+int gk_procgen_brick(float* positions, float* normals, uint32_t* indices, uint32_t* tri_material,
+                     uint32_t* num_vertices, uint32_t* num_triangles);
+
+/* ---------------------------------------------------------------- differentiation
+ * Path-replay backpropagation of an image loss with respect to per-material
+ * parameters (albedo rgb, roughness, metalness): 5 floats per material, in
+ * material order. No counterpart in the reference (it has no differentiable
+ * path). The primal is the PT_MIS estimator at params->samples spp; paths are
+ * replayed from their seeds, never stored.
+ *   d_dl_dimage: device float4 per pixel (dL/d radiance rgb, w ignored)
+ *   d_grad:      device 5*num_materials floats, overwritten                    */
+int gk_adjoint(gk_ctx* ctx, const float* d_dl_dimage, float* d_grad);
+
+/* Renders the PT_MIS primal at params->samples spp into a device float4 image
+ * (the same estimator gk_adjoint differentiates). */
+int gk_render_radiance(gk_ctx* ctx, float* d_image);
+
+/* ---------------------------------------------------------------- learned denoiser
+ * 5-level convolutional U-Net on BF16 MFMA with FP32 accumulation. */
+typedef struct gk_unet gk_unet;
+
+int gk_unet_create(gk_ctx* ctx, uint32_t base_channels, uint64_t seed, gk_unet** out);
+int gk_unet_destroy(gk_unet* net);
+int gk_unet_num_params(gk_unet* net, uint64_t* out);
+/* Weights as FP32 master copy (device array of gk_unet_num_params floats). */
+int gk_unet_params(gk_unet* net, float** d_params);
+int gk_unet_save(gk_unet* net, const char* path);
+int gk_unet_load(gk_unet* net, const char* path);
+/* One Adam step on a batch: d_input n*h*w*10 (NHWC, fp32), d_target n*h*w*3.
+ * Writes the batch loss (relMSE) to *loss. */
+int gk_unet_train_step(gk_unet* net, const float* d_input, const float* d_target,
+                       uint32_t n, uint32_t h, uint32_t w, float lr, float* loss);
+/* Inference on one image: d_input h*w*10 NHWC fp32 -> d_output h*w*3. */
+int gk_unet_infer(gk_unet* net, const float* d_input, float* d_output, uint32_t h, uint32_t w);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GKNX_RT_H_ */

@@ -1,0 +1,1022 @@
+// oracle_render.cpp - CPU restatement of the path-tracing passes (test infrastructure only).
+//
+//   primary rays   FVisibilityBufferRayCasterV2::TracePrimaryRay ray construction
+//                  (Shading.slang:350-356), aperture 0, traced instead of rasterised.
+//   REF_V2         FPathTracingRendererV2 Init/PrimaryHit/Render/GetRayColor
+//                  (Shading.slang:995-1241) with FHardwareRayTracerV2 hit reconstruction
+//                  (Shading.slang:784-826), CalculateMotionVectorV2 (Shading.slang:50-72),
+//                  FHardwareDirectIlluminatorV2 (Shading.slang:886-907) and the sampling
+//                  helpers of Const_Func.slang:8-49,227-340. Probe termination
+//                  (interpolateAmbientCubesV2) returns 0: no probes are baked.
+//   PT_MIS         the build's GGX/Lambert path tracer with next-event estimation to
+//                  emissive triangles and power-heuristic MIS, templated on float/double
+//                  (double = the convergence oracle).
+#include <algorithm>
+#include <atomic>
+#include <thread>
+
+#include "oracle_core.hpp"
+
+namespace orc {
+float blas_sah_cost(const Mesh& m, uint32_t idx);
+uint32_t blas_node_count(const Mesh& m);
+}  // namespace orc
+
+using namespace orc;
+
+namespace {
+
+struct InstanceABI {  // gk_instance
+    float world[16], prev_world[16];
+    uint32_t instance_id, mesh_id, visible, nort, mat_id[16], mat_offset, use_remap, reserved[2];
+};
+static_assert(sizeof(InstanceABI) == 224, "gk_instance layout");
+
+template <class F>
+void parallel_rows(uint32_t n, uint32_t nthreads, F&& f) {
+    if (nthreads <= 1) { for (uint32_t i = 0; i < n; ++i) f(i); return; }
+    std::atomic<uint32_t> next{0};
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < nthreads; ++t)
+        pool.emplace_back([&] { for (uint32_t i; (i = next.fetch_add(1)) < n;) f(i); });
+    for (auto& th : pool) th.join();
+}
+
+constexpr float PI_F = 3.14159265358979323846f;
+constexpr float EPS = 1e-3f;                     // PreProcessor.slang:13
+constexpr float PT_MAX_TRACE_DISTANCE = 1000.f;  // Shading.slang:15
+
+// ---------------------------------------------------------------- sampling (float)
+// Schlick, Const_Func.slang:8-14
+float schlick(float cosine, float ior) {
+    float r0 = (1.f - ior) / (1.f + ior);
+    r0 = r0 * r0;
+    const float x = 1.f - cosine;
+    return r0 + (1.f - r0) * (x * x * x * x * x);
+}
+// ONB (Duff et al. 2017), Const_Func.slang:17-22
+void onb(F3 n, F3& b1, F3& b2) {
+    const float sz = n.z < 0.f ? -1.f : 1.f;
+    const float a = -1.0f / (sz + n.z);
+    b2 = F3(n.x * n.y * a, sz + n.y * n.y * a, -n.y);
+    b1 = F3(1.0f + sz * n.x * n.x * a, sz * b2.x, -sz * n.x);
+}
+// to_world(v, T, B, N) = v.x*T + v.y*B + v.z*N (PreProcessor.slang:2)
+F3 to_world(F3 v, F3 T, F3 B, F3 N) {
+    return F3(v.x * T.x + v.y * B.x + v.z * N.x, v.x * T.y + v.y * B.y + v.z * N.y,
+              v.x * T.z + v.y * B.z + v.z * N.z);
+}
+// to_local(v, t, b, n) = (dot(t,v), dot(b,v), dot(n,v)) (PreProcessor.slang:1)
+F3 to_local(F3 v, F3 T, F3 B, F3 N) { return F3(dot(T, v), dot(B, v), dot(N, v)); }
+// AlignWithNormal, Const_Func.slang:44-49
+F3 align_with_normal(F3 ray, F3 normal) {
+    F3 T, B;
+    onb(normal, T, B);
+    return to_world(ray, T, B, normal);
+}
+// RandomInHemiSphere1 (cosine-weighted), Const_Func.slang:299-305
+F3 random_hemisphere1(Rng& s) {
+    float ux, uy;
+    rnd2(s, ux, uy);
+    float sp, cp;
+    sincos_2pi(ux, sp, cp);
+    const float r = std::sqrt(uy);
+    return F3(r * cp, r * sp, std::sqrt(1.0f - uy));
+}
+// RandomInCone, Const_Func.slang:282-289
+F3 random_in_cone(Rng& s, float cos_theta) {
+    float ux, uy;
+    rnd2(s, ux, uy);
+    float sp, cp;
+    sincos_2pi(ux, sp, cp);
+    const float c = 1.0f + uy * (cos_theta - 1.f);
+    const float r = std::sqrt(1.0f - c * c);
+    return F3(r * cp, r * sp, c);
+}
+float saturate(float x) { return x < 0.f ? 0.f : (x > 1.f ? 1.f : x); }
+// ggx_sample_vndf (bounded VNDF, Eto & Tokuyoshi 2023), Const_Func.slang:310-328, isotropic
+F3 ggx_sample_vndf(float alpha, F3 wi_, float ux, float uy) {
+    const F3 wi = normalize(F3(wi_.x * alpha, wi_.y * alpha, wi_.z));
+    float b = wi.z;
+    if (wi_.z > 0.f) {
+        const float a = saturate(alpha);
+        const float awiz_s = a * wi_.z / (1.0f + std::sqrt(wi_.x * wi_.x + wi_.y * wi_.y));
+        b = b * ((1.0f - a * a) / (1.0f + awiz_s * awiz_s));
+    }
+    const float z = std::fma(1.0f - uy, 1.0f + b, -b);
+    float sp, cp;
+    sincos_2pi(ux, sp, cp);
+    const float r = std::sqrt(saturate(1.0f - z * z));
+    const F3 o(r * cp, r * sp, z);
+    const F3 m = wi + o;
+    return normalize(F3(m.x * alpha, m.y * alpha, m.z));
+}
+// ggxSampling, Const_Func.slang:330-340
+F3 ggx_sampling(Rng& s, float roughness, F3 normal) {
+    F3 T, B;
+    onb(normal, T, B);
+    const float alpha = roughness * roughness;
+    const F3 wl = to_local(normal, T, B, normal);
+    float ux, uy;
+    rnd2(s, ux, uy);
+    return to_world(ggx_sample_vndf(alpha, wl, ux, uy), T, B, normal);
+}
+// HLSL reflect / refract
+F3 reflect(F3 i, F3 n) { return i - n * (2.f * dot(n, i)); }
+F3 refract(F3 i, F3 n, float eta) {
+    const float d = dot(n, i);
+    const float k = 1.f - eta * eta * (1.f - d * d);
+    if (k < 0.f) return F3(0.f, 0.f, 0.f);
+    return i * eta - n * (eta * d + std::sqrt(k));
+}
+
+struct F4 {
+    float v[4];
+};
+F4 mul4(F4 a, F4 b) { return F4{{a.v[0] * b.v[0], a.v[1] * b.v[1], a.v[2] * b.v[2], a.v[3] * b.v[3]}}; }
+
+// ---------------------------------------------------------------- primary rays
+// uv = ipos/size*2-1 (no half-pixel offset), origin = MVinv*(0,0,0,1),
+// target = Pinv*(uv,1,1), dir = MVinv*(normalize(target.xyz),0), ray = normalize(dir.xyz)
+void primary_ray(const FrameParams& P, uint32_t x, uint32_t y, uint32_t W, uint32_t H, F3& O, F3& D) {
+    const float ux = (float(x) / float(W)) * 2.0f - 1.0f;
+    const float uy = (float(y) / float(H)) * 2.0f - 1.0f;
+    const float o4[4] = {0.f, 0.f, 0.f, 1.f};
+    float org[4];
+    mat_mul4(P.model_view_inverse, o4, org);
+    const float t4[4] = {ux, uy, 1.f, 1.f};
+    float tgt[4];
+    mat_mul4(P.projection_inverse, t4, tgt);
+    const F3 tn = normalize(F3(tgt[0], tgt[1], tgt[2]));
+    const float d4[4] = {tn.x, tn.y, tn.z, 0.f};
+    float dir[4];
+    mat_mul4(P.model_view_inverse, d4, dir);
+    O = F3(org[0], org[1], org[2]);
+    D = normalize(F3(dir[0], dir[1], dir[2]));
+}
+
+// hit reconstruction ----------------------------------------------------------
+struct Vtx {
+    F3 pos, nrm;
+    uint32_t mat;
+};
+
+// FHardwareRayTracerV2::TraceRay: normal = normalize(WorldToObject^T * Mix(n0,n1,n2,bary)),
+// position = origin + dir * t (Shading.slang:803-821).
+Vtx hit_vertex(const Scene& sc, const Hit& h, F3 O, F3 D) {
+    const Instance& I = sc.inst[h.inst];
+    const Mesh& m = sc.meshes[I.mesh_id];
+    const F3 n0 = m.n0[h.prim], n1 = m.n1[h.prim], n2 = m.n2[h.prim];
+    // Mix(a,b,c,bary) = a + (b-a)*bary.y + (c-a)*bary.z  (PreProcessor.slang:24)
+    const F3 mix = (n0 + (n1 - n0) * h.u) + (n2 - n0) * h.v;
+    const float* W = I.inv;  // row-major world->object
+    F3 nw;
+    for (int j = 0; j < 3; ++j) nw[j] = W[0 * 4 + j] * mix.x + W[1 * 4 + j] * mix.y + W[2 * 4 + j] * mix.z;
+    Vtx v;
+    v.nrm = normalize(nw);
+    v.pos = O + D * h.t;
+    v.mat = material_of(sc, h.inst, h.prim);
+    return v;
+}
+
+// primary vertex as get_material_dataV2 (GeneralFunc.slang:20-53): world-space vertex
+// normals blended with (1-u-v, u, v): bx*n0 + (by*n1 + bz*n2).
+Vtx primary_vertex(const Scene& sc, const Hit& h, F3 O, F3 D) {
+    const Instance& I = sc.inst[h.inst];
+    const Mesh& m = sc.meshes[I.mesh_id];
+    const F3 n0 = xform_vector(I.world, m.n0[h.prim]);
+    const F3 n1 = xform_vector(I.world, m.n1[h.prim]);
+    const F3 n2 = xform_vector(I.world, m.n2[h.prim]);
+    const float by = h.u, bz = h.v, bx = 1.0f - (by + bz);
+    const F3 nb = n0 * bx + (n1 * by + n2 * bz);
+    Vtx v;
+    v.nrm = normalize(nb);
+    v.pos = O + D * h.t;
+    v.mat = material_of(sc, h.inst, h.prim);
+    return v;
+}
+
+const Material& mat_at(const Scene& sc, uint32_t i) { return sc.mats[i]; }
+
+// 4x4 column-major product a*b
+void mat_mul44(const float* a, const float* b, float* o) {
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) {
+            float acc = a[0 * 4 + r] * b[c * 4 + 0];
+            acc = acc + a[1 * 4 + r] * b[c * 4 + 1];
+            acc = acc + a[2 * 4 + r] * b[c * 4 + 2];
+            acc = acc + a[3 * 4 + r] * b[c * 4 + 3];
+            o[c * 4 + r] = acc;
+        }
+}
+
+// combinedPrevTS = prev * inverse(world) (Node::TickVelocity, Node.cpp:82-106), computed
+// in double by Gauss-Jordan elimination with partial pivoting, identity when the origin
+// moves by at most 0.001.
+void combined_prev(const float* world, const float* prev, float* out) {
+    double a[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) { a[r][c] = world[c * 4 + r]; a[r][4 + c] = r == c ? 1.0 : 0.0; }
+    for (int c = 0; c < 4; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 4; ++r) if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+        if (p != c) for (int k = 0; k < 8; ++k) std::swap(a[c][k], a[p][k]);
+        const double d = a[c][c];
+        if (d == 0.0) { for (int i = 0; i < 16; ++i) out[i] = (i % 5 == 0) ? 1.f : 0.f; return; }
+        for (int k = 0; k < 8; ++k) a[c][k] /= d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            const double f = a[r][c];
+            if (f != 0.0) for (int k = 0; k < 8; ++k) a[r][k] -= f * a[c][k];
+        }
+    }
+    double res[16];  // column-major prev * inv
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) {
+            double acc = 0.0;
+            for (int k = 0; k < 4; ++k) acc += double(prev[k * 4 + r]) * a[k][4 + c];
+            res[c * 4 + r] = acc;
+        }
+    const double tx = res[12], ty = res[13], tz = res[14];
+    if (std::sqrt(tx * tx + ty * ty + tz * tz) > 0.001) {
+        for (int i = 0; i < 16; ++i) out[i] = float(res[i]);
+    } else {
+        for (int i = 0; i < 16; ++i) out[i] = (i % 5 == 0) ? 1.f : 0.f;
+    }
+}
+
+// ---------------------------------------------------------------- REF_V2 per pixel
+struct RefOut {
+    float diffuse[4], spec[4], albedo[4], normal[4];
+    uint32_t objid;
+    float motion[2];
+    float depth;
+    uint32_t rays;
+};
+
+struct RefV2 {
+    const Scene& sc;
+    const FrameParams& P;
+    const std::vector<std::vector<float>>& combined;  // per instance, column-major
+
+    bool trace(F3 O, F3 D, float tmax, Hit& h, uint32_t& rays) const {
+        rays++;
+        return trace_closest(sc, O, D, EPS, tmax, h);
+    }
+    bool occluded(F3 O, F3 D, uint32_t& rays) const {
+        rays++;
+        return trace_any(sc, O, D, EPS, PT_MAX_TRACE_DISTANCE);
+    }
+
+    // GetRayColor, Shading.slang:1003-1082
+    bool get_ray_color(Vtx& v, F3& ray_dir, F4& color, Rng& seed, bool& hit_reflect, bool& hit_metal,
+                       uint32_t& rays) const {
+        const Material& mat = mat_at(sc, v.mat);
+        const float start_off = mat.material_model == MAT_DIELECTRIC ? 0.0f : 1.0f;
+        const float roughness = mat.fuzziness;
+        const float dv = dot(ray_dir, v.nrm);
+        const bool back = dv > 0.f;
+        const F3 outward = back ? -v.nrm : v.nrm;
+        const float ni_over_nt = back ? mat.refraction_index2 : (1.f / mat.refraction_index2);
+        const float cosine = dv > 0.f ? mat.refraction_index * dv : -dv;
+        const float reflect_prob = schlick(cosine, mat.refraction_index);
+        const float metal_prob = mat.metalness;
+        const bool chance_reflect = rnd1(seed) < reflect_prob;
+        const bool chance_metal = rnd1(seed) < metal_prob;
+        const bool chance_ggx = chance_reflect || chance_metal;
+        const F3 trace_next = chance_ggx ? reflect(ray_dir, outward) : outward;
+        F3 trace_dir = chance_ggx ? ggx_sampling(seed, std::sqrt(roughness), trace_next)
+                                  : align_with_normal(random_hemisphere1(seed), trace_next);
+        hit_reflect = chance_ggx;
+        hit_metal = chance_metal;
+        if (mat.material_model == MAT_DIELECTRIC && !chance_reflect)
+            trace_dir = refract(ray_dir, outward, ni_over_nt);
+        ray_dir = trace_dir;
+        const F3 org = v.pos + (v.nrm * P.hit_normal_offset) * start_off;
+        Hit h;
+        if (trace(org, trace_dir, PT_MAX_TRACE_DISTANCE, h, rays)) {
+            v = hit_vertex(sc, h, org, trace_dir);
+            const Material& hm = mat_at(sc, v.mat);
+            const F4 alb{{hm.diffuse[0], hm.diffuse[1], hm.diffuse[2], hm.diffuse[3]}};
+            if (hm.material_model == MAT_DIFFUSE_LIGHT || !chance_reflect) color = mul4(color, alb);
+            if (back && mat.material_model != MAT_DIELECTRIC) {
+                color = F4{{0.5f, 0.5f, 0.5f, 0.f}};
+                return true;
+            }
+            return hm.material_model == MAT_DIFFUSE_LIGHT;
+        }
+        const F4 sky = P.has_sky ? F4{{P.sky_color[0], P.sky_color[1], P.sky_color[2], P.sky_color[3]}}
+                                 : F4{{0.f, 0.f, 0.f, 0.f}};
+        color = mul4(color, sky);
+        return true;
+    }
+
+    void pixel(uint32_t x, uint32_t y, uint32_t W, uint32_t H, uint16_t& moment, RefOut& out) const {
+        Rng seed{{x, y, P.total_frames, P.global_seed}};
+        out = RefOut{};
+        uint32_t rays = 0;
+        F3 O, D;
+        primary_ray(P, x, y, W, H, O, D);
+        Hit h;
+        rays++;
+        if (!trace_closest(sc, O, D, 0.0f, 1e30f, h)) {
+            const F4 sky = P.has_sky ? F4{{P.sky_color[0], P.sky_color[1], P.sky_color[2], P.sky_color[3]}}
+                                     : F4{{0.f, 0.f, 0.f, 0.f}};
+            for (int i = 0; i < 4; ++i) out.diffuse[i] = sky.v[i];
+            out.albedo[0] = out.albedo[1] = out.albedo[2] = out.albedo[3] = 1.f;
+            out.normal[1] = 1.f;
+            out.normal[3] = 1.f;
+            out.objid = 65535u;
+            out.depth = 1e5f;
+            out.rays = rays;
+            return;
+        }
+        Vtx pv = primary_vertex(sc, h, O, D);
+        const Instance& I = sc.inst[h.inst];
+        const Material& mat = mat_at(sc, pv.mat);
+        // PrimaryHit, Shading.slang:1204-1220
+        {
+            const float p4[4] = {pv.pos.x, pv.pos.y, pv.pos.z, 1.f};
+            float clip[4];
+            mat_mul4(P.view_projection, p4, clip);
+            out.depth = clip[2] / clip[3];
+            float ch[4], ph[4], pp[4], m[16];
+            mat_mul4(P.view_projection_unjit, p4, ch);
+            const float* comb = combined[h.inst].data();
+            mat_mul44(P.prev_view_projection_unjit, comb, m);
+            mat_mul4(m, p4, ph);
+            const float cfx = ch[0] / ch[3] * 0.5f, cfy = ch[1] / ch[3] * 0.5f;
+            const float pfx = ph[0] / ph[3] * 0.5f, pfy = ph[1] / ph[3] * 0.5f;
+            mat_mul4(comb, p4, pp);
+            const F3 dpos = pv.pos - F3(pp[0], pp[1], pp[2]);
+            if (length(dpos) > 0.02f) moment = uint16_t(P.temporal_frames);
+            else moment = moment > 0 ? uint16_t(moment - 1) : 0;
+            out.motion[0] = (pfx - cfx) * float(W);
+            out.motion[1] = (pfy - cfy) * float(H);
+            for (int i = 0; i < 4; ++i) out.albedo[i] = mat.diffuse[i];
+            out.normal[0] = pv.nrm.x; out.normal[1] = pv.nrm.y; out.normal[2] = pv.nrm.z;
+            out.normal[3] = mat.fuzziness;
+            out.objid = I.instance_id;
+            if (mat.material_model != MAT_DIELECTRIC) pv.pos = pv.pos - D * 0.03f;
+        }
+        const F4 primary_albedo{{mat.diffuse[0], mat.diffuse[1], mat.diffuse[2], mat.diffuse[3]}};
+        // Render, Shading.slang:1085-1184
+        if (mat.material_model == MAT_DIFFUSE_LIGHT) {
+            for (int i = 0; i < 4; ++i) out.diffuse[i] = mat.diffuse[i];
+            out.rays = rays;
+            return;
+        }
+        const uint32_t mult = moment > 0 ? 4u : 1u;
+        const uint32_t nsamp = std::max(1u, P.samples * mult);
+        F4 fc{{0, 0, 0, 0}}, fr{{0, 0, 0, 0}};
+        const float o4[4] = {0.f, 0.f, 0.f, 1.f};
+        float eye4[4];
+        mat_mul4(P.model_view_inverse, o4, eye4);
+        const F3 eye(eye4[0], eye4[1], eye4[2]);
+        const float cos_cone = float(std::cos(0.25 / 180.0 * 3.14159265358979323846));
+        const F3 sun_dir(P.sun_direction[0], P.sun_direction[1], P.sun_direction[2]);
+        const F4 sun_col{{P.sun_color[0], P.sun_color[1], P.sun_color[2], P.sun_color[3]}};
+        for (uint32_t s = 0; s < nsamp; ++s) {
+            F4 color{{1, 1, 1, 1}};
+            F3 dir = normalize(pv.pos - eye);
+            Vtx vs = pv;
+            const uint32_t maxb = mat.material_model == MAT_DIELECTRIC ? P.max_bounces : P.bounces;
+            bool hit_reflect = false, hit_metal = false;
+            const bool exit_first = get_ray_color(vs, dir, color, seed, hit_reflect, hit_metal, rays);
+            if (!exit_first) {
+                for (uint32_t b = 1; b < maxb; ++b) {
+                    bool dc = false;
+                    if (get_ray_color(vs, dir, color, seed, dc, dc, rays)) break;
+                    if (P.has_sun && rnd1(seed) < 0.5f) {
+                        const F3 cone = align_with_normal(random_in_cone(seed, cos_cone), sun_dir);
+                        if (!occluded(vs.pos + vs.nrm * 0.0f, cone, rays)) {
+                            color = mul4(color, sun_col);
+                            break;
+                        }
+                    }
+                    const bool early = mat.material_model != MAT_DIELECTRIC && rnd1(seed) < P.exit_probability;
+                    if (b == maxb - 1 || early) {
+                        // probe termination: no baked probes -> (0,0,0,1)
+                        color.v[0] = 0.f; color.v[1] = 0.f; color.v[2] = 0.f;
+                        break;
+                    }
+                }
+            }
+            if (hit_metal) color = mul4(color, primary_albedo);
+            F4& acc = hit_reflect ? fr : fc;
+            for (int i = 0; i < 4; ++i) acc.v[i] = acc.v[i] + color.v[i];
+        }
+        for (int i = 0; i < 4; ++i) { fc.v[i] = fc.v[i] / float(nsamp); fr.v[i] = fr.v[i] / float(nsamp); }
+        // FHardwareDirectIlluminatorV2, Shading.slang:888-905
+        {
+            const float d = std::max(dot(sun_dir, normalize(pv.nrm)), 0.0f) * (1.0f / PI_F);
+            float shadow = 0.f;
+            if (P.has_sun) {
+                const F3 cone = align_with_normal(random_in_cone(seed, cos_cone), sun_dir);
+                shadow = 1.f;
+                if (occluded(pv.pos, cone, rays)) shadow = 0.f;
+            }
+            for (int i = 0; i < 4; ++i) fc.v[i] = fc.v[i] + sun_col.v[i] * d * shadow;
+        }
+        for (int i = 0; i < 4; ++i) { out.diffuse[i] = fc.v[i]; out.spec[i] = fr.v[i]; }
+        out.rays = rays;
+    }
+};
+
+// ---------------------------------------------------------------- PT_MIS (templated)
+template <class T>
+struct PtMis {
+    const Scene& sc;
+    const FrameParams& P;
+    using V = V3<T>;
+    static constexpr T PI = T(3.14159265358979323846);
+
+    static V cvt(F3 a) { return V(T(a.x), T(a.y), T(a.z)); }
+
+    T rnd(Rng& s) const { return T(rnd1(s)); }
+    void rnd2t(Rng& s, T& a, T& b) const { float x, y; rnd2(s, x, y); a = T(x); b = T(y); }
+
+    static void onb_t(V n, V& b1, V& b2) {
+        const T sz = n.z < T(0) ? T(-1) : T(1);
+        const T a = T(-1) / (sz + n.z);
+        b2 = V(n.x * n.y * a, sz + n.y * n.y * a, -n.y);
+        b1 = V(T(1) + sz * n.x * n.x * a, sz * b2.x, -sz * n.x);
+    }
+    static T sat(T x) { return x < T(0) ? T(0) : (x > T(1) ? T(1) : x); }
+    static T ggx_d(T a2, T cz) {
+        const T d = cz * cz * (a2 - T(1)) + T(1);
+        return a2 / (PI * d * d);
+    }
+    static T ggx_lambda(T a2, V w) {
+        const T z2 = w.z * w.z;
+        if (z2 <= T(0)) return T(0);
+        const T t2 = (w.x * w.x + w.y * w.y) / z2;
+        return (std::sqrt(T(1) + a2 * t2) - T(1)) * T(0.5);
+    }
+    // bounded VNDF sample (Eto & Tokuyoshi 2023), isotropic alpha, wo.z > 0
+    static V vndf_sample(T alpha, V wo, T ux, T uy) {
+        const V wi = normalize(V(wo.x * alpha, wo.y * alpha, wo.z));
+        T b = wi.z;
+        const T a = sat(alpha);
+        const T s = T(1) + std::sqrt(wo.x * wo.x + wo.y * wo.y);
+        const T awiz = a * wo.z / s;
+        b = b * ((T(1) - a * a) / (T(1) + awiz * awiz));
+        const T z = (T(1) - uy) * (T(1) + b) - b;
+        T sp, cp;
+        sincos_2pi<T>(ux, sp, cp);
+        const T r = std::sqrt(sat(T(1) - z * z));
+        const V m = wi + V(r * cp, r * sp, z);
+        return normalize(V(m.x * alpha, m.y * alpha, m.z));
+    }
+    // pdf of the reflected direction for the bounded VNDF sampler (Eto & Tokuyoshi 2023, eq. 8)
+    static T vndf_pdf(T alpha, V wo, V m) {
+        const T a2 = alpha * alpha;
+        const T D = ggx_d(a2, m.z);
+        const T len2 = a2 * (wo.x * wo.x + wo.y * wo.y);
+        const T t = std::sqrt(len2 + wo.z * wo.z);
+        const T a = sat(alpha);
+        const T s = T(1) + std::sqrt(wo.x * wo.x + wo.y * wo.y);
+        const T s2 = s * s, aa = a * a;
+        const T k = (T(1) - aa) * s2 / (s2 + aa * wo.z * wo.z);
+        return D / (T(2) * (k * wo.z + t));
+    }
+
+    struct Bsdf {
+        V albedo;
+        T metal, alpha, ps;
+    };
+    Bsdf bsdf_of(const Material& m) const {
+        Bsdf b;
+        b.albedo = V(T(m.diffuse[0]), T(m.diffuse[1]), T(m.diffuse[2]));
+        const T r = std::max(T(m.fuzziness), T(0.02));
+        b.alpha = r * r;
+        if (m.material_model == MAT_METALLIC) b.metal = T(1);
+        else if (m.material_model == MAT_MIXTURE) b.metal = T(m.metalness);
+        else b.metal = T(0);
+        b.ps = m.material_model == MAT_LAMBERTIAN || m.material_model == MAT_ISOTROPIC
+                   ? T(0) : T(0.5) + T(0.5) * b.metal;
+        return b;
+    }
+    // f_d and f_s for local wo, wi (z = shading normal); returns cos-free BSDF values
+    void eval(const Bsdf& b, V wo, V wi, V& fd, V& fs, T& pdf_d, T& pdf_s) const {
+        fd = b.albedo * ((T(1) - b.metal) / PI);
+        pdf_d = wi.z / PI;
+        fs = V(T(0), T(0), T(0));
+        pdf_s = T(0);
+        if (b.ps <= T(0)) return;
+        V h = wo + wi;
+        const T hl = length(h);
+        if (hl <= T(0)) return;
+        h = h * (T(1) / hl);
+        const T a2 = b.alpha * b.alpha;
+        const T D = ggx_d(a2, h.z);
+        const T G = T(1) / (T(1) + ggx_lambda(a2, wo) + ggx_lambda(a2, wi));
+        const T c = std::max(T(0), dot(wo, h));
+        const T x = T(1) - c;
+        const T x5 = x * x * x * x * x;
+        const V f0 = V(T(0.04), T(0.04), T(0.04)) * (T(1) - b.metal) + b.albedo * b.metal;
+        const V F = f0 + (V(T(1), T(1), T(1)) - f0) * x5;
+        fs = F * (D * G / (T(4) * wo.z * wi.z));
+        pdf_s = vndf_pdf(b.alpha, wo, h);
+    }
+    static T power(T a, T b) { const T a2 = a * a, b2 = b * b; return a2 + b2 > T(0) ? a2 / (a2 + b2) : T(0); }
+
+    // radiance of one path; rgb accumulated into Ld (primary diffuse lobe) and Ls (rest)
+    void path(uint32_t x, uint32_t y, uint32_t W, uint32_t H, Rng& seed, V& Ld, V& Ls, uint64_t& rays) const {
+        F3 Of, Df;
+        primary_ray(P, x, y, W, H, Of, Df);
+        V O = cvt(Of), D = cvt(Df);
+        V thr(T(1), T(1), T(1));
+        int lobe0 = -1;          // 0 diffuse, 1 specular/other at the primary vertex
+        T last_pdf = T(0);
+        bool last_delta = true;  // camera ray: no MIS
+        V last_pos = O;
+        const uint32_t max_depth = P.bounces;
+        for (uint32_t depth = 0;; ++depth) {
+            HitT<T> h;
+            rays++;
+            const T tmin = T(0);
+            if (!trace_closest_t<T>(sc, O, D, tmin, T(1e30), h)) {
+                if (P.has_sky) {
+                    const V sky(T(P.sky_color[0]), T(P.sky_color[1]), T(P.sky_color[2]));
+                    (lobe0 == 0 ? Ld : Ls) = (lobe0 == 0 ? Ld : Ls) + thr * sky;
+                }
+                break;
+            }
+            const Instance& I = sc.inst[h.inst];
+            const Mesh& m = sc.meshes[I.mesh_id];
+            const uint32_t mi = material_of(sc, h.inst, h.prim);
+            const Material& mat = sc.mats[mi];
+            const V P_ = O + D * h.t;
+            // world normals: geometric (inverse transpose of e1 x e2) and shading
+            const float* Wi = I.inv;
+            auto nrm_w = [&](F3 n) {
+                V o;
+                for (int j = 0; j < 3; ++j)
+                    o[j] = T(Wi[0 * 4 + j]) * T(n.x) + T(Wi[1 * 4 + j]) * T(n.y) + T(Wi[2 * 4 + j]) * T(n.z);
+                return normalize(o);
+            };
+            V Ng = nrm_w(cross(m.e1[h.prim], m.e2[h.prim]));
+            const F3 n0 = m.n0[h.prim], n1 = m.n1[h.prim], n2 = m.n2[h.prim];
+            const T bu = h.u, bv = h.v, bw = T(1) - bu - bv;
+            V nsum = cvt(n0) * bw + cvt(n1) * bu + cvt(n2) * bv;
+            const float* Wr = I.inv;
+            V Ns;
+            for (int j = 0; j < 3; ++j)
+                Ns[j] = T(Wr[0 * 4 + j]) * nsum.x + T(Wr[1 * 4 + j]) * nsum.y + T(Wr[2 * 4 + j]) * nsum.z;
+            Ns = normalize(Ns);
+            if (dot(Ng, D) > T(0)) Ng = -Ng;
+            if (dot(Ns, Ng) < T(0)) Ns = -Ns;
+            if (mat.material_model == MAT_DIFFUSE_LIGHT) {
+                const V Le(T(mat.diffuse[0]), T(mat.diffuse[1]), T(mat.diffuse[2]));
+                T w = T(1);
+                if (!last_delta && sc.light_total > 0.0) {
+                    const T cosl = std::fabs(dot(Ng, D));
+                    const T area = T(0.5) * length(nrm_raw(m, h.prim, I));
+                    const T psel = light_select_pdf(h.inst, h.prim);
+                    const T d2 = h.t * h.t * dot(D, D);
+                    const T pl = cosl > T(0) && area > T(0) ? psel * d2 / (cosl * area) : T(0);
+                    w = power(last_pdf, pl);
+                }
+                if (depth == 0) Ls = Ls + Le;  // emitter seen directly: specular target
+                else (lobe0 == 0 ? Ld : Ls) = (lobe0 == 0 ? Ld : Ls) + thr * Le * w;
+                break;
+            }
+            if (depth >= max_depth) break;
+            const T scale = T(1) + std::max(std::fabs(P_.x), std::max(std::fabs(P_.y), std::fabs(P_.z)));
+            const T off = T(1e-4) * scale;
+            if (mat.material_model == MAT_DIELECTRIC) {
+                const T ior = T(mat.refraction_index);
+                const T cosi = -dot(D, Ng);
+                const bool entering = dot(D, cvt_face(m, h.prim, I)) < T(0);
+                const T eta = entering ? T(1) / ior : ior;
+                T r0 = (T(1) - ior) / (T(1) + ior);
+                r0 = r0 * r0;
+                const T xx = T(1) - cosi;
+                const T F = r0 + (T(1) - r0) * xx * xx * xx * xx * xx;
+                const T k = T(1) - eta * eta * (T(1) - cosi * cosi);
+                const T u = rnd(seed);
+                V wi;
+                bool refl = k < T(0) || u < F;
+                if (refl) wi = D + Ng * (T(2) * cosi);
+                else wi = D * eta + Ng * (eta * cosi - std::sqrt(k));
+                wi = normalize(wi);
+                if (depth == 0) lobe0 = 1;
+                thr = thr * V(T(mat.diffuse[0]), T(mat.diffuse[1]), T(mat.diffuse[2]));
+                last_delta = true;
+                O = P_ + (refl ? Ng : -Ng) * off;
+                D = wi;
+            } else {
+                const Bsdf b = bsdf_of(mat);
+                V T1, B1;
+                onb_t(Ns, T1, B1);
+                auto to_loc = [&](V v) { return V(dot(T1, v), dot(B1, v), dot(Ns, v)); };
+                auto to_wld = [&](V v) { return T1 * v.x + B1 * v.y + Ns * v.z; };
+                const V wo = to_loc(-D);
+                if (wo.z <= T(0)) break;
+                // next-event estimation
+                if (sc.light_total > 0.0) {
+                    const T ul = rnd(seed);
+                    T s1, s2;
+                    rnd2t(seed, s1, s2);
+                    uint32_t li = light_pick(ul);
+                    const Light& L = sc.lights[li];
+                    const T sq = std::sqrt(s1);
+                    const T bu2 = T(1) - sq, bv2 = s2 * sq;
+                    const V Pl = cvt(L.p0) + cvt(L.e1) * bu2 + cvt(L.e2) * bv2;
+                    const V Po = P_ + Ng * off;
+                    V dl = Pl - Po;
+                    const T d2 = dot(dl, dl);
+                    const T dist = std::sqrt(d2);
+                    const V wl = dl * (T(1) / dist);
+                    const T cosl = std::fabs(dot(cvt(L.n), wl));
+                    const V wil = to_loc(wl);
+                    if (cosl > T(0) && wil.z > T(0) && dot(Ng, wl) > T(0)) {
+                        const T psel = T(sc.light_cdf[li + 1] - sc.light_cdf[li]);
+                        const T pl = psel * d2 / (cosl * T(L.area));
+                        rays++;
+                        if (!trace_any_t<T>(sc, Po, wl, T(0), dist * (T(1) - T(1e-4)) - off)) {
+                            V fd, fs;
+                            T pdd, pds;
+                            eval(b, wo, wil, fd, fs, pdd, pds);
+                            const V Le = cvt(L.le);
+                            const T wd = power(pl, (T(1) - b.ps) * pdd);
+                            const T ws = power(pl, b.ps * pds);
+                            const V cd = thr * Le * fd * (wil.z * wd / pl);
+                            const V cs = thr * Le * fs * (wil.z * ws / pl);
+                            if (depth == 0) { Ld = Ld + cd; Ls = Ls + cs; }
+                            else { V& acc = lobe0 == 0 ? Ld : Ls; acc = acc + cd + cs; }
+                        }
+                    }
+                }
+                // BSDF sampling
+                const T ulobe = rnd(seed);
+                T u1, u2;
+                rnd2t(seed, u1, u2);
+                const bool spec = ulobe < b.ps;
+                V wi;
+                if (spec) {
+                    const V mh = vndf_sample(b.alpha, wo, u1, u2);
+                    wi = mh * (T(2) * dot(wo, mh)) - wo;
+                } else {
+                    T sp, cp;
+                    sincos_2pi<T>(u1, sp, cp);
+                    const T r = std::sqrt(u2);
+                    wi = V(r * cp, r * sp, std::sqrt(T(1) - u2));
+                }
+                if (wi.z <= T(0)) break;
+                V fd, fs;
+                T pdd, pds;
+                eval(b, wo, wi, fd, fs, pdd, pds);
+                T pdf;
+                V f;
+                if (spec) { pdf = b.ps * pds; f = fs; }
+                else { pdf = (T(1) - b.ps) * pdd; f = fd; }
+                if (!(pdf > T(0))) break;
+                thr = thr * f * (wi.z / pdf);
+                if (depth == 0) lobe0 = spec ? 1 : 0;
+                last_pdf = pdf;
+                last_delta = false;
+                const V wiw = normalize(to_wld(wi));
+                O = P_ + (dot(wiw, Ng) > T(0) ? Ng : -Ng) * off;
+                D = wiw;
+            }
+            last_pos = P_;
+            if (depth >= 3) {
+                const T q = std::min(T(0.95), std::max(thr.x, std::max(thr.y, thr.z)));
+                if (rnd(seed) >= q) break;
+                thr = thr * (T(1) / q);
+            }
+        }
+        (void)last_pos;
+    }
+
+    V nrm_raw(const Mesh& m, uint32_t p, const Instance& I) const {
+        // world-space e1 x e2 (for the emitter area)
+        const F3 a = xform_vector(I.world, m.e1[p]), c = xform_vector(I.world, m.e2[p]);
+        return cvt(cross(a, c));
+    }
+    V cvt_face(const Mesh& m, uint32_t p, const Instance& I) const { return nrm_raw(m, p, I); }
+    T light_select_pdf(uint32_t inst, uint32_t prim) const {
+        // lights are sorted by (inst, prim); binary search
+        size_t lo = 0, hi = sc.lights.size();
+        while (lo < hi) {
+            const size_t mid = (lo + hi) / 2;
+            const Light& L = sc.lights[mid];
+            if (L.inst < inst || (L.inst == inst && L.prim < prim)) lo = mid + 1; else hi = mid;
+        }
+        if (lo >= sc.lights.size() || sc.lights[lo].inst != inst || sc.lights[lo].prim != prim) return T(0);
+        return T(sc.light_cdf[lo + 1] - sc.light_cdf[lo]);
+    }
+    uint32_t light_pick(T u) const {
+        // first i with cdf[i+1] > u
+        size_t lo = 0, hi = sc.lights.size() - 1;
+        while (lo < hi) {
+            const size_t mid = (lo + hi) / 2;
+            if (T(sc.light_cdf[mid + 1]) > u) hi = mid; else lo = mid + 1;
+        }
+        return uint32_t(lo);
+    }
+};
+
+void build_lights(Scene& sc) {
+    sc.lights.clear();
+    for (uint32_t ii = 0; ii < sc.inst.size(); ++ii) {
+        const Instance& I = sc.inst[ii];
+        if (!I.visible || I.nort) continue;
+        const Mesh& m = sc.meshes[I.mesh_id];
+        for (uint32_t p = 0; p < m.v0.size(); ++p) {
+            const uint32_t mi = material_of(sc, ii, p);
+            const Material& mat = sc.mats[mi];
+            if (mat.material_model != MAT_DIFFUSE_LIGHT) continue;
+            Light L;
+            L.p0 = xform_point(I.world, m.v0[p]);
+            L.e1 = xform_vector(I.world, m.e1[p]);
+            L.e2 = xform_vector(I.world, m.e2[p]);
+            const F3 c = cross(L.e1, L.e2);
+            L.area = 0.5f * length(c);
+            L.n = normalize(c);
+            L.le = F3(mat.diffuse[0], mat.diffuse[1], mat.diffuse[2]);
+            L.inst = ii;
+            L.prim = p;
+            if (L.area > 0.f) sc.lights.push_back(L);
+        }
+    }
+    sc.light_cdf.assign(sc.lights.size() + 1, 0.0);
+    double acc = 0.0;
+    for (size_t i = 0; i < sc.lights.size(); ++i) {
+        const Light& L = sc.lights[i];
+        const double lum = 0.2126 * L.le.x + 0.7152 * L.le.y + 0.0722 * L.le.z;
+        acc += double(L.area) * lum;
+        sc.light_cdf[i + 1] = acc;
+    }
+    sc.light_total = acc;
+    if (acc > 0.0) for (auto& c : sc.light_cdf) c /= acc;
+}
+
+}  // namespace
+
+// =================================================================== C API
+extern "C" {
+
+void* oracle_scene_new() { return new Scene(); }
+void oracle_scene_free(void* s) { delete static_cast<Scene*>(s); }
+
+// positions/normals: 3*nv floats (normals may be NULL: face normals); idx: 3*nt; tri_mat: nt or NULL
+int oracle_scene_add_mesh(void* s, const float* pos, const float* nrm, const uint32_t* idx,
+                          const uint32_t* tri_mat, uint32_t nv, uint32_t nt) {
+    Scene& sc = *static_cast<Scene*>(s);
+    Mesh m;
+    m.v0.resize(nt); m.e1.resize(nt); m.e2.resize(nt);
+    m.n0.resize(nt); m.n1.resize(nt); m.n2.resize(nt); m.mat.resize(nt);
+    for (uint32_t t = 0; t < nt; ++t) {
+        const uint32_t a = idx[3 * t], b = idx[3 * t + 1], c = idx[3 * t + 2];
+        if (a >= nv || b >= nv || c >= nv) return -1;
+        const F3 pa(pos[3 * a], pos[3 * a + 1], pos[3 * a + 2]);
+        const F3 pb(pos[3 * b], pos[3 * b + 1], pos[3 * b + 2]);
+        const F3 pc(pos[3 * c], pos[3 * c + 1], pos[3 * c + 2]);
+        m.v0[t] = pa;
+        m.e1[t] = pb - pa;
+        m.e2[t] = pc - pa;
+        if (nrm) {
+            m.n0[t] = F3(nrm[3 * a], nrm[3 * a + 1], nrm[3 * a + 2]);
+            m.n1[t] = F3(nrm[3 * b], nrm[3 * b + 1], nrm[3 * b + 2]);
+            m.n2[t] = F3(nrm[3 * c], nrm[3 * c + 1], nrm[3 * c + 2]);
+        } else {
+            const F3 fn = normalize(cross(m.e1[t], m.e2[t]));
+            m.n0[t] = m.n1[t] = m.n2[t] = fn;
+        }
+        m.mat[t] = tri_mat ? tri_mat[t] : 0u;
+    }
+    sc.meshes.push_back(std::move(m));
+    return int(sc.meshes.size() - 1);
+}
+
+int oracle_scene_set_materials(void* s, const Material* mats, uint32_t n) {
+    Scene& sc = *static_cast<Scene*>(s);
+    sc.mats.assign(mats, mats + n);
+    return 0;
+}
+
+int oracle_scene_add_instance(void* s, const InstanceABI* in) {
+    Scene& sc = *static_cast<Scene*>(s);
+    Instance I{};
+    std::memcpy(I.world, in->world, 64);
+    std::memcpy(I.prev_world, in->prev_world, 64);
+    I.instance_id = in->instance_id;
+    I.mesh_id = in->mesh_id;
+    I.visible = in->visible;
+    I.nort = in->nort;
+    std::memcpy(I.mat_id, in->mat_id, 64);
+    I.mat_offset = in->mat_offset;
+    I.use_remap = in->use_remap;
+    sc.inst.push_back(I);
+    return int(sc.inst.size() - 1);
+}
+
+int oracle_scene_build(void* s) {
+    Scene& sc = *static_cast<Scene*>(s);
+    for (const Instance& I : sc.inst) if (I.mesh_id >= sc.meshes.size()) return -1;
+    if (sc.mats.empty()) return -1;
+    scene_build(sc);
+    build_lights(sc);
+    return 0;
+}
+
+// New world matrices (column-major, n*16) for instances [first, first+n); prev <- old world.
+int oracle_scene_update_instances(void* s, const float* world16, uint32_t first, uint32_t n) {
+    Scene& sc = *static_cast<Scene*>(s);
+    if (size_t(first) + n > sc.inst.size()) return -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        Instance& I = sc.inst[first + i];
+        std::memcpy(I.prev_world, I.world, 64);
+        std::memcpy(I.world, world16 + size_t(i) * 16, 64);
+    }
+    scene_update_instances(sc);
+    build_lights(sc);
+    return 0;
+}
+
+void oracle_blas_stats(void* s, uint32_t mesh, uint32_t* nodes, float* sah) {
+    Scene& sc = *static_cast<Scene*>(s);
+    *nodes = blas_node_count(sc.meshes[mesh]);
+    *sah = sc.meshes[mesh].nodes.empty() ? 0.f : blas_sah_cost(sc.meshes[mesh], 0);
+}
+
+// rays: n gk_ray; hits: n*{t,u,v,prim}; inst may be NULL. any != 0: occlusion (hits[i].prim = 0/1)
+void oracle_trace(void* s, const float* rays, uint32_t n, float* hits, uint32_t* inst, int any,
+                  uint32_t nthreads) {
+    const Scene& sc = *static_cast<Scene*>(s);
+    const uint32_t chunk = 4096;
+    parallel_rows((n + chunk - 1) / chunk, nthreads, [&](uint32_t c) {
+        for (uint32_t i = c * chunk; i < std::min(n, (c + 1) * chunk); ++i) {
+            const float* r = rays + size_t(i) * 8;
+            const F3 O(r[0], r[1], r[2]), D(r[4], r[5], r[6]);
+            float* o = hits + size_t(i) * 4;
+            if (any) {
+                const uint32_t occ = trace_any(sc, O, D, r[3], r[7]) ? 1u : 0u;
+                o[0] = o[1] = o[2] = 0.f;
+                std::memcpy(&o[3], &occ, 4);
+                continue;
+            }
+            Hit h;
+            if (trace_closest(sc, O, D, r[3], r[7], h)) {
+                o[0] = h.t; o[1] = h.u; o[2] = h.v;
+                std::memcpy(&o[3], &h.prim, 4);
+                if (inst) inst[i] = h.inst;
+            } else {
+                const float inf = std::numeric_limits<float>::infinity();
+                const uint32_t miss = 0xffffffffu;
+                o[0] = inf; o[1] = 0.f; o[2] = 0.f;
+                std::memcpy(&o[3], &miss, 4);
+                if (inst) inst[i] = miss;
+            }
+        }
+    });
+}
+
+// Primary rays of a W x H frame as gk_ray records (tmin 0, tmax 1e30).
+void oracle_primary_rays(const FrameParams* P, uint32_t W, uint32_t H, float* rays) {
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            F3 O, D;
+            primary_ray(*P, x, y, W, H, O, D);
+            float* r = rays + (size_t(y) * W + x) * 8;
+            r[0] = O.x; r[1] = O.y; r[2] = O.z; r[3] = 0.f;
+            r[4] = D.x; r[5] = D.y; r[6] = D.z; r[7] = 1e30f;
+        }
+}
+
+// REF_V2 frame over the pixel rectangle [x0,x0+w) x [y0,y0+h) of a W x H image.
+// Outputs are w*h arrays (row-major within the rectangle). moment is in/out (u16).
+// Returns the number of rays traced.
+uint64_t oracle_render_refv2(void* s, const FrameParams* P, uint32_t W, uint32_t H, uint32_t x0,
+                             uint32_t y0, uint32_t w, uint32_t h, float* diffuse, float* spec,
+                             float* albedo, float* normal, uint32_t* objid, float* motion,
+                             float* depth, uint16_t* moment, uint32_t nthreads) {
+    const Scene& sc = *static_cast<Scene*>(s);
+    std::vector<std::vector<float>> comb(sc.inst.size(), std::vector<float>(16));
+    for (size_t i = 0; i < sc.inst.size(); ++i) combined_prev(sc.inst[i].world, sc.inst[i].prev_world, comb[i].data());
+    RefV2 r{sc, *P, comb};
+    std::atomic<uint64_t> rays{0};
+    parallel_rows(h, nthreads, [&](uint32_t yy) {
+        uint64_t local = 0;
+        for (uint32_t xx = 0; xx < w; ++xx) {
+            const size_t o = size_t(yy) * w + xx;
+            RefOut out;
+            r.pixel(x0 + xx, y0 + yy, W, H, moment[o], out);
+            std::memcpy(diffuse + 4 * o, out.diffuse, 16);
+            std::memcpy(spec + 4 * o, out.spec, 16);
+            std::memcpy(albedo + 4 * o, out.albedo, 16);
+            std::memcpy(normal + 4 * o, out.normal, 16);
+            objid[o] = out.objid;
+            motion[2 * o] = out.motion[0];
+            motion[2 * o + 1] = out.motion[1];
+            depth[o] = out.depth;
+            local += out.rays;
+        }
+        rays += local;
+    });
+    return rays.load();
+}
+
+}  // extern "C"
+
+// PT_MIS over a rectangle: radiance (rgb) averaged over spp samples drawn from frames
+// [frame0, frame0+frames) (seed z = frame), spp_per_frame = P->samples paths each.
+// diffuse/spec receive the primary-lobe split (w*h*3 each, may be NULL).
+template <class T>
+uint64_t render_ptmis(const Scene& sc, const FrameParams* P, uint32_t W, uint32_t H, uint32_t x0,
+                      uint32_t y0, uint32_t w, uint32_t h, uint32_t frame0, uint32_t frames,
+                      double* rgb, double* diffuse, double* spec, uint32_t nthreads) {
+    PtMis<T> pt{sc, *P};
+    std::atomic<uint64_t> rays{0};
+    const uint32_t spp = std::max(1u, P->samples);
+    parallel_rows(h, nthreads, [&](uint32_t yy) {
+        uint64_t local = 0;
+        for (uint32_t xx = 0; xx < w; ++xx) {
+            V3<T> sd(T(0), T(0), T(0)), ss(T(0), T(0), T(0));
+            for (uint32_t f = 0; f < frames; ++f) {
+                Rng seed{{x0 + xx, y0 + yy, frame0 + f, P->global_seed}};
+                for (uint32_t s = 0; s < spp; ++s) {
+                    V3<T> Ld(T(0), T(0), T(0)), Ls(T(0), T(0), T(0));
+                    pt.path(x0 + xx, y0 + yy, W, H, seed, Ld, Ls, local);
+                    sd = sd + Ld;
+                    ss = ss + Ls;
+                }
+            }
+            const T inv = T(1) / T(frames * spp);
+            const size_t o = size_t(yy) * w + xx;
+            for (int c = 0; c < 3; ++c) {
+                rgb[3 * o + c] = double((sd[c] + ss[c]) * inv);
+                if (diffuse) diffuse[3 * o + c] = double(sd[c] * inv);
+                if (spec) spec[3 * o + c] = double(ss[c] * inv);
+            }
+        }
+        rays += local;
+    });
+    return rays.load();
+}
+
+extern "C" {
+
+uint64_t oracle_render_ptmis(void* s, const FrameParams* P, uint32_t W, uint32_t H, uint32_t x0,
+                             uint32_t y0, uint32_t w, uint32_t h, uint32_t frame0, uint32_t frames,
+                             int fp64, double* rgb, double* diffuse, double* spec, uint32_t nthreads) {
+    const Scene& sc = *static_cast<Scene*>(s);
+    if (fp64) return render_ptmis<double>(sc, P, W, H, x0, y0, w, h, frame0, frames, rgb, diffuse, spec, nthreads);
+    return render_ptmis<float>(sc, P, W, H, x0, y0, w, h, frame0, frames, rgb, diffuse, spec, nthreads);
+}
+
+// pcg4d known answers: n steps from seed, writes 4*n words
+void oracle_pcg4d(const uint32_t* seed, uint32_t n, uint32_t* out) {
+    Rng s{{seed[0], seed[1], seed[2], seed[3]}};
+    for (uint32_t i = 0; i < n; ++i) {
+        pcg4d(s);
+        std::memcpy(out + 4 * i, s.v, 16);
+    }
+}
+float oracle_u2f(uint32_t x) { return u2f(x); }
+void oracle_sincos_2pi(const float* u, uint32_t n, float* s, float* c) {
+    for (uint32_t i = 0; i < n; ++i) sincos_2pi<float>(u[i], s[i], c[i]);
+}
+// sampler draws for distribution tests: kind 0 = RandomInHemiSphere1 (around +z),
+// 1 = RandomInCone(cos_theta = param), 2 = ggx_sample_vndf(alpha = param, wi = dir)
+void oracle_sample(int kind, float param, const float* dir, uint32_t n, uint32_t seed, float* out) {
+    Rng s{{seed, 0u, 0u, 0u}};
+    for (uint32_t i = 0; i < n; ++i) {
+        F3 v;
+        if (kind == 0) v = random_hemisphere1(s);
+        else if (kind == 1) v = random_in_cone(s, param);
+        else {
+            float a, b;
+            rnd2(s, a, b);
+            v = ggx_sample_vndf(param, F3(dir[0], dir[1], dir[2]), a, b);
+        }
+        out[3 * i] = v.x; out[3 * i + 1] = v.y; out[3 * i + 2] = v.z;
+    }
+}
+// bounded-VNDF reflected-direction sampling and its pdf in double (PT_MIS):
+// writes n directions (3 floats) and pdfs.
+void oracle_ptmis_spec_sample(double alpha, const double* wo, uint32_t n, uint32_t seed, double* dirs,
+                              double* pdfs) {
+    Rng s{{seed, 1u, 2u, 3u}};
+    const D3 w(wo[0], wo[1], wo[2]);
+    for (uint32_t i = 0; i < n; ++i) {
+        float a, b;
+        rnd2(s, a, b);
+        const D3 m = PtMis<double>::vndf_sample(alpha, w, a, b);
+        const D3 wi = m * (2.0 * dot(w, m)) - w;
+        dirs[3 * i] = wi.x; dirs[3 * i + 1] = wi.y; dirs[3 * i + 2] = wi.z;
+        pdfs[i] = PtMis<double>::vndf_pdf(alpha, w, m);
+    }
+}
+double oracle_ptmis_spec_pdf(double alpha, const double* wo, const double* wi) {
+    const D3 o(wo[0], wo[1], wo[2]), i(wi[0], wi[1], wi[2]);
+    D3 h = o + i;
+    h = normalize(h);
+    return PtMis<double>::vndf_pdf(alpha, o, h);
+}
+uint32_t oracle_num_lights(void* s) { return uint32_t(static_cast<Scene*>(s)->lights.size()); }
+
+}  // extern "C"

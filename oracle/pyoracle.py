@@ -1,0 +1,273 @@
+"""Python binding of the CPU oracle (test infrastructure only).
+
+Loads ``oracle/_build/liboracle.so`` (the C++ restatement of the reference's path
+tracing math) and ``oracle/_ref/libtinybvh_ref.so`` (the reference's tinybvh, compiled
+from the reference tree). Imported only by tests/, ``__graft_entry__.smoke`` and
+bench.py's cpu_baseline leg -- never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_LIB = HERE / "_build" / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libtinybvh_ref.so"
+
+_P = C.c_void_p
+_U = C.c_uint32
+_o = None
+_r = None
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def oracle():
+    global _o
+    if _o is None:
+        if not ORACLE_LIB.exists():
+            raise RuntimeError(f"{ORACLE_LIB} missing: run `make -C oracle`")
+        L = C.CDLL(str(ORACLE_LIB))
+        sig = {
+            "oracle_scene_new": (_P, []),
+            "oracle_scene_free": (None, [_P]),
+            "oracle_scene_add_mesh": (C.c_int, [_P, _P, _P, _P, _P, _U, _U]),
+            "oracle_scene_set_materials": (C.c_int, [_P, _P, _U]),
+            "oracle_scene_add_instance": (C.c_int, [_P, _P]),
+            "oracle_scene_build": (C.c_int, [_P]),
+            "oracle_scene_update_instances": (C.c_int, [_P, _P, _U, _U]),
+            "oracle_blas_stats": (None, [_P, _U, C.POINTER(_U), C.POINTER(C.c_float)]),
+            "oracle_trace": (None, [_P, _P, _U, _P, _P, C.c_int, _U]),
+            "oracle_primary_rays": (None, [_P, _U, _U, _P]),
+            "oracle_render_refv2": (C.c_uint64, [_P, _P, _U, _U, _U, _U, _U, _U, _P, _P, _P, _P, _P, _P, _P, _P, _U]),
+            "oracle_render_ptmis": (C.c_uint64, [_P, _P, _U, _U, _U, _U, _U, _U, _U, _U, C.c_int, _P, _P, _P, _U]),
+            "oracle_pcg4d": (None, [_P, _U, _P]),
+            "oracle_u2f": (C.c_float, [_U]),
+            "oracle_sincos_2pi": (None, [_P, _U, _P, _P]),
+            "oracle_sample": (None, [C.c_int, C.c_float, _P, _U, _U, _P]),
+            "oracle_ptmis_spec_sample": (None, [C.c_double, _P, _U, _U, _P, _P]),
+            "oracle_ptmis_spec_pdf": (C.c_double, [C.c_double, _P, _P]),
+            "oracle_num_lights": (_U, [_P]),
+        }
+        optional = {
+            "oracle_reproject": (None, [_P, _U, _U] + [_P] * 13),
+            "oracle_jbf": (None, [_P, _U, _U] + [_P] * 6),
+            "oracle_gt_tonemap": (C.c_float, [C.c_float]),
+        }
+        for name, (res, args) in {**sig, **optional}.items():
+            if not hasattr(L, name):
+                if name in sig:
+                    raise RuntimeError(f"{name} missing from {ORACLE_LIB}")
+                continue
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _o = L
+    return _o
+
+
+def ref_available() -> bool:
+    return REF_LIB.exists()
+
+
+def ref():
+    global _r
+    if _r is None:
+        if not REF_LIB.exists():
+            raise RuntimeError(f"{REF_LIB} missing: run `make -C oracle ref` where the reference tree exists")
+        L = C.CDLL(str(REF_LIB))
+        sig = {
+            "ref_hardware_concurrency": (C.c_uint, []),
+            "ref_blas_build": (_P, [_P, _U, C.POINTER(C.c_double)]),
+            "ref_blas_free": (None, [_P]),
+            "ref_blas_stats": (None, [_P, C.POINTER(_U), C.POINTER(C.c_double), _P, _P]),
+            "ref_intersect": (None, [_P, _P, _U, _P, _U, C.c_int, C.POINTER(C.c_double)]),
+            "ref_occluded": (None, [_P, _P, _U, _P, _U, C.POINTER(C.c_double)]),
+            "ref_tlas_build": (_P, [_P, _U, _P, _P, _U, C.POINTER(C.c_double)]),
+            "ref_tlas_rebuild": (None, [_P, _P, C.POINTER(C.c_double)]),
+            "ref_tlas_free": (None, [_P]),
+            "ref_tlas_intersect": (None, [_P, _P, _U, _P, _P, _U, C.POINTER(C.c_double)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _r = L
+    return _r
+
+
+# ---------------------------------------------------------------- oracle scene
+class OracleScene:
+    """Oracle copy of a gknextengine_amd.SceneData (same arrays and structs)."""
+
+    def __init__(self, scene):
+        L = oracle()
+        self.h = L.oracle_scene_new()
+        self.scene = scene
+        for m in scene.meshes:
+            r = L.oracle_scene_add_mesh(self.h, ptr(m["positions"]), ptr(m["normals"]), ptr(m["indices"]),
+                                        ptr(m["tri_material"]), m["positions"].shape[0], m["indices"].shape[0])
+            assert r >= 0
+        mats = scene.c_materials()
+        L.oracle_scene_set_materials(self.h, C.addressof(mats), len(scene.materials))
+        for inst in scene.instances:
+            L.oracle_scene_add_instance(self.h, C.addressof(inst))
+        assert L.oracle_scene_build(self.h) == 0
+
+    def __del__(self):
+        try:
+            oracle().oracle_scene_free(self.h)
+        except Exception:
+            pass
+
+    def update_instances(self, worlds, first=0):
+        w = np.ascontiguousarray(np.transpose(np.asarray(worlds, np.float32), (0, 2, 1)))
+        assert oracle().oracle_scene_update_instances(self.h, ptr(w), first, w.shape[0]) == 0
+
+    def blas_stats(self, mesh=0):
+        n, s = _U(0), C.c_float(0)
+        oracle().oracle_blas_stats(self.h, mesh, C.byref(n), C.byref(s))
+        return n.value, s.value
+
+    def trace(self, rays, any_hit=False, threads=None):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), np.float32)
+        inst = np.zeros(n, np.uint32)
+        oracle().oracle_trace(self.h, ptr(rays), n, ptr(hits), ptr(inst), int(any_hit), threads or os.cpu_count())
+        return hits, inst
+
+    def num_lights(self):
+        return oracle().oracle_num_lights(self.h)
+
+    def render_refv2(self, params, W, H, rect=None, moment=None, threads=None):
+        x0, y0, w, h = rect or (0, 0, W, H)
+        out = {
+            "diffuse": np.zeros((h, w, 4), np.float32), "spec": np.zeros((h, w, 4), np.float32),
+            "albedo": np.zeros((h, w, 4), np.float32), "normal": np.zeros((h, w, 4), np.float32),
+            "objid": np.zeros((h, w), np.uint32), "motion": np.zeros((h, w, 2), np.float32),
+            "depth": np.zeros((h, w), np.float32),
+        }
+        mom = np.zeros((h, w), np.uint16) if moment is None else moment
+        rays = oracle().oracle_render_refv2(self.h, C.addressof(params), W, H, x0, y0, w, h,
+                                            *(ptr(out[k]) for k in ("diffuse", "spec", "albedo", "normal", "objid",
+                                                                    "motion", "depth")),
+                                            ptr(mom), threads or os.cpu_count())
+        out["moment"] = mom
+        out["rays"] = rays
+        return out
+
+    def render_ptmis(self, params, W, H, rect=None, frame0=0, frames=1, fp64=False, threads=None):
+        x0, y0, w, h = rect or (0, 0, W, H)
+        rgb = np.zeros((h, w, 3), np.float64)
+        d = np.zeros((h, w, 3), np.float64)
+        s = np.zeros((h, w, 3), np.float64)
+        rays = oracle().oracle_render_ptmis(self.h, C.addressof(params), W, H, x0, y0, w, h, frame0, frames,
+                                            int(fp64), ptr(rgb), ptr(d), ptr(s), threads or os.cpu_count())
+        return {"rgb": rgb, "diffuse": d, "spec": s, "rays": rays}
+
+
+def primary_rays(params, W, H):
+    rays = np.zeros((H * W, 8), np.float32)
+    oracle().oracle_primary_rays(C.addressof(params), W, H, ptr(rays))
+    return rays
+
+
+def pcg4d(seed, n):
+    s = np.ascontiguousarray(seed, np.uint32)
+    out = np.zeros((n, 4), np.uint32)
+    oracle().oracle_pcg4d(ptr(s), n, ptr(out))
+    return out
+
+
+def sincos_2pi(u):
+    u = np.ascontiguousarray(u, np.float32)
+    s = np.zeros_like(u)
+    c = np.zeros_like(u)
+    oracle().oracle_sincos_2pi(ptr(u), u.size, ptr(s), ptr(c))
+    return s, c
+
+
+def sample(kind, n, param=0.0, direction=(0, 0, 1), seed=1):
+    d = np.asarray(direction, np.float32)
+    out = np.zeros((n, 3), np.float32)
+    oracle().oracle_sample(kind, param, ptr(d), n, seed, ptr(out))
+    return out
+
+
+# ---------------------------------------------------------------- tinybvh (reference)
+def scene_triangles(mesh) -> np.ndarray:
+    """(n, 9) float32 triangle corners of a SceneData mesh dict."""
+    return np.ascontiguousarray(mesh["positions"][mesh["indices"]].reshape(-1, 9), np.float32)
+
+
+class TinyBVH:
+    """The reference's CPU BVH (tinybvh 1.3.8, binned SAH) over one mesh."""
+
+    def __init__(self, tris9):
+        self.tris = np.ascontiguousarray(tris9, np.float32)
+        ms = C.c_double(0)
+        self.h = ref().ref_blas_build(ptr(self.tris), self.tris.shape[0], C.byref(ms))
+        self.build_ms = ms.value
+
+    def __del__(self):
+        try:
+            ref().ref_blas_free(self.h)
+        except Exception:
+            pass
+
+    def stats(self):
+        n, s = _U(0), C.c_double(0)
+        ref().ref_blas_stats(self.h, C.byref(n), C.byref(s), None, None)
+        return n.value, s.value
+
+    def intersect(self, rays, threads=1, normalize=False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.zeros((rays.shape[0], 4), np.float32)
+        ms = C.c_double(0)
+        ref().ref_intersect(self.h, ptr(rays), rays.shape[0], ptr(hits), threads, int(normalize), C.byref(ms))
+        return hits, ms.value
+
+    def occluded(self, rays, threads=1):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(rays.shape[0], np.uint32)
+        ms = C.c_double(0)
+        ref().ref_occluded(self.h, ptr(rays), rays.shape[0], ptr(out), threads, C.byref(ms))
+        return out, ms.value
+
+
+class TinyTLAS:
+    def __init__(self, blases, worlds_rowmajor, blas_idx):
+        self.blases = blases
+        arr = (C.c_void_p * len(blases))(*[b.h for b in blases])
+        self._arr = arr
+        self.xf = np.ascontiguousarray(worlds_rowmajor, np.float32).reshape(-1, 16)
+        self.idx = np.ascontiguousarray(blas_idx, np.uint32)
+        ms = C.c_double(0)
+        self.h = ref().ref_tlas_build(arr, len(blases), ptr(self.xf), ptr(self.idx), self.idx.size, C.byref(ms))
+        self.build_ms = ms.value
+
+    def rebuild(self, worlds_rowmajor):
+        self.xf = np.ascontiguousarray(worlds_rowmajor, np.float32).reshape(-1, 16)
+        ms = C.c_double(0)
+        ref().ref_tlas_rebuild(self.h, ptr(self.xf), C.byref(ms))
+        return ms.value
+
+    def __del__(self):
+        try:
+            ref().ref_tlas_free(self.h)
+        except Exception:
+            pass
+
+    def intersect(self, rays, threads=1):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.zeros((rays.shape[0], 4), np.float32)
+        inst = np.zeros(rays.shape[0], np.uint32)
+        ms = C.c_double(0)
+        ref().ref_tlas_intersect(self.h, ptr(rays), rays.shape[0], ptr(hits), ptr(inst), threads, C.byref(ms))
+        return hits, inst, ms.value

@@ -1,0 +1,76 @@
+"""Writes the golden vectors of tests/golden/ from the reference's own tinybvh.
+
+Runs in the development container, where the reference tree exists: oracle/Makefile
+compiles src/ThirdParty/tinybvh/tiny_bvh.h into oracle/_ref/libtinybvh_ref.so, which
+this script calls (BVH::Build, Build(BLASInstance*), Intersect, IntersectTLAS,
+IsOccluded). Inputs and outputs are stored; the scene geometry is identified by a
+SHA-256 of its triangle array.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parents[1]))
+
+import gknextengine_amd as gk  # noqa: E402
+from oracle import pyoracle as po  # noqa: E402
+from tests import scenes  # noqa: E402
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def main():
+    meta = {}
+    # ---- config 1: Cornell box, 256x256 primary rays (every 4th pixel stored)
+    s = gk.procgen_cornell()
+    tris = po.scene_triangles(s.meshes[0])
+    W = H = 256
+    p = gk.make_frame_params(scenes.CORNELL_CAM["eye"], scenes.CORNELL_CAM["target"], W, H, fov=40.0)
+    rays = po.primary_rays(p, W, H).reshape(H, W, 8)[::4, ::4].reshape(-1, 8)
+    bvh = po.TinyBVH(tris)
+    hits, _ = bvh.intersect(rays)
+    occ, _ = bvh.occluded(scenes.random_rays(4096, 0.05, 0.95, seed=3, tmax=0.5))
+    nodes, sah = bvh.stats()
+    np.savez_compressed(HERE / "cornell_tinybvh.npz", tris=tris, rays=rays, hits=hits,
+                        occ_rays=scenes.random_rays(4096, 0.05, 0.95, seed=3, tmax=0.5), occ=occ)
+    meta["cornell"] = {"tris_sha256": sha(tris), "nodes": nodes, "sah": sah, "triangles": int(tris.shape[0])}
+    # ---- config 2: city, BVH statistics and 4096 rays
+    c = gk.procgen_city()
+    ctris = po.scene_triangles(c.meshes[0])
+    cb = po.TinyBVH(ctris)
+    nodes, sah = cb.stats()
+    pc = gk.make_frame_params(scenes.CITY_CAM["eye"], scenes.CITY_CAM["target"], 1920, 1080,
+                              fov=scenes.CITY_CAM["fov"])
+    crays = po.primary_rays(pc, 1920, 1080)[::507][:2048]
+    rr = scenes.random_rays(2048, -30.0, 30.0, seed=5)
+    rr[:, 1] = np.abs(rr[:, 1]) * 0.3
+    crays = np.concatenate([crays, rr])
+    chits, _ = cb.intersect(crays)
+    np.savez_compressed(HERE / "city_tinybvh.npz", rays=crays, hits=chits)
+    meta["city"] = {"tris_sha256": sha(ctris), "nodes": nodes, "sah": sah, "triangles": int(ctris.shape[0])}
+    # ---- config 3: 4096 instanced bricks (TLAS), frame 0
+    b = scenes.Bricks()
+    btris = po.scene_triangles(b.scene.meshes[0])
+    bb = po.TinyBVH(btris)
+    worlds = b.worlds(0.0)
+    tl = po.TinyTLAS([bb], worlds.reshape(-1, 16), np.zeros(b.n, np.uint32))
+    brays = scenes.random_rays(4096, -14.0, 14.0, seed=11)
+    bh, binst, _ = tl.intersect(brays)
+    np.savez_compressed(HERE / "bricks_tinybvh.npz", rays=brays, hits=bh, inst=binst, worlds=worlds)
+    meta["bricks"] = {"tris_sha256": sha(btris), "instances": int(b.n)}
+    (HERE / "golden_meta.json").write_text(json.dumps(meta, indent=1))
+    print(json.dumps(meta, indent=1))
+
+
+if __name__ == "__main__":
+    main()

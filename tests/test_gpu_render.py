@@ -1,0 +1,122 @@
+"""Parity of the wavefront path tracer (gk_render) with the CPU oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import gknextengine_amd as gk
+from oracle import pyoracle as po
+from tests import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def cornell_params(W, H, frame=0, estimator=gk.ESTIMATOR_REF_V2, bounces=4, samples=1, prev=None):
+    p = gk.make_frame_params(scenes.CORNELL_CAM["eye"], scenes.CORNELL_CAM["target"], W, H, fov=40.0,
+                             frame=frame, prev=prev)
+    p.estimator = estimator
+    p.bounces = bounces
+    p.max_bounces = bounces
+    p.samples = samples
+    p.global_seed = 0
+    p.denoiser = gk.DENOISER_NONE
+    return p
+
+
+def relmse(x, ref):
+    return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def test_ref_v2_cornell_frames_match_oracle():
+    """Config 1: 256x256, 16 frames x 1 spp, 4 bounces, REF_V2 estimator."""
+    W = H = 256
+    s = gk.procgen_cornell()
+    osc = po.OracleScene(s)
+    moment = np.zeros((H, W), np.uint16)
+    exact_frac = []
+    acc_gpu = np.zeros((H, W, 3))
+    acc_cpu = np.zeros((H, W, 3))
+    with gk.Context(W, H) as ctx:
+        ctx.upload(s)
+        for f in range(16):
+            p = cornell_params(W, H, frame=f)
+            ctx.frame(p)
+            ctx.render()
+            ctx.sync()
+            o = osc.render_refv2(p, W, H, moment=moment)
+            gd, gs = ctx.read("SINGLE_DIFFUSE"), ctx.read("SINGLE_SPECULAR")
+            od, os_ = o["diffuse"].astype(np.float16), o["spec"].astype(np.float16)
+            same = np.all(gd.view(np.uint16) == od.view(np.uint16), axis=2) & \
+                np.all(gs.view(np.uint16) == os_.view(np.uint16), axis=2)
+            exact_frac.append(same.mean())
+            acc_gpu += gd[..., :3].astype(np.float64) + gs[..., :3]
+            acc_cpu += o["diffuse"][..., :3] + o["spec"][..., :3]
+            if f == 0:
+                # G-buffer of the primary hit
+                assert np.array_equal(ctx.read("OBJECTID_0"), o["objid"])
+                assert np.array_equal(ctx.read("ALBEDO").view(np.uint16), o["albedo"].astype(np.float16).view(np.uint16))
+                assert np.array_equal(ctx.read("NORMAL").view(np.uint16), o["normal"].astype(np.float16).view(np.uint16))
+                assert np.array_equal(ctx.read("PREV_DEPTH").view(np.uint32), o["depth"].view(np.uint32))
+                assert np.array_equal(ctx.read("MOTIONVECTOR").view(np.uint32), o["motion"].view(np.uint32))
+        assert np.array_equal(ctx.read("MOTIONMOMENT"), moment)
+    # per-pixel identical fp16 outputs for nearly every pixel; the rest differ where a
+    # transcendental result differs in the last ulp between the device and libm
+    assert min(exact_frac) > 0.98, exact_frac
+    err = relmse(acc_gpu / 16, acc_cpu / 16)
+    assert err < 1e-4, err
+
+
+def test_ref_v2_deterministic():
+    W = H = 128
+    s = gk.procgen_cornell()
+    with gk.Context(W, H) as ctx:
+        ctx.upload(s)
+        outs = []
+        for _ in range(2):
+            ctx.frame(cornell_params(W, H, frame=3))
+            ctx.render()
+            outs.append((ctx.read("SINGLE_DIFFUSE").copy(), ctx.read("SINGLE_SPECULAR").copy()))
+    assert np.array_equal(outs[0][0].view(np.uint16), outs[1][0].view(np.uint16))
+    assert np.array_equal(outs[0][1].view(np.uint16), outs[1][1].view(np.uint16))
+
+
+def test_pt_mis_cornell_matches_oracle_same_seeds():
+    W = H = 128
+    s = gk.procgen_cornell()
+    osc = po.OracleScene(s)
+    frames = 8
+    with gk.Context(W, H) as ctx:
+        ctx.upload(s)
+        for f in range(frames):
+            ctx.frame(cornell_params(W, H, frame=f, estimator=gk.ESTIMATOR_PT_MIS, bounces=4))
+            ctx.render()
+        ctx.sync()
+        rad = ctx.read("RADIANCE")
+    gpu = rad[..., :3] / rad[..., 3:4]
+    o = osc.render_ptmis(cornell_params(W, H, estimator=gk.ESTIMATOR_PT_MIS, bounces=4), W, H, frame0=0,
+                         frames=frames)
+    cpu = o["rgb"]
+    close = np.all(np.abs(gpu - cpu) <= 1e-4 * (1 + np.abs(cpu)), axis=2)
+    assert close.mean() > 0.97, close.mean()
+    assert relmse(gpu, cpu) < 1e-3
+
+
+def test_pt_mis_white_furnace_closed_form():
+    """Closed box, Lambertian albedo a on every face, emission Le on every face, B bounces:
+    every path sees Le * sum_{k=0..B} a^k (emission is collected on emitters only at the
+    first hit, so here every face is an emitter: L = Le)."""
+    W = H = 32
+    s = gk.procgen_cornell()
+    # all faces emissive -> the primary hit sees Le exactly
+    m = gk.Material()
+    m.diffuse[:] = [2.0, 2.0, 2.0, 1.0]
+    m.diffuse_texture_id = m.mra_texture_id = m.normal_texture_id = -1
+    m.material_model = gk.MAT_DIFFUSE_LIGHT
+    s.materials = [m, m, m, m]
+    with gk.Context(W, H) as ctx:
+        ctx.upload(s)
+        ctx.frame(cornell_params(W, H, estimator=gk.ESTIMATOR_PT_MIS))
+        ctx.render()
+        rad = ctx.read("RADIANCE")
+        hit = ctx.read("PRIMARY_INST") != 0xFFFFFFFF
+    assert np.allclose(rad[hit][:, :3], 2.0)

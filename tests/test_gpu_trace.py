@@ -1,0 +1,132 @@
+"""Parity of the HIP traversal (gk_trace) with the oracle and the reference's tinybvh."""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import gknextengine_amd as gk
+from oracle import pyoracle as po
+from tests import scenes
+from tests.test_oracle import _compare_hits, _u32
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+pytestmark = pytest.mark.gpu
+
+
+def gpu_trace(ctx, rays, device, any_hit=False):
+    import torch
+    r = torch.from_numpy(np.ascontiguousarray(rays, np.float32)).to(device)
+    h = torch.zeros((rays.shape[0], 4), dtype=torch.float32, device=device)
+    inst = torch.zeros(rays.shape[0], dtype=torch.int32, device=device)
+    torch.cuda.synchronize()
+    ctx.trace(r.data_ptr(), rays.shape[0], h.data_ptr(), inst.data_ptr(), any_hit=any_hit)
+    ctx.sync()
+    return h.cpu().numpy(), inst.cpu().numpy().view(np.uint32)
+
+
+def test_cornell_primary_bit_exact(device):
+    s = gk.procgen_cornell()
+    W = H = 256
+    p = gk.make_frame_params(scenes.CORNELL_CAM["eye"], scenes.CORNELL_CAM["target"], W, H, fov=40.0)
+    rays = po.primary_rays(p, W, H)
+    with gk.Context(W, H) as ctx:
+        ctx.upload(s)
+        h, inst = gpu_trace(ctx, rays, device)
+    o, _ = po.OracleScene(s).trace(rays)
+    # identical to the oracle bit for bit (same canonical tie-break)
+    assert np.array_equal(_u32(h), _u32(o))
+    assert np.all(inst[_u32(h[:, 3]) != 0xFFFFFFFF] == 0)
+    # and to the reference's tinybvh up to ties
+    g = np.load(GOLDEN / "cornell_tinybvh.npz")
+    sub = h.reshape(H, W, 4)[::4, ::4].reshape(-1, 4)
+    _compare_hits(sub, g["hits"], "gpu vs tinybvh")
+
+
+def test_cornell_random_and_occlusion(device):
+    s = gk.procgen_cornell()
+    rays = scenes.random_rays(100000, 0.02, 0.98, seed=21, tmin=1e-3, tmax=1000.0)
+    osc = po.OracleScene(s)
+    with gk.Context(64, 64) as ctx:
+        ctx.upload(s)
+        h, _ = gpu_trace(ctx, rays, device)
+        occ_rays = scenes.random_rays(100000, 0.02, 0.98, seed=22, tmax=0.4)
+        ho, _ = gpu_trace(ctx, occ_rays, device, any_hit=True)
+    o, _ = osc.trace(rays)
+    assert np.array_equal(_u32(h), _u32(o))
+    oo, _ = osc.trace(occ_rays, any_hit=True)
+    assert np.array_equal(_u32(ho[:, 3]), _u32(oo[:, 3]))
+    g = np.load(GOLDEN / "cornell_tinybvh.npz")
+    with gk.Context(64, 64) as ctx:
+        ctx.upload(s)
+        hg, _ = gpu_trace(ctx, g["occ_rays"], device, any_hit=True)
+    assert np.array_equal(_u32(hg[:, 3]), g["occ"])
+
+
+def test_city_camera_and_random_rays(device):
+    c = gk.procgen_city()
+    g = np.load(GOLDEN / "city_tinybvh.npz")
+    p = gk.make_frame_params(scenes.CITY_CAM["eye"], scenes.CITY_CAM["target"], 1920, 1080,
+                             fov=scenes.CITY_CAM["fov"])
+    cam = po.primary_rays(p, 1920, 1080)[::16]
+    rr = scenes.random_rays(60000, -30.0, 30.0, seed=8, tmin=1e-3, tmax=1000.0)
+    rr[:, 1] = np.abs(rr[:, 1]) * 0.3
+    rays = np.concatenate([g["rays"], cam, rr])
+    with gk.Context(64, 64) as ctx:
+        ctx.upload(c)
+        h, _ = gpu_trace(ctx, rays, device)
+        st = ctx.bvh_stats()
+    assert st.blas_nodes > 0 and st.sah_cost > 0
+    o, _ = po.OracleScene(c).trace(rays)
+    assert np.array_equal(_u32(h), _u32(o))
+    _compare_hits(h[: len(g["rays"])], g["hits"], "city gpu vs tinybvh")
+
+
+def test_tlas_bricks_bit_exact(device):
+    b = scenes.Bricks()
+    g = np.load(GOLDEN / "bricks_tinybvh.npz")
+    osc = po.OracleScene(b.scene)
+    rays = np.concatenate([g["rays"], scenes.random_rays(30000, -14.0, 14.0, seed=12, tmin=1e-3, tmax=1000.0)])
+    with gk.Context(64, 64) as ctx:
+        ctx.upload(b.scene)
+        h, inst = gpu_trace(ctx, rays, device)
+        o, oi = osc.trace(rays)
+        hit = _u32(o[:, 3]) != 0xFFFFFFFF
+        assert np.array_equal(_u32(h), _u32(o))
+        assert np.array_equal(inst[hit], oi[hit])
+        # reference TLAS (tinybvh IntersectTLAS) up to ties
+        n = len(g["rays"])
+        same = (_u32(h[:n, 3]) == _u32(g["hits"][:, 3])) & (inst[:n] == g["inst"])
+        assert same.mean() > 0.99
+        # after a refit with moved instances
+        w1 = b.worlds(0.5)
+        ctx.update_instances(w1)
+        osc.update_instances(w1)
+        h2, i2 = gpu_trace(ctx, rays, device)
+    o2, oi2 = osc.trace(rays)
+    assert np.array_equal(_u32(h2), _u32(o2))
+    hit2 = _u32(o2[:, 3]) != 0xFFFFFFFF
+    assert np.array_equal(i2[hit2], oi2[hit2])
+
+
+def test_empty_and_edge_inputs(device):
+    s = gk.procgen_cornell()
+    with gk.Context(8, 8) as ctx:
+        ctx.upload(s)
+        ctx.trace(0, 0, 0)  # n = 0 is a no-op
+        ctx.sync()
+        # rays along the axes, through edges and from outside
+        rays = np.zeros((6, 8), np.float32)
+        rays[:, 3] = 0.0
+        rays[:, 7] = 1e30
+        rays[0, :3] = [0.5, 0.5, -1.0]; rays[0, 4:7] = [0, 0, 1]
+        rays[1, :3] = [0.5, 0.5, 0.5]; rays[1, 4:7] = [1, 0, 0]
+        rays[2, :3] = [0.5, 2.0, 0.5]; rays[2, 4:7] = [0, -1, 0]
+        rays[3, :3] = [0.5, 0.5, 0.5]; rays[3, 4:7] = [0, 0, -1]   # leaves through the open face
+        rays[4, :3] = [0.0, 0.0, -1.0]; rays[4, 4:7] = [0, 0, 1]   # along an edge of the box
+        rays[5, :3] = [0.3, 0.3, 0.3]; rays[5, 4:7] = [0, 1, 0]; rays[5, 7] = 0.0  # empty interval
+        h, _ = gpu_trace(ctx, rays, device)
+    o, _ = po.OracleScene(s).trace(rays)
+    assert np.array_equal(_u32(h), _u32(o))
+    assert _u32(h[5, 3]) == 0xFFFFFFFF

@@ -1,0 +1,289 @@
+"""Pins the CPU oracle: golden vectors of the reference's tinybvh, brute force,
+closed forms and distribution tests. CPU only."""
+from __future__ import annotations
+
+import hashlib
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import gknextengine_amd as gk
+from oracle import pyoracle as po
+from tests import scenes
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+META = json.loads((GOLDEN / "golden_meta.json").read_text())
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def brute_force(tris9, rays):
+    """Closest hit over all triangles with tinybvh's Moeller-Trumbore (float32 numpy),
+    ties in t broken towards the lower primitive index."""
+    v0 = tris9[:, 0:3]
+    e1 = (tris9[:, 3:6] - v0).astype(np.float32)
+    e2 = (tris9[:, 6:9] - v0).astype(np.float32)
+    out = np.full((rays.shape[0], 4), 0, np.float32)
+    out[:, 0] = np.inf
+    out[:, 3] = np.float32(0).view(np.float32)
+    prim_out = np.full(rays.shape[0], 0xFFFFFFFF, np.uint32)
+    f32 = np.float32
+    for i, r in enumerate(rays):
+        O, D = r[0:3].astype(f32), r[4:7].astype(f32)
+        h = np.stack([D[1] * e2[:, 2] - D[2] * e2[:, 1], D[2] * e2[:, 0] - D[0] * e2[:, 2],
+                      D[0] * e2[:, 1] - D[1] * e2[:, 0]], 1).astype(f32)
+        a = (e1[:, 0] * h[:, 0] + e1[:, 1] * h[:, 1] + e1[:, 2] * h[:, 2]).astype(f32)
+        with np.errstate(all="ignore"):
+            f = (f32(1) / a).astype(f32)
+            s = (O - v0).astype(f32)
+            u = (f * (s[:, 0] * h[:, 0] + s[:, 1] * h[:, 1] + s[:, 2] * h[:, 2]).astype(f32)).astype(f32)
+            q = np.stack([s[:, 1] * e1[:, 2] - s[:, 2] * e1[:, 1], s[:, 2] * e1[:, 0] - s[:, 0] * e1[:, 2],
+                          s[:, 0] * e1[:, 1] - s[:, 1] * e1[:, 0]], 1).astype(f32)
+            v = (f * (D[0] * q[:, 0] + D[1] * q[:, 1] + D[2] * q[:, 2]).astype(f32)).astype(f32)
+            t = (f * (e2[:, 0] * q[:, 0] + e2[:, 1] * q[:, 1] + e2[:, 2] * q[:, 2]).astype(f32)).astype(f32)
+            ok = ~(np.abs(a) < f32(1e-7)) & ~((u < 0) | (u > 1)) & ~((v < 0) | ((u + v).astype(f32) > 1))
+            ok &= (t > r[3]) & (t < r[7])
+        if ok.any():
+            cand = np.nonzero(ok)[0]
+            tt = t[cand]
+            best = cand[tt == tt.min()].min()
+            out[i, :3] = [t[best], u[best], v[best]]
+            prim_out[i] = best
+    out[:, 3] = prim_out.view(np.float32)
+    return out
+
+
+# ------------------------------------------------------------------ geometry / BVH
+def test_cornell_geometry_matches_golden():
+    s = gk.procgen_cornell()
+    tris = po.scene_triangles(s.meshes[0])
+    assert tris.shape[0] == 36  # 5 walls + 2 boxes (34 triangles) + 2 light triangles
+    assert hashlib.sha256(tris.tobytes()).hexdigest() == META["cornell"]["tris_sha256"]
+
+
+def test_oracle_bvh_build_equals_tinybvh():
+    """Node count and SAH cost of the oracle's binned-SAH restatement equal tinybvh's."""
+    s = gk.procgen_cornell()
+    n, sah = po.OracleScene(s).blas_stats()
+    assert n == META["cornell"]["nodes"]
+    assert sah == pytest.approx(META["cornell"]["sah"], rel=1e-6)
+
+
+def test_oracle_bvh_build_equals_tinybvh_city():
+    c = gk.procgen_city()
+    tris = po.scene_triangles(c.meshes[0])
+    assert tris.shape[0] == META["city"]["triangles"]
+    assert hashlib.sha256(tris.tobytes()).hexdigest() == META["city"]["tris_sha256"]
+    n, sah = po.OracleScene(c).blas_stats()
+    assert n == META["city"]["nodes"]
+    assert sah == pytest.approx(META["city"]["sah"], rel=1e-5)
+
+
+def _compare_hits(h, g, what):
+    """prim ids equal except ties; t/u/v bit-identical where prims agree.
+
+    A tie is a ray that hits two triangles at the same t (a shared edge) or at t one or
+    two ulps apart: tinybvh keeps the first one its traversal meets and its box test
+    (tmin < hit.t) may cull a triangle that is closer by an ulp, while the canonical
+    answer keeps the smallest (t, prim)."""
+    pa, pb = _u32(h[:, 3]), _u32(g[:, 3])
+    same = pa == pb
+    hit = same & (pa != 0xFFFFFFFF)
+    assert np.array_equal(_u32(h[hit, :3]), _u32(g[hit, :3])), what
+    diff = ~same
+    assert np.all(pa[diff] != 0xFFFFFFFF) and np.all(pb[diff] != 0xFFFFFFFF), what
+    ulps = _u32(g[diff, 0]).astype(np.int64) - _u32(h[diff, 0]).astype(np.int64)
+    assert np.all((ulps >= 0) & (ulps <= 2)), (what, ulps)  # ours is never farther
+    return int(diff.sum())
+
+
+def test_oracle_primary_hits_vs_tinybvh_golden():
+    d = np.load(GOLDEN / "cornell_tinybvh.npz")
+    osc = po.OracleScene(gk.procgen_cornell())
+    h, _ = osc.trace(d["rays"])
+    ties = _compare_hits(h, d["hits"], "cornell primary")
+    assert ties <= 0.01 * len(h)
+
+
+def test_oracle_primary_rays_reproduce_golden_inputs():
+    d = np.load(GOLDEN / "cornell_tinybvh.npz")
+    p = gk.make_frame_params(scenes.CORNELL_CAM["eye"], scenes.CORNELL_CAM["target"], 256, 256, fov=40.0)
+    rays = po.primary_rays(p, 256, 256).reshape(256, 256, 8)[::4, ::4].reshape(-1, 8)
+    assert np.array_equal(_u32(rays), _u32(d["rays"]))
+
+
+def test_oracle_vs_brute_force_cornell():
+    d = np.load(GOLDEN / "cornell_tinybvh.npz")
+    osc = po.OracleScene(gk.procgen_cornell())
+    rays = np.concatenate([d["rays"][:600], scenes.random_rays(600, 0.02, 0.98, seed=1)])
+    h, _ = osc.trace(rays)
+    bf = brute_force(d["tris"], rays)
+    assert np.array_equal(_u32(h), _u32(bf))
+
+
+def test_oracle_occlusion_vs_tinybvh_golden():
+    d = np.load(GOLDEN / "cornell_tinybvh.npz")
+    osc = po.OracleScene(gk.procgen_cornell())
+    h, _ = osc.trace(d["occ_rays"], any_hit=True)
+    assert np.array_equal(_u32(h[:, 3]), d["occ"])
+
+
+def test_oracle_city_vs_tinybvh_golden():
+    d = np.load(GOLDEN / "city_tinybvh.npz")
+    osc = po.OracleScene(gk.procgen_city())
+    h, _ = osc.trace(d["rays"])
+    _compare_hits(h, d["hits"], "city")
+
+
+def test_oracle_tlas_vs_tinybvh_golden():
+    d = np.load(GOLDEN / "bricks_tinybvh.npz")
+    b = scenes.Bricks()
+    assert np.array_equal(b.worlds(0.0), d["worlds"])
+    osc = po.OracleScene(b.scene)
+    h, inst = osc.trace(d["rays"])
+    prim_same = _u32(h[:, 3]) == _u32(d["hits"][:, 3])
+    same = prim_same & (inst == d["inst"])
+    hit = same & (_u32(h[:, 3]) != 0xFFFFFFFF)
+    assert np.array_equal(_u32(h[hit, :3]), _u32(d["hits"][hit, :3]))
+    assert np.array_equal(_u32(h[~same, 0]), _u32(d["hits"][~same, 0]))  # ties only
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="reference tinybvh library not built")
+def test_oracle_full_frame_vs_live_tinybvh():
+    s = gk.procgen_cornell()
+    p = gk.make_frame_params(scenes.CORNELL_CAM["eye"], scenes.CORNELL_CAM["target"], 256, 256, fov=40.0)
+    rays = po.primary_rays(p, 256, 256)
+    h, _ = po.OracleScene(s).trace(rays)
+    g, _ = po.TinyBVH(po.scene_triangles(s.meshes[0])).intersect(rays)
+    ties = _compare_hits(h, g, "cornell 256^2")
+    assert ties < 400
+
+
+# ------------------------------------------------------------------ RNG and samplers
+def test_pcg4d_known_answer_and_uniformity():
+    # one step from (0,0,0,0), evaluated by hand: v = 1013904223 in every lane, then
+    # the two cross-mix rounds and the xorshift of Const_Func.slang:227-233
+    M = 0xFFFFFFFF
+    v = [1013904223] * 4
+
+    def mix(v):
+        v[0] = (v[0] + v[1] * v[3]) & M
+        v[1] = (v[1] + v[2] * v[0]) & M
+        v[2] = (v[2] + v[0] * v[1]) & M
+        v[3] = (v[3] + v[1] * v[2]) & M
+
+    mix(v)
+    v = [x ^ (x >> 16) for x in v]
+    mix(v)
+    got = po.pcg4d([0, 0, 0, 0], 1)[0]
+    assert got.tolist() == v
+    # uniformity of uint_to_float over 2^16 draws of lane x
+    x = po.pcg4d([17, 23, 5, 0], 1 << 16)[:, 0]
+    f = (((x >> 9) | 0x3F800000).view(np.float32) - 1.0)
+    assert f.min() >= 0.0 and f.max() < 1.0
+    hist, _ = np.histogram(f, bins=64, range=(0, 1))
+    chi2 = ((hist - hist.mean()) ** 2 / hist.mean()).sum()
+    assert chi2 < 120  # 63 dof, p ~ 1e-5
+
+
+def test_sincos_2pi_accuracy():
+    u = np.linspace(0, 1, 200001, endpoint=False).astype(np.float32)
+    s, c = po.sincos_2pi(u)
+    ref_s = np.sin(2 * np.pi * u.astype(np.float64))
+    ref_c = np.cos(2 * np.pi * u.astype(np.float64))
+    assert np.abs(s - ref_s).max() < 4e-7
+    assert np.abs(c - ref_c).max() < 4e-7
+
+
+def test_cosine_hemisphere_distribution():
+    v = po.sample(0, 200000, seed=3).astype(np.float64)
+    assert np.allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-6)
+    cz = v[:, 2]
+    assert cz.min() >= 0
+    # cosine-weighted: cos(theta)^2 is uniform on [0,1]
+    hist, _ = np.histogram(cz ** 2, bins=50, range=(0, 1))
+    chi2 = ((hist - hist.mean()) ** 2 / hist.mean()).sum()
+    assert chi2 < 100
+    assert abs(np.arctan2(v[:, 1], v[:, 0]).mean()) < 0.02
+
+
+def test_cone_distribution():
+    ct = np.cos(np.deg2rad(10.0))
+    v = po.sample(1, 100000, param=ct, seed=4).astype(np.float64)
+    assert v[:, 2].min() >= ct - 1e-6
+    hist, _ = np.histogram(v[:, 2], bins=40, range=(ct, 1))  # uniform in cos(theta) on the cap
+    chi2 = ((hist - hist.mean()) ** 2 / hist.mean()).sum()
+    assert chi2 < 90
+
+
+def _ggx_d(alpha, cz):
+    a2 = alpha * alpha
+    d = cz * cz * (a2 - 1) + 1
+    return a2 / (np.pi * d * d)
+
+
+def test_bounded_vndf_pdf_normalised_and_matches_histogram():
+    """PT_MIS specular sampler: its pdf integrates to one over the reflected hemisphere
+    (closed form of Eto & Tokuyoshi 2023) and matches the histogram of its samples."""
+    # sharp lobe: E[1{x in cap} / pdf(x)] = solid angle of a cap around the mirror direction
+    o = po.oracle()
+    for alpha, th in [(0.05, 0.2), (0.1, 0.9)]:
+        wo = np.array([np.sin(th), 0.0, np.cos(th)])
+        n = 200000
+        dirs = np.zeros((n, 3))
+        pdfs = np.zeros(n)
+        o.oracle_ptmis_spec_sample(alpha, po.ptr(wo), n, 13, po.ptr(dirs), po.ptr(pdfs))
+        mirror = np.array([-wo[0], -wo[1], wo[2]])
+        cap = (dirs @ mirror > np.cos(0.25)) & (dirs[:, 2] > 0)
+        est = np.mean(cap / pdfs)
+        assert est == pytest.approx(2 * np.pi * (1 - np.cos(0.25)), rel=0.02), (alpha, th)
+    for alpha, th in [(0.3, 0.4), (0.7, 1.1)]:
+        wo = np.array([np.sin(th), 0.0, np.cos(th)])
+        n = 200000
+        dirs = np.zeros((n, 3))
+        pdfs = np.zeros(n)
+        o = po.oracle()
+        o.oracle_ptmis_spec_sample(alpha, po.ptr(wo), n, 9, po.ptr(dirs), po.ptr(pdfs))
+        up = dirs[:, 2] > 0
+        # E[1/pdf] over samples with positive z = measure of the sampled set <= 2*pi
+        # and the pdf at each sample agrees with the explicit pdf function
+        pdf2 = np.array([o.oracle_ptmis_spec_pdf(alpha, po.ptr(wo), po.ptr(np.ascontiguousarray(d))) for d in dirs[:200]])
+        assert np.allclose(pdf2, pdfs[:200], rtol=1e-9)
+        # histogram in (cos theta_i) vs integral of the pdf over the same bins
+        ct = dirs[up, 2]
+        edges = np.linspace(0, 1, 21)
+        hist, _ = np.histogram(ct, bins=edges)
+        # integrate pdf numerically on a grid over each bin
+        phi = np.linspace(0, 2 * np.pi, 721)[:-1]
+        expect = []
+        for a, b in zip(edges[:-1], edges[1:]):
+            cz = np.linspace(a, b, 41)[:-1] + (b - a) / 80
+            acc = 0.0
+            for z in cz:
+                r = np.sqrt(max(0.0, 1 - z * z))
+                wi = np.stack([r * np.cos(phi), r * np.sin(phi), np.full_like(phi, z)], 1)
+                h = wi + wo
+                h /= np.linalg.norm(h, axis=1, keepdims=True)
+                vals = [o.oracle_ptmis_spec_pdf(alpha, po.ptr(wo), po.ptr(np.ascontiguousarray(w))) for w in wi[::8]]
+                acc += np.mean(vals) * 2 * np.pi * (b - a) / 40
+            expect.append(acc * n)
+        expect = np.array(expect)
+        mask = expect > 50
+        rel = np.abs(hist[mask] - expect[mask]) / expect[mask]
+        assert rel.max() < 0.08, (alpha, th, rel.max())
+
+
+def test_reference_ggx_sampler_is_unit_and_centred():
+    # ggx_sample_vndf of Const_Func.slang:310-328 around +z: unit vectors in the upper
+    # hemisphere whose spread grows with alpha
+    spreads = []
+    for alpha in (0.05, 0.3, 0.9):
+        v = po.sample(2, 50000, param=alpha, direction=(0, 0, 1), seed=5).astype(np.float64)
+        assert np.allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-6)
+        assert v[:, 2].min() > 0
+        spreads.append(1 - v[:, 2].mean())
+    assert spreads[0] < spreads[1] < spreads[2]
