@@ -425,10 +425,26 @@ int gk_target_info(gk_ctx* ctx, int target, size_t* bytes, void** device_ptr) {
     return GK_OK;
 }
 
+// Statistics since the context was created: ray counts from device counters, trace
+// time summed over the HIP events bracketing every trace launch, and the stage times of
+// the last frame (events ev[0..6] of render_frame / post_process).
 int gk_get_stats(gk_ctx* ctx, gk_stats* out) {
     GK_CHECK_CTX(ctx);
     if (!out) return GK_ERR_ARG;
-    *out = ctx->c.stats;
+    Ctx& c = ctx->c;
+    GK_HIP(hipStreamSynchronize(c.stream));
+    const int s = collect_timing(c);
+    if (s != GK_OK) return s;
+    if (c.frames_rendered > 0) {
+        float a = 0.f, b = 0.f, r = 0.f, d = 0.f;
+        if (hipEventElapsedTime(&a, c.ev[0], c.ev[1]) == hipSuccess) c.stats.ms_tlas = a;
+        if (hipEventElapsedTime(&b, c.ev[0], c.ev[3]) == hipSuccess) c.stats.ms_frame = b;
+        if (hipEventElapsedTime(&r, c.ev[4], c.ev[5]) == hipSuccess) c.stats.ms_reproject = r;
+        if (hipEventElapsedTime(&d, c.ev[5], c.ev[6]) == hipSuccess) c.stats.ms_denoise = d;
+    }
+    c.stats.extension_rays = c.stats.rays_traced - c.stats.shadow_rays;
+    *out = c.stats;
+    if (c.stack_overflow) return GK_ERR_LIMIT;  // a traversal stack overflowed: results are invalid
     return GK_OK;
 }
 

@@ -168,8 +168,12 @@ struct Ctx {
         uint32_t spill_threads = 0;
     } wf;
 
-    // timing
+    // timing: per-launch trace events (collected after a stream synchronisation)
     hipEvent_t ev[16] = {};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> tev;
+    uint32_t tev_used = 0;
+    uint64_t trace_launches = 0;
+    bool stack_overflow = false;
     gk_stats stats{};
 };
 
@@ -198,6 +202,7 @@ int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_
                  uint32_t* occluded_flags);
 int launch_trace_user(Ctx& c, const gk_ray* rays, uint32_t n, gk_hit* hits, uint32_t* inst, uint32_t flags);
 int ensure_spill(Ctx& c, uint32_t threads);
+int collect_timing(Ctx& c);
 
 }  // namespace gk
 

@@ -147,12 +147,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(
     const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
     const uint32_t* __restrict__ count_dev, uint32_t max_count, uint32_t* __restrict__ next_ray,
     float4* __restrict__ hit_out, uint32_t* __restrict__ inst_out, uint32_t* __restrict__ spill,
-    uint32_t* __restrict__ overflow_flag) {
+    uint32_t* __restrict__ overflow_flag, unsigned long long* __restrict__ ray_stats) {
     __shared__ uint32_t ring[kRing][kBlock];
     const uint32_t stride = gridDim.x * kBlock;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     uint32_t count = max_count;
     if (count_dev) { const uint32_t c = *count_dev; count = c < max_count ? c : max_count; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ray_stats[ANY ? 1 : 0], (unsigned long long)count);
 
     Lane L;
     L.active = false;
@@ -326,13 +327,26 @@ uint32_t persistent_blocks(int device) {
 
 int ensure_spill(Ctx& c, uint32_t threads) {
     const size_t need = size_t(threads) * kSpill;
-    if (c.wf.spill.count < need + 2) GK_HIP(c.wf.spill.resize(need + 2));
+    if (c.wf.spill.count < need + 8) {
+        GK_HIP(c.wf.spill.resize(need + 8));
+        GK_HIP(hipMemsetAsync(c.wf.spill.ptr + need, 0, 32, c.stream));
+    }
     c.wf.spill_threads = threads;
+    if (c.tev.empty()) {
+        c.tev.resize(1024);
+        for (auto& e : c.tev) {
+            GK_HIP(hipEventCreate(&e.first));
+            GK_HIP(hipEventCreate(&e.second));
+        }
+    }
     return GK_OK;
 }
 
 // Trace a queue of rays. count_dev (device) caps the number of rays when non-null.
 // hit: float4 (t,u,v,prim) per ray (any-hit: w = 1 when occluded).
+// Trace a queue of rays. count_dev (device) caps the number of rays when non-null.
+// hit: float4 (t,u,v,prim) per ray (any-hit: w = 1 when occluded). Each launch is
+// bracketed by HIP events on the context stream (collected by collect_timing).
 int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev,
                  uint32_t max_count, float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t*) {
     if (max_count == 0) return GK_OK;
@@ -340,28 +354,47 @@ int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_
     const uint32_t threads = blocks * kBlock;
     int st = ensure_spill(c, threads);
     if (st != GK_OK) return st;
-    uint32_t* next = c.wf.spill.ptr + size_t(threads) * kSpill;  // 2 words after the spill area
-    uint32_t* ovf = next + 1;
+    uint32_t* tail = c.wf.spill.ptr + size_t(threads) * kSpill;  // [next ray, overflow, stats u64 x2]
+    uint32_t* next = tail;
+    uint32_t* ovf = tail + 1;
+    unsigned long long* rs = reinterpret_cast<unsigned long long*>(tail + 2);
     k_reset_counter<<<1, 1, 0, c.stream>>>(next);
     const bool two = !c.blas_only;
     const uint32_t root = c.h_meshes.empty() ? 0u : c.h_meshes[0].root;
     const Bvh4Node* tl = c.tlas_nodes.ptr;
-    if (any_hit) {
-        if (two)
-            k_trace<true, true><<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, tl, c.tlas_prims.ptr,
-                c.inst.ptr, root, ray_o, ray_d, count_dev, max_count, next, hit, hit_inst, c.wf.spill.ptr, ovf);
-        else
-            k_trace<true, false><<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, tl, c.tlas_prims.ptr,
-                c.inst.ptr, root, ray_o, ray_d, count_dev, max_count, next, hit, hit_inst, c.wf.spill.ptr, ovf);
-    } else {
-        if (two)
-            k_trace<false, true><<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, tl, c.tlas_prims.ptr,
-                c.inst.ptr, root, ray_o, ray_d, count_dev, max_count, next, hit, hit_inst, c.wf.spill.ptr, ovf);
-        else
-            k_trace<false, false><<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, tl, c.tlas_prims.ptr,
-                c.inst.ptr, root, ray_o, ray_d, count_dev, max_count, next, hit, hit_inst, c.wf.spill.ptr, ovf);
-    }
+    auto* kern = any_hit ? (two ? k_trace<true, true> : k_trace<true, false>)
+                         : (two ? k_trace<false, true> : k_trace<false, false>);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c.tev_used < c.tev.size()) { e0 = c.tev[c.tev_used].first; e1 = c.tev[c.tev_used].second; c.tev_used++; }
+    if (e0) GK_HIP(hipEventRecord(e0, c.stream));
+    kern<<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, tl, c.tlas_prims.ptr, c.inst.ptr, root,
+                                         ray_o, ray_d, count_dev, max_count, next, hit, hit_inst, c.wf.spill.ptr,
+                                         ovf, rs);
     GK_HIP(hipGetLastError());
+    if (e1) GK_HIP(hipEventRecord(e1, c.stream));
+    return GK_OK;
+}
+
+// Sums the elapsed time of the trace launches recorded since the last call (the stream
+// must have been synchronised) and reads the device ray counters.
+int collect_timing(Ctx& c) {
+    for (uint32_t i = 0; i < c.tev_used; ++i) {
+        float ms = 0.f;
+        GK_HIP(hipEventElapsedTime(&ms, c.tev[i].first, c.tev[i].second));
+        c.stats.ms_trace += ms;
+        c.trace_launches++;
+    }
+    c.tev_used = 0;
+    if (c.wf.spill_threads) {
+        unsigned long long v[2] = {0, 0};
+        uint32_t ovf = 0;
+        const uint32_t* tail = c.wf.spill.ptr + size_t(c.wf.spill_threads) * kSpill;
+        GK_HIP(hipMemcpy(v, tail + 2, 16, hipMemcpyDeviceToHost));
+        GK_HIP(hipMemcpy(&ovf, tail + 1, 4, hipMemcpyDeviceToHost));
+        c.stats.rays_traced = v[0] + v[1];
+        c.stats.shadow_rays = v[1];
+        c.stack_overflow = ovf != 0;
+    }
     return GK_OK;
 }
 

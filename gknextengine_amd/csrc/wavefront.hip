@@ -984,6 +984,12 @@ int render_frame(Ctx& c) {
     auto& w = c.wf;
     hipStream_t s = c.stream;
     const uint32_t n = c.width * c.height;
+    if (c.tev_used + 256u > c.tev.size() && c.tev_used > 0) {
+        // keep room for this frame's trace events: fold the recorded ones into the totals
+        GK_HIP(hipStreamSynchronize(s));
+        st = collect_timing(c);
+        if (st != GK_OK) return st;
+    }
     hipEventRecord(c.ev[0], s);
     if (c.tlas_dirty) { st = refit_tlas(c); if (st != GK_OK) return st; }
     hipEventRecord(c.ev[1], s);
