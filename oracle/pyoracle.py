@@ -55,8 +55,8 @@ def oracle():
             "oracle_num_lights": (_U, [_P]),
         }
         optional = {
-            "oracle_reproject": (None, [_P, _U, _U] + [_P] * 13),
-            "oracle_jbf": (None, [_P, _U, _U] + [_P] * 6),
+            "oracle_reproject": (None, [_P, _U, _U] + [_P] * 12),
+            "oracle_jbf": (None, [_P, _U, _U] + [_P] * 5),
             "oracle_gt_tonemap": (C.c_float, [C.c_float]),
         }
         for name, (res, args) in {**sig, **optional}.items():
@@ -271,3 +271,34 @@ class TinyTLAS:
         ms = C.c_double(0)
         ref().ref_tlas_intersect(self.h, ptr(rays), rays.shape[0], ptr(hits), ptr(inst), threads, C.byref(ms))
         return hits, inst, ms.value
+
+
+# ---------------------------------------------------------------- post passes
+def _f4(a):
+    return np.ascontiguousarray(np.asarray(a, np.float32))
+
+
+def reproject(src_d, src_s, src_a, normal, prev_d, prev_s, prev_a, obj0, obj1, motion, moment, acc_init,
+              temporal_frames=16, progressive=False):
+    """Process.ReProject restatement. All colour inputs (H, W, 4) float32 (fp16 values);
+    acc_init supplies the alpha kept by the .rgb-only writes. Returns (diffuse, spec, albedo)."""
+    H, W = obj0.shape
+    params = np.array([int(progressive), temporal_frames], np.uint32)
+    acc = np.ascontiguousarray(np.stack([_f4(a) for a in acc_init]), np.float32)
+    args = [_f4(x) for x in (src_d, src_s, src_a, normal, prev_d, prev_s, prev_a)]
+    o0 = np.ascontiguousarray(obj0, np.uint32)
+    o1 = np.ascontiguousarray(obj1, np.uint32)
+    mv = _f4(motion)
+    mo = np.ascontiguousarray(moment, np.uint16)
+    oracle().oracle_reproject(ptr(params), W, H, *(ptr(a) for a in args), ptr(o0), ptr(o1), ptr(mv), ptr(mo), ptr(acc))
+    return acc[0], acc[1], acc[2]
+
+
+def jbf(acc_d, acc_s, acc_a, sigma=2.0, sigma_lum=3.0, paper_white=600.0, bf_size=5):
+    H, W = acc_d.shape[:2]
+    fp = np.array([sigma, sigma_lum, paper_white], np.float32)
+    bs = np.array([bf_size], np.uint32)
+    out = np.zeros((H, W, 4), np.float32)
+    d, s, a = _f4(acc_d), _f4(acc_s), _f4(acc_a)
+    oracle().oracle_jbf(ptr(fp), W, H, ptr(d), ptr(s), ptr(a), ptr(bs), ptr(out))
+    return out
