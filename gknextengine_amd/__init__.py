@@ -23,13 +23,13 @@ GK_ERR_ARG, GK_ERR_HIP, GK_ERR_OOM, GK_ERR_STATE, GK_ERR_LIMIT, GK_ERR_NO_DEVICE
 MAT_LAMBERTIAN, MAT_METALLIC, MAT_DIELECTRIC, MAT_ISOTROPIC, MAT_DIFFUSE_LIGHT, MAT_MIXTURE = range(6)
 ESTIMATOR_REF_V2, ESTIMATOR_PT_MIS = 0, 1
 DENOISER_NONE, DENOISER_BILATERAL, DENOISER_UNET = 0, 1, 2
-FRAME_NO_MATERIAL_SORT, FRAME_SKIP_POST = 1, 2
+FRAME_NO_MATERIAL_SORT, FRAME_SKIP_POST, FRAME_ACCUM_FP64 = 1, 2, 4
 TRACE_CLOSEST, TRACE_ANY = 0, 1
 
 TARGETS = [
     "SINGLE_DIFFUSE", "SINGLE_SPECULAR", "ALBEDO", "NORMAL", "OBJECTID_0", "OBJECTID_1",
     "MOTIONVECTOR", "MOTIONMOMENT", "PREV_DEPTH", "ACCUM_DIFFUSE", "ACCUM_SPECULAR",
-    "ACCUM_ALBEDO", "DENOISED", "MINIGBUFFER", "PRIMARY_HIT", "PRIMARY_INST", "RADIANCE",
+    "ACCUM_ALBEDO", "DENOISED", "MINIGBUFFER", "PRIMARY_HIT", "PRIMARY_INST", "RADIANCE", "RADIANCE64",
 ]
 RT = {name: i for i, name in enumerate(TARGETS)}
 # numpy dtype and channel count of each target
@@ -39,7 +39,7 @@ TARGET_FORMAT = {
     "MOTIONVECTOR": (np.float32, 2), "MOTIONMOMENT": (np.uint16, 1), "PREV_DEPTH": (np.float32, 1),
     "ACCUM_DIFFUSE": (np.float16, 4), "ACCUM_SPECULAR": (np.float16, 4), "ACCUM_ALBEDO": (np.float16, 4),
     "DENOISED": (np.float16, 4), "MINIGBUFFER": (np.uint32, 1), "PRIMARY_HIT": (np.float32, 4),
-    "PRIMARY_INST": (np.uint32, 1), "RADIANCE": (np.float32, 4),
+    "PRIMARY_INST": (np.uint32, 1), "RADIANCE": (np.float32, 4), "RADIANCE64": (np.float64, 4),
 }
 
 
@@ -163,6 +163,13 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m gknextengine_amd.build`")
+        # PyTorch-ROCm ships its own libamdhip64 (SONAME libamdhip64.so.7, loaded under the
+        # unversioned name). Loading it first makes libgknx_rt bind to that same runtime, so
+        # the process has one HIP runtime and torch tensors and gk_* calls share the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         for name, res, args in SIGNATURES:
             fn = getattr(L, name)

@@ -37,6 +37,7 @@ size_t target_bpp(int t) {
         case GK_RT_MOTIONVECTOR: return 8;
         case GK_RT_MOTIONMOMENT: return 2;
         case GK_RT_PRIMARY_HIT: case GK_RT_RADIANCE: return 16;
+        case GK_RT_RADIANCE64: return 32;
         default: return 0;
     }
 }
@@ -395,8 +396,8 @@ int gk_reset_accumulation(gk_ctx* ctx) {
     GK_CHECK_CTX(ctx);
     Ctx& c = ctx->c;
     GK_HIP(hipStreamSynchronize(c.stream));
-    for (int t : {GK_RT_RADIANCE, GK_RT_ACCUM_DIFFUSE, GK_RT_ACCUM_SPECULAR, GK_RT_ACCUM_ALBEDO, GK_RT_MOTIONMOMENT,
-                  GK_RT_OBJECTID_1}) {
+    for (int t : {GK_RT_RADIANCE, GK_RT_RADIANCE64, GK_RT_ACCUM_DIFFUSE, GK_RT_ACCUM_SPECULAR, GK_RT_ACCUM_ALBEDO,
+                  GK_RT_MOTIONMOMENT, GK_RT_OBJECTID_1}) {
         const size_t n16 = c.target_bytes[t] / 2;
         k_fill_u16<<<uint32_t((n16 + 255) / 256), 256, 0, c.stream>>>(static_cast<uint16_t*>(c.target_ptr[t]), n16, 0);
     }

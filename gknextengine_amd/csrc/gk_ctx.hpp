@@ -148,6 +148,8 @@ struct Ctx {
         DevBuf<float4> hit;                   // per queue entry: t,u,v,prim
         DevBuf<uint32_t> hit_inst;
         DevBuf<uint32_t> sorted_idx;          // material-sorted permutation of queue entries
+        DevBuf<uint32_t> sort_keys;           // material class per queue entry
+        DevBuf<uint32_t> sort_cnt;            // 16 class counts + 16 cursors
         DevBuf<float4> shadow_o, shadow_d;    // shadow queue
         DevBuf<float4> shadow_c;              // contribution (rgb) + target (w)
         DevBuf<uint32_t> shadow_path;

@@ -55,6 +55,8 @@ struct Targets {
     float4* phit;
     uint32_t* pinst;
     float4* radiance;
+    double4* radiance64;  // written when fp64 != 0
+    uint32_t fp64;
 };
 
 struct Queues {
@@ -69,6 +71,9 @@ struct Queues {
     float4* sc_s;   // shadow contribution to Ls (rgb)
     uint32_t* spath;
     uint32_t* cnt;  // [0],[1] extension queue sizes, [2] shadow queue size, [4..] ray statistics
+    uint32_t* sort_keys;    // material class of each queue entry
+    uint32_t* sorted;       // queue entries in material order (nullptr: path order)
+    uint32_t* sort_cnt;     // 16 class counts, then 16 cursors
 };
 
 struct PathState {
@@ -380,6 +385,12 @@ __device__ inline void ref_end_sample(const DScene& s, const FrameK& K, const Ta
     float4 r = T.radiance[px];
     r.x += fc.x + fr.x; r.y += fc.y + fr.y; r.z += fc.z + fr.z; r.w += 1.f;
     T.radiance[px] = r;
+    if (T.fp64) {
+        double4 q = T.radiance64[px];
+        q.x += double(fc.x) + double(fr.x); q.y += double(fc.y) + double(fr.y); q.z += double(fc.z) + double(fr.z);
+        q.w += 1.0;
+        T.radiance64[px] = q;
+    }
     emit = false;
 }
 
@@ -658,13 +669,14 @@ __device__ inline uint32_t light_pick(const DScene& s, float u) {
 // PT_MIS flags (misc.y): bits 0-7 depth, bit8 lobe0 decided, bit9 lobe0 = diffuse,
 // bit10 last vertex was delta (or camera), bits 16-27 sample index
 __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T, Queues Q, PathState S, int cur) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = Q.cnt[cur];
     const int nxt = cur ^ 1;
     bool emit = false, emit_sh = false;
     float4 ro{}, rd{}, so{}, sd{}, scd{}, scs{};
     uint32_t p = 0;
-    if (i < n) {
+    if (i0 < n) {
+        const uint32_t i = Q.sorted ? Q.sorted[i0] : i0;  // material order
         p = Q.path[cur][i];
         uint4 misc = S.misc[p];
         const uint32_t px = misc.x;
@@ -885,6 +897,59 @@ __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T,
     count_rays(Q.cnt, 6, emit_sh);
 }
 
+// ---------------------------------------------------------------- material sort
+// Counting sort of the hit queue by material class (model x metal, misses last), so
+// that a wave shades one BSDF type: per-block LDS histograms, one scan, per-block
+// range reservation. The order inside a class is arbitrary; results are not affected
+// because every path owns its state.
+__device__ inline uint32_t sort_key(const DScene& s, float4 h, uint32_t inst) {
+    const uint32_t prim = __float_as_uint(h.w);
+    if (prim == kMiss) return 15u;
+    const gk_material& m = s.mats[material_of(s, inst, prim)];
+    return (m.material_model * 2u + (m.metalness >= 0.5f ? 1u : 0u)) & 15u;
+}
+
+__global__ __launch_bounds__(256) void k_sort_count(DScene s, Queues Q, int cur) {
+    __shared__ uint32_t hist[16];
+    if (threadIdx.x < 16) hist[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < Q.cnt[cur]) {
+        const uint32_t k = sort_key(s, Q.hit[i], Q.hit_inst[i]);
+        Q.sort_keys[i] = k;
+        atomicAdd(&hist[k], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 16 && hist[threadIdx.x]) atomicAdd(&Q.sort_cnt[threadIdx.x], hist[threadIdx.x]);
+}
+
+__global__ void k_sort_scan(uint32_t* cnt) {
+    if (threadIdx.x != 0) return;
+    uint32_t acc = 0;
+    for (int k = 0; k < 16; ++k) {
+        cnt[16 + k] = acc;
+        acc += cnt[k];
+        cnt[k] = 0u;  // ready for the next sort
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sort_scatter(Queues Q, int cur) {
+    __shared__ uint32_t hist[16], base[16];
+    if (threadIdx.x < 16) hist[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < Q.cnt[cur];
+    uint32_t k = 0, rank = 0;
+    if (live) {
+        k = Q.sort_keys[i];
+        rank = atomicAdd(&hist[k], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 16 && hist[threadIdx.x]) base[threadIdx.x] = atomicAdd(&Q.sort_cnt[16 + threadIdx.x], hist[threadIdx.x]);
+    __syncthreads();
+    if (live) Q.sorted[base[k] + rank] = i;
+}
+
 // shadow rays: add the NEE contribution of unoccluded rays
 __global__ void k_pt_shadow_resolve(Queues Q, PathState S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -914,6 +979,11 @@ __global__ void k_pt_finalize(FrameK K, Targets T, PathState S, uint32_t n) {
     float4 r = T.radiance[px];
     r.x += dx + sx; r.y += dy + sy; r.z += dz + sz; r.w += 1.f;
     T.radiance[px] = r;
+    if (T.fp64) {
+        double4 q = T.radiance64[px];
+        q.x += double(dx) + double(sx); q.y += double(dy) + double(sy); q.z += double(dz) + double(sz); q.w += 1.0;
+        T.radiance64[px] = q;
+    }
 }
 
 __global__ void k_copy_objid(const uint32_t* a, uint32_t* b, uint32_t n) {
@@ -941,6 +1011,10 @@ int ensure_wavefront(Ctx& c) {
     GK_HIP(w.shadow_o.resize(n)); GK_HIP(w.shadow_d.resize(n));
     GK_HIP(w.shadow_c.resize(2 * size_t(n))); GK_HIP(w.shadow_path.resize(n));
     GK_HIP(w.counters.resize(64));
+    GK_HIP(w.sort_keys.resize(n));
+    GK_HIP(w.sorted_idx.resize(n));
+    GK_HIP(w.sort_cnt.resize(32));
+    GK_HIP(hipMemsetAsync(w.sort_cnt.ptr, 0, 32 * 4, c.stream));
     GK_HIP(w.s_thr.resize(n)); GK_HIP(w.s_ld.resize(n)); GK_HIP(w.s_ls.resize(n));
     GK_HIP(w.s_rng.resize(n)); GK_HIP(w.s_misc.resize(n));
     GK_HIP(w.s_pos.resize(n)); GK_HIP(w.s_nrm.resize(n)); GK_HIP(w.s_dir.resize(n));
@@ -1008,6 +1082,8 @@ int render_frame(Ctx& c) {
     T.phit = static_cast<float4*>(c.target_ptr[GK_RT_PRIMARY_HIT]);
     T.pinst = static_cast<uint32_t*>(c.target_ptr[GK_RT_PRIMARY_INST]);
     T.radiance = static_cast<float4*>(c.target_ptr[GK_RT_RADIANCE]);
+    T.radiance64 = static_cast<double4*>(c.target_ptr[GK_RT_RADIANCE64]);
+    T.fp64 = (c.params.flags & GK_FRAME_ACCUM_FP64) ? 1u : 0u;
     Queues Q;
     for (int k = 0; k < 2; ++k) { Q.o[k] = w.ray_o[k].ptr; Q.d[k] = w.ray_d[k].ptr; Q.path[k] = w.ray_path[k].ptr; }
     Q.hit = w.hit.ptr;
@@ -1018,6 +1094,10 @@ int render_frame(Ctx& c) {
     Q.sc_s = w.shadow_c.ptr + n;
     Q.spath = w.shadow_path.ptr;
     Q.cnt = w.counters.ptr;
+    Q.sort_keys = w.sort_keys.ptr;
+    Q.sort_cnt = w.sort_cnt.ptr;
+    const bool sort = !(c.params.flags & GK_FRAME_NO_MATERIAL_SORT);
+    Q.sorted = sort ? w.sorted_idx.ptr : nullptr;
     PathState P;
     P.thr = w.s_thr.ptr; P.ld = w.s_ld.ptr; P.ls = w.s_ls.ptr; P.rng = w.s_rng.ptr; P.misc = w.s_misc.ptr;
     P.pos = w.s_pos.ptr; P.nrm = w.s_nrm.ptr; P.dir = w.s_dir.ptr; P.ppos = w.s_ppos.ptr; P.pnrm = w.s_pnrm.ptr;
@@ -1062,6 +1142,11 @@ int render_frame(Ctx& c) {
             }
             k_reset_counter<<<1, 1, 0, s>>>(Q.cnt + (cur ^ 1));
             k_reset_counter<<<1, 1, 0, s>>>(Q.cnt + 2);
+            if (sort) {
+                k_sort_count<<<nblk(n), 256, 0, s>>>(S0, Q, cur);
+                k_sort_scan<<<1, 64, 0, s>>>(Q.sort_cnt);
+                k_sort_scatter<<<nblk(n), 256, 0, s>>>(Q, cur);
+            }
             k_pt_shade<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, cur);
             GK_HIP(hipGetLastError());
             st = launch_trace(c, Q.so, Q.sd, Q.cnt + 2, n, Q.hit, nullptr, true, nullptr);

@@ -89,7 +89,8 @@ enum gk_target {
     GK_RT_PRIMARY_HIT = 14,     /* gk_hit   primary-ray hit {t,u,v,prim}         */
     GK_RT_PRIMARY_INST = 15,    /* R32UI    primary-ray hit instance             */
     GK_RT_RADIANCE = 16,        /* RGBA32F  running FP32 sum of diffuse+specular  */
-    GK_RT_COUNT = 17
+    GK_RT_RADIANCE64 = 17,      /* RGBA64F  FP64 sum, written with GK_FRAME_ACCUM_FP64 */
+    GK_RT_COUNT = 18
 };
 
 /* ---------------------------------------------------------------- data types */
@@ -203,6 +204,7 @@ typedef struct gk_frame_params {
 /* gk_frame_params.flags */
 #define GK_FRAME_NO_MATERIAL_SORT 1u   /* shade in path order instead of material order */
 #define GK_FRAME_SKIP_POST 2u          /* stop after the tracer (no reproject/denoise)  */
+#define GK_FRAME_ACCUM_FP64 4u         /* also add the frame's radiance to GK_RT_RADIANCE64 */
 
 /* Camera description for gk_make_frame_params (Assets::Camera, Scene.hpp). */
 typedef struct gk_camera_desc {
