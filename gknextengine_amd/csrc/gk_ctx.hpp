@@ -129,6 +129,7 @@ struct Ctx {
     DevBuf<float4> light_obj;       // 3 per light: object-space v0, e1, e2
     BuildScratch scratch;
     gk_bvh_stats bvh_stats{};
+    DevBuf<float> grad_partial;     // adjoint: per-work-group gradient rows
 
     // frame
     gk_frame_params params{};

@@ -70,6 +70,100 @@ __device__ inline float4 from_h4(ushort4 v) {
                        __half2float(__ushort_as_half(v.z)), __half2float(__ushort_as_half(v.w)));
 }
 
+// ---------------------------------------------------------------- single-ray closest hit
+// Per-thread traversal with a private stack for per-pixel kernels (the differentiable
+// renderer). Same triangle test, widened box test and (t, instance, primitive)
+// tie-break as k_trace, so it returns the same hit for the same ray.
+struct SHit {
+    float t, u, v;
+    uint32_t prim, inst;
+};
+
+__device__ inline bool trace_closest_single(const DScene& s, f3 O, f3 D, float tmin, float tmax, SHit& best) {
+    uint32_t stack[96];
+    int sp = 0;
+    const f3 wO = O, wD = D;
+    f3 rD = f3{safercp(D.x), safercp(D.y), safercp(D.z)};
+    const f3 wrD = rD;
+    uint32_t level = s.two_level ? 0u : 1u;
+    uint32_t node = s.two_level ? 0u : s.blas_root;
+    uint32_t cur_inst = 0;
+    bool found = false;
+    best.t = tmax;
+    best.prim = best.inst = 0xffffffffu;
+    best.u = best.v = 0.f;
+    for (;;) {
+        if (node == 0xfffffffeu) {
+            if (sp == 0) return found;
+            const uint32_t e = stack[--sp];
+            if (e == kStackMark) { O = wO; D = wD; rD = wrD; level = 0; continue; }
+            if (e & kInstFlag) {
+                cur_inst = e & ~kInstFlag;
+                const InstDev& I = s.inst[cur_inst];
+                const float* M = I.inv;
+                f3 r = f3{M[0] * wO.x + M[1] * wO.y + M[2] * wO.z + M[3], M[4] * wO.x + M[5] * wO.y + M[6] * wO.z + M[7],
+                          M[8] * wO.x + M[9] * wO.y + M[10] * wO.z + M[11]};
+                const float w = M[12] * wO.x + M[13] * wO.y + M[14] * wO.z + M[15];
+                O = w == 1.0f ? r : r * (1.f / w);
+                D = f3{M[0] * wD.x + M[1] * wD.y + M[2] * wD.z, M[4] * wD.x + M[5] * wD.y + M[6] * wD.z,
+                       M[8] * wD.x + M[9] * wD.y + M[10] * wD.z};
+                rD = f3{safercp(D.x), safercp(D.y), safercp(D.z)};
+                if (sp < 96) stack[sp++] = kStackMark;
+                level = 1;
+                node = I.root;
+                continue;
+            }
+            node = e;
+            continue;
+        }
+        const Bvh4Node n = (level == 0) ? s.tlas[node] : s.blas[node];
+        const float lx[4] = {n.lox.x, n.lox.y, n.lox.z, n.lox.w}, hx[4] = {n.hix.x, n.hix.y, n.hix.z, n.hix.w};
+        const float ly[4] = {n.loy.x, n.loy.y, n.loy.z, n.loy.w}, hy[4] = {n.hiy.x, n.hiy.y, n.hiy.z, n.hiy.w};
+        const float lz[4] = {n.loz.x, n.loz.y, n.loz.z, n.loz.w}, hz[4] = {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w};
+        const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+        const uint32_t me[4] = {n.meta.x, n.meta.y, n.meta.z, n.meta.w};
+        node = 0xfffffffeu;
+        for (int i = 0; i < 4; ++i) {
+            float a = (lx[i] - O.x) * rD.x, b = (hx[i] - O.x) * rD.x;
+            float lo = fminf(a, b), hi = fmaxf(a, b);
+            a = (ly[i] - O.y) * rD.y; b = (hy[i] - O.y) * rD.y;
+            lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
+            a = (lz[i] - O.z) * rD.z; b = (hz[i] - O.z) * rD.z;
+            lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
+            lo = lo > 0.f ? lo * 0.99999904632568359375f : lo * 1.00000095367431640625f;
+            hi = hi > 0.f ? hi * 1.00000095367431640625f : hi * 0.99999904632568359375f;
+            if (!(hi >= lo && hi >= tmin && lo <= best.t)) continue;
+            if (me[i] == 0u) { if (sp < 96) stack[sp++] = ch[i]; continue; }
+            if (level == 0) {
+                for (uint32_t k = 0; k < me[i]; ++k) if (sp < 96) stack[sp++] = kInstFlag | s.tlas_prims[ch[i] + k];
+                continue;
+            }
+            for (uint32_t k = 0; k < me[i]; ++k) {
+                const TriRec tr = s.tris[ch[i] + k];
+                const f3 v0 = ld3(tr.v0), e1 = ld3(tr.e1), e2 = ld3(tr.e2);
+                const f3 h = cross(D, e2);
+                const float aa = dot(e1, h);
+                if (fabsf(aa) < 0.0000001f) continue;
+                const float f = 1 / aa;
+                const f3 sv = O - v0;
+                const float u = f * dot(sv, h);
+                if (u < 0 || u > 1) continue;
+                const f3 q = cross(sv, e1);
+                const float v = f * dot(D, q);
+                if (v < 0 || u + v > 1) continue;
+                const float t = f * dot(e2, q);
+                if (!(t > tmin && t < tmax)) continue;
+                const uint32_t prim = __float_as_uint(tr.v0.w);
+                const uint32_t ci = s.two_level ? cur_inst : 0u;
+                if (!found || t < best.t || (t == best.t && (ci < best.inst || (ci == best.inst && prim < best.prim)))) {
+                    best.t = t; best.u = u; best.v = v; best.prim = prim; best.inst = ci;
+                    found = true;
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- single-ray any-hit
 // Occlusion query for rare inline uses (REF_V2 sun cone); private stack.
 __device__ inline bool occluded_single(const DScene& s, f3 O, f3 D, float tmin, float tmax) {

@@ -138,9 +138,9 @@ F4 mul4(F4 a, F4 b) { return F4{{a.v[0] * b.v[0], a.v[1] * b.v[1], a.v[2] * b.v[
 // ---------------------------------------------------------------- primary rays
 // uv = ipos/size*2-1 (no half-pixel offset), origin = MVinv*(0,0,0,1),
 // target = Pinv*(uv,1,1), dir = MVinv*(normalize(target.xyz),0), ray = normalize(dir.xyz)
-void primary_ray(const FrameParams& P, uint32_t x, uint32_t y, uint32_t W, uint32_t H, F3& O, F3& D) {
-    const float ux = (float(x) / float(W)) * 2.0f - 1.0f;
-    const float uy = (float(y) / float(H)) * 2.0f - 1.0f;
+void primary_ray_at(const FrameParams& P, float fx, float fy, uint32_t W, uint32_t H, F3& O, F3& D) {
+    const float ux = (fx / float(W)) * 2.0f - 1.0f;
+    const float uy = (fy / float(H)) * 2.0f - 1.0f;
     const float o4[4] = {0.f, 0.f, 0.f, 1.f};
     float org[4];
     mat_mul4(P.model_view_inverse, o4, org);
@@ -153,6 +153,9 @@ void primary_ray(const FrameParams& P, uint32_t x, uint32_t y, uint32_t W, uint3
     mat_mul4(P.model_view_inverse, d4, dir);
     O = F3(org[0], org[1], org[2]);
     D = normalize(F3(dir[0], dir[1], dir[2]));
+}
+void primary_ray(const FrameParams& P, uint32_t x, uint32_t y, uint32_t W, uint32_t H, F3& O, F3& D) {
+    primary_ray_at(P, float(x), float(y), W, H, O, D);
 }
 
 // hit reconstruction ----------------------------------------------------------
@@ -719,6 +722,134 @@ struct PtMis {
     }
 };
 
+// Differentiable PT_MIS without Russian roulette, in double, with detached sampling:
+// lobe choices, directions, pdfs and MIS weights come from the scene's materials (the
+// sampling set); BSDF values come from `eval` (the evaluation set). Central differences
+// of this function with respect to `eval` are the detached-sampling gradient that the
+// HIP adjoint computes.
+struct DiffPath {
+    const Scene& sc;
+    const FrameParams& P;
+    const std::vector<Material>& eval;
+    using T = double;
+    using V = D3;
+    using PT = PtMis<double>;
+
+    V path(uint32_t x, uint32_t y, uint32_t W, uint32_t H, Rng& seed) const {
+        PT pt{sc, P};
+        F3 Of, Df;
+        float jx, jy;  // sub-pixel position of this sample
+        rnd2(seed, jx, jy);
+        primary_ray_at(P, float(x) + jx, float(y) + jy, W, H, Of, Df);
+        V O = PT::cvt(Of), D = PT::cvt(Df);
+        V thr(1, 1, 1), L(0, 0, 0);
+        T last_pdf = 0;
+        bool last_delta = true;
+        for (uint32_t depth = 0;; ++depth) {
+            HitT<T> h;
+            if (!trace_closest_t<T>(sc, O, D, T(0), T(1e30), h)) {
+                if (P.has_sky) L = L + thr * V(P.sky_color[0], P.sky_color[1], P.sky_color[2]);
+                break;
+            }
+            const Instance& I = sc.inst[h.inst];
+            const Mesh& m = sc.meshes[I.mesh_id];
+            const uint32_t mi = material_of(sc, h.inst, h.prim);
+            const Material& ms = sc.mats[mi];
+            const Material& me = eval[mi];
+            const V Pp = O + D * h.t;
+            auto nrm_w = [&](V n) {
+                V o;
+                for (int j = 0; j < 3; ++j)
+                    o[j] = T(I.inv[0 * 4 + j]) * n.x + T(I.inv[1 * 4 + j]) * n.y + T(I.inv[2 * 4 + j]) * n.z;
+                return normalize(o);
+            };
+            V Ng = nrm_w(PT::cvt(cross(m.e1[h.prim], m.e2[h.prim])));
+            const T bu = h.u, bv = h.v, bw = T(1) - bu - bv;
+            V Ns = nrm_w(PT::cvt(m.n0[h.prim]) * bw + PT::cvt(m.n1[h.prim]) * bu + PT::cvt(m.n2[h.prim]) * bv);
+            if (dot(Ng, D) > 0) Ng = -Ng;
+            if (dot(Ns, Ng) < 0) Ns = -Ns;
+            if (ms.material_model == MAT_DIFFUSE_LIGHT) {
+                const V Le(ms.diffuse[0], ms.diffuse[1], ms.diffuse[2]);  // emission is not a parameter
+                T w = 1;
+                if (!last_delta && sc.light_total > 0.0) {
+                    const T cosl = std::fabs(dot(Ng, D));
+                    const T area = T(0.5) * length(pt.nrm_raw(m, h.prim, I));
+                    const T psel = pt.light_select_pdf(h.inst, h.prim);
+                    const T pl = cosl > 0 && area > 0 ? psel * h.t * h.t / (cosl * area) : T(0);
+                    w = PT::power(last_pdf, pl);
+                }
+                L = L + (depth == 0 ? Le : thr * Le * w);
+                break;
+            }
+            if (depth >= P.bounces) break;
+            const T off = T(1e-4) * (T(1) + std::max(std::fabs(Pp.x), std::max(std::fabs(Pp.y), std::fabs(Pp.z))));
+            const typename PT::Bsdf bs = pt.bsdf_of(ms), be = pt.bsdf_of(me);
+            V T1, B1;
+            PT::onb_t(Ns, T1, B1);
+            auto to_loc = [&](V v) { return V(dot(T1, v), dot(B1, v), dot(Ns, v)); };
+            const V wo = to_loc(-D);
+            if (wo.z <= 0) break;
+            if (sc.light_total > 0.0) {
+                const T ul = pt.rnd(seed);
+                T s1, s2;
+                pt.rnd2t(seed, s1, s2);
+                const uint32_t li = pt.light_pick(ul);
+                const Light& Lt = sc.lights[li];
+                const T sq = std::sqrt(s1);
+                const V Pl = PT::cvt(Lt.p0) + PT::cvt(Lt.e1) * (T(1) - sq) + PT::cvt(Lt.e2) * (s2 * sq);
+                const V Po = Pp + Ng * off;
+                const V dl = Pl - Po;
+                const T d2 = dot(dl, dl), dist = std::sqrt(d2);
+                const V wl = dl * (T(1) / dist);
+                const T cosl = std::fabs(dot(PT::cvt(Lt.n), wl));
+                const V wil = to_loc(wl);
+                if (cosl > 0 && wil.z > 0 && dot(Ng, wl) > 0) {
+                    const T psel = T(sc.light_cdf[li + 1] - sc.light_cdf[li]);
+                    const T pl = psel * d2 / (cosl * T(Lt.area));
+                    if (!trace_any_t<T>(sc, Po, wl, T(0), dist * (T(1) - T(1e-4)) - off)) {
+                        V fd, fs, fd_s, fs_s;
+                        T pdd, pds;
+                        pt.eval(bs, wo, wil, fd_s, fs_s, pdd, pds);   // pdfs of the sampling set
+                        T pdd2, pds2;
+                        pt.eval(be, wo, wil, fd, fs, pdd2, pds2);     // values of the evaluation set
+                        const T wd = PT::power(pl, (T(1) - bs.ps) * pdd);
+                        const T ws = PT::power(pl, bs.ps * pds);
+                        L = L + thr * PT::cvt(Lt.le) * (fd * wd + fs * ws) * (wil.z / pl);
+                    }
+                }
+            }
+            const T ulobe = pt.rnd(seed);
+            T u1, u2;
+            pt.rnd2t(seed, u1, u2);
+            const bool spec = ulobe < bs.ps;
+            V wi;
+            if (spec) {
+                const V mh = PT::vndf_sample(bs.alpha, wo, u1, u2);
+                wi = mh * (T(2) * dot(wo, mh)) - wo;
+            } else {
+                T sp, cp;
+                sincos_2pi<T>(u1, sp, cp);
+                const T r = std::sqrt(u2);
+                wi = V(r * cp, r * sp, std::sqrt(T(1) - u2));
+            }
+            if (wi.z <= 0) break;
+            V fd, fs, fde, fse;
+            T pdd, pds, a, b;
+            pt.eval(bs, wo, wi, fd, fs, pdd, pds);
+            pt.eval(be, wo, wi, fde, fse, a, b);
+            const T pdf = spec ? bs.ps * pds : (T(1) - bs.ps) * pdd;
+            if (!(pdf > 0)) break;
+            thr = thr * (spec ? fse : fde) * (wi.z / pdf);
+            last_pdf = pdf;
+            last_delta = false;
+            const V wiw = normalize(T1 * wi.x + B1 * wi.y + Ns * wi.z);
+            O = Pp + (dot(wiw, Ng) > 0 ? Ng : -Ng) * off;
+            D = wiw;
+        }
+        return L;
+    }
+};
+
 void build_lights(Scene& sc) {
     sc.lights.clear();
     for (uint32_t ii = 0; ii < sc.inst.size(); ++ii) {
@@ -966,6 +1097,27 @@ uint64_t oracle_render_ptmis(void* s, const FrameParams* P, uint32_t W, uint32_t
     const Scene& sc = *static_cast<Scene*>(s);
     if (fp64) return render_ptmis<double>(sc, P, W, H, x0, y0, w, h, frame0, frames, rgb, diffuse, spec, nthreads);
     return render_ptmis<float>(sc, P, W, H, x0, y0, w, h, frame0, frames, rgb, diffuse, spec, nthreads);
+}
+
+// Detached differentiable PT_MIS (DiffPath) over a rectangle: mean of spp paths seeded
+// (x, y, frame0 + s, global_seed); eval: num_materials materials or NULL (= scene's).
+void oracle_diff_render(void* s, const FrameParams* P, const Material* eval, uint32_t W, uint32_t H, uint32_t x0,
+                        uint32_t y0, uint32_t w, uint32_t h, uint32_t frame0, uint32_t spp, double* rgb,
+                        uint32_t nthreads) {
+    const Scene& sc = *static_cast<Scene*>(s);
+    std::vector<Material> ev = eval ? std::vector<Material>(eval, eval + sc.mats.size()) : sc.mats;
+    DiffPath dp{sc, *P, ev};
+    parallel_rows(h, nthreads, [&](uint32_t yy) {
+        for (uint32_t xx = 0; xx < w; ++xx) {
+            D3 acc(0, 0, 0);
+            for (uint32_t k = 0; k < spp; ++k) {
+                Rng seed{{x0 + xx, y0 + yy, frame0 + k, P->global_seed}};
+                acc = acc + dp.path(x0 + xx, y0 + yy, W, H, seed);
+            }
+            const size_t o = size_t(yy) * w + xx;
+            for (int c = 0; c < 3; ++c) rgb[3 * o + c] = acc[c] / double(spp);
+        }
+    });
 }
 
 // pcg4d known answers: n steps from seed, writes 4*n words

@@ -53,6 +53,7 @@ def oracle():
             "oracle_ptmis_spec_sample": (None, [C.c_double, _P, _U, _U, _P, _P]),
             "oracle_ptmis_spec_pdf": (C.c_double, [C.c_double, _P, _P]),
             "oracle_num_lights": (_U, [_P]),
+            "oracle_diff_render": (None, [_P, _P, _P, _U, _U, _U, _U, _U, _U, _U, _U, _P, _U]),
         }
         optional = {
             "oracle_reproject": (None, [_P, _U, _U] + [_P] * 12),
@@ -161,6 +162,18 @@ class OracleScene:
         out["moment"] = mom
         out["rays"] = rays
         return out
+
+    def diff_render(self, params, W, H, eval_materials=None, rect=None, frame0=0, spp=1, threads=None):
+        """Detached differentiable PT_MIS in double: sampling from the scene's materials,
+        BSDF values from eval_materials (list of Material, same length) when given."""
+        x0, y0, w, h = rect or (0, 0, W, H)
+        rgb = np.zeros((h, w, 3), np.float64)
+        ev = None
+        if eval_materials is not None:
+            ev = (type(eval_materials[0]) * len(eval_materials))(*eval_materials)
+        oracle().oracle_diff_render(self.h, C.addressof(params), None if ev is None else C.addressof(ev), W, H,
+                                    x0, y0, w, h, frame0, spp, ptr(rgb), threads or os.cpu_count())
+        return rgb
 
     def render_ptmis(self, params, W, H, rect=None, frame0=0, frames=1, fp64=False, threads=None):
         x0, y0, w, h = rect or (0, 0, W, H)
