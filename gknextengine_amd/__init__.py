@@ -23,7 +23,7 @@ GK_ERR_ARG, GK_ERR_HIP, GK_ERR_OOM, GK_ERR_STATE, GK_ERR_LIMIT, GK_ERR_NO_DEVICE
 MAT_LAMBERTIAN, MAT_METALLIC, MAT_DIELECTRIC, MAT_ISOTROPIC, MAT_DIFFUSE_LIGHT, MAT_MIXTURE = range(6)
 ESTIMATOR_REF_V2, ESTIMATOR_PT_MIS = 0, 1
 DENOISER_NONE, DENOISER_BILATERAL, DENOISER_UNET = 0, 1, 2
-FRAME_NO_MATERIAL_SORT, FRAME_SKIP_POST, FRAME_ACCUM_FP64 = 1, 2, 4
+FRAME_NO_MATERIAL_SORT, FRAME_SKIP_POST, FRAME_ACCUM_FP64, FRAME_LINEAR_OUTPUT = 1, 2, 4, 8
 TRACE_CLOSEST, TRACE_ANY = 0, 1
 
 TARGETS = [
@@ -102,6 +102,11 @@ class Stats(C.Structure):
         (n, C.c_double) for n in ("ms_tlas", "ms_trace", "ms_shade", "ms_reproject", "ms_denoise", "ms_frame")]
 
 
+class UNetLayer(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("cin_a", "cin_b", "cout", "cout_real", "kpad", "up", "relu", "level")] + [
+        ("offset", C.c_uint64)]
+
+
 assert C.sizeof(Material) == 64 and C.sizeof(Instance) == 224 and C.sizeof(FrameParams) == 656
 assert C.sizeof(Light) == 80 and C.sizeof(Mesh) == 48
 
@@ -141,11 +146,15 @@ SIGNATURES = [
     ("gk_unet_create", C.c_int, [_P, _U, C.c_uint64, C.POINTER(_P)]),
     ("gk_unet_destroy", C.c_int, [_P]),
     ("gk_unet_num_params", C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    ("gk_unet_layer_info", C.c_int, [_P, _U, C.POINTER(UNetLayer)]),
     ("gk_unet_params", C.c_int, [_P, C.POINTER(_P)]),
+    ("gk_unet_params_updated", C.c_int, [_P]),
     ("gk_unet_save", C.c_int, [_P, C.c_char_p]),
     ("gk_unet_load", C.c_int, [_P, C.c_char_p]),
+    ("gk_unet_grad", C.c_int, [_P, _P, _P, _U, _U, _U, _P, C.POINTER(C.c_float)]),
     ("gk_unet_train_step", C.c_int, [_P, _P, _P, _U, _U, _U, C.c_float, C.POINTER(C.c_float)]),
     ("gk_unet_infer", C.c_int, [_P, _P, _P, _U, _U]),
+    ("gk_denoise_input", C.c_int, [_P, _P]),
 ]
 
 _lib = None
@@ -324,9 +333,13 @@ class Context:
         self.h = h
         self.width, self.height = width, height
         self._keep = None
+        self.unet = None
 
     def close(self):
         if self.h:
+            if getattr(self, "unet", None) is not None:
+                self.unet.h = None  # gk_destroy destroys the context's network
+                self.unet = None
             lib().gk_destroy(self.h)
             self.h = None
 
@@ -341,6 +354,10 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def denoise_input(self, d_out: int):
+        """Writes the current frame's 10 denoiser features (width*height*10 FP32) to d_out."""
+        check(lib().gk_denoise_input(self.h, d_out), "gk_denoise_input")
 
     def upload(self, scene: SceneData):
         meshes, mats, insts = scene.c_meshes(), scene.c_materials(), scene.c_instances()
@@ -394,3 +411,101 @@ class Context:
         """Device-pointer ray cast (gk_ray in, gk_hit out); enqueue only."""
         check(lib().gk_trace(self.h, d_rays, n, d_hits, d_inst, TRACE_ANY if any_hit else TRACE_CLOSEST),
               "gk_trace")
+
+
+class UNet:
+    """The learned denoiser of a Context (gk_unet_*). Arrays are device pointers."""
+
+    IN_CHANNELS = 10
+    LAYERS = 12
+
+    def __init__(self, ctx: Context, base_channels: int = 32, seed: int = 1):
+        h = C.c_void_p()
+        check(lib().gk_unet_create(ctx.h, base_channels, seed, C.byref(h)), "gk_unet_create")
+        if getattr(ctx, "unet", None) is not None:
+            ctx.unet.h = None  # replaced (and destroyed) by the new network
+        self.h = h
+        self.ctx = ctx
+        self.base = base_channels
+        ctx.unet = self
+
+    def close(self):
+        if self.h:
+            lib().gk_unet_destroy(self.h)
+            self.h = None
+            if self.ctx.unet is self:
+                self.ctx.unet = None
+
+    def num_params(self) -> int:
+        n = C.c_uint64()
+        check(lib().gk_unet_num_params(self.h, C.byref(n)), "gk_unet_num_params")
+        return n.value
+
+    def layer(self, i: int) -> UNetLayer:
+        L = UNetLayer()
+        check(lib().gk_unet_layer_info(self.h, i, C.byref(L)), "gk_unet_layer_info")
+        return L
+
+    def params_ptr(self) -> int:
+        p = C.c_void_p()
+        check(lib().gk_unet_params(self.h, C.byref(p)), "gk_unet_params")
+        return p.value
+
+    def params_updated(self):
+        check(lib().gk_unet_params_updated(self.h), "gk_unet_params_updated")
+
+    def save(self, path):
+        check(lib().gk_unet_save(self.h, str(path).encode()), "gk_unet_save")
+
+    def load(self, path):
+        check(lib().gk_unet_load(self.h, str(path).encode()), "gk_unet_load")
+
+    HEADER = 32  # "GKUNET01", u32 base, u32 reserved, u64 count, u64 adam step
+
+    def get_params(self) -> np.ndarray:
+        """FP32 parameter vector on the host (through gk_unet_save)."""
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            f = Path(d) / "p.bin"
+            self.save(f)
+            return np.fromfile(f, np.float32, offset=self.HEADER)
+
+    def set_params(self, params: np.ndarray, step: int = 0):
+        """Replaces the parameters (through gk_unet_load; Adam moments restart)."""
+        import tempfile
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == self.num_params()
+        with tempfile.TemporaryDirectory() as d:
+            f = Path(d) / "p.bin"
+            with open(f, "wb") as fh:
+                fh.write(b"GKUNET01")
+                fh.write(np.array([self.base, 0], np.uint32).tobytes())
+                fh.write(np.array([p.size, step], np.uint64).tobytes())
+                fh.write(p.tobytes())
+            self.load(f)
+
+    # The network runs on its context's HIP stream. Inputs produced by torch (on torch's
+    # stream) are made visible with a device synchronisation first; grad and train_step
+    # return after the stream has drained (they read the loss back), so torch may free or
+    # reuse their inputs right after.
+    @staticmethod
+    def _sync_torch():
+        import torch
+        torch.cuda.synchronize()
+
+    def grad(self, d_in: int, d_target: int, n: int, h: int, w: int, d_grad: int) -> float:
+        self._sync_torch()
+        loss = C.c_float()
+        check(lib().gk_unet_grad(self.h, d_in, d_target, n, h, w, d_grad, C.byref(loss)), "gk_unet_grad")
+        return loss.value
+
+    def train_step(self, d_in: int, d_target: int, n: int, h: int, w: int, lr: float) -> float:
+        self._sync_torch()
+        loss = C.c_float()
+        check(lib().gk_unet_train_step(self.h, d_in, d_target, n, h, w, lr, C.byref(loss)), "gk_unet_train_step")
+        return loss.value
+
+    def infer(self, d_in: int, d_out: int, h: int, w: int):
+        """Enqueues inference on the context's stream (call ctx.sync() before reading d_out)."""
+        self._sync_torch()
+        check(lib().gk_unet_infer(self.h, d_in, d_out, h, w), "gk_unet_infer")

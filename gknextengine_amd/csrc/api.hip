@@ -250,6 +250,7 @@ int gk_destroy(gk_ctx* ctx) {
     Ctx& c = ctx->c;
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
+    if (c.unet) (void)gk_unet_destroy(c.unet);  // detaches itself from c
     free_targets(c);
     for (auto& e : c.ev) if (e) (void)hipEventDestroy(e);
     if (c.stream) (void)hipStreamDestroy(c.stream);

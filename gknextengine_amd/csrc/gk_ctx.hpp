@@ -130,6 +130,7 @@ struct Ctx {
     BuildScratch scratch;
     gk_bvh_stats bvh_stats{};
     DevBuf<float> grad_partial;     // adjoint: per-work-group gradient rows
+    ::gk_unet* unet = nullptr;      // learned denoiser (GK_DENOISER_UNET), owned by the context
 
     // frame
     gk_frame_params params{};

@@ -9,6 +9,7 @@
 // Storage images read out of bounds return zero, as robust Vulkan image loads do.
 #include <hip/hip_runtime.h>
 
+#include "gk_post.hpp"
 #include "gk_scene.hpp"
 
 namespace gk {
@@ -18,7 +19,7 @@ int unet_denoise_frame(Ctx& c);  // unet.hip
 namespace {
 
 struct PostK {
-    uint32_t W, H, progressive, temporal, bf_size, hdr;
+    uint32_t W, H, progressive, temporal, bf_size, hdr, linear;
     float bf_sigma, bf_sigma_lum, paper_white;
 };
 
@@ -136,33 +137,6 @@ __global__ __launch_bounds__(64) void k_reproject(PostK K, PostT T) {
     T.acc_a[i] = to_h4(lerpf(ha.x, ca.x, keep), lerpf(ha.y, ca.y, keep), lerpf(ha.z, ca.z, keep), oa.w);
 }
 
-// W_f / H_f / GranTurismoTonemapper, Const_Func.slang:84-121
-__device__ inline float w_f(float x, float e0, float e1) {
-    if (x <= e0) return 0.f;
-    if (x >= e1) return 1.f;
-    const float a = (x - e0) / (e1 - e0);
-    return a * a * (3.f - 2.f * a);
-}
-__device__ inline float h_f(float x, float e0, float e1) {
-    if (x <= e0) return 0.f;
-    if (x >= e1) return 1.f;
-    return (x - e0) / (e1 - e0);
-}
-__device__ inline float gt_tonemap(float x) {
-    const float P = 1.f, a = 0.7f, m = 0.22f, l = 0.4f, c = 1.33f, b = 0.f;
-    const float l0 = (P - m) * l / a;
-    const float L_x = m + a * (x - m);
-    const float T_x = m * powf(x / m, c) + b;
-    const float S0 = m + l0;
-    const float S1 = m + a * l0;
-    const float C2 = a * P / (P - S1);
-    const float S_x = P - (P - S1) * powf(2.71828f, -(C2 * (x - S0) / P));
-    const float w0 = 1.f - w_f(x, 0.f, m);
-    const float w2 = h_f(x, m + l0, m + l0);
-    const float w1 = 1.f - w0 - w2;
-    return T_x * w0 + L_x * w1 + S_x * w2;
-}
-
 constexpr int kR = 5, kTile = 8 + 2 * kR;
 
 __global__ __launch_bounds__(64) void k_jbf(PostK K, PostT T) {
@@ -213,7 +187,9 @@ __global__ __launch_bounds__(64) void k_jbf(PostK K, PostT T) {
         total = ld3(from_h4(T.acc_d[i])) * ld3(from_h4(T.acc_a[i])) + ld3(from_h4(T.acc_s[i]));
     }
     const float sc = K.paper_white / 40000.0f;
-    T.out[size_t(y) * K.W + x] = to_h4(gt_tonemap(total.x * sc), gt_tonemap(total.y * sc), gt_tonemap(total.z * sc), 1.f);
+    T.out[size_t(y) * K.W + x] = K.linear ? to_h4(total.x, total.y, total.z, 1.f)
+                                          : to_h4(gt_tonemap(total.x * sc), gt_tonemap(total.y * sc),
+                                                  gt_tonemap(total.z * sc), 1.f);
 }
 
 __global__ void k_copy_history(const ushort4* a, const ushort4* b, const ushort4* c, ushort4* pa, ushort4* pb,
@@ -249,6 +225,7 @@ int post_process(Ctx& c) {
     K.temporal = c.params.temporal_frames;
     K.bf_size = c.params.denoiser == GK_DENOISER_BILATERAL ? c.params.bf_size : 0u;
     K.hdr = c.params.hdr;
+    K.linear = (c.params.flags & GK_FRAME_LINEAR_OUTPUT) ? 1u : 0u;
     K.bf_sigma = c.params.bf_sigma;
     K.bf_sigma_lum = c.params.bf_sigma_lum;
     K.paper_white = c.params.paper_white_nit;

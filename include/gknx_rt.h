@@ -84,7 +84,7 @@ enum gk_target {
     GK_RT_ACCUM_DIFFUSE = 9,    /* RGBA16F  reprojected/accumulated diffuse      */
     GK_RT_ACCUM_SPECULAR = 10,  /* RGBA16F                                        */
     GK_RT_ACCUM_ALBEDO = 11,    /* RGBA16F                                        */
-    GK_RT_DENOISED = 12,        /* RGBA16F  tonemapped output                    */
+    GK_RT_DENOISED = 12,        /* RGBA16F  tonemapped output (linear radiance with GK_FRAME_LINEAR_OUTPUT) */
     GK_RT_MINIGBUFFER = 13,     /* R32UI    (inst+1)<<17 | tri, visibility buffer */
     GK_RT_PRIMARY_HIT = 14,     /* gk_hit   primary-ray hit {t,u,v,prim}         */
     GK_RT_PRIMARY_INST = 15,    /* R32UI    primary-ray hit instance             */
@@ -205,6 +205,7 @@ typedef struct gk_frame_params {
 #define GK_FRAME_NO_MATERIAL_SORT 1u   /* shade in path order instead of material order */
 #define GK_FRAME_SKIP_POST 2u          /* stop after the tracer (no reproject/denoise)  */
 #define GK_FRAME_ACCUM_FP64 4u         /* also add the frame's radiance to GK_RT_RADIANCE64 */
+#define GK_FRAME_LINEAR_OUTPUT 8u      /* GK_RT_DENOISED holds linear radiance, not the tonemapped image */
 
 /* Camera description for gk_make_frame_params (Assets::Camera, Scene.hpp). */
 typedef struct gk_camera_desc {
@@ -386,22 +387,72 @@ int gk_adjoint(gk_ctx* ctx, const float* d_dl_dimage, float* d_grad);
 int gk_render_radiance(gk_ctx* ctx, float* d_image);
 
 /* ---------------------------------------------------------------- learned denoiser
- * 5-level convolutional U-Net on BF16 MFMA with FP32 accumulation. */
+ * No counterpart in the reference renderer, whose learned denoiser is a prebuilt
+ * library called from src/Rendering/PathTracing/PathTracingRenderer.cpp:244-268.
+ *
+ * A 5-level convolutional U-Net. Channels per level: base * {1, 1.5, 2, 3, 4}
+ * rounded up to multiples of 16 (base 32: 32/48/64/96/128). Twelve 3x3 "same"
+ * convolutions (zero padding), ReLU after all but the last, 2x2 average pooling
+ * down, nearest-neighbour upsampling with skip concatenation up:
+ *   L0 in->c0  L1 c0->c0 | L2 c0->c1 | L3 c1->c2 | L4 c2->c3 | L5 c3->c4 (1/16 res)
+ *   L6 up(c4)+c3->c3 | L7 up(c3)+c2->c2 | L8 up(c2)+c1->c1 | L9 up(c1)+c0->c0
+ *   L10 c0->c0  L11 c0->3 (linear)
+ * Input features per pixel (10 floats, NHWC): linear radiance rgb, albedo rgb,
+ * normal xyz, depth feature log(1+min(t,1e4))/10 of the primary hit distance t.
+ * The network sees log(1+radiance) and predicts z = log(1+radiance); outputs are
+ * max(exp(z)-1, 0). Training loss: mean over pixels and rgb of
+ * (z - log(1+max(target,0)))^2.
+ *
+ * Parameters: one FP32 vector (Adam master copy, moments beside it); layer l is
+ * a row-major [cout_pad][kpad] block at its offset, column tap*cin_pad + ci with
+ * tap = 3*(dy+1) + (dx+1), the bias in column 9*cin_pad, zero elsewhere in the
+ * padding. Kernels use a BF16 copy refreshed after every update.
+ * Training sizes: h and w multiples of 16. Inference: any size (zero padded). */
 typedef struct gk_unet gk_unet;
 
+#define GK_UNET_IN_CHANNELS 10u
+#define GK_UNET_LAYERS 12u
+
+typedef struct gk_unet_layer {
+    uint32_t cin_a;      /* channels of the first input (upsampled when up = 1) */
+    uint32_t cin_b;      /* channels of the skip input (0: none)                 */
+    uint32_t cout;       /* output channels, padded to a multiple of 16          */
+    uint32_t cout_real;  /* 3 for the last layer, else cout                      */
+    uint32_t kpad;       /* row length: 9*(cin_a+cin_b)+1 rounded up to 32       */
+    uint32_t up;         /* first input read at half resolution                  */
+    uint32_t relu;
+    uint32_t level;      /* output resolution = input resolution >> level        */
+    uint64_t offset;     /* first parameter of the layer                         */
+} gk_unet_layer;
+
+/* Creates a network on ctx's device and stream with He-normal weights drawn from
+ * seed, and makes it ctx's GK_DENOISER_UNET. A context owns at most one network:
+ * creating another destroys the previous one, and gk_destroy destroys it. */
 int gk_unet_create(gk_ctx* ctx, uint32_t base_channels, uint64_t seed, gk_unet** out);
 int gk_unet_destroy(gk_unet* net);
 int gk_unet_num_params(gk_unet* net, uint64_t* out);
-/* Weights as FP32 master copy (device array of gk_unet_num_params floats). */
+int gk_unet_layer_info(gk_unet* net, uint32_t layer, gk_unet_layer* out);
+/* Device pointer to the FP32 parameter vector; after writing it call
+ * gk_unet_params_updated. */
 int gk_unet_params(gk_unet* net, float** d_params);
+int gk_unet_params_updated(gk_unet* net);
+/* File: "GKUNET01", u32 base_channels, u32 reserved, u64 num_params, u64 adam step,
+ * then num_params FP32 values. */
 int gk_unet_save(gk_unet* net, const char* path);
 int gk_unet_load(gk_unet* net, const char* path);
-/* One Adam step on a batch: d_input n*h*w*10 (NHWC, fp32), d_target n*h*w*3.
- * Writes the batch loss (relMSE) to *loss. */
-int gk_unet_train_step(gk_unet* net, const float* d_input, const float* d_target,
-                       uint32_t n, uint32_t h, uint32_t w, float lr, float* loss);
-/* Inference on one image: d_input h*w*10 NHWC fp32 -> d_output h*w*3. */
+/* Gradient of the training loss on a batch (no update). d_input n*h*w*10 and
+ * d_target n*h*w*3 (FP32 NHWC, device); d_grad num_params floats (overwritten);
+ * *loss receives the loss (host). */
+int gk_unet_grad(gk_unet* net, const float* d_input, const float* d_target, uint32_t n, uint32_t h,
+                 uint32_t w, float* d_grad, float* loss);
+/* gk_unet_grad followed by one Adam step (beta 0.9/0.999, eps 1e-8) with rate lr. */
+int gk_unet_train_step(gk_unet* net, const float* d_input, const float* d_target, uint32_t n, uint32_t h,
+                       uint32_t w, float lr, float* loss);
+/* Inference: d_input h*w*10 -> d_output h*w*3 linear radiance (device, FP32). */
 int gk_unet_infer(gk_unet* net, const float* d_input, float* d_output, uint32_t h, uint32_t w);
+/* The 10 input features of ctx's current frame (accumulation targets after
+ * reprojection, normal, primary hit distance) as d_input, width*height*10 FP32. */
+int gk_denoise_input(gk_ctx* ctx, float* d_input);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,171 @@
+"""Learned denoiser on the GPU against the CPU oracle (oracle/unet_oracle.py).
+
+The oracle runs in its bf16 mode (rounding where the GPU stores bfloat16), so the GPU and
+the oracle differ by the summation order of FP32 accumulations and by the rare bf16
+rounding that order flips; the tolerances below are set for that.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import gknextengine_amd as gk
+from oracle import unet_oracle as U
+
+pytestmark = pytest.mark.gpu
+
+
+def features(rng, n, h, w):
+    nrm = rng.normal(size=(n, h, w, 3))
+    nrm /= np.linalg.norm(nrm, axis=3, keepdims=True)
+    return np.concatenate([rng.gamma(0.4, 2.5, (n, h, w, 3)), rng.uniform(0, 1, (n, h, w, 3)), nrm,
+                           rng.uniform(0, 1, (n, h, w, 1))], axis=3).astype(np.float32)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_layer_table_matches_oracle():
+    with gk.Context(16, 16) as ctx:
+        net = gk.UNet(ctx, 32, seed=1)
+        assert net.num_params() == U.num_params(32)
+        for i, L in enumerate(U.layers(32)):
+            g = net.layer(i)
+            assert {k: getattr(g, k) for k in L} == L, i
+        p = net.get_params()
+        # He-normal over the real inputs, zero bias and padding
+        L0 = U.layers(32)[0]
+        M = p[:L0["cout"] * L0["kpad"]].reshape(L0["cout"], L0["kpad"])
+        assert np.all(M[:, 9 * 16:] == 0) and np.all(M[:, :144].reshape(32, 9, 16)[:, :, 10:] == 0)
+        assert abs(M[:, :144].reshape(32, 9, 16)[:, :, :10].std() - np.sqrt(2 / 90)) < 0.02
+
+
+def test_forward_matches_oracle_with_padding():
+    import torch
+    rng = np.random.default_rng(0)
+    h, w = 40, 56  # padded to 48 x 64 inside
+    f = features(rng, 1, h, w)
+    with gk.Context(16, 16) as ctx:
+        net = gk.UNet(ctx, 32, seed=2)
+        p = net.get_params()
+        p += (rng.normal(0, 0.02, p.shape) * (p != 0)).astype(np.float32)  # move off the init
+        for L in U.layers(32):  # biases
+            cin = L["cin_a"] + L["cin_b"]
+            idx = L["offset"] + np.arange(L["cout_real"]) * L["kpad"] + 9 * cin
+            p[idx] = rng.normal(0, 0.05, L["cout_real"])
+        net.set_params(p)
+        d_in = torch.from_numpy(f).cuda()
+        d_out = torch.empty(h * w * 3, device="cuda")
+        net.infer(d_in.data_ptr(), d_out.data_ptr(), h, w)
+        torch.cuda.synchronize()
+        gpu = d_out.cpu().numpy().reshape(h, w, 3)
+    fp = np.zeros((1, 48, 64, 10), np.float32)
+    fp[:, :h, :w] = f
+    ref = U.infer(p, fp, 32, bf16_mode=True)[0, :h, :w]
+    assert rel(gpu, ref) < 5e-3, rel(gpu, ref)
+    assert np.abs(gpu - ref).max() < 2e-2 * np.abs(ref).max()
+
+
+def test_gradient_matches_oracle():
+    import torch
+    rng = np.random.default_rng(1)
+    n, h, w = 2, 32, 32
+    f = features(rng, n, h, w)
+    t = rng.gamma(1.0, 1.0, (n, h, w, 3)).astype(np.float32)
+    with gk.Context(16, 16) as ctx:
+        net = gk.UNet(ctx, 32, seed=3)
+        p = net.get_params()
+        d_in, d_t = torch.from_numpy(f).cuda(), torch.from_numpy(t).cuda()
+        d_g = torch.empty(net.num_params(), device="cuda")
+        loss = net.grad(d_in.data_ptr(), d_t.data_ptr(), n, h, w, d_g.data_ptr())
+        g = d_g.cpu().numpy()
+    lo, go = U.loss_grad(p, f, t, 32, bf16_mode=True)
+    assert loss == pytest.approx(lo, rel=2e-3)
+    for i, L in enumerate(U.layers(32)):
+        sl = slice(L["offset"], L["offset"] + L["cout"] * L["kpad"])
+        a, b = g[sl], go[sl]
+        cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+        assert rel(a, b) < 3e-2 and cos > 0.999, (i, rel(a, b), cos)
+    pad = np.ones(go.size, bool)  # padding parameters get exactly zero gradient
+    for i, L in enumerate(U.layers(32)):
+        cin = L["cin_a"] + L["cin_b"]
+        cin_real = U.IN_CH if i == 0 else cin
+        M = pad[L["offset"]:L["offset"] + L["cout"] * L["kpad"]].reshape(L["cout"], L["kpad"])
+        for tap in range(9):
+            M[:L["cout_real"], tap * cin:tap * cin + cin_real] = False
+        M[:L["cout_real"], 9 * cin] = False
+    assert np.all(g[pad] == 0)
+
+
+def test_adam_step_and_save_load():
+    import torch
+    rng = np.random.default_rng(2)
+    n, h, w = 1, 32, 32
+    f = features(rng, n, h, w)
+    t = rng.gamma(1.0, 1.0, (n, h, w, 3)).astype(np.float32)
+    with gk.Context(16, 16) as ctx:
+        net = gk.UNet(ctx, 32, seed=4)
+        p0 = net.get_params()
+        d_in, d_t = torch.from_numpy(f).cuda(), torch.from_numpy(t).cuda()
+        d_g = torch.empty(net.num_params(), device="cuda")
+        net.grad(d_in.data_ptr(), d_t.data_ptr(), n, h, w, d_g.data_ptr())
+        g = d_g.cpu().numpy()
+        net.train_step(d_in.data_ptr(), d_t.data_ptr(), n, h, w, 1e-3)
+        p1 = net.get_params()
+        ref, _, _ = U.adam(p0, g, np.zeros_like(p0), np.zeros_like(p0), 1, 1e-3)
+        assert np.abs(p1 - ref).max() < 1e-6
+        assert np.all(p1[g == 0] == p0[g == 0])
+        net.set_params(p0)
+        assert np.array_equal(net.get_params(), p0)
+
+
+def test_training_reduces_loss():
+    import torch
+    rng = np.random.default_rng(3)
+    n, h, w = 4, 64, 64
+    # smooth radiance + noise: the network has to learn to smooth
+    yy, xx = np.mgrid[0:h, 0:w] / 64.0
+    clean = np.stack([1 + np.sin(6 * xx + 3 * k) * np.cos(4 * yy) for k in range(3)], axis=2)
+    clean = np.broadcast_to(clean, (n, h, w, 3)).astype(np.float32)
+    f = features(rng, n, h, w)
+    f[..., :3] = clean * rng.gamma(1.0, 1.0, clean.shape)
+    with gk.Context(16, 16) as ctx:
+        net = gk.UNet(ctx, 32, seed=5)
+        d_in, d_t = torch.from_numpy(f).cuda(), torch.from_numpy(np.ascontiguousarray(clean)).cuda()
+        losses = [net.train_step(d_in.data_ptr(), d_t.data_ptr(), n, h, w, 1e-3) for _ in range(60)]
+    assert np.all(np.isfinite(losses))
+    assert np.mean(losses[-5:]) < 0.3 * losses[0], losses[::10]
+
+
+def test_frame_denoiser_path_equals_infer():
+    import torch
+
+    from bench import gt_tonemap
+    from gknextengine_amd import scenes
+    W = H = 64
+    s = gk.procgen_cornell()
+    cam = scenes.CORNELL_CAM
+    p = gk.make_frame_params(cam["eye"], cam["target"], W, H, fov=cam["fov"])
+    p.estimator = gk.ESTIMATOR_PT_MIS
+    p.temporal_frames = 1
+    p.denoiser = gk.DENOISER_UNET
+    with gk.Context(W, H) as ctx:
+        ctx.upload(s)
+        net = gk.UNet(ctx, 32, seed=6)
+        ctx.frame(p)
+        ctx.render()
+        ctx.sync()
+        den = ctx.read("DENOISED")[..., :3].astype(np.float32)
+        d_f = torch.empty(W * H * 10, device="cuda")
+        ctx.denoise_input(d_f.data_ptr())
+        d_o = torch.empty(W * H * 3, device="cuda")
+        net.infer(d_f.data_ptr(), d_o.data_ptr(), H, W)
+        torch.cuda.synchronize()
+        out = d_o.cpu().numpy().reshape(H, W, 3)
+        feats = d_f.cpu().numpy().reshape(H, W, 10)
+    expect = gt_tonemap(out * (p.paper_white_nit / 40000.0))
+    assert np.abs(den - expect).max() < 2e-3
+    # features: radiance = accumulated diffuse * albedo + specular; albedo; normal; depth
+    assert np.all(feats[..., :3] >= 0) and np.all(np.abs(np.linalg.norm(feats[..., 6:9], axis=2) - 1) < 1e-2)
+    assert np.all((feats[..., 9] >= 0) & (feats[..., 9] <= 1))
