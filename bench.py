@@ -12,8 +12,12 @@ region; ``cpu_baseline`` is the reference's tinybvh BVH::Intersect on host cores
 
 Other configurations (``--config``): ``dynamic`` (config 3: 4096 moving instanced bricks,
 TLAS refit every frame, 4 bounces, reprojection + bilateral, 120 frames) and ``cornell``
-(config 1: 256x256, 16 frames x 1 spp, 4 bounces). ``--relmse N`` adds the relMSE of the
-measured frame against an N-spp FP64-accumulated render of the same frame.
+(config 1: 256x256, 16 frames x 1 spp, 4 bounces), ``diff`` (config 4: path-replay adjoint
+at 512x512, 4 spp, with its finite-difference and 4096-spp reference-gradient checks) and
+``denoise`` (config 5: learned vs bilateral denoiser on 1-spp 1080p frames of the city and
+the Cornell box). ``--relmse N``
+adds the relMSE of the measured frame against an N-spp FP64-accumulated render of the
+same frame (``denoise`` defaults to N = 4096).
 """
 from __future__ import annotations
 
@@ -59,6 +63,18 @@ def relmse(x, ref):
     x = np.asarray(x, np.float64)
     ref = np.asarray(ref, np.float64)
     return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def relmse_stats(x, ref):
+    """relMSE with non-finite pixels shown as black, relMSE over the finite pixels only, and
+    the number of pixels with a non-finite channel (the bilateral filter divides by a weight
+    sum that is zero when every tap fails the luminance test, as the reference's does)."""
+    import numpy as np
+    x = np.asarray(x, np.float64)
+    bad = ~np.all(np.isfinite(x), axis=-1)
+    xs = np.where(np.isfinite(x), x, 0.0)
+    ok = ~bad
+    return relmse(xs, ref), relmse(x[ok], np.asarray(ref)[ok]), int(bad.sum())
 
 
 def cpu_baseline(scene, params, W, H):
@@ -313,6 +329,241 @@ def run_cornell(args):
             "config": {"workload": "Cornell box (config 1), 256x256, 16 frames x 1 spp, 4 bounces", **out}}
 
 
+def denoise_compare(ctx, net, p, ref_spp, reps=20):
+    """One 1-spp frame of the current view (TemporalFrames = 1) through the bilateral and the
+    learned denoisers, both writing linear radiance; relMSE of the raw frame and of each
+    denoised image against an FP64 ref_spp render; post-pass milliseconds of each."""
+    import numpy as np
+
+    import gknextengine_amd as gk
+    q = gk.FrameParams.from_buffer_copy(p)
+    q.temporal_frames = 1
+    q.flags |= gk.FRAME_LINEAR_OUTPUT
+    out, ms = {}, {}
+    for name, den in (("bilateral", gk.DENOISER_BILATERAL), ("unet", gk.DENOISER_UNET)):
+        q.denoiser = den
+        t = []
+        for r in range(reps):  # the same frame (same seeds) every time; HIP events around the denoiser
+            ctx.reset()
+            ctx.frame(q)
+            ctx.render()
+            ctx.sync()
+            t.append(ctx.stats().ms_denoise)
+        ms[name] = float(np.median(t[2:]))
+        out[name] = ctx.read("DENOISED")[..., :3].astype(np.float64)
+        rad = ctx.read("RADIANCE")
+        out["noisy"] = rad[..., :3] / np.maximum(rad[..., 3:4], 1.0)
+    ref = reference_frame(ctx, q, ref_spp)
+    res = {}
+    for k, v in out.items():
+        a, b, nbad = relmse_stats(v, ref)
+        res[f"relmse_{k}"] = a
+        if nbad:
+            res[f"relmse_{k}_finite_only"] = b
+            res[f"nonfinite_pixels_{k}"] = nbad
+    res.update({"reference_spp": ref_spp, "denoise_ms_bilateral": round(ms["bilateral"], 3),
+                "denoise_ms_unet": round(ms["unet"], 3),
+                "unet_extra_ms_per_frame": round(ms["unet"] - ms["bilateral"], 3),
+                "unet_params": net.num_params()})
+    return res
+
+
+def run_diff(args):
+    """Config 4: gradient of an L2 image loss with respect to 256 materials x (albedo rgb,
+    roughness, metallic) of the tiled Cornell box at 512x512, 4 spp, 4 bounces, by the
+    path-replay adjoint. The target is rendered (256 spp, other seeds) with the parameters
+    perturbed by N(0, 0.05). Checks: central differences (h = 1e-3) of the same-seed 4-spp
+    loss for albedo (its sampling does not depend on albedo, so the fixed-seed loss is
+    smooth in it); a 4096-spp reference gradient (adjoint of the 4096-spp residual); and a
+    brute-force gradient for all five parameter kinds, central differences of the 4096-spp
+    render at 128x128 with common seeds (it sees how roughness and metallic move the
+    sampling), against the adjoint of the same linearised loss."""
+    import numpy as np
+    import torch
+
+    import gknextengine_amd as gk
+    from gknextengine_amd import diff, scenes
+    scene, nmat = scenes.cornell_tiles()
+    base = list(scene.materials)
+    cam = scenes.CORNELL_CAM
+
+    def params(W, H, spp, frame, seed=3):
+        p = gk.make_frame_params(cam["eye"], cam["target"], W, H, fov=cam["fov"])
+        p.estimator = gk.ESTIMATOR_PT_MIS
+        p.samples = spp
+        p.bounces = p.max_bounces = 4
+        p.total_frames = frame
+        p.global_seed = seed
+        return p
+
+    def render_big(ctx, theta, W, H, spp, frame0, chunk=64):
+        """Mean of spp paths per pixel, chunk spp per launch."""
+        acc = torch.zeros((H, W, 3), device="cuda")
+        for k in range(0, spp, chunk):
+            acc += diff.render(theta, ctx, base, params(W, H, chunk, frame0 + k)).detach() * chunk
+        return acc / spp
+
+    def adjoint(ctx, theta, W, H, spp, frame0, dL, chunk=64):
+        """Gradient of <dL, image> with the image at spp paths per pixel."""
+        g = torch.zeros_like(theta)
+        for k in range(0, spp, chunk):
+            t = theta.clone().requires_grad_(True)
+            (diff.render(t, ctx, base, params(W, H, chunk, frame0 + k)) * dL).sum().backward()
+            g += t.grad * (chunk / spp)
+        return g
+
+    W = H = 512
+    spp = 4
+    ctx = gk.Context(W, H)
+    ctx.upload(scene)
+    theta = diff.params_of(base).cuda()
+    gen = torch.Generator().manual_seed(5)
+    pert = theta.clone()
+    pert[:nmat] = (pert[:nmat] + 0.05 * torch.randn(nmat, 5, generator=gen).cuda()).clamp(0.02, 0.98)
+    log("rendering target")
+    target = render_big(ctx, pert, W, H, 256, 50_000_000)
+    p4 = params(W, H, spp, 1)
+
+    def loss_grad(t):
+        t = t.clone().requires_grad_(True)
+        img = diff.render(t, ctx, base, p4)
+        loss = ((img - target) ** 2).mean()
+        loss.backward()
+        return ((img.detach().double() - target.double()) ** 2).mean().item(), t.grad
+
+    def loss_only(t):
+        with torch.no_grad():
+            img = diff.render(t, ctx, base, p4)
+        return ((img.double() - target.double()) ** 2).mean().item()
+
+    for _ in range(2):
+        loss_grad(theta)
+    times = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loss, g4 = loss_grad(theta)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    ms = float(np.median(times) * 1e3)
+    # central differences of the same-seed 4-spp loss in albedo
+    rng = np.random.default_rng(9)
+    lit = np.nonzero(g4[:nmat, :3].abs().sum(dim=1).cpu().numpy() > 0)[0]
+    h = 1e-3
+    fd_err = []
+    for m in rng.choice(lit, 32, replace=False):
+        k = int(rng.integers(3))
+        tp, tm = theta.clone(), theta.clone()
+        tp[m, k] += h
+        tm[m, k] -= h
+        fd = (loss_only(tp) - loss_only(tm)) / (2 * h)
+        fd_err.append(abs(fd - g4[m, k].item()) / max(abs(fd), 1e-12))
+    # 4096-spp reference gradient of the same loss: independent seeds for image and adjoint
+    log("4096-spp reference gradient")
+    t0 = time.perf_counter()
+    img_ref = render_big(ctx, theta, W, H, 4096, 60_000_000)
+    dL = 2.0 * (img_ref - target) / target.numel()
+    g_ref = adjoint(ctx, theta, W, H, 4096, 70_000_000, dL)
+    t_ref = time.perf_counter() - t0
+
+    def cos(x, y):
+        x, y = x[:nmat].flatten(), y[:nmat].flatten()
+        return float((x @ y) / (x.norm() * y.norm()))
+
+    # The 4-spp gradient two ways. Correlated: loss.backward() on one 4-spp image, residual
+    # and adjoint from the same paths; biased by the gradient of the image variance.
+    # Decorrelated: residual from one 4-spp render, adjoint with independent seeds; an
+    # unbiased estimate of the reference gradient (its average over draws converges to it).
+    cos_corr = cos(g4, g_ref)
+    g_dec = []
+    for d in range(16):
+        img_a = diff.render(theta, ctx, base, params(W, H, spp, 1000 + 2 * d)).detach()
+        g_dec.append(adjoint(ctx, theta, W, H, spp, 1001 + 2 * d, 2.0 * (img_a - target) / target.numel(), chunk=spp))
+    cos_dec1 = cos(g_dec[0], g_ref)
+    cos_dec16 = cos(sum(g_dec) / 16, g_ref)
+    # Brute force at 128x128 for every parameter kind: central differences (h = 0.02) of the
+    # mean radiance rendered with common seeds, 8 chunks of 512 spp giving a standard error,
+    # against the adjoint of the same functional (also in 8 chunks). The finite difference
+    # includes how roughness and metallic move the sampled directions and lobe choices.
+    log("brute-force gradient at 128x128")
+    ctx.close()
+    Ws = 128
+    cs = gk.Context(Ws, Ws)
+    cs.upload(scene)
+    dLs = torch.full((Ws, Ws, 3), 1.0 / (Ws * Ws * 3), device="cuda")
+    chunks, cspp, f0 = 8, 512, 80_000_000
+    g_ch = [adjoint(cs, theta, Ws, Ws, cspp, f0 + 100_000 + c * cspp, dLs) for c in range(chunks)]
+    g_adj = sum(g_ch) / chunks
+    g_adj_se = torch.stack(g_ch).std(dim=0) / np.sqrt(chunks)
+    hb = 0.02
+    bf = []
+    order = torch.argsort(g_adj[:nmat].abs(), dim=0, descending=True).cpu().numpy()
+    for k in range(5):
+        for m in order[:8, k]:
+            tp, tm = theta.clone(), theta.clone()
+            tp[m, k] += hb
+            tm[m, k] -= hb
+            fds = []
+            for c in range(chunks):
+                fr = f0 + c * cspp
+                fds.append(float(((render_big(cs, tp, Ws, Ws, cspp, fr) - render_big(cs, tm, Ws, Ws, cspp, fr))
+                                  * dLs).sum().item() / (2 * hb)))
+            fd, se = float(np.mean(fds)), float(np.std(fds, ddof=1) / np.sqrt(chunks))
+            a, sa = g_adj[m, k].item(), g_adj_se[m, k].item()
+            bf.append((k, fd, a, (a - fd) / max(np.hypot(se, sa), 1e-12)))
+    cs.close()
+    kinds = ["albedo_r", "albedo_g", "albedo_b", "roughness", "metallic"]
+    by_kind = {}
+    for k in range(5):
+        rows = [x for x in bf if x[0] == k]
+        fdv = np.array([x[1] for x in rows])
+        adv = np.array([x[2] for x in rows])
+        z = np.array([x[3] for x in rows])
+        by_kind[kinds[k]] = {"rel_err": round(float(np.linalg.norm(fdv - adv) / np.linalg.norm(fdv)), 4),
+                             "mean_abs_z": round(float(np.abs(z).mean()), 2), "max_abs_z": round(float(np.abs(z).max()), 2)}
+    return {"metric": METRIC, "value": round(ms, 3), "unit": "ms per forward+adjoint (512x512, 4 spp)",
+            "n_gpus": 1, "steps": args.steps, "warmup": 2, "ms_per_step": round(ms, 3), "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "differentiable render (config 4): tiled Cornell box, 256 materials x 5 params, "
+                                   "512x512, 4 spp, 4 bounces, L2 loss, path-replay adjoint",
+                       "loss": loss, "paths_per_pass": W * H * spp,
+                       "fd_albedo_h1e-3_median_rel_err": float(np.median(fd_err)),
+                       "fd_albedo_h1e-3_max_rel_err": float(np.max(fd_err)), "fd_params": len(fd_err),
+                       "reference_gradient_seconds": round(t_ref, 1),
+                       "cosine_vs_4096spp_reference": {"4spp_correlated": round(cos_corr, 4),
+                                                       "4spp_decorrelated": round(cos_dec1, 4),
+                                                       "mean_of_16_decorrelated_4spp": round(cos_dec16, 4)},
+                       "bruteforce_128px_4096spp_h0.02": by_kind, "bruteforce_params": len(bf)}}
+
+
+def run_denoise(args):
+    """Config 5: the learned denoiser against the bilateral filter on 1-spp frames at 1080p
+    (the city at the benchmark camera, the Cornell box), relMSE against FP64 renders."""
+    import gknextengine_amd as gk
+    from gknextengine_amd import scenes
+    W, H = args.width, args.height
+    res = {}
+    for name, scene, cam, bounces in (("city", lambda: gk.procgen_city(64, 4, 256, 64, 2024), scenes.CITY_CAM, 8),
+                                      ("cornell", gk.procgen_cornell, scenes.CORNELL_CAM, 4)):
+        ctx = gk.Context(W, H)
+        ctx.upload(scene())
+        net = gk.UNet.pretrained(ctx)
+        p = gk.make_frame_params(cam["eye"], cam["target"], W, H, fov=cam["fov"])
+        p.estimator = gk.ESTIMATOR_PT_MIS
+        p.bounces = p.max_bounces = bounces
+        p.global_seed = 11
+        p.total_frames = 5
+        res[name] = denoise_compare(ctx, net, p, args.relmse or 4096)
+        log(name, res[name])
+        ctx.close()
+    c = res["city"]
+    return {"metric": METRIC, "value": c["relmse_unet"], "unit": "relMSE (linear, vs FP64 render)", "n_gpus": 1,
+            "steps": 20, "warmup": 0, "ms_per_step": c["denoise_ms_unet"], "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+            "config": {"workload": "learned denoiser (config 5) vs bilateral on 1-spp PT_MIS frames, 1920x1080",
+                       "resolution": [W, H], **{f"{k}_{m}": v for k, d in res.items() for m, v in d.items()}}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -321,7 +572,7 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--bounces", type=int, default=8)
-    ap.add_argument("--config", default="city", choices=["city", "dynamic", "cornell"])
+    ap.add_argument("--config", default="city", choices=["city", "dynamic", "cornell", "denoise", "diff"])
     ap.add_argument("--relmse", type=int, default=0, help="reference spp for relMSE (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sort", action="store_true", help="shade in path order (no material sort)")
@@ -331,7 +582,8 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (DESIGN.md describes how it would "
               "shard).", file=sys.stderr)
         sys.exit(2)
-    run = {"city": run_city, "dynamic": run_dynamic, "cornell": run_cornell}[args.config]
+    run = {"city": run_city, "dynamic": run_dynamic, "cornell": run_cornell, "denoise": run_denoise,
+           "diff": run_diff}[args.config]
     print(json.dumps(run(args)), flush=True)
 
 

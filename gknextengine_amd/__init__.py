@@ -429,6 +429,17 @@ class UNet:
         self.base = base_channels
         ctx.unet = self
 
+    WEIGHTS = PKG / "weights" / "unet_b32.bin"
+
+    @classmethod
+    def pretrained(cls, ctx: Context) -> "UNet":
+        """The network trained by gknextengine_amd.denoise_train (weights shipped in the package)."""
+        if not cls.WEIGHTS.exists():
+            raise RuntimeError(f"{cls.WEIGHTS} is missing")
+        net = cls(ctx, 32, seed=0)
+        net.load(cls.WEIGHTS)
+        return net
+
     def close(self):
         if self.h:
             lib().gk_unet_destroy(self.h)

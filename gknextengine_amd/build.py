@@ -62,9 +62,24 @@ def build(verbose: bool = True) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_host_demo()
     if verbose:
-        print(f"built {LIB} ({LIB.stat().st_size // 1024} KiB) from {len(srcs)} sources")
+        print(f"built {LIB} ({LIB.stat().st_size // 1024} KiB) from {len(srcs)} sources, and {DEMO.name}")
     return LIB
+
+
+DEMO = OUT_DIR / "gk_cornell_demo"
+
+
+def build_host_demo() -> Path:
+    """The C++ adapter demo (host/), a plain g++ build against the C ABI."""
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", f"-I{PKG.parent / 'include'}", str(PKG / "host" / "cornell_demo.cpp"),
+           f"-L{OUT_DIR}", "-lgknx_rt", "-Wl,-rpath,$ORIGIN", "-o", str(DEMO)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed for the host demo:\n{res.stdout}\n{res.stderr}")
+    return DEMO
 
 
 if __name__ == "__main__":

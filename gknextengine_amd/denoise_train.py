@@ -195,8 +195,11 @@ def evaluate(net: gk.UNet, d_feat, d_tgt, idx, bil, paper_white=600.0):
                "unet": out.cpu().numpy().reshape(TILE, TILE, 3)}
         t = d_tgt[i].cpu().numpy()
         for k, v in img.items():
+            bad = ~np.all(np.isfinite(v), axis=-1)  # bilateral: zero weight sum at isolated outliers
+            v = np.where(np.isfinite(v), v, 0.0)
             res.setdefault(k, []).append(relmse(v, t))
             res.setdefault(k + "_tonemapped", []).append(relmse(gt_tonemap(v * sc), gt_tonemap(t * sc)))
+            res.setdefault(k + "_nonfinite_pixels", []).append(float(bad.sum()))
     return {k: float(np.mean(v)) for k, v in res.items()}
 
 
