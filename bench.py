@@ -475,10 +475,13 @@ def run_diff(args):
     # Decorrelated: residual from one 4-spp render, adjoint with independent seeds; an
     # unbiased estimate of the reference gradient (its average over draws converges to it).
     cos_corr = cos(g4, g_ref)
+    # (pcg4d seeds are (x, y, frame + sample): a call with `spp` samples at frame f uses
+    # frames f .. f+spp-1, so the two renders of a pair start spp frames apart)
     g_dec = []
     for d in range(16):
-        img_a = diff.render(theta, ctx, base, params(W, H, spp, 1000 + 2 * d)).detach()
-        g_dec.append(adjoint(ctx, theta, W, H, spp, 1001 + 2 * d, 2.0 * (img_a - target) / target.numel(), chunk=spp))
+        f = 2_000_000 + 2 * spp * d
+        img_a = diff.render(theta, ctx, base, params(W, H, spp, f)).detach()
+        g_dec.append(adjoint(ctx, theta, W, H, spp, f + spp, 2.0 * (img_a - target) / target.numel(), chunk=spp))
     cos_dec1 = cos(g_dec[0], g_ref)
     cos_dec16 = cos(sum(g_dec) / 16, g_ref)
     # Brute force at 128x128 for every parameter kind: central differences (h = 0.02) of the
