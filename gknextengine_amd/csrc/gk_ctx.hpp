@@ -154,10 +154,13 @@ struct Ctx {
         DevBuf<uint32_t> sort_cnt;            // 16 class counts + 16 cursors
         DevBuf<float4> shadow_o, shadow_d;    // shadow queue
         DevBuf<float4> shadow_c;              // contribution (rgb) + target (w)
+        DevBuf<float4> shadow_hit;            // occlusion results (w = 1 when occluded)
         DevBuf<uint32_t> shadow_path;
         DevBuf<uint32_t> counters;            // [0..1] queue sizes [2] shadow size [3..] sort
         DevBuf<uint32_t> stats;               // ray counters (u64 pairs)
         // path state (SoA)
+        DevBuf<float4> q_thr[2];              // PT_MIS state travelling with queue entries:
+        DevBuf<uint4> q_rng[2], q_misc[2];    //   throughput + pdf, rng, (pixel, flags)
         DevBuf<float4> s_thr;                 // throughput rgb, last bsdf pdf
         DevBuf<float4> s_ld;                  // primary-diffuse-lobe radiance, w = flags
         DevBuf<float4> s_ls;                  // other radiance
@@ -204,6 +207,10 @@ int update_lights(Ctx& c);  // world-space emissive triangles from the instance 
 int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev,
                  uint32_t max_count, float4* hit, uint32_t* hit_inst, bool any_hit,
                  uint32_t* occluded_flags);
+// a closest-hit queue (o, d -> hit, hit_inst) and an occlusion queue (so, sd -> shit) in one launch
+int launch_trace_pair(Ctx& c, const float4* o, const float4* d, const uint32_t* count_dev, uint32_t max_count,
+                      float4* hit, uint32_t* hit_inst, const float4* so, const float4* sd,
+                      const uint32_t* scount_dev, uint32_t smax_count, float4* shit);
 int launch_trace_user(Ctx& c, const gk_ray* rays, uint32_t n, gk_hit* hits, uint32_t* inst, uint32_t flags);
 int ensure_spill(Ctx& c, uint32_t threads);
 int collect_timing(Ctx& c);

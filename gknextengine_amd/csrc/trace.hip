@@ -45,7 +45,24 @@ struct Lane {
     uint32_t head, count, gsp;       // ring head, entries in ring, spilled entries
     uint32_t ray;
     bool active, found, overflow;
+    bool any;                        // occlusion ray: the first hit ends it
+    bool job1;                       // ray of the second job (two-job launches)
 };
+
+// One queue of rays for a trace launch. count_dev (device, may be null) caps max_count.
+struct TraceJob {
+    const float4* o;                 // origin, tmin (w)
+    const float4* d;                 // direction, tmax (w)
+    const uint32_t* count_dev;
+    uint32_t max_count;
+    float4* hit;                     // closest: (t, u, v, prim); any: w = 1 when occluded
+    uint32_t* inst;                  // closest: instance of the hit (may be null)
+};
+
+// MODE 0: one closest-hit job; 1: one any-hit job; 2: a closest-hit job then an any-hit
+// job drained by the same persistent waves (their latency tails overlap).
+template <int MODE>
+__device__ inline bool lane_any(const bool any_flag) { return MODE == 1 ? true : (MODE == 0 ? false : any_flag); }
 
 constexpr uint32_t kNone = 0xfffffffeu;
 
@@ -139,25 +156,33 @@ __device__ inline bool mt(const TriRec& tr, f3 O, f3 D, float& t, float& u, floa
     return true;
 }
 
-template <bool ANY, bool TWO>
+template <int MODE, bool TWO>
 __global__ __launch_bounds__(kBlock) void k_trace(
     const Bvh4Node* __restrict__ blas, const TriRec* __restrict__ tris,
     const Bvh4Node* __restrict__ tlas, const uint32_t* __restrict__ tlas_prims,
-    const InstDev* __restrict__ inst, uint32_t blas_root,
-    const float4* __restrict__ ray_o, const float4* __restrict__ ray_d,
-    const uint32_t* __restrict__ count_dev, uint32_t max_count, uint32_t* __restrict__ next_ray,
-    float4* __restrict__ hit_out, uint32_t* __restrict__ inst_out, uint32_t* __restrict__ spill,
+    const InstDev* __restrict__ inst, uint32_t blas_root, TraceJob j0, TraceJob j1,
+    uint32_t* __restrict__ next_ray, uint32_t* __restrict__ spill,
     uint32_t* __restrict__ overflow_flag, unsigned long long* __restrict__ ray_stats) {
     __shared__ uint32_t ring[kRing][kBlock];
     const uint32_t stride = gridDim.x * kBlock;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-    uint32_t count = max_count;
-    if (count_dev) { const uint32_t c = *count_dev; count = c < max_count ? c : max_count; }
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ray_stats[ANY ? 1 : 0], (unsigned long long)count);
+    uint32_t c0 = j0.max_count, c1 = 0;
+    if (j0.count_dev) { const uint32_t c = *j0.count_dev; c0 = c < c0 ? c : c0; }
+    if (MODE == 2) {
+        c1 = j1.max_count;
+        if (j1.count_dev) { const uint32_t c = *j1.count_dev; c1 = c < c1 ? c : c1; }
+    }
+    const uint32_t count = c0 + c1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&ray_stats[MODE == 1 ? 1 : 0], (unsigned long long)c0);
+        if (MODE == 2) atomicAdd(&ray_stats[1], (unsigned long long)c1);
+    }
 
     Lane L;
     L.active = false;
     L.overflow = false;
+    L.any = MODE == 1;
+    L.job1 = false;
     bool exhausted = false;
     for (;;) {
         // ---- refill idle lanes: one atomic per wave, ranks from the ballot
@@ -173,8 +198,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(
                     const uint32_t rank = __popcll(need & ((1ull << __lane_id()) - 1ull));
                     const uint32_t r = base + rank;
                     if (r < count) {
-                        const float4 o = ray_o[r], d = ray_d[r];
-                        L.ray = r;
+                        const bool second = MODE == 2 && r >= c0;
+                        const uint32_t ri = second ? r - c0 : r;
+                        const float4 o = second ? j1.o[ri] : j0.o[ri];
+                        const float4 d = second ? j1.d[ri] : j0.d[ri];
+                        L.ray = ri;
+                        L.job1 = second;
+                        L.any = MODE == 1 || second;
                         L.O = ld3(o);
                         L.D = ld3(d);
                         L.tmin = o.w;
@@ -227,12 +257,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(
             }
             if (done) {
                 // ---- ray finished
-                if (ANY) {
-                    hit_out[L.ray] = make_float4(0.f, 0.f, 0.f, __uint_as_float(L.found ? 1u : 0u));
+                if (lane_any<MODE>(L.any)) {
+                    float4* out = (MODE == 2 && L.job1) ? j1.hit : j0.hit;
+                    out[L.ray] = make_float4(0.f, 0.f, 0.f, __uint_as_float(L.found ? 1u : 0u));
                 } else {
-                    hit_out[L.ray] = L.found ? make_float4(L.bt, L.bu, L.bv, __uint_as_float(L.bprim))
-                                             : make_float4(__builtin_inff(), 0.f, 0.f, __uint_as_float(0xffffffffu));
-                    if (inst_out) inst_out[L.ray] = L.found ? L.binst : 0xffffffffu;
+                    j0.hit[L.ray] = L.found ? make_float4(L.bt, L.bu, L.bv, __uint_as_float(L.bprim))
+                                            : make_float4(__builtin_inff(), 0.f, 0.f, __uint_as_float(0xffffffffu));
+                    if (j0.inst) j0.inst[L.ray] = L.found ? L.binst : 0xffffffffu;
                 }
                 if (L.overflow) atomicOr(overflow_flag, 1u);
                 L.active = false;
@@ -263,16 +294,16 @@ __global__ __launch_bounds__(kBlock) void k_trace(
                 if (!mt(tr, L.O, L.D, t, u, v)) continue;
                 if (!(t > L.tmin && t < L.tmax)) continue;
                 const uint32_t prim = __float_as_uint(tr.v0.w);
-                if (ANY) { L.found = true; terminate = true; break; }
+                if (lane_any<MODE>(L.any)) { L.found = true; terminate = true; break; }
                 const uint32_t ci = TWO ? L.cur_inst : 0u;
                 if (!L.found || t < L.bt ||
                     (t == L.bt && (ci < L.binst || (ci == L.binst && prim < L.bprim)))) {
                     L.bt = t; L.bu = u; L.bv = v; L.bprim = prim; L.binst = ci; L.found = true;
                 }
             }
-            if (ANY && terminate) break;
+            if (MODE != 0 && terminate) break;
         }
-        if (ANY && terminate) {
+        if (MODE != 0 && terminate) {
             L.count = 0; L.gsp = 0; L.node = kNone;
             continue;
         }
@@ -342,14 +373,9 @@ int ensure_spill(Ctx& c, uint32_t threads) {
     return GK_OK;
 }
 
-// Trace a queue of rays. count_dev (device) caps the number of rays when non-null.
-// hit: float4 (t,u,v,prim) per ray (any-hit: w = 1 when occluded).
-// Trace a queue of rays. count_dev (device) caps the number of rays when non-null.
-// hit: float4 (t,u,v,prim) per ray (any-hit: w = 1 when occluded). Each launch is
+// Launches one persistent trace over up to two jobs (mode as k_trace). Each launch is
 // bracketed by HIP events on the context stream (collected by collect_timing).
-int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev,
-                 uint32_t max_count, float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t*) {
-    if (max_count == 0) return GK_OK;
+static int launch_jobs(Ctx& c, int mode, const TraceJob& j0, const TraceJob& j1) {
     const uint32_t blocks = persistent_blocks(c.device);
     const uint32_t threads = blocks * kBlock;
     int st = ensure_spill(c, threads);
@@ -362,17 +388,35 @@ int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_
     const bool two = !c.blas_only;
     const uint32_t root = c.h_meshes.empty() ? 0u : c.h_meshes[0].root;
     const Bvh4Node* tl = c.tlas_nodes.ptr;
-    auto* kern = any_hit ? (two ? k_trace<true, true> : k_trace<true, false>)
-                         : (two ? k_trace<false, true> : k_trace<false, false>);
+    auto* kern = mode == 0 ? (two ? k_trace<0, true> : k_trace<0, false>)
+               : mode == 1 ? (two ? k_trace<1, true> : k_trace<1, false>)
+                           : (two ? k_trace<2, true> : k_trace<2, false>);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c.tev_used < c.tev.size()) { e0 = c.tev[c.tev_used].first; e1 = c.tev[c.tev_used].second; c.tev_used++; }
     if (e0) GK_HIP(hipEventRecord(e0, c.stream));
-    kern<<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, tl, c.tlas_prims.ptr, c.inst.ptr, root,
-                                         ray_o, ray_d, count_dev, max_count, next, hit, hit_inst, c.wf.spill.ptr,
-                                         ovf, rs);
+    kern<<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, tl, c.tlas_prims.ptr, c.inst.ptr, root, j0,
+                                         j1, next, c.wf.spill.ptr, ovf, rs);
     GK_HIP(hipGetLastError());
     if (e1) GK_HIP(hipEventRecord(e1, c.stream));
     return GK_OK;
+}
+
+// Trace a queue of rays. count_dev (device) caps the number of rays when non-null.
+// hit: float4 (t,u,v,prim) per ray (any-hit: w = 1 when occluded).
+int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev,
+                 uint32_t max_count, float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t*) {
+    if (max_count == 0) return GK_OK;
+    const TraceJob j0{ray_o, ray_d, count_dev, max_count, hit, any_hit ? nullptr : hit_inst};
+    return launch_jobs(c, any_hit ? 1 : 0, j0, TraceJob{});
+}
+
+// A closest-hit queue and an occlusion queue in one launch.
+int launch_trace_pair(Ctx& c, const float4* o, const float4* d, const uint32_t* count_dev, uint32_t max_count,
+                      float4* hit, uint32_t* hit_inst, const float4* so, const float4* sd,
+                      const uint32_t* scount_dev, uint32_t smax_count, float4* shit) {
+    const TraceJob j0{o, d, count_dev, max_count, hit, hit_inst};
+    const TraceJob j1{so, sd, scount_dev, smax_count, shit, nullptr};
+    return launch_jobs(c, 2, j0, j1);
 }
 
 // Sums the elapsed time of the trace launches recorded since the last call (the stream

@@ -78,10 +78,16 @@ struct Queues {
     float4* sc_d;   // shadow contribution to Ld (rgb)
     float4* sc_s;   // shadow contribution to Ls (rgb)
     uint32_t* spath;
+    float4* shit;   // occlusion result per shadow entry (w = 1 when occluded)
     uint32_t* cnt;  // [0],[1] extension queue sizes, [2] shadow queue size, [4..] ray statistics
     uint32_t* sort_keys;    // material class of each queue entry
     uint32_t* sorted;       // queue entries in material order (nullptr: path order)
     uint32_t* sort_cnt;     // 16 class counts, then 16 cursors
+    // PT_MIS path state stored with each queue entry (read where the hit is read, written
+    // where the extension ray is appended), so shading reads and writes it coalesced
+    float4* qthr[2];        // throughput rgb, pdf of the last bsdf sample (w)
+    uint4* qrng[2];
+    uint4* qmisc[2];        // x pixel, y flags
 };
 
 struct PathState {
@@ -284,7 +290,9 @@ __device__ inline void write_miss_gbuffer(const FrameK& K, const Targets& T, uin
 }
 
 // ---------------------------------------------------------------- raygen
-__global__ void k_raygen(FrameK K, Queues Q, PathState S, uint32_t n) {
+// queue_state: PT_MIS keeps its per-path state with the queue entries (Queues::q*),
+// REF_V2 in PathState
+__global__ void k_raygen(FrameK K, Queues Q, PathState S, uint32_t n, uint32_t queue_state) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0) { Q.cnt[0] = n; Q.cnt[1] = 0u; Q.cnt[2] = 0u; }
     if (p >= n) return;
@@ -295,9 +303,15 @@ __global__ void k_raygen(FrameK K, Queues Q, PathState S, uint32_t n) {
     Q.o[0][p] = make_float4(O.x, O.y, O.z, 0.f);
     Q.d[0][p] = make_float4(D.x, D.y, D.z, 1e30f);
     Q.path[0][p] = p;
-    S.rng[p] = make_uint4(x, y, K.frame, K.seed);
-    S.misc[p] = make_uint4(px, 0u, 0u, 0u);
-    S.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    if (queue_state) {
+        Q.qrng[0][p] = make_uint4(x, y, K.frame, K.seed);
+        Q.qmisc[0][p] = make_uint4(px, 0u, 0u, 0u);
+        Q.qthr[0][p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    } else {
+        S.rng[p] = make_uint4(x, y, K.frame, K.seed);
+        S.misc[p] = make_uint4(px, 0u, 0u, 0u);
+        S.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    }
     S.ld[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     S.ls[p] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
@@ -583,29 +597,33 @@ __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T,
     const uint32_t n = Q.cnt[cur];
     const int nxt = cur ^ 1;
     bool emit = false, emit_sh = false;
-    float4 ro{}, rd{}, so{}, sd{}, scd{}, scs{};
+    float4 ro{}, rd{}, so{}, sd{}, scd{}, scs{}, nthr{};
+    uint4 nrng{}, nmisc{};
     uint32_t p = 0;
     if (i0 < n) {
         const uint32_t i = Q.sorted ? Q.sorted[i0] : i0;  // material order
         p = Q.path[cur][i];
-        uint4 misc = S.misc[p];
+        const uint4 misc = Q.qmisc[cur][i];
         const uint32_t px = misc.x;
         uint32_t flags = misc.y;
         const uint32_t depth = flags & 0xffu;
-        const uint4 sr = S.rng[p];
+        const uint4 sr = Q.qrng[cur][i];
         rng4 seed{sr.x, sr.y, sr.z, sr.w};
-        float4 thr4 = S.thr[p];
+        const float4 thr4 = Q.qthr[cur][i];
         f3 thr = ld3(thr4);
         float last_pdf = thr4.w;
         const bool last_delta = depth == 0u || (flags & 1024u);
-        float4 Ld = S.ld[p], Ls = S.ls[p];
+        // radiance found at this vertex; added to the path's sums only when non-zero
+        f3 dLd = f3{0.f, 0.f, 0.f}, dLs = f3{0.f, 0.f, 0.f};
+        bool added = false;
         const float4 h = Q.hit[i];
         const uint32_t inst = Q.hit_inst[i];
         const f3 O = ld3(Q.o[cur][i]), D = ld3(Q.d[cur][i]);
         const bool lobe0_diffuse = (flags & 512u) != 0u;
         auto add = [&](f3 v, bool to_diffuse) {
-            if (to_diffuse) { Ld.x += v.x; Ld.y += v.y; Ld.z += v.z; }
-            else { Ls.x += v.x; Ls.y += v.y; Ls.z += v.z; }
+            if (to_diffuse) dLd = dLd + v;
+            else dLs = dLs + v;
+            added = true;
         };
         bool alive = false;
         if (depth == 0u && ((flags >> 16) & 0xfffu) == 0u) {
@@ -775,11 +793,16 @@ __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T,
                 }
             }
         }
-        S.ld[p] = Ld;
-        S.ls[p] = Ls;
+        if (added) {
+            float4 Ld = S.ld[p], Ls = S.ls[p];
+            Ld.x += dLd.x; Ld.y += dLd.y; Ld.z += dLd.z;
+            Ls.x += dLs.x; Ls.y += dLs.y; Ls.z += dLs.z;
+            S.ld[p] = Ld;
+            S.ls[p] = Ls;
+        }
         if (alive) {
             flags = (flags & ~0xffu) | (depth + 1u);
-            S.thr[p] = make_float4(thr.x, thr.y, thr.z, last_pdf);
+            nthr = make_float4(thr.x, thr.y, thr.z, last_pdf);
             emit = true;
         } else {
             // path finished: restart for the next sample of this frame, if any
@@ -792,15 +815,22 @@ __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T,
                 rd = make_float4(D2.x, D2.y, D2.z, 1e30f);
                 flags = (flags & 0x300u) | (sample << 16) | 1024u;  // keep the primary lobe, depth 0 -> 1 below
                 flags = (flags & ~0xffu);
-                S.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+                nthr = make_float4(1.f, 1.f, 1.f, 0.f);
                 emit = true;
             }
         }
-        S.misc[p] = make_uint4(px, flags, misc.z, misc.w);
-        S.rng[p] = make_uint4(seed.x, seed.y, seed.z, seed.w);
+        nmisc = make_uint4(px, flags, misc.z, misc.w);
+        nrng = make_uint4(seed.x, seed.y, seed.z, seed.w);
     }
     const uint32_t slot = wave_append(&Q.cnt[nxt], emit);
-    if (emit) { Q.o[nxt][slot] = ro; Q.d[nxt][slot] = rd; Q.path[nxt][slot] = p; }
+    if (emit) {
+        Q.o[nxt][slot] = ro;
+        Q.d[nxt][slot] = rd;
+        Q.path[nxt][slot] = p;
+        Q.qthr[nxt][slot] = nthr;
+        Q.qrng[nxt][slot] = nrng;
+        Q.qmisc[nxt][slot] = nmisc;
+    }
     const uint32_t ss = wave_append(&Q.cnt[2], emit_sh);
     if (emit_sh) { Q.so[ss] = so; Q.sd[ss] = sd; Q.sc_d[ss] = scd; Q.sc_s[ss] = scs; Q.spath[ss] = p; }
     count_rays(Q.cnt, 5, emit);
@@ -864,7 +894,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(Queues Q, int cur) {
 __global__ void k_pt_shadow_resolve(Queues Q, PathState S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Q.cnt[2]) return;
-    if (__float_as_uint(Q.hit[i].w) != 0u) return;  // occluded
+    if (__float_as_uint(Q.shit[i].w) != 0u) return;  // occluded
     const uint32_t p = Q.spath[i];
     const float4 a = Q.sc_d[i], b = Q.sc_s[i];
     float4 ld = S.ld[p], ls = S.ls[p];
@@ -878,7 +908,7 @@ __global__ void k_pt_shadow_resolve(Queues Q, PathState S) {
 __global__ void k_pt_finalize(FrameK K, Targets T, PathState S, uint32_t n) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const uint32_t px = S.misc[p].x;
+    const uint32_t px = pixel_of(K, p);
     const float inv = 1.f / float(K.samples > 0u ? K.samples : 1u);
     const float4 ld = S.ld[p], ls = S.ls[p];
     const float4 alb = from_h4(T.albedo[px]);
@@ -919,12 +949,15 @@ int ensure_wavefront(Ctx& c) {
     }
     GK_HIP(w.hit.resize(n)); GK_HIP(w.hit_inst.resize(n));
     GK_HIP(w.shadow_o.resize(n)); GK_HIP(w.shadow_d.resize(n));
-    GK_HIP(w.shadow_c.resize(2 * size_t(n))); GK_HIP(w.shadow_path.resize(n));
+    GK_HIP(w.shadow_c.resize(2 * size_t(n))); GK_HIP(w.shadow_path.resize(n)); GK_HIP(w.shadow_hit.resize(n));
     GK_HIP(w.counters.resize(64));
     GK_HIP(w.sort_keys.resize(n));
     GK_HIP(w.sorted_idx.resize(n));
     GK_HIP(w.sort_cnt.resize(32));
     GK_HIP(hipMemsetAsync(w.sort_cnt.ptr, 0, 32 * 4, c.stream));
+    for (int k = 0; k < 2; ++k) {
+        GK_HIP(w.q_thr[k].resize(n)); GK_HIP(w.q_rng[k].resize(n)); GK_HIP(w.q_misc[k].resize(n));
+    }
     GK_HIP(w.s_thr.resize(n)); GK_HIP(w.s_ld.resize(n)); GK_HIP(w.s_ls.resize(n));
     GK_HIP(w.s_rng.resize(n)); GK_HIP(w.s_misc.resize(n));
     GK_HIP(w.s_pos.resize(n)); GK_HIP(w.s_nrm.resize(n)); GK_HIP(w.s_dir.resize(n));
@@ -995,7 +1028,10 @@ int render_frame(Ctx& c) {
     T.radiance64 = static_cast<double4*>(c.target_ptr[GK_RT_RADIANCE64]);
     T.fp64 = (c.params.flags & GK_FRAME_ACCUM_FP64) ? 1u : 0u;
     Queues Q;
-    for (int k = 0; k < 2; ++k) { Q.o[k] = w.ray_o[k].ptr; Q.d[k] = w.ray_d[k].ptr; Q.path[k] = w.ray_path[k].ptr; }
+    for (int k = 0; k < 2; ++k) {
+        Q.o[k] = w.ray_o[k].ptr; Q.d[k] = w.ray_d[k].ptr; Q.path[k] = w.ray_path[k].ptr;
+        Q.qthr[k] = w.q_thr[k].ptr; Q.qrng[k] = w.q_rng[k].ptr; Q.qmisc[k] = w.q_misc[k].ptr;
+    }
     Q.hit = w.hit.ptr;
     Q.hit_inst = w.hit_inst.ptr;
     Q.so = w.shadow_o.ptr;
@@ -1003,6 +1039,7 @@ int render_frame(Ctx& c) {
     Q.sc_d = w.shadow_c.ptr;
     Q.sc_s = w.shadow_c.ptr + n;
     Q.spath = w.shadow_path.ptr;
+    Q.shit = w.shadow_hit.ptr;
     Q.cnt = w.counters.ptr;
     Q.sort_keys = w.sort_keys.ptr;
     Q.sort_cnt = w.sort_cnt.ptr;
@@ -1015,7 +1052,7 @@ int render_frame(Ctx& c) {
     // previous object ids for reprojection (copy pass, PathTracingRenderer.cpp:160-216)
     k_copy_objid<<<nblk(n), 256, 0, s>>>(T.objid, static_cast<uint32_t*>(c.target_ptr[GK_RT_OBJECTID_1]), n);
     k_zero_stats<<<1, 64, 0, s>>>(Q.cnt);
-    k_raygen<<<nblk(n), 256, 0, s>>>(K, Q, P, n);
+    k_raygen<<<nblk(n), 256, 0, s>>>(K, Q, P, n, c.params.estimator == GK_ESTIMATOR_PT_MIS ? 1u : 0u);
     GK_HIP(hipGetLastError());
     float ms_trace = 0.f, ms_shade = 0.f;
     (void)ms_trace;
@@ -1044,12 +1081,11 @@ int render_frame(Ctx& c) {
                 break;
         }
     } else {
+        // Each iteration shades the hits of queue cur, then one launch traces both the
+        // extension rays it produced (closest hit, for the next iteration) and its shadow
+        // rays (occlusion), whose contributions are resolved before the next shade.
         const uint32_t iters = (K.bounces + 1u) * K.samples;
         for (uint32_t it = 0; it < iters; ++it) {
-            if (it > 0) {
-                st = launch_trace(c, Q.o[cur], Q.d[cur], Q.cnt + cur, n, Q.hit, Q.hit_inst, false, nullptr);
-                if (st != GK_OK) return st;
-            }
             k_reset_counter<<<1, 1, 0, s>>>(Q.cnt + (cur ^ 1));
             k_reset_counter<<<1, 1, 0, s>>>(Q.cnt + 2);
             if (sort) {
@@ -1059,11 +1095,16 @@ int render_frame(Ctx& c) {
             }
             k_pt_shade<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, cur);
             GK_HIP(hipGetLastError());
-            st = launch_trace(c, Q.so, Q.sd, Q.cnt + 2, n, Q.hit, nullptr, true, nullptr);
+            const int nxt = cur ^ 1;
+            if (it + 1 < iters)
+                st = launch_trace_pair(c, Q.o[nxt], Q.d[nxt], Q.cnt + nxt, n, Q.hit, Q.hit_inst, Q.so, Q.sd, Q.cnt + 2,
+                                       n, Q.shit);
+            else
+                st = launch_trace(c, Q.so, Q.sd, Q.cnt + 2, n, Q.shit, nullptr, true, nullptr);
             if (st != GK_OK) return st;
             k_pt_shadow_resolve<<<nblk(n), 256, 0, s>>>(Q, P);
             GK_HIP(hipGetLastError());
-            cur ^= 1;
+            cur = nxt;
         }
         k_pt_finalize<<<nblk(n), 256, 0, s>>>(K, T, P, n);
         GK_HIP(hipGetLastError());
