@@ -656,8 +656,11 @@ __global__ void k_update_lights(LightTri* __restrict__ lights, const float4* __r
 // ---------------------------------------------------------------- BLAS
 int build_blas(Ctx& c, const gk_mesh* meshes, uint32_t num_meshes) {
     hipStream_t st = c.stream;
-    uint32_t total = 0;
-    for (uint32_t m = 0; m < num_meshes; ++m) total += meshes[m].num_triangles;
+    uint64_t total64 = 0;
+    for (uint32_t m = 0; m < num_meshes; ++m) total64 += meshes[m].num_triangles;
+    // traversal stack entries address a leaf's first triangle with 27 bits
+    if (total64 >= (uint64_t(1) << 27)) return GK_ERR_LIMIT;
+    const uint32_t total = uint32_t(total64);
     c.num_tris = total;
     GK_HIP(c.tris.resize(std::max<uint32_t>(total, 1)));
     GK_HIP(c.tri_nrm.resize(3 * size_t(std::max<uint32_t>(total, 1))));

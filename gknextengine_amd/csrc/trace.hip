@@ -8,11 +8,15 @@
 //   * 4-wide BVH, one 128-byte node per visit (gk_math.hpp Bvh4Node), 48-byte triangle
 //     records stored in leaf order.
 //   * Persistent waves with dynamic ray fetch: lanes whose ray has finished are refilled
-//     together -- one ballot, one atomicAdd per wave, ranks from mbcnt -- so a wave keeps
-//     all 64 lanes busy until the queue drains (wave-level compaction).
+//     together -- one ballot, one atomicAdd per wave, ranks from mbcnt -- once kRefill of
+//     the 64 lanes are idle (wave-level compaction).
 //   * Traversal stack: a 16-entry ring per lane in LDS; when a push would overwrite the
 //     oldest entry that entry is spilled to a per-thread global array (coalesced
 //     [entry][thread] layout) and read back when the ring runs empty.
+//   * One step tests either one node (4 boxes) or one leaf (<= 4 triangles): BLAS leaf
+//     children become stack entries (0x40000000 | count << 27 | first triangle) ordered
+//     near to far with the internal children, which bounds the divergence of a step and
+//     lets near hits cull far subtrees (measured on the 1080p city: trace 4.46 -> 3.94 ms).
 //   * Two-level scenes: TLAS leaves push instance entries; entering an instance pushes a
 //     return mark and continues in the instance's BLAS with the ray transformed by the
 //     same inverse-matrix arithmetic tinybvh uses (tiny_bvh.h:396-409,2309-2316).
@@ -32,6 +36,7 @@ constexpr int kBlock = 256;
 constexpr int kRing = 16;            // LDS stack entries per lane
 constexpr int kSpill = 64;           // global spill entries per thread
 constexpr int kBlocksPerCU = 4;
+constexpr uint32_t kRefill = 48;     // idle lanes that trigger a wave refill (<= 64)
 
 struct Lane {
     f3 O, D, rD;
@@ -185,11 +190,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(
     L.job1 = false;
     bool exhausted = false;
     for (;;) {
-        // ---- refill idle lanes: one atomic per wave, ranks from the ballot
+        // ---- refill idle lanes: one atomic per wave, ranks from the ballot. A refill
+        // stalls the whole wave on the ray loads, so it waits until kRefill lanes are idle
+        // (measured on the 1080p city: 16 -> 9.3 ms/frame, 48 -> 8.4 ms, 64 -> 8.5 ms).
         if (!exhausted) {
             const uint64_t need = __ballot(!L.active);
             const uint32_t n_need = __popcll(need);
-            if (n_need >= 16 || n_need == 64) {
+            if (n_need >= kRefill) {
                 uint32_t base = 0;
                 if (__lane_id() == __ffsll((unsigned long long)need) - 1) base = atomicAdd(next_ray, n_need);
                 base = __shfl(base, __ffsll((unsigned long long)need) - 1);
@@ -272,48 +279,48 @@ __global__ __launch_bounds__(kBlock) void k_trace(
         }
 
         const bool tl_level = TWO && L.level == 0;
-        const Bvh4Node& N = tl_level ? tlas[L.node] : blas[L.node];
-        const Bvh4Node n = N;
-        float tl[4];
-        slab4(n, L, L.bt, tl);
-        const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
-        const uint32_t mt4[4] = {n.meta.x, n.meta.y, n.meta.z, n.meta.w};
-        bool terminate = false;
-        // leaves first
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (!(tl[i] < __builtin_inff()) || mt4[i] == 0u) continue;
-            if (tl_level) {
-                for (uint32_t k = 0; k < mt4[i]; ++k)
-                    push(L, ring, spill, gtid, stride, kInstFlag | tlas_prims[ch[i] + k]);
-                continue;
-            }
-            for (uint32_t k = 0; k < mt4[i]; ++k) {
-                const TriRec tr = tris[ch[i] + k];
+        if (!tl_level && (L.node & 0xc0000000u) == 0x40000000u) {  // a postponed BLAS leaf
+            const uint32_t first = L.node & 0x07ffffffu, cnt = (L.node >> 27) & 7u;
+            bool term = false;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const TriRec tr = tris[first + k];
                 float t, u, v;
                 if (!mt(tr, L.O, L.D, t, u, v)) continue;
                 if (!(t > L.tmin && t < L.tmax)) continue;
                 const uint32_t prim = __float_as_uint(tr.v0.w);
-                if (lane_any<MODE>(L.any)) { L.found = true; terminate = true; break; }
+                if (lane_any<MODE>(L.any)) { L.found = true; term = true; break; }
                 const uint32_t ci = TWO ? L.cur_inst : 0u;
                 if (!L.found || t < L.bt ||
                     (t == L.bt && (ci < L.binst || (ci == L.binst && prim < L.bprim)))) {
                     L.bt = t; L.bu = u; L.bv = v; L.bprim = prim; L.binst = ci; L.found = true;
                 }
             }
-            if (MODE != 0 && terminate) break;
-        }
-        if (MODE != 0 && terminate) {
-            L.count = 0; L.gsp = 0; L.node = kNone;
+            if (MODE != 0 && term) { L.count = 0; L.gsp = 0; }
+            L.node = kNone;
             continue;
         }
-        // internal children, near to far
+        const Bvh4Node& N = tl_level ? tlas[L.node] : blas[L.node];
+        const Bvh4Node n = N;
+        float tl[4];
+        slab4(n, L, L.bt, tl);
+        const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+        const uint32_t mt4[4] = {n.meta.x, n.meta.y, n.meta.z, n.meta.w};
+        // TLAS leaves push their instances; BLAS leaves join the internal children as
+        // entries (flag 0x40000000, count << 27, first triangle), all near to far
         float kt[4];
         uint32_t kc[4];
         int nk = 0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            if (mt4[i] == 0u && tl[i] <= L.bt) { kt[nk] = tl[i]; kc[nk] = ch[i]; nk++; }
+            if (!(tl[i] <= L.bt)) continue;
+            if (mt4[i] != 0u && tl_level) {
+                for (uint32_t k = 0; k < mt4[i]; ++k)
+                    push(L, ring, spill, gtid, stride, kInstFlag | tlas_prims[ch[i] + k]);
+                continue;
+            }
+            kt[nk] = tl[i];
+            kc[nk] = mt4[i] == 0u ? ch[i] : (0x40000000u | (mt4[i] << 27) | ch[i]);
+            nk++;
         }
         // insertion sort (<= 4 entries)
         for (int i = 1; i < nk; ++i) {

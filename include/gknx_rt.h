@@ -283,7 +283,8 @@ int gk_resize(gk_ctx* ctx, uint32_t width, uint32_t height);
 
 /* Replaces Scene::Reload + RebuildMeshBuffer + OnPostLoadScene's
  * CreateAccelerationStructures (Scene.cpp:92-289; RayTraceBaseRenderer.cpp:106-124,
- * 276-386): uploads geometry, builds one BLAS per mesh on the device and the TLAS. */
+ * 276-386): uploads geometry, builds one BLAS per mesh on the device and the TLAS.
+ * GK_ERR_LIMIT when the meshes hold 2^27 triangles or more in total. */
 int gk_scene_upload(gk_ctx* ctx,
                     const gk_mesh* meshes, uint32_t num_meshes,
                     const gk_material* materials, uint32_t num_materials,
