@@ -35,7 +35,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kRing = 16;            // LDS stack entries per lane
 constexpr int kSpill = 64;           // global spill entries per thread
-constexpr int kBlocksPerCU = 4;
+constexpr int kMaxBlocksPerCU = 8;   // spill-area capacity; launches use the kernel's occupancy
 constexpr uint32_t kRefill = 48;     // idle lanes that trigger a wave refill (<= 64)
 
 struct Lane {
@@ -351,14 +351,24 @@ __global__ void k_unpack_rays(const gk_ray* __restrict__ r, uint32_t n, float4* 
     d[i] = make_float4(x.dir[0], x.dir[1], x.dir[2], x.tmax);
 }
 
-uint32_t persistent_blocks(int device) {
+uint32_t cu_count(int device) {
     static int cus = 0;
     if (!cus) {
         hipDeviceProp_t p;
         if (hipGetDeviceProperties(&p, device) == hipSuccess) cus = p.multiProcessorCount;
         if (cus <= 0) cus = 256;
     }
-    return uint32_t(cus) * kBlocksPerCU;
+    return uint32_t(cus);
+}
+
+// Persistent grid: as many blocks as the kernel keeps resident per CU (its VGPR and LDS
+// use decide: 5 for the single-level kernels at 92 VGPRs, 4 for the two-level ones).
+template <class K>
+uint32_t persistent_blocks(int device, K kern) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, 0) != hipSuccess || nb < 1) nb = 4;
+    if (nb > kMaxBlocksPerCU) nb = kMaxBlocksPerCU;
+    return cu_count(device) * uint32_t(nb);
 }
 
 }  // namespace
@@ -383,11 +393,11 @@ int ensure_spill(Ctx& c, uint32_t threads) {
 // Launches one persistent trace over up to two jobs (mode as k_trace). Each launch is
 // bracketed by HIP events on the context stream (collected by collect_timing).
 static int launch_jobs(Ctx& c, int mode, const TraceJob& j0, const TraceJob& j1) {
-    const uint32_t blocks = persistent_blocks(c.device);
-    const uint32_t threads = blocks * kBlock;
-    int st = ensure_spill(c, threads);
+    // the spill area is laid out for the largest grid, so its tail (counters) never moves
+    const uint32_t cap_threads = cu_count(c.device) * kMaxBlocksPerCU * kBlock;
+    int st = ensure_spill(c, cap_threads);
     if (st != GK_OK) return st;
-    uint32_t* tail = c.wf.spill.ptr + size_t(threads) * kSpill;  // [next ray, overflow, stats u64 x2]
+    uint32_t* tail = c.wf.spill.ptr + size_t(cap_threads) * kSpill;  // [next ray, overflow, stats u64 x2]
     uint32_t* next = tail;
     uint32_t* ovf = tail + 1;
     unsigned long long* rs = reinterpret_cast<unsigned long long*>(tail + 2);
@@ -398,6 +408,7 @@ static int launch_jobs(Ctx& c, int mode, const TraceJob& j0, const TraceJob& j1)
     auto* kern = mode == 0 ? (two ? k_trace<0, true> : k_trace<0, false>)
                : mode == 1 ? (two ? k_trace<1, true> : k_trace<1, false>)
                            : (two ? k_trace<2, true> : k_trace<2, false>);
+    const uint32_t blocks = persistent_blocks(c.device, kern);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c.tev_used < c.tev.size()) { e0 = c.tev[c.tev_used].first; e1 = c.tev[c.tev_used].second; c.tev_used++; }
     if (e0) GK_HIP(hipEventRecord(e0, c.stream));
