@@ -157,7 +157,6 @@ struct Ctx {
         DevBuf<float4> shadow_hit;            // occlusion results (w = 1 when occluded)
         DevBuf<uint32_t> shadow_path;
         DevBuf<uint32_t> counters;            // [0..1] queue sizes [2] shadow size [3..] sort
-        DevBuf<uint32_t> stats;               // ray counters (u64 pairs)
         // path state (SoA)
         DevBuf<float4> q_thr[2];              // PT_MIS state travelling with queue entries:
         DevBuf<uint4> q_rng[2], q_misc[2];    //   throughput + pdf, rng, (pixel, flags)
