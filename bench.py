@@ -215,7 +215,8 @@ def run_city(args):
                    "blas_nodes": bvh.blas_nodes, **extra},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "kernel": "k_trace (closest + any hit)", "bytes_per_ray": BYTES_PER_RAY},
+                     "kernel": "k_trace (primary rays; per-bounce extension + shadow rays)",
+                     "bytes_per_ray": BYTES_PER_RAY},
         "cpu_baseline": cpu,
     }
 
