@@ -141,6 +141,15 @@ def run_city(args):
     log("building city scene")
     scene = gk.procgen_city(64, 4, 256, 64, 2024)
     ntris = scene.meshes[0]["indices"].shape[0]
+    if args.mixed_materials:
+        # same geometry, materials spread over four models (dielectric shading differs)
+        import numpy as np
+        rng = np.random.default_rng(5)
+        models = [gk.MAT_LAMBERTIAN, gk.MAT_METALLIC, gk.MAT_DIELECTRIC, gk.MAT_MIXTURE]
+        for m in scene.materials:
+            if m.material_model != gk.MAT_DIFFUSE_LIGHT:
+                m.material_model = int(models[rng.integers(4)])
+    models_used = {m.material_model for m in scene.materials if m.material_model != gk.MAT_DIFFUSE_LIGHT}
     ctx = gk.Context(W, H)
     ctx.upload(scene)
     bvh = ctx.bvh_stats()
@@ -151,8 +160,11 @@ def run_city(args):
     p.samples = 1
     p.global_seed = 1
     p.denoiser = gk.DENOISER_BILATERAL if args.denoiser == "bilateral" else gk.DENOISER_NONE
-    if args.no_sort:
+    if args.sort == "off":
         p.flags |= gk.FRAME_NO_MATERIAL_SORT
+    elif args.sort == "on":
+        p.flags |= gk.FRAME_MATERIAL_SORT
+    sorted_shading = args.sort == "on"
     frame = 0
 
     def step():
@@ -208,7 +220,8 @@ def run_city(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "procedural city (config 2), PT_MIS 1 spp 8 bounces + reproject + bilateral",
                    "resolution": [W, H], "triangles": int(ntris), "bounces": args.bounces, "spp": 1,
-                   "estimator": "PT_MIS", "material_sort": not args.no_sort,
+                   "estimator": "PT_MIS", "material_sort": sorted_shading,
+                   "material_models": len(models_used),
                    "rays_per_frame": round(rays / args.steps), "trace_ms_per_frame": round(ms_trace / args.steps, 3),
                    "reproject_ms": round(s1.ms_reproject, 3), "denoise_ms": round(s1.ms_denoise, 3),
                    "blas_build_ms": round(bvh.build_ms, 2), "bvh_sah_cost": round(bvh.sah_cost, 2),
@@ -579,7 +592,9 @@ def main():
     ap.add_argument("--config", default="city", choices=["city", "dynamic", "cornell", "denoise", "diff"])
     ap.add_argument("--relmse", type=int, default=0, help="reference spp for relMSE (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-sort", action="store_true", help="shade in path order (no material sort)")
+    ap.add_argument("--sort", default="off", choices=["on", "off"], help="material-ordered shading")
+    ap.add_argument("--mixed-materials", action="store_true",
+                    help="city materials spread over Lambertian/Metallic/Dielectric/Mixture")
     ap.add_argument("--denoiser", default="bilateral", choices=["none", "bilateral"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

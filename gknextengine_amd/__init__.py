@@ -23,7 +23,7 @@ GK_ERR_ARG, GK_ERR_HIP, GK_ERR_OOM, GK_ERR_STATE, GK_ERR_LIMIT, GK_ERR_NO_DEVICE
 MAT_LAMBERTIAN, MAT_METALLIC, MAT_DIELECTRIC, MAT_ISOTROPIC, MAT_DIFFUSE_LIGHT, MAT_MIXTURE = range(6)
 ESTIMATOR_REF_V2, ESTIMATOR_PT_MIS = 0, 1
 DENOISER_NONE, DENOISER_BILATERAL, DENOISER_UNET = 0, 1, 2
-FRAME_NO_MATERIAL_SORT, FRAME_SKIP_POST, FRAME_ACCUM_FP64, FRAME_LINEAR_OUTPUT = 1, 2, 4, 8
+FRAME_NO_MATERIAL_SORT, FRAME_SKIP_POST, FRAME_ACCUM_FP64, FRAME_LINEAR_OUTPUT, FRAME_MATERIAL_SORT = 1, 2, 4, 8, 16
 TRACE_CLOSEST, TRACE_ANY = 0, 1
 
 TARGETS = [

@@ -1058,7 +1058,11 @@ int render_frame(Ctx& c) {
     Q.cnt = w.counters.ptr;
     Q.sort_keys = w.sort_keys.ptr;
     Q.sort_cnt = w.sort_cnt.ptr;
-    const bool sort = !(c.params.flags & GK_FRAME_NO_MATERIAL_SORT);
+    // Material-ordered shading is opt-in: PT_MIS shades every non-dielectric model with one
+    // BSDF code path, so the sort (two passes over the queue, 0.57 ms per 1080p city frame)
+    // costs more than the divergence it removes, also with four material models in the
+    // scene (1080p city: 5.50 ms per frame sorted, 4.92 ms unsorted)
+    const bool sort = (c.params.flags & GK_FRAME_MATERIAL_SORT) && !(c.params.flags & GK_FRAME_NO_MATERIAL_SORT);
     Q.sorted = sort ? w.sorted_idx.ptr : nullptr;
     PathState P;
     P.thr = w.s_thr.ptr; P.ld = w.s_ld.ptr; P.ls = w.s_ls.ptr; P.rng = w.s_rng.ptr; P.misc = w.s_misc.ptr;

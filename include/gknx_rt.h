@@ -202,7 +202,9 @@ typedef struct gk_frame_params {
 } gk_frame_params;
 
 /* gk_frame_params.flags */
-#define GK_FRAME_NO_MATERIAL_SORT 1u   /* shade in path order instead of material order */
+#define GK_FRAME_NO_MATERIAL_SORT 1u   /* shade in path order (the default; overrides 16u) */
+#define GK_FRAME_MATERIAL_SORT 16u     /* sort each hit queue by material class (model x
+                                          metal, misses last) before shading               */
 #define GK_FRAME_SKIP_POST 2u          /* stop after the tracer (no reproject/denoise)  */
 #define GK_FRAME_ACCUM_FP64 4u         /* also add the frame's radiance to GK_RT_RADIANCE64 */
 #define GK_FRAME_LINEAR_OUTPUT 8u      /* GK_RT_DENOISED holds linear radiance, not the tonemapped image */
