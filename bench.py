@@ -203,15 +203,17 @@ def run_city(args):
         ctx.reset()
         p.total_frames = 7
         p.temporal_frames = 1
+        p.flags |= gk.FRAME_LINEAR_OUTPUT
         ctx.frame(p)
         ctx.render()
         raw = ctx.read("RADIANCE")[..., :3]
         den = ctx.read("DENOISED")[..., :3].astype(np.float64)
         ref = reference_frame(ctx, p, args.relmse)
         sc = p.paper_white_nit / 40000.0
-        extra = {"relmse_1spp": relmse(raw, ref), "reference_spp": args.relmse,
-                 "relmse_tonemapped_1spp": relmse(gt_tonemap(raw * sc), gt_tonemap(ref * sc)),
-                 "relmse_tonemapped_bilateral": relmse(den, gt_tonemap(ref * sc))}
+        b, b_finite, nbad = relmse_stats(den, ref)
+        extra = {"relmse_1spp": relmse(raw, ref), "relmse_bilateral": b, "relmse_bilateral_finite_only": b_finite,
+                 "nonfinite_pixels_bilateral": nbad, "reference_spp": args.relmse,
+                 "relmse_tonemapped_1spp": relmse(gt_tonemap(raw * sc), gt_tonemap(ref * sc))}
     cpu = None if args.no_cpu_baseline else cpu_baseline(scene, p, W, H)
     ctx.close()
     return {
@@ -282,8 +284,9 @@ def run_dynamic(args):
         den = ctx.read("DENOISED")[..., :3].astype(np.float64)
         ref = reference_frame(ctx, p, args.relmse)
         sc = p.paper_white_nit / 40000.0
-        extra = {"relmse_tonemapped_bilateral_frame120": relmse(den, gt_tonemap(ref * sc)),
-                 "reference_spp": args.relmse}
+        b, b_finite, nbad = relmse_stats(den, gt_tonemap(ref * sc))
+        extra = {"relmse_tonemapped_bilateral_frame120": b, "relmse_tonemapped_bilateral_frame120_finite_only": b_finite,
+                 "nonfinite_pixels_bilateral": nbad, "reference_spp": args.relmse}
     ctx.close()
     return {"metric": METRIC, "value": round(rays / (ms.sum() * 1e-3) / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
             "steps": len(ms), "warmup": 9, "ms_per_step": round(float(np.median(ms)), 3), "higher_is_better": True,

@@ -24,6 +24,7 @@ MAT_LAMBERTIAN, MAT_METALLIC, MAT_DIELECTRIC, MAT_ISOTROPIC, MAT_DIFFUSE_LIGHT, 
 ESTIMATOR_REF_V2, ESTIMATOR_PT_MIS = 0, 1
 DENOISER_NONE, DENOISER_BILATERAL, DENOISER_UNET = 0, 1, 2
 FRAME_NO_MATERIAL_SORT, FRAME_SKIP_POST, FRAME_ACCUM_FP64, FRAME_LINEAR_OUTPUT, FRAME_MATERIAL_SORT = 1, 2, 4, 8, 16
+FRAME_PRIMARY_FROM_VISBUFFER = 32
 TRACE_CLOSEST, TRACE_ANY = 0, 1
 
 TARGETS = [
@@ -130,6 +131,7 @@ SIGNATURES = [
     ("gk_sync", C.c_int, [_P]),
     ("gk_reset_accumulation", C.c_int, [_P]),
     ("gk_read_target", C.c_int, [_P, C.c_int, _P, C.c_size_t]),
+    ("gk_write_target", C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     ("gk_target_info", C.c_int, [_P, C.c_int, C.POINTER(C.c_size_t), C.POINTER(_P)]),
     ("gk_get_stats", C.c_int, [_P, C.POINTER(Stats)]),
     ("gk_trace", C.c_int, [_P, _P, _U, _P, _P, _U]),
@@ -391,6 +393,11 @@ class Context:
         out = np.zeros(shape, dt)
         check(lib().gk_read_target(self.h, RT[name], ptr(out), out.nbytes), f"gk_read_target({name})")
         return out
+
+    def write(self, name: str, data: np.ndarray):
+        dt, ch = TARGET_FORMAT[name]
+        a = np.ascontiguousarray(data, dt)
+        check(lib().gk_write_target(self.h, RT[name], ptr(a), a.nbytes), f"gk_write_target({name})")
 
     def target_ptr(self, name: str) -> int:
         n, p = C.c_size_t(0), C.c_void_p()

@@ -370,6 +370,10 @@ int gk_frame(gk_ctx* ctx, const gk_frame_params* p) {
     for (int k = 0; k < 16; ++k)
         if (!std::isfinite(p->model_view_inverse[k]) || !std::isfinite(p->projection_inverse[k])) return GK_ERR_ARG;
     Ctx& c = ctx->c;
+    // the visibility-buffer packing holds 15 bits of instance+1 and 17 bits of triangle
+    if ((p->flags & GK_FRAME_PRIMARY_FROM_VISBUFFER) &&
+        (c.h_instances.size() >= 0x7fffu || c.max_mesh_tris > 0x20000u))
+        return GK_ERR_LIMIT;
     c.prev_params = c.params;
     c.params = *p;
     c.frame_ready = true;
@@ -414,6 +418,16 @@ int gk_read_target(gk_ctx* ctx, int target, void* host, size_t bytes) {
     if (target < 0 || target >= GK_RT_COUNT || !host) return GK_ERR_ARG;
     if (bytes != c.target_bytes[target]) return GK_ERR_ARG;
     GK_HIP(hipMemcpyAsync(host, c.target_ptr[target], bytes, hipMemcpyDeviceToHost, c.stream));
+    GK_HIP(hipStreamSynchronize(c.stream));
+    return GK_OK;
+}
+
+int gk_write_target(gk_ctx* ctx, int target, const void* host, size_t bytes) {
+    GK_CHECK_CTX(ctx);
+    Ctx& c = ctx->c;
+    if (target < 0 || target >= GK_RT_COUNT || !host) return GK_ERR_ARG;
+    if (bytes != c.target_bytes[target]) return GK_ERR_ARG;
+    GK_HIP(hipMemcpyAsync(c.target_ptr[target], host, bytes, hipMemcpyHostToDevice, c.stream));
     GK_HIP(hipStreamSynchronize(c.stream));
     return GK_OK;
 }

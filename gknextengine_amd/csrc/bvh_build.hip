@@ -375,10 +375,11 @@ __global__ void k_set_root(uint2* q, uint32_t* counters, uint32_t root_elem) {
 __global__ void k_zero_word(uint32_t* p) { *p = 0u; }
 
 __global__ void k_reorder_tris(const TriRec* __restrict__ tmp, const uint32_t* __restrict__ vals, uint32_t n,
-                               TriRec* __restrict__ out) {
+                               TriRec* __restrict__ out, uint32_t* __restrict__ rec_of) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     out[k] = tmp[vals[k]];
+    rec_of[vals[k]] = k;
 }
 
 __global__ void k_copy_u32(const uint32_t* __restrict__ a, uint32_t* __restrict__ b, uint32_t n) {
@@ -665,6 +666,9 @@ int build_blas(Ctx& c, const gk_mesh* meshes, uint32_t num_meshes) {
     GK_HIP(c.tris.resize(std::max<uint32_t>(total, 1)));
     GK_HIP(c.tri_nrm.resize(3 * size_t(std::max<uint32_t>(total, 1))));
     GK_HIP(c.tri_mat.resize(std::max<uint32_t>(total, 1)));
+    GK_HIP(c.tri_rec.resize(std::max<uint32_t>(total, 1)));
+    c.max_mesh_tris = 0;
+    for (uint32_t m = 0; m < num_meshes; ++m) c.max_mesh_tris = std::max(c.max_mesh_tris, meshes[m].num_triangles);
     size_t node_cap = 0;
     for (uint32_t m = 0; m < num_meshes; ++m) node_cap += meshes[m].num_triangles + 1;
     GK_HIP(c.blas_nodes.resize(std::max<size_t>(node_cap, 1)));
@@ -722,7 +726,7 @@ int build_blas(Ctx& c, const gk_mesh* meshes, uint32_t num_meshes) {
         int s = build_wide(c, c.scratch.prim_lo.ptr, c.scratch.prim_hi.ptr, nt, 4, c.blas_nodes.ptr, node_base,
                            tri_base, nullptr, r);
         if (s != GK_OK) return s;
-        k_reorder_tris<<<nblk(nt), 256, 0, st>>>(tmp.ptr, r.vals, nt, c.tris.ptr + tri_base);
+        k_reorder_tris<<<nblk(nt), 256, 0, st>>>(tmp.ptr, r.vals, nt, c.tris.ptr + tri_base, c.tri_rec.ptr + tri_base);
         GK_HIP(hipGetLastError());
         // mesh bounds = union of the root's children
         Bvh4Node root;

@@ -112,6 +112,8 @@ struct Ctx {
     DevBuf<TriRec> tris;            // all meshes, leaf order (absolute record index)
     DevBuf<float4> tri_nrm;         // 3 per triangle, original order: n0, n1, n2 (object space)
     DevBuf<uint32_t> tri_mat;       // local material slot per triangle, original order
+    DevBuf<uint32_t> tri_rec;       // original order -> leaf-order record index (within the mesh)
+    uint32_t max_mesh_tris = 0;     // largest mesh (visibility-buffer packing limit)
     DevBuf<Bvh4Node> blas_nodes;
     DevBuf<MeshInfo> meshes;
     DevBuf<InstDev> inst;

@@ -48,6 +48,7 @@ struct FrameK {
     float sun_dir[4], sun_col[4], sky[4];
     float exit_prob, hit_off;
     uint32_t W, H, frame, seed, samples, bounces, max_bounces, temporal, has_sky, has_sun, tiled;
+    uint32_t vis_primary;  // primary hits come from GK_RT_MINIGBUFFER (left unwritten)
 };
 
 struct Targets {
@@ -299,7 +300,7 @@ __device__ inline void write_gbuffer(const DScene& s, const FrameK& K, const Tar
     T.albedo[px] = to_h4(m.diffuse[0], m.diffuse[1], m.diffuse[2], m.diffuse[3]);
     T.normal[px] = to_h4(nrm.x, nrm.y, nrm.z, m.fuzziness);
     T.objid[px] = I.instance_id;
-    T.minigbuffer[px] = ((inst + 1u) << 17) | (prim & 0x1ffffu);
+    if (!K.vis_primary) T.minigbuffer[px] = ((inst + 1u) << 17) | (prim & 0x1ffffu);
     T.phit[px] = hit;
     T.pinst[px] = inst;
 }
@@ -310,7 +311,7 @@ __device__ inline void write_miss_gbuffer(const FrameK& K, const Targets& T, uin
     T.normal[px] = to_h4(0.f, 1.f, 0.f, 1.f);
     T.objid[px] = 65535u;
     T.depth[px] = 1e5f;
-    T.minigbuffer[px] = 0u;
+    if (!K.vis_primary) T.minigbuffer[px] = 0u;
     T.phit[px] = hit;
     T.pinst[px] = kMiss;
 }
@@ -340,6 +341,56 @@ __global__ void k_raygen(FrameK K, Queues Q, PathState S, uint32_t n, uint32_t q
     }
     S.ld[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     S.ls[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Primary hits from a visibility buffer (GK_FRAME_PRIMARY_FROM_VISBUFFER): the camera ray of
+// path p hits the triangle GK_RT_MINIGBUFFER names for its pixel ((inst+1) << 17 | tri, 0 = no
+// geometry; Rast.VisibilityPass.vert.slang:18-22), as in FVisibilityBufferRayCasterV2::
+// TracePrimaryRay (Shading.slang:336-370). The barycentrics are get_material_dataV2's
+// Moeller-Trumbore form without rejection (GeneralFunc.slang:31-45); t follows the same
+// form. They are evaluated in the triangle's object space with the tracer's arithmetic
+// (the ray moved by the instance inverse as tinybvh does), so a buffer written by a traced
+// frame of this core reproduces that frame's primary hits bit for bit.
+__global__ void k_vis_primary(DScene s, FrameK K, Queues Q, const uint32_t* __restrict__ vis,
+                              const uint32_t* __restrict__ tri_rec, const MeshInfo* __restrict__ meshes,
+                              uint32_t ninst, uint32_t n) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t px = pixel_of(K, p);
+    const uint32_t pk = vis[px];
+    const uint32_t ip1 = (pk >> 17) & 0x7fffu, tri = pk & 0x1ffffu;
+    float4 hit = make_float4(__builtin_inff(), 0.f, 0.f, __uint_as_float(kMiss));
+    uint32_t hinst = kMiss;
+    if (ip1 != 0u && ip1 - 1u < ninst) {
+        const uint32_t inst = ip1 - 1u;
+        const InstDev& I = s.inst[inst];
+        if (tri < meshes[I.mesh].tri_count) {
+            f3 O = ld3(Q.o[0][p]), D = ld3(Q.d[0][p]);
+            if (s.two_level) {
+                const float* M = I.inv;
+                const f3 r = f3{M[0] * O.x + M[1] * O.y + M[2] * O.z + M[3], M[4] * O.x + M[5] * O.y + M[6] * O.z + M[7],
+                                M[8] * O.x + M[9] * O.y + M[10] * O.z + M[11]};
+                const float w = M[12] * O.x + M[13] * O.y + M[14] * O.z + M[15];
+                const f3 d = f3{M[0] * D.x + M[1] * D.y + M[2] * D.z, M[4] * D.x + M[5] * D.y + M[6] * D.z,
+                                M[8] * D.x + M[9] * D.y + M[10] * D.z};
+                O = w == 1.0f ? r : r * (1.f / w);
+                D = d;
+            }
+            const TriRec tr = s.tris[I.tri_base + tri_rec[I.tri_base + tri]];
+            const f3 v0 = ld3(tr.v0), e1 = ld3(tr.e1), e2 = ld3(tr.e2);
+            const f3 h = cross(D, e2);
+            const float f = 1 / dot(e1, h);
+            const f3 sv = O - v0;
+            const float u = f * dot(sv, h);
+            const f3 q = cross(sv, e1);
+            const float v = f * dot(D, q);
+            const float t = f * dot(e2, q);
+            hit = make_float4(t, u, v, __uint_as_float(__float_as_uint(tr.v0.w)));
+            hinst = s.two_level ? inst : 0u;
+        }
+    }
+    Q.hit[p] = hit;
+    Q.hit_inst[p] = hinst;
 }
 
 __global__ void k_reset_counter(uint32_t* p) { *p = 0u; }
@@ -1007,6 +1058,7 @@ static FrameK make_k(const Ctx& c) {
     K.has_sky = P.has_sky;
     K.has_sun = P.has_sun;
     K.tiled = (c.width % 8 == 0 && c.height % 8 == 0) ? 1u : 0u;
+    K.vis_primary = (P.flags & GK_FRAME_PRIMARY_FROM_VISBUFFER) ? 1u : 0u;
     return K;
 }
 
@@ -1076,9 +1128,15 @@ int render_frame(Ctx& c) {
     (void)ms_trace;
     (void)ms_shade;
     const bool ref = c.params.estimator == GK_ESTIMATOR_REF_V2;
-    // primary rays
-    st = launch_trace(c, Q.o[0], Q.d[0], Q.cnt + 0, n, Q.hit, Q.hit_inst, false, nullptr);
-    if (st != GK_OK) return st;
+    // primary rays: traced, or their hits taken from the caller's visibility buffer
+    if (K.vis_primary) {
+        k_vis_primary<<<nblk(n), 256, 0, s>>>(S0, K, Q, T.minigbuffer, c.tri_rec.ptr, c.meshes.ptr,
+                                               uint32_t(c.h_instances.size()), n);
+        GK_HIP(hipGetLastError());
+    } else {
+        st = launch_trace(c, Q.o[0], Q.d[0], Q.cnt + 0, n, Q.hit, Q.hit_inst, false, nullptr);
+        if (st != GK_OK) return st;
+    }
     int cur = 0;
     if (ref) {
         k_ref_primary<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P);

@@ -205,6 +205,14 @@ typedef struct gk_frame_params {
 #define GK_FRAME_NO_MATERIAL_SORT 1u   /* shade in path order (the default; overrides 16u) */
 #define GK_FRAME_MATERIAL_SORT 16u     /* sort each hit queue by material class (model x
                                           metal, misses last) before shading               */
+/* Primary hits from the caller's visibility buffer instead of traced camera rays: before
+ * gk_render the caller writes GK_RT_MINIGBUFFER (device pointer from gk_target_info) with
+ * (instance+1) << 17 | triangle per pixel, 0 where no geometry covers it, as the
+ * reference's raster visibility pass does (Rast.VisibilityPass.vert.slang:18-22,
+ * FVisibilityBufferRayCasterV2::TracePrimaryRay, Shading.slang:336-370); the frame then
+ * leaves that target unwritten. GK_ERR_LIMIT from gk_frame when the scene has 0x7fff
+ * instances or more, or a mesh of more than 2^17 triangles (the packing's limits). */
+#define GK_FRAME_PRIMARY_FROM_VISBUFFER 32u
 #define GK_FRAME_SKIP_POST 2u          /* stop after the tracer (no reproject/denoise)  */
 #define GK_FRAME_ACCUM_FP64 4u         /* also add the frame's radiance to GK_RT_RADIANCE64 */
 #define GK_FRAME_LINEAR_OUTPUT 8u      /* GK_RT_DENOISED holds linear radiance, not the tonemapped image */
@@ -327,6 +335,9 @@ int gk_reset_accumulation(gk_ctx* ctx);
 
 /* Copies a render target to host memory. bytes must equal gk_target_bytes(). */
 int gk_read_target(gk_ctx* ctx, int target, void* host, size_t bytes);
+/* Copies host memory into a render target (same size rule); for inputs such as the
+ * visibility buffer of GK_FRAME_PRIMARY_FROM_VISBUFFER. */
+int gk_write_target(gk_ctx* ctx, int target, const void* host, size_t bytes);
 /* Size in bytes and device address of a render target (for zero-copy readers). */
 int gk_target_info(gk_ctx* ctx, int target, size_t* bytes, void** device_ptr);
 
