@@ -103,6 +103,19 @@ class Stats(C.Structure):
         (n, C.c_double) for n in ("ms_tlas", "ms_trace", "ms_shade", "ms_reproject", "ms_denoise", "ms_frame")]
 
 
+class ProbeGrid(C.Structure):
+    _fields_ = [("size_xy", C.c_uint32), ("size_z", C.c_uint32), ("unit", C.c_float), ("offset", C.c_float * 3)]
+
+
+# gk_ambient_cube (56 bytes) and gk_voxel (16 bytes) as numpy records
+CUBE_FIELDS = ("pos_z", "neg_z", "pos_y", "neg_y", "pos_x", "neg_x", "pos_z_d", "neg_z_d", "pos_y_d", "neg_y_d",
+               "pos_x_d", "neg_x_d", "sky_vis_pznzpyny", "sky_vis_pxnxs0s1")
+CUBE_DTYPE = np.dtype([(n, np.uint32) for n in CUBE_FIELDS])
+VOXEL_DTYPE = np.dtype([(n, np.uint32) for n in ("mat_id", "age", "dist_gg_z01", "dist_x01_y01")])
+# the reference's grid: 192 x 48 x 192 probes, 0.25 m apart (AmbientCube.slang:10-14)
+REFERENCE_PROBE_GRID = (192, 48, 0.25, (-24.0, -0.34375, -24.0))
+
+
 class UNetLayer(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("cin_a", "cin_b", "cout", "cout_real", "kpad", "up", "relu", "level")] + [
         ("offset", C.c_uint64)]
@@ -157,6 +170,12 @@ SIGNATURES = [
     ("gk_unet_train_step", C.c_int, [_P, _P, _P, _U, _U, _U, C.c_float, C.POINTER(C.c_float)]),
     ("gk_unet_infer", C.c_int, [_P, _P, _P, _U, _U]),
     ("gk_denoise_input", C.c_int, [_P, _P]),
+    ("gk_probes_init", C.c_int, [_P, C.POINTER(ProbeGrid)]),
+    ("gk_probes_voxelize", C.c_int, [_P]),
+    ("gk_probes_bake", C.c_int, [_P, _U, _U]),
+    ("gk_probes_bake_frame", C.c_int, [_P, _U, _U]),
+    ("gk_probes_read", C.c_int, [_P, _P, _P, _U, _U]),
+    ("gk_probes_write", C.c_int, [_P, _P, _P, _U, _U]),
 ]
 
 _lib = None
@@ -223,6 +242,7 @@ class SceneData:
         self.meshes: list[dict] = []
         self.materials: list[Material] = []
         self.instances: list[Instance] = []
+        self.lights: list[Light] = []    # LightObject quads (the probe bake uses the first)
 
     def add_mesh(self, positions, indices, normals=None, tri_material=None) -> int:
         m = {"positions": np.ascontiguousarray(positions, np.float32).reshape(-1, 3),
@@ -272,6 +292,14 @@ class SceneData:
             arr[i] = m
         return arr
 
+    def c_lights(self):
+        if not self.lights:
+            return None
+        arr = (Light * len(self.lights))()
+        for i, m in enumerate(self.lights):
+            arr[i] = m
+        return arr
+
 
 def procgen_cornell() -> SceneData:
     L = lib()
@@ -283,12 +311,14 @@ def procgen_cornell() -> SceneData:
     mat = np.zeros(nt.value, np.uint32)
     check(L.gk_procgen_cornell(ptr(pos), ptr(nrm), ptr(idx), ptr(mat), C.byref(nv), C.byref(nt)),
           "gk_procgen_cornell")
-    nm = C.c_uint32(8)
+    nm, nl = C.c_uint32(8), C.c_uint32(4)
     mats = (Material * 8)()
-    check(L.gk_procgen_cornell_materials(mats, C.byref(nm), None, None), "gk_procgen_cornell_materials")
+    lights = (Light * 4)()
+    check(L.gk_procgen_cornell_materials(mats, C.byref(nm), lights, C.byref(nl)), "gk_procgen_cornell_materials")
     s = SceneData()
     s.add_mesh(pos, idx, nrm, mat)
     s.materials = [mats[i] for i in range(nm.value)]
+    s.lights = [lights[i] for i in range(nl.value)]
     s.add_instance(0)
     return s
 
@@ -363,9 +393,10 @@ class Context:
 
     def upload(self, scene: SceneData):
         meshes, mats, insts = scene.c_meshes(), scene.c_materials(), scene.c_instances()
-        self._keep = (scene, meshes, mats, insts)
-        check(lib().gk_scene_upload(self.h, meshes, len(scene.meshes), mats, len(scene.materials), None, 0,
-                                    insts, len(scene.instances)), "gk_scene_upload")
+        lights = scene.c_lights()
+        self._keep = (scene, meshes, mats, insts, lights)
+        check(lib().gk_scene_upload(self.h, meshes, len(scene.meshes), mats, len(scene.materials), lights,
+                                    len(scene.lights), insts, len(scene.instances)), "gk_scene_upload")
 
     def update_instances(self, worlds: np.ndarray, first: int = 0):
         w = np.ascontiguousarray(np.transpose(np.asarray(worlds, np.float32), (0, 2, 1)))  # column-major
@@ -403,6 +434,38 @@ class Context:
         n, p = C.c_size_t(0), C.c_void_p()
         check(lib().gk_target_info(self.h, RT[name], C.byref(n), C.byref(p)), "gk_target_info")
         return p.value
+
+    # ambient-cube probe grid
+    def probes_init(self, size_xy=None, size_z=None, unit=None, offset=None):
+        """Allocate and clear the probe grid (the reference's grid by default)."""
+        d = REFERENCE_PROBE_GRID
+        g = ProbeGrid(size_xy or d[0], size_z or d[1], d[2] if unit is None else unit,
+                      (C.c_float * 3)(*(d[3] if offset is None else offset)))
+        check(lib().gk_probes_init(self.h, C.byref(g)), "gk_probes_init")
+        self.probe_count = g.size_xy * g.size_xy * g.size_z
+
+    def probes_voxelize(self):
+        check(lib().gk_probes_voxelize(self.h), "gk_probes_voxelize")
+
+    def probes_bake(self, first=0, count=None):
+        n = self.probe_count - first if count is None else count
+        check(lib().gk_probes_bake(self.h, first, n), "gk_probes_bake")
+
+    def probes_bake_frame(self, frame_count, temporal_frames=120):
+        check(lib().gk_probes_bake_frame(self.h, frame_count, temporal_frames), "gk_probes_bake_frame")
+
+    def probes_read(self, first=0, count=None):
+        n = self.probe_count - first if count is None else count
+        cubes = np.zeros(n, CUBE_DTYPE)
+        vox = np.zeros(n, VOXEL_DTYPE)
+        check(lib().gk_probes_read(self.h, ptr(cubes), ptr(vox), first, n), "gk_probes_read")
+        return cubes, vox
+
+    def probes_write(self, cubes=None, voxels=None, first=0):
+        n = len(cubes) if cubes is not None else len(voxels)
+        c = None if cubes is None else np.ascontiguousarray(cubes, CUBE_DTYPE)
+        v = None if voxels is None else np.ascontiguousarray(voxels, VOXEL_DTYPE)
+        check(lib().gk_probes_write(self.h, ptr(c), ptr(v), first, n), "gk_probes_write")
 
     def bvh_stats(self) -> BvhStats:
         s = BvhStats()

@@ -200,7 +200,6 @@ float halton(int index, int base) {
 
 using namespace gk;
 
-#define GK_CHECK_CTX(ctx) do { if (!(ctx)) return GK_ERR_ARG; } while (0)
 
 extern "C" {
 
@@ -276,8 +275,9 @@ int gk_scene_upload(gk_ctx* ctx, const gk_mesh* meshes, uint32_t num_meshes, con
                     const gk_instance* instances, uint32_t num_instances) {
     GK_CHECK_CTX(ctx);
     Ctx& c = ctx->c;
-    (void)lights;
-    (void)num_lights;  // emitters are the DIFFUSE_LIGHT triangles; LightObject quads are not needed
+    // path tracing finds emitters as DIFFUSE_LIGHT triangles; the LightObject quads are kept
+    // for the probe bake, which lights probes with the first one
+    if (num_lights && !lights) return GK_ERR_ARG;
     if (num_meshes == 0 || !meshes || num_materials == 0 || !materials || num_instances == 0 || !instances)
         return GK_ERR_ARG;
     if (num_meshes > GK_MAX_MESHES || num_materials > GK_MAX_MATERIALS || num_instances > GK_MAX_INSTANCES)
@@ -298,6 +298,7 @@ int gk_scene_upload(gk_ctx* ctx, const gk_mesh* meshes, uint32_t num_meshes, con
     GK_HIP(hipStreamSynchronize(c.stream));
     c.scene_ready = false;
     c.h_materials.assign(materials, materials + num_materials);
+    c.h_lights.assign(lights, lights + num_lights);
     GK_HIP(c.materials.resize(num_materials));
     GK_HIP(hipMemcpyAsync(c.materials.ptr, materials, sizeof(gk_material) * num_materials, hipMemcpyHostToDevice, c.stream));
     int s = build_blas(c, meshes, num_meshes);

@@ -133,6 +133,22 @@ struct Ctx {
     gk_bvh_stats bvh_stats{};
     DevBuf<float> grad_partial;     // adjoint: per-work-group gradient rows
     ::gk_unet* unet = nullptr;      // learned denoiser (GK_DENOISER_UNET), owned by the context
+    std::vector<gk_light> h_lights; // LightObject quads (the probe bake lights with the first)
+
+    // ambient-cube probe grid (probe.hip)
+    struct Probes {
+        gk_probe_grid grid{};
+        uint32_t count = 0;             // probes; 0 = no grid
+        DevBuf<gk_ambient_cube> cubes;
+        DevBuf<gk_ambient_cube> next;   // bake results, applied when the pass ends
+        DevBuf<gk_voxel> voxels;
+        DevBuf<uint32_t> list;          // probes of the chunk being baked
+        DevBuf<uint32_t> counters;      // [0] probes in list, [1] closest rays, [2] occlusion rays
+        DevBuf<float4> ro, rd, hit;
+        DevBuf<uint32_t> hit_inst;
+        DevBuf<float4> so, sd, shit;
+        DevBuf<uint32_t> face_sky;      // per (probe, face): sky byte, sun flag and bit
+    } probes;
 
     // frame
     gk_frame_params params{};
@@ -221,3 +237,4 @@ int collect_timing(Ctx& c);
 struct gk_ctx {
     gk::Ctx c;
 };
+#define GK_CHECK_CTX(ctx) do { if (!(ctx)) return GK_ERR_ARG; } while (0)

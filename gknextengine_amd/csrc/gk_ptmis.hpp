@@ -16,12 +16,6 @@ namespace ptmis {
 constexpr float kPi = 3.14159265358979323846f;
 
 __device__ inline float saturate(float x) { return x < 0.f ? 0.f : (x > 1.f ? 1.f : x); }
-__device__ inline void onb(f3 n, f3& b1, f3& b2) {
-    const float sz = n.z < 0.f ? -1.f : 1.f;
-    const float a = -1.0f / (sz + n.z);
-    b2 = f3{n.x * n.y * a, sz + n.y * n.y * a, -n.y};
-    b1 = f3{1.0f + sz * n.x * n.x * a, sz * b2.x, -sz * n.x};
-}
 
 struct Bsdf {
     f3 albedo;

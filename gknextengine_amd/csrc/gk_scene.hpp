@@ -61,6 +61,34 @@ __device__ inline f3 inv_transpose(const float* W, f3 v) {
     return o;
 }
 
+// branchless orthonormal basis around n (Duff et al. 2017) and AlignWithNormal
+// (Const_Func.slang:8-49): r expressed in the frame (T, B, n)
+__device__ inline void onb(f3 n, f3& b1, f3& b2) {
+    const float sz = n.z < 0.f ? -1.f : 1.f;
+    const float a = -1.0f / (sz + n.z);
+    b2 = f3{n.x * n.y * a, sz + n.y * n.y * a, -n.y};
+    b1 = f3{1.0f + sz * n.x * n.x * a, sz * b2.x, -sz * n.x};
+}
+__device__ inline f3 to_world(f3 v, f3 T, f3 B, f3 N) {
+    return f3{v.x * T.x + v.y * B.x + v.z * N.x, v.x * T.y + v.y * B.y + v.z * N.y, v.x * T.z + v.y * B.z + v.z * N.z};
+}
+__device__ inline f3 align_with_normal(f3 r, f3 n) {
+    f3 T, B;
+    onb(n, T, B);
+    return to_world(r, T, B, n);
+}
+
+// FHardwareRayTracerV2 hit vertex (Shading.slang:803-821)
+__device__ inline void hit_vertex(const DScene& s, uint32_t inst, uint32_t prim, float t, float u, float v, f3 O,
+                                  f3 D, f3& pos, f3& nrm) {
+    const InstDev& I = s.inst[inst];
+    const size_t b = 3 * size_t(I.tri_base + prim);
+    const f3 n0 = ld3(s.tri_nrm[b]), n1 = ld3(s.tri_nrm[b + 1]), n2 = ld3(s.tri_nrm[b + 2]);
+    const f3 mix = (n0 + (n1 - n0) * u) + (n2 - n0) * v;
+    nrm = normalize(inv_transpose(I.inv, mix));
+    pos = O + D * t;
+}
+
 __device__ inline ushort4 to_h4(float a, float b, float c, float d) {
     return make_ushort4(__half_as_ushort(__float2half_rn(a)), __half_as_ushort(__float2half_rn(b)),
                         __half_as_ushort(__float2half_rn(c)), __half_as_ushort(__float2half_rn(d)));

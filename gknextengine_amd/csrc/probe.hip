@@ -1,0 +1,612 @@
+// probe.hip - the ambient-cube probe grid: voxelization and bake.
+//
+//   voxelize  VoxelizeCube (CPUAccelerationStructure.cpp:85-160), which the reference runs
+//             on CPU threads over 16x16-probe columns (AsyncProcessFull/AsyncProcessGroup,
+//             :340-431), here one wavefront pass over the grid: 6 axis rays and 8 diagonal
+//             rays per probe, all closest-hit, traced by the persistent traversal kernel.
+//   bake      Bake.HwAmbientCube (Bake.HwAmbientCube.comp.slang:1-19) ->
+//             FGpuProbeGenerator::RenderAmbientCubeOnly / FaceTask (AmbientCube.slang:
+//             472-566,650-690): per probe within 8 units of a surface and per face, 16
+//             hemisphere rays on a jittered 4x4 grid, a segment to the first quad light and
+//             a sun ray; hits add albedo x the direct-light lookup x 1.25, misses add sky.
+//             The new face value is lerp(old, new, 1/8) in RGB10.
+//
+// Work is done in chunks of kChunk probes: rays of a chunk are written to queues, traced,
+// then reduced per (probe, face). A pass writes its results to a second array that is
+// copied over the grid when the pass ends, so every lookup of the pass reads the grid as
+// it was when the pass started.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "gk_probe.hpp"
+#include "gk_scene.hpp"
+
+namespace gk {
+
+int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev, uint32_t max_count,
+                 float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t* occluded_flags);
+int launch_trace_pair(Ctx& c, const float4* o, const float4* d, const uint32_t* count_dev, uint32_t max_count,
+                      float4* hit, uint32_t* hit_inst, const float4* so, const float4* sd,
+                      const uint32_t* scount_dev, uint32_t smax_count, float4* shit);
+int collect_timing(Ctx& c);
+
+namespace {
+
+constexpr uint32_t kChunk = 65536;          // probes per chunk
+constexpr uint32_t kVoxRays = 14;            // 6 axis + 8 diagonal rays per probe
+constexpr uint32_t kFaceRays = 16;           // FACE_TRACING (AmbientCube.slang:27,57)
+constexpr uint32_t kProbeRays = 6 * kFaceRays;
+constexpr float kEps = 1e-3f;                // EPS (PreProcessor.slang:13)
+constexpr float kMaxTrace = 1000.f;          // PT_MAX_TRACE_DISTANCE (Shading.slang:15)
+constexpr float kFastMaxTrace = 20.f;        // FAST_MAX_TRACE_DISTANCE (Shading.slang:16)
+constexpr uint32_t kMiss = 0xffffffffu;
+
+// grid4x4 and grid3x3 sample offsets (AmbientCube.slang:37-49)
+__constant__ float kGrid4[16][2] = {
+    {-0.75f, -0.75f}, {-0.25f, -0.75f}, {0.25f, -0.75f}, {0.75f, -0.75f},
+    {-0.75f, -0.25f}, {-0.25f, -0.25f}, {0.25f, -0.25f}, {0.75f, -0.25f},
+    {-0.75f, 0.25f},  {-0.25f, 0.25f},  {0.25f, 0.25f},  {0.75f, 0.25f},
+    {-0.75f, 0.75f},  {-0.25f, 0.75f},  {0.25f, 0.75f},  {0.75f, 0.75f}};
+__constant__ float kGrid3[9][2] = {{-0.667f, -0.667f}, {0.0f, -0.667f}, {0.667f, -0.667f},
+                                   {-0.667f, 0.0f},    {0.0f, 0.0f},    {0.667f, 0.0f},
+                                   {-0.667f, 0.667f},  {0.0f, 0.667f},  {0.667f, 0.667f}};
+
+// faces in FaceTask call order (AmbientCube.slang:674-679): +y, -y, +x, -x, +z, -z
+__device__ inline f3 face_basis(uint32_t f) {
+    switch (f) {
+        case 0: return f3{0.f, 1.f, 0.f};
+        case 1: return f3{0.f, -1.f, 0.f};
+        case 2: return f3{1.f, 0.f, 0.f};
+        case 3: return f3{-1.f, 0.f, 0.f};
+        case 4: return f3{0.f, 0.f, 1.f};
+        default: return f3{0.f, 0.f, -1.f};
+    }
+}
+
+// VoxelizeCube ray k: axis rays +y, -y, +x, -x, +z, -z, then DetectDistance's eight
+// diagonals (CPUAccelerationStructure.cpp:126-144, directions as written there,
+// normalised by tinybvh::Ray)
+__device__ inline f3 vox_dir(uint32_t k) {
+    if (k < 6) return face_basis(k);
+    // (1,1,1) (-1,1,1) (-1,-1,1) (-1,1,1) (1,1,-1) (-1,1,-1) (-1,-1,-1) (-1,1,-1)
+    const uint32_t j = k - 6u;
+    const float x = (j == 0u || j == 4u) ? 1.f : -1.f;
+    const float y = (j == 2u || j == 6u) ? -1.f : 1.f;
+    const float z = j < 4u ? 1.f : -1.f;
+    return normalize(f3{x, y, z});
+}
+
+struct GridK {
+    uint32_t sxy, sz, count;
+    float unit, off[3];
+};
+
+__device__ inline f3 probe_pos(const GridK& g, uint32_t idx) {
+    const uint32_t plane = g.sxy * g.sxy;
+    const uint32_t y = idx / plane;
+    const uint32_t rem = idx - y * plane;
+    const uint32_t z = rem / g.sxy;
+    const uint32_t x = rem - z * g.sxy;
+    return f3{float(x) * g.unit + g.off[0], float(y) * g.unit + g.off[1], float(z) * g.unit + g.off[2]};
+}
+
+// ---------------------------------------------------------------- voxelize
+__global__ void k_vox_rays(GridK g, uint32_t first, uint32_t n, float4* __restrict__ ro, float4* __restrict__ rd) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n * kVoxRays) return;
+    const uint32_t v = r / kVoxRays, k = r - v * kVoxRays;
+    const f3 o = probe_pos(g, first + v);
+    const f3 d = vox_dir(k);
+    ro[r] = make_float4(o.x, o.y, o.z, 0.f);
+    rd[r] = make_float4(d.x, d.y, d.z, g.unit * 64.f);
+}
+
+// InsideGeometry + VoxelizeCube's encoding (CPUAccelerationStructure.cpp:85-160). The
+// solid test uses the hit triangle's geometric normal in world space.
+__global__ void k_vox_finish(DScene s, GridK g, uint32_t first, uint32_t n, const float4* __restrict__ rd,
+                             const float4* __restrict__ hit, const uint32_t* __restrict__ hit_inst,
+                             const uint32_t* __restrict__ tri_rec, gk_voxel* __restrict__ vox) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    uint32_t mat_id = 0;
+    float dist[6];
+    for (uint32_t k = 0; k < 6; ++k) {
+        const size_t r = size_t(v) * kVoxRays + k;
+        const float4 h = hit[r];
+        dist[k] = 255.0f;
+        if (__float_as_uint(h.w) == kMiss) continue;
+        dist[k] = h.x;
+        if (h.x <= g.unit) {
+            const uint32_t inst = s.two_level ? hit_inst[r] : 0u;
+            const uint32_t prim = __float_as_uint(h.w);
+            const uint32_t mi = material_of(s, inst, prim);
+            mat_id = mi;
+            const InstDev& I = s.inst[inst];
+            const TriRec& tr = s.tris[I.tri_base + tri_rec[I.tri_base + prim]];
+            const f3 nl = normalize(cross(ld3(tr.e1), ld3(tr.e2)));
+            float nw[4];
+            mul44v(I.world, nl.x, nl.y, nl.z, 0.f, nw);
+            const f3 d = ld3(rd[r]);
+            if (nw[0] * d.x + nw[1] * d.y + nw[2] * d.z > 0.0f ||
+                s.mats[mi].material_model == GK_MATERIAL_DIFFUSE_LIGHT)
+                dist[k] = 0.f;
+        }
+    }
+    // order of the reference's distances: +y, -y, +x, -x, +z, -z
+    float mn = dist[0];
+    for (int k = 1; k < 6; ++k) mn = dist[k] < mn ? dist[k] : mn;
+    if (mn > 254.0f) {
+        for (uint32_t k = 6; k < kVoxRays; ++k) {
+            const float4 h = hit[size_t(v) * kVoxRays + k];
+            const float dd = __float_as_uint(h.w) == kMiss ? 255.0f : h.x;
+            mn = dd < mn ? dd : mn;
+        }
+    }
+    const float py = clamp01(dist[0] / g.unit), ny = clamp01(dist[1] / g.unit);
+    const float px = clamp01(dist[2] / g.unit), nx = clamp01(dist[3] / g.unit);
+    const float pz = clamp01(dist[4] / g.unit), nz = clamp01(dist[5] / g.unit);
+    const float inside = py * ny * px * nx * pz * nz;
+    gk_voxel out;
+    out.mat_id = mat_id;
+    out.age = 0u;
+    out.dist_gg_z01 = (uint32_t(mn / g.unit) & 0xffu) | ((uint32_t(inside * 255.0f) & 0xffu) << 8) |
+                      ((uint32_t(pz * 255.0f) & 0xffu) << 16) | ((uint32_t(nz * 255.0f) & 0xffu) << 24);
+    out.dist_x01_y01 = (uint32_t(px * 255.0f) & 0xffu) | ((uint32_t(nx * 255.0f) & 0xffu) << 8) |
+                       ((uint32_t(py * 255.0f) & 0xffu) << 16) | ((uint32_t(ny * 255.0f) & 0xffu) << 24);
+    vox[first + v] = out;
+}
+
+// ---------------------------------------------------------------- bake
+struct BakeK {
+    GridK g;
+    float sky[4], sun_dir[4], sun_col[4];
+    uint32_t has_sky, has_sun, has_light, nany;  // nany: occlusion rays per face (light, sun)
+    float light_pos[3], light_area;
+    uint32_t light_mat;
+};
+
+__device__ inline bool probe_active(const gk_voxel& v) { return (v.dist_gg_z01 & 0xffu) < 8u; }
+
+// probes of [first, first+n) that get baked (RenderAmbientCubeOnly's test), in order
+// within each work-group
+__global__ __launch_bounds__(256) void k_bake_select(const gk_voxel* __restrict__ vox, uint32_t first, uint32_t n,
+                                                     uint32_t* __restrict__ list, uint32_t* __restrict__ counters) {
+    __shared__ uint32_t cnt[5];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool want = i < n && probe_active(vox[first + i]);
+    const uint64_t m = __ballot(want);
+    const uint32_t wv = threadIdx.x >> 6;
+    if (__lane_id() == 0) cnt[wv] = uint32_t(__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+        cnt[4] = t ? atomicAdd(counters, t) : 0u;
+    }
+    __syncthreads();
+    uint32_t b = cnt[4];
+    for (uint32_t w = 0; w < wv; ++w) b += cnt[w];
+    if (want) list[b + uint32_t(__popcll(m & ((1ull << __lane_id()) - 1ull)))] = first + i;
+}
+
+__global__ void k_bake_counts(uint32_t* counters, uint32_t nany) {
+    counters[1] = counters[0] * kProbeRays;
+    counters[2] = counters[0] * 6u * nany;
+}
+
+__device__ inline f3 face_origin(const GridK& g, uint32_t idx, f3 basis) {
+    return probe_pos(g, idx) + (basis * g.unit) * 0.25f;
+}
+
+// FaceTask's rays (AmbientCube.slang:488-549)
+__global__ void k_bake_rays(BakeK K, const gk_voxel* __restrict__ vox, const uint32_t* __restrict__ list,
+                            const uint32_t* __restrict__ counters, float4* __restrict__ ro, float4* __restrict__ rd,
+                            float4* __restrict__ so, float4* __restrict__ sd) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t np = counters[0];
+    if (r < np * kProbeRays) {
+        const uint32_t s = r / kProbeRays, rem = r - s * kProbeRays;
+        const uint32_t f = rem / kFaceRays, i = rem - f * kFaceRays;
+        const uint32_t idx = list[s];
+        const uint32_t it = vox[idx].age % 9u;
+        const f3 basis = face_basis(f);
+        const f3 o = face_origin(K.g, idx, basis);
+        const float jx = kGrid3[it][0] * 0.25f, jy = kGrid3[it][1] * 0.25f;
+        const f3 hemi = normalize(f3{kGrid4[i][0] + jx, kGrid4[i][1] + jy, 1.0f});
+        const f3 d = align_with_normal(hemi, basis);
+        ro[r] = make_float4(o.x, o.y, o.z, kEps);
+        rd[r] = make_float4(d.x, d.y, d.z, kFastMaxTrace);
+    }
+    if (K.nany && r < np * 6u * K.nany) {
+        const uint32_t sf = r / K.nany, j = r - sf * K.nany;
+        const uint32_t s = sf / 6u, f = sf - s * 6u;
+        const uint32_t idx = list[s];
+        const f3 o = face_origin(K.g, idx, face_basis(f));
+        if (K.has_light && j == 0u) {
+            // TraceSegment(origin, lightPos, CUBE_UNIT * 0.5) (Shading.slang:759-782)
+            const float e = K.g.unit * 0.5f;
+            const f3 dir = f3{K.light_pos[0], K.light_pos[1], K.light_pos[2]} - o;
+            const float len = length(dir);
+            so[r] = make_float4(o.x, o.y, o.z, e);
+            sd[r] = make_float4(dir.x / len, dir.y / len, dir.z / len, len - e);
+        } else {
+            // TraceOcclusion(origin, SunDirection) (Shading.slang:737-757)
+            so[r] = make_float4(o.x, o.y, o.z, kEps);
+            sd[r] = make_float4(K.sun_dir[0], K.sun_dir[1], K.sun_dir[2], kMaxTrace);
+        }
+    }
+}
+
+__device__ inline float4 lerp4(float4 a, float4 b, float t) {
+    return make_float4(a.x + (b.x - a.x) * t, a.y + (b.y - a.y) * t, a.z + (b.z - a.z) * t, a.w + (b.w - a.w) * t);
+}
+
+__device__ inline uint32_t& face_direct(gk_ambient_cube& c, uint32_t f) {
+    switch (f) {
+        case 0: return c.pos_y_d;
+        case 1: return c.neg_y_d;
+        case 2: return c.pos_x_d;
+        case 3: return c.neg_x_d;
+        case 4: return c.pos_z_d;
+        default: return c.neg_z_d;
+    }
+}
+__device__ inline uint32_t& face_bounce(gk_ambient_cube& c, uint32_t f) {
+    switch (f) {
+        case 0: return c.pos_y;
+        case 1: return c.neg_y;
+        case 2: return c.pos_x;
+        case 3: return c.neg_x;
+        case 4: return c.pos_z;
+        default: return c.neg_z;
+    }
+}
+// sky-visibility byte of face f: (word, shift)
+__device__ inline void face_sky_slot(uint32_t f, uint32_t& word, uint32_t& shift) {
+    switch (f) {
+        case 0: word = 0; shift = 16; break;  // +y -> pznzpyny.z
+        case 1: word = 0; shift = 24; break;  // -y -> pznzpyny.w
+        case 2: word = 1; shift = 0; break;   // +x -> pxnxs0s1.x
+        case 3: word = 1; shift = 8; break;   // -x -> pxnxs0s1.y
+        case 4: word = 0; shift = 0; break;   // +z -> pznzpyny.x
+        default: word = 0; shift = 8; break;  // -z -> pznzpyny.y
+    }
+}
+
+// FaceTask's shading and accumulation (AmbientCube.slang:497-566) for one (probe, face)
+__global__ void k_bake_face(DScene s, BakeK K, ProbeDev P, const gk_voxel* __restrict__ vox,
+                            const uint32_t* __restrict__ list, const uint32_t* __restrict__ counters,
+                            const float4* __restrict__ ro, const float4* __restrict__ rd,
+                            const float4* __restrict__ hit, const uint32_t* __restrict__ hit_inst,
+                            const float4* __restrict__ shit, gk_ambient_cube* __restrict__ next,
+                            uint32_t* __restrict__ face_sky) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= counters[0] * 6u) return;
+    const uint32_t sidx = t / 6u, f = t - sidx * 6u;
+    const uint32_t idx = list[sidx];
+    const f3 basis = face_basis(f);
+    const f3 o = face_origin(K.g, idx, basis);
+    float4 direct = make_float4(0.f, 0.f, 0.f, 0.f), bounce = make_float4(0.f, 0.f, 0.f, 0.f);
+    float skyv = 0.f;
+    const float4 sky = K.has_sky ? make_float4(K.sky[0], K.sky[1], K.sky[2], K.sky[3]) : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (uint32_t i = 0; i < kFaceRays; ++i) {
+        const size_t r = size_t(sidx) * kProbeRays + f * kFaceRays + i;
+        const float4 h = hit[r];
+        if (__float_as_uint(h.w) != kMiss) {
+            const uint32_t inst = s.two_level ? hit_inst[r] : 0u;
+            const uint32_t prim = __float_as_uint(h.w);
+            f3 pos, nrm;
+            hit_vertex(s, inst, prim, h.x, h.y, h.z, ld3(ro[r]), ld3(rd[r]), pos, nrm);
+            const gk_material& m = s.mats[material_of(s, inst, prim)];
+            const float4 li = probe_lookup<false>(P, pos, nrm);
+            bounce.x = bounce.x + (m.diffuse[0] * li.x) * 1.25f;
+            bounce.y = bounce.y + (m.diffuse[1] * li.y) * 1.25f;
+            bounce.z = bounce.z + (m.diffuse[2] * li.z) * 1.25f;
+            bounce.w = bounce.w + (m.diffuse[3] * li.w) * 1.25f;
+        } else {
+            direct.x = direct.x + sky.x; direct.y = direct.y + sky.y;
+            direct.z = direct.z + sky.z; direct.w = direct.w + sky.w;
+            skyv = skyv + 1.0f;
+        }
+    }
+    const float nf = float(kFaceRays);
+    direct = make_float4(direct.x / nf, direct.y / nf, direct.z / nf, direct.w / nf);
+    bounce = make_float4(bounce.x / nf, bounce.y / nf, bounce.z / nf, bounce.w / nf);
+    uint32_t j = 0;
+    if (K.has_light) {
+        const bool occ = shit[(size_t(sidx) * 6u + f) * K.nany + j].w != 0.f;
+        ++j;
+        const float atten_l = occ ? 0.0f : 1.0f;
+        const f3 lp = f3{K.light_pos[0], K.light_pos[1], K.light_pos[2]};
+        const f3 ldir = normalize(lp - o);
+        const float ndotl = clamp01(dot(basis, ldir));
+        const float dist = length(lp - o);
+        const float att = ndotl * K.light_area / (dist * dist * 3.14159f);
+        const gk_material& lm = s.mats[K.light_mat];
+        direct.x = direct.x + (lm.diffuse[0] * att) * atten_l;
+        direct.y = direct.y + (lm.diffuse[1] * att) * atten_l;
+        direct.z = direct.z + (lm.diffuse[2] * att) * atten_l;
+        direct.w = direct.w + (lm.diffuse[3] * att) * atten_l;
+    }
+    uint32_t sun_bits = 0;
+    if (K.has_sun) {
+        const bool occ = shit[(size_t(sidx) * 6u + f) * K.nany + j].w != 0.f;
+        const float atten_s = occ ? 0.0f : 1.0f;
+        const float ndotl = clamp01(dot(basis, f3{K.sun_dir[0], K.sun_dir[1], K.sun_dir[2]}));
+        sun_bits = 0x100u | (atten_s > 0.0f ? 0x200u : 0u);
+        direct.x = direct.x + (((K.sun_col[0] * atten_s) * ndotl) * 1.0f) * 0.25f;
+        direct.y = direct.y + (((K.sun_col[1] * atten_s) * ndotl) * 1.0f) * 0.25f;
+        direct.z = direct.z + (((K.sun_col[2] * atten_s) * ndotl) * 1.0f) * 0.25f;
+        direct.w = direct.w + (((K.sun_col[3] * atten_s) * ndotl) * 1.0f) * 0.25f;
+    }
+    // lerp towards this pass with weight 1/8 (AmbientCube.slang:561-565)
+    const float w = 0.125f;
+    gk_ambient_cube old = P.cubes[idx];
+    uint32_t word, shift;
+    face_sky_slot(f, word, shift);
+    const uint32_t sky_old = ((word ? old.sky_vis_pxnxs0s1 : old.sky_vis_pznzpyny) >> shift) & 0xffu;
+    const float sky_new = 255.0f * skyv / nf;
+    const float sky_l = float(sky_old) + (sky_new - float(sky_old)) * w;
+    const uint32_t d_new = probe_pack(lerp4(probe_unpack(face_direct(old, f)), direct, w));
+    const uint32_t b_new = probe_pack(lerp4(probe_unpack(face_bounce(old, f)), bounce, w));
+    gk_ambient_cube& out = next[idx];
+    face_direct(out, f) = d_new;
+    face_bounce(out, f) = b_new;
+    face_sky[t] = (uint32_t(sky_l) & 0xffu) | (f == 0u ? sun_bits : 0u);
+}
+
+// sky bytes of the six faces, the +y face's sun bit, and the probe's age
+__global__ void k_bake_commit(const uint32_t* __restrict__ list, const uint32_t* __restrict__ counters,
+                              const gk_ambient_cube* __restrict__ cubes, const uint32_t* __restrict__ face_sky,
+                              gk_ambient_cube* __restrict__ next, gk_voxel* __restrict__ vox) {
+    const uint32_t sidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sidx >= counters[0]) return;
+    const uint32_t idx = list[sidx];
+    uint32_t w[2] = {cubes[idx].sky_vis_pznzpyny, cubes[idx].sky_vis_pxnxs0s1};
+    for (uint32_t f = 0; f < 6; ++f) {
+        const uint32_t v = face_sky[sidx * 6u + f];
+        uint32_t word, shift;
+        face_sky_slot(f, word, shift);
+        w[word] = (w[word] & ~(0xffu << shift)) | ((v & 0xffu) << shift);
+        if (v & 0x100u) w[1] = (w[1] & ~(0xffu << 16)) | (((v >> 9) & 1u) << 16);
+    }
+    next[idx].sky_vis_pznzpyny = w[0];
+    next[idx].sky_vis_pxnxs0s1 = w[1];
+    vox[idx].age = vox[idx].age + 1u;
+}
+
+__global__ void k_bake_apply(const gk_voxel* __restrict__ vox, uint32_t first, uint32_t n,
+                             const gk_ambient_cube* __restrict__ next, gk_ambient_cube* __restrict__ cubes) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (probe_active(vox[first + i])) cubes[first + i] = next[first + i];
+}
+
+__global__ void k_zero_u32(uint32_t* p, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+inline uint32_t nblk(size_t n, uint32_t b = 256) { return uint32_t((n + b - 1) / b); }
+
+GridK grid_k(const Ctx& c) {
+    const auto& g = c.probes.grid;
+    GridK k;
+    k.sxy = g.size_xy;
+    k.sz = g.size_z;
+    k.count = c.probes.count;
+    k.unit = g.unit;
+    memcpy(k.off, g.offset, 12);
+    return k;
+}
+
+// keep room for the trace launches' timing events (folded into the frame statistics)
+int reserve_trace_events(Ctx& c) {
+    if (c.tev_used + 64u > c.tev.size() && c.tev_used > 0) {
+        GK_HIP(hipStreamSynchronize(c.stream));
+        return collect_timing(c);
+    }
+    return GK_OK;
+}
+
+}  // namespace
+
+ProbeDev probe_dev(const Ctx& c) {
+    ProbeDev P{};
+    if (c.probes.count == 0) return P;
+    P.cubes = c.probes.cubes.ptr;
+    P.voxels = c.probes.voxels.ptr;
+    P.sxy = c.probes.grid.size_xy;
+    P.sz = c.probes.grid.size_z;
+    P.unit = c.probes.grid.unit;
+    memcpy(P.off, c.probes.grid.offset, 12);
+    return P;
+}
+
+int probes_init(Ctx& c, const gk_probe_grid& g) {
+    if (g.size_xy < 2 || g.size_z < 2 || !(g.unit > 0.f) || !std::isfinite(g.unit)) return GK_ERR_ARG;
+    for (int i = 0; i < 3; ++i)
+        if (!std::isfinite(g.offset[i])) return GK_ERR_ARG;
+    const uint64_t n = uint64_t(g.size_xy) * g.size_xy * g.size_z;
+    if (n > (1ull << 26)) return GK_ERR_LIMIT;
+    auto& P = c.probes;
+    GK_HIP(hipStreamSynchronize(c.stream));
+    P.count = 0;
+    GK_HIP(P.cubes.resize(n));
+    GK_HIP(P.next.resize(n));
+    GK_HIP(P.voxels.resize(n));
+    k_zero_u32<<<nblk(n * 14), 256, 0, c.stream>>>(reinterpret_cast<uint32_t*>(P.cubes.ptr), n * 14);
+    k_zero_u32<<<nblk(n * 4), 256, 0, c.stream>>>(reinterpret_cast<uint32_t*>(P.voxels.ptr), n * 4);
+    GK_HIP(hipGetLastError());
+    P.grid = g;
+    P.count = uint32_t(n);
+    return GK_OK;
+}
+
+int probes_voxelize(Ctx& c) {
+    auto& P = c.probes;
+    if (P.count == 0 || !c.scene_ready) return GK_ERR_STATE;
+    int st = reserve_trace_events(c);
+    if (st != GK_OK) return st;
+    const size_t nr = size_t(kChunk) * kVoxRays;
+    GK_HIP(P.ro.resize(std::max(P.ro.count, nr)));
+    GK_HIP(P.rd.resize(std::max(P.rd.count, nr)));
+    GK_HIP(P.hit.resize(std::max(P.hit.count, nr)));
+    GK_HIP(P.hit_inst.resize(std::max(P.hit_inst.count, nr)));
+    const GridK g = grid_k(c);
+    const DScene s = make_dscene(c);
+    for (uint32_t first = 0; first < P.count; first += kChunk) {
+        const uint32_t n = std::min(kChunk, P.count - first);
+        k_vox_rays<<<nblk(size_t(n) * kVoxRays), 256, 0, c.stream>>>(g, first, n, P.ro.ptr, P.rd.ptr);
+        GK_HIP(hipGetLastError());
+        st = launch_trace(c, P.ro.ptr, P.rd.ptr, nullptr, n * kVoxRays, P.hit.ptr, P.hit_inst.ptr, false, nullptr);
+        if (st != GK_OK) return st;
+        k_vox_finish<<<nblk(n), 256, 0, c.stream>>>(s, g, first, n, P.rd.ptr, P.hit.ptr, P.hit_inst.ptr,
+                                                    c.tri_rec.ptr, P.voxels.ptr);
+        GK_HIP(hipGetLastError());
+        st = reserve_trace_events(c);
+        if (st != GK_OK) return st;
+    }
+    return GK_OK;
+}
+
+int probes_bake(Ctx& c, uint32_t first, uint32_t count) {
+    auto& P = c.probes;
+    if (P.count == 0 || !c.scene_ready || !c.frame_ready) return GK_ERR_STATE;
+    if (first > P.count || count > P.count - first) return GK_ERR_ARG;
+    if (count == 0) return GK_OK;
+    int st = reserve_trace_events(c);
+    if (st != GK_OK) return st;
+    const gk_frame_params& fp = c.params;
+    BakeK K{};
+    K.g = grid_k(c);
+    memcpy(K.sky, fp.sky_color, 16);
+    memcpy(K.sun_dir, fp.sun_direction, 16);
+    memcpy(K.sun_col, fp.sun_color, 16);
+    K.has_sky = fp.has_sky ? 1u : 0u;
+    K.has_sun = fp.has_sun ? 1u : 0u;
+    K.has_light = c.h_lights.empty() ? 0u : 1u;
+    if (K.has_light) {
+        const gk_light& L = c.h_lights[0];
+        // lerp(p1, p3, 0.5)
+        for (int i = 0; i < 3; ++i) K.light_pos[i] = L.p1[i] + (L.p3[i] - L.p1[i]) * 0.5f;
+        K.light_area = L.normal_area[3];
+        const uint32_t nm = uint32_t(c.h_materials.size());
+        K.light_mat = L.light_mat_idx > nm - 1u ? nm - 1u : L.light_mat_idx;
+    }
+    K.nany = K.has_light + K.has_sun;
+    GK_HIP(P.list.resize(kChunk));
+    GK_HIP(P.counters.resize(4));
+    GK_HIP(P.face_sky.resize(size_t(kChunk) * 6));
+    const size_t nr = size_t(kChunk) * kProbeRays;
+    GK_HIP(P.ro.resize(std::max(P.ro.count, nr)));
+    GK_HIP(P.rd.resize(std::max(P.rd.count, nr)));
+    GK_HIP(P.hit.resize(std::max(P.hit.count, nr)));
+    GK_HIP(P.hit_inst.resize(std::max(P.hit_inst.count, nr)));
+    const size_t ns = size_t(kChunk) * 6u * std::max(1u, K.nany);
+    GK_HIP(P.so.resize(ns));
+    GK_HIP(P.sd.resize(ns));
+    GK_HIP(P.shit.resize(ns));
+    const DScene s = make_dscene(c);
+    const ProbeDev PD = probe_dev(c);
+    for (uint32_t c0 = first; c0 < first + count; c0 += kChunk) {
+        const uint32_t n = std::min(kChunk, first + count - c0);
+        k_zero_u32<<<1, 64, 0, c.stream>>>(P.counters.ptr, 4);
+        k_bake_select<<<nblk(n), 256, 0, c.stream>>>(P.voxels.ptr, c0, n, P.list.ptr, P.counters.ptr);
+        k_bake_counts<<<1, 1, 0, c.stream>>>(P.counters.ptr, K.nany);
+        k_bake_rays<<<nblk(size_t(n) * kProbeRays), 256, 0, c.stream>>>(K, P.voxels.ptr, P.list.ptr, P.counters.ptr,
+                                                                         P.ro.ptr, P.rd.ptr, P.so.ptr, P.sd.ptr);
+        GK_HIP(hipGetLastError());
+        if (K.nany)
+            st = launch_trace_pair(c, P.ro.ptr, P.rd.ptr, P.counters.ptr + 1, n * kProbeRays, P.hit.ptr,
+                                   P.hit_inst.ptr, P.so.ptr, P.sd.ptr, P.counters.ptr + 2, n * 6u * K.nany,
+                                   P.shit.ptr);
+        else
+            st = launch_trace(c, P.ro.ptr, P.rd.ptr, P.counters.ptr + 1, n * kProbeRays, P.hit.ptr, P.hit_inst.ptr,
+                              false, nullptr);
+        if (st != GK_OK) return st;
+        k_bake_face<<<nblk(size_t(n) * 6u), 256, 0, c.stream>>>(s, K, PD, P.voxels.ptr, P.list.ptr, P.counters.ptr,
+                                                                 P.ro.ptr, P.rd.ptr, P.hit.ptr, P.hit_inst.ptr,
+                                                                 P.shit.ptr, P.next.ptr, P.face_sky.ptr);
+        k_bake_commit<<<nblk(n), 256, 0, c.stream>>>(P.list.ptr, P.counters.ptr, P.cubes.ptr, P.face_sky.ptr,
+                                                     P.next.ptr, P.voxels.ptr);
+        GK_HIP(hipGetLastError());
+        st = reserve_trace_events(c);
+        if (st != GK_OK) return st;
+    }
+    k_bake_apply<<<nblk(count), 256, 0, c.stream>>>(P.voxels.ptr, first, count, P.next.ptr, P.cubes.ptr);
+    GK_HIP(hipGetLastError());
+    return GK_OK;
+}
+
+}  // namespace gk
+
+extern "C" {
+
+int gk_probes_init(gk_ctx* ctx, const gk_probe_grid* grid) {
+    GK_CHECK_CTX(ctx);
+    gk::Ctx& c = ctx->c;
+    GK_HIP(hipSetDevice(c.device));
+    const gk_probe_grid def{192u, 48u, 0.25f, {-24.0f, -0.34375f, -24.0f}};
+    return gk::probes_init(c, grid ? *grid : def);
+}
+
+int gk_probes_voxelize(gk_ctx* ctx) {
+    GK_CHECK_CTX(ctx);
+    GK_HIP(hipSetDevice(ctx->c.device));
+    return gk::probes_voxelize(ctx->c);
+}
+
+int gk_probes_bake(gk_ctx* ctx, uint32_t first, uint32_t count) {
+    GK_CHECK_CTX(ctx);
+    GK_HIP(hipSetDevice(ctx->c.device));
+    return gk::probes_bake(ctx->c, first, count);
+}
+
+int gk_probes_bake_frame(gk_ctx* ctx, uint32_t frame_count, uint32_t temporal_frames) {
+    GK_CHECK_CTX(ctx);
+    if (temporal_frames == 0) return GK_ERR_ARG;
+    const uint32_t n = ctx->c.probes.count;
+    if (n == 0) return GK_ERR_STATE;
+    const uint32_t groups = n / 64u;
+    const uint32_t per_frame = groups / temporal_frames;
+    const uint32_t first = (frame_count % temporal_frames) * per_frame * 64u;
+    return gk_probes_bake(ctx, first, per_frame * 64u);
+}
+
+int gk_probes_read(gk_ctx* ctx, gk_ambient_cube* cubes, gk_voxel* voxels, uint32_t first, uint32_t count) {
+    GK_CHECK_CTX(ctx);
+    gk::Ctx& c = ctx->c;
+    if (c.probes.count == 0) return GK_ERR_STATE;
+    if (first > c.probes.count || count > c.probes.count - first) return GK_ERR_ARG;
+    GK_HIP(hipSetDevice(c.device));
+    if (cubes)
+        GK_HIP(hipMemcpyAsync(cubes, c.probes.cubes.ptr + first, sizeof(gk_ambient_cube) * size_t(count),
+                              hipMemcpyDeviceToHost, c.stream));
+    if (voxels)
+        GK_HIP(hipMemcpyAsync(voxels, c.probes.voxels.ptr + first, sizeof(gk_voxel) * size_t(count),
+                              hipMemcpyDeviceToHost, c.stream));
+    GK_HIP(hipStreamSynchronize(c.stream));
+    return GK_OK;
+}
+
+int gk_probes_write(gk_ctx* ctx, const gk_ambient_cube* cubes, const gk_voxel* voxels, uint32_t first,
+                    uint32_t count) {
+    GK_CHECK_CTX(ctx);
+    gk::Ctx& c = ctx->c;
+    if (c.probes.count == 0) return GK_ERR_STATE;
+    if (first > c.probes.count || count > c.probes.count - first) return GK_ERR_ARG;
+    GK_HIP(hipSetDevice(c.device));
+    if (cubes)
+        GK_HIP(hipMemcpyAsync(c.probes.cubes.ptr + first, cubes, sizeof(gk_ambient_cube) * size_t(count),
+                              hipMemcpyHostToDevice, c.stream));
+    if (voxels)
+        GK_HIP(hipMemcpyAsync(c.probes.voxels.ptr + first, voxels, sizeof(gk_voxel) * size_t(count),
+                              hipMemcpyHostToDevice, c.stream));
+    GK_HIP(hipStreamSynchronize(c.stream));
+    return GK_OK;
+}
+
+}  // extern "C"

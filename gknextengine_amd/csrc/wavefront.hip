@@ -9,7 +9,8 @@
 //           GetRayColor is split at the trace call into "begin" (material, RNG draws,
 //           new direction) and "finish" (hit albedo, back-face and emitter rules); the
 //           sample loop, sun cone, early exit and DirectIlluminate follow Render.
-//           Probe termination returns 0 (no probes are baked).
+//           A terminated path is multiplied by the ambient-cube probe lookup
+//           (gk_probe.hpp); without a probe grid that lookup is (0,0,0,1).
 //   PT_MIS  GGX (bounded VNDF) + Lambert with next-event estimation to emissive
 //           triangles, power-heuristic MIS, Russian roulette after 3 bounces.
 // Every RNG draw happens in the same order as in the per-pixel formulation, so a path's
@@ -19,6 +20,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "gk_probe.hpp"
 #include "gk_ptmis.hpp"
 
 namespace gk {
@@ -32,6 +34,7 @@ using ptmis::power_h;
 using ptmis::vndf_sample;
 
 int post_process(Ctx& c);  // post.hip
+ProbeDev probe_dev(const Ctx& c);  // probe.hip
 int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev, uint32_t max_count,
                  float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t* occluded_flags);
 
@@ -49,6 +52,7 @@ struct FrameK {
     float exit_prob, hit_off;
     uint32_t W, H, frame, seed, samples, bounces, max_bounces, temporal, has_sky, has_sun, tiled;
     uint32_t vis_primary;  // primary hits come from GK_RT_MINIGBUFFER (left unwritten)
+    ProbeDev probes;       // REF_V2 path termination
 };
 
 struct Targets {
@@ -175,20 +179,6 @@ __device__ inline float schlick(float cosine, float ior) {
     const float x = 1.f - cosine;
     return r0 + (1.f - r0) * (x * x * x * x * x);
 }
-__device__ inline void onb(f3 n, f3& b1, f3& b2) {
-    const float sz = n.z < 0.f ? -1.f : 1.f;
-    const float a = -1.0f / (sz + n.z);
-    b2 = f3{n.x * n.y * a, sz + n.y * n.y * a, -n.y};
-    b1 = f3{1.0f + sz * n.x * n.x * a, sz * b2.x, -sz * n.x};
-}
-__device__ inline f3 to_world(f3 v, f3 T, f3 B, f3 N) {
-    return f3{v.x * T.x + v.y * B.x + v.z * N.x, v.x * T.y + v.y * B.y + v.z * N.y, v.x * T.z + v.y * B.z + v.z * N.z};
-}
-__device__ inline f3 align_with_normal(f3 r, f3 n) {
-    f3 T, B;
-    onb(n, T, B);
-    return to_world(r, T, B, n);
-}
 __device__ inline f3 random_hemisphere1(rng4& s) {
     float ux, uy;
     rnd2(s, ux, uy);
@@ -237,17 +227,6 @@ __device__ inline f3 refract(f3 i, f3 n, float eta) {
     const float k = 1.f - eta * eta * (1.f - d * d);
     if (k < 0.f) return f3{0.f, 0.f, 0.f};
     return i * eta - n * (eta * d + sqrtf(k));
-}
-
-// FHardwareRayTracerV2 hit vertex (Shading.slang:803-821)
-__device__ inline void hit_vertex(const DScene& s, uint32_t inst, uint32_t prim, float t, float u, float v, f3 O,
-                                  f3 D, f3& pos, f3& nrm) {
-    const InstDev& I = s.inst[inst];
-    const size_t b = 3 * size_t(I.tri_base + prim);
-    const f3 n0 = ld3(s.tri_nrm[b]), n1 = ld3(s.tri_nrm[b + 1]), n2 = ld3(s.tri_nrm[b + 2]);
-    const f3 mix = (n0 + (n1 - n0) * u) + (n2 - n0) * v;
-    nrm = normalize(inv_transpose(I.inv, mix));
-    pos = O + D * t;
 }
 
 // get_material_dataV2 normal (GeneralFunc.slang:20-53): bx*n0 + (by*n1 + bz*n2), world normals
@@ -629,7 +608,10 @@ __global__ __launch_bounds__(256) void k_ref_shade(DScene s, FrameK K, Targets T
                 if (!done) {
                     const bool early = !prim_diel && rnd1(seed) < K.exit_prob;
                     if (b == maxb - 1u || early) {
-                        color.x = 0.f; color.y = 0.f; color.z = 0.f;  // no probes: (0,0,0,1)
+                        // RayColor *= interpolateAmbientCubesV2<FullAmbientCubeSampler>(vertex)
+                        const float4 pr = probe_lookup<true>(K.probes, pos, nrm);
+                        color.x = color.x * pr.x; color.y = color.y * pr.y;
+                        color.z = color.z * pr.z; color.w = color.w * pr.w;
                         done = true;
                     }
                 }
@@ -1059,6 +1041,7 @@ static FrameK make_k(const Ctx& c) {
     K.has_sun = P.has_sun;
     K.tiled = (c.width % 8 == 0 && c.height % 8 == 0) ? 1u : 0u;
     K.vis_primary = (P.flags & GK_FRAME_PRIMARY_FROM_VISBUFFER) ? 1u : 0u;
+    K.probes = probe_dev(c);
     return K;
 }
 

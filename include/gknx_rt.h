@@ -468,6 +468,68 @@ int gk_unet_infer(gk_unet* net, const float* d_input, float* d_output, uint32_t 
  * reprojection, normal, primary hit distance) as d_input, width*height*10 FP32. */
 int gk_denoise_input(gk_ctx* ctx, float* d_input);
 
+/* ---------------------------------------------------------------- ambient-cube probes
+ * A regular grid of probes (AmbientCube.slang:10-14: 192 x 48 x 192 probes, 0.25 m
+ * apart, first probe at (-24, -0.34375, -24)); probe (x, y, z) is element
+ * y*size_xy*size_xy + z*size_xy + x. Each probe has a voxel record (scene distance
+ * field, FCPUProbeBaker) and an ambient cube (six faces of direct and of bounced light,
+ * RGB10 at 512/1023 per step, plus sky visibility bytes). REF_V2 multiplies a path
+ * ended by early exit or the bounce limit by the probes' trilinear, occlusion-tested
+ * lookup (interpolateAmbientCubesV2<FullAmbientCubeSampler>, AmbientCube.slang:268-345;
+ * Shading.slang:1147-1153). Without gk_probes_init a terminated path gets (0,0,0,1), the
+ * value of a lookup outside the grid. */
+typedef struct gk_probe_grid {
+    uint32_t size_xy;     /* probes along x and z (CUBE_SIZE_XY) */
+    uint32_t size_z;      /* probes along y (CUBE_SIZE_Z) */
+    float unit;           /* spacing in metres (CUBE_UNIT) */
+    float offset[3];      /* position of probe (0,0,0) (CUBE_OFFSET) */
+} gk_probe_grid;
+
+/* AmbientCube, byte-identical (BasicTypes.slang:116-138), 56 bytes. Colours are packed
+ * r | g << 10 | b << 20 | a << 30 of clamp(c / 512, 0, 1) * 1023 (a: * 3). Sky
+ * visibility bytes: sky_vis_pznzpyny = +z, -z, +y, -y (low byte first),
+ * sky_vis_pxnxs0s1 = +x, -x, sun visibility of the +y face, unused. */
+typedef struct gk_ambient_cube {
+    uint32_t pos_z, neg_z, pos_y, neg_y, pos_x, neg_x;              /* bounced light */
+    uint32_t pos_z_d, neg_z_d, pos_y_d, neg_y_d, pos_x_d, neg_x_d;  /* direct light  */
+    uint32_t sky_vis_pznzpyny, sky_vis_pxnxs0s1;
+} gk_ambient_cube;
+
+/* VoxelData, byte-identical (BasicTypes.slang:145-151), 16 bytes. dist_gg_z01 bytes:
+ * min(hit distance)/unit, 255 * product of the six clamped axis distances ("not inside"),
+ * +z, -z; dist_x01_y01 bytes: +x, -x, +y, -y axis distances clamped to one unit, * 255. */
+typedef struct gk_voxel {
+    uint32_t mat_id;
+    uint32_t age;         /* bake passes this probe has received */
+    uint32_t dist_gg_z01;
+    uint32_t dist_x01_y01;
+} gk_voxel;
+
+/* Allocates the grid of ctx and clears it, as FCPUProbeBaker::Init + ClearAmbientCubes
+ * (CPUAccelerationStructure.cpp:165-170,491-497). grid NULL: the reference's grid.
+ * At most 2^26 probes (GK_ERR_LIMIT). */
+int gk_probes_init(gk_ctx* ctx, const gk_probe_grid* grid);
+/* Voxelizes every probe against the current scene: VoxelizeCube
+ * (CPUAccelerationStructure.cpp:85-160), run on the device instead of CPU threads
+ * (AsyncProcessFull / AsyncProcessGroup, :340-431). Resets every probe's age. */
+int gk_probes_voxelize(gk_ctx* ctx);
+/* One bake pass over probes [first, first+count) of the grid: Bake.HwAmbientCube
+ * (Bake.HwAmbientCube.comp.slang:1-19 -> FGpuProbeGenerator::RenderAmbientCubeOnly,
+ * AmbientCube.slang:472-690) with sky and sun of the last gk_frame parameters and the
+ * first gk_light of the scene. Only probes whose voxel lies within 8 units of a surface
+ * are baked. Reads of the grid inside one pass see it as it was when the pass started
+ * (the reference reads probes that other threads of the same dispatch are writing). */
+int gk_probes_bake(gk_ctx* ctx, uint32_t first, uint32_t count);
+/* The per-frame slice RayTraceBaseRenderer::PostRender bakes (RayTraceBaseRenderer.cpp:
+ * 225-274): groups of 64 probes, grid/64/temporal_frames groups per frame, slice
+ * frame_count % temporal_frames (temporal_frames 30, 120 or 300 for BakeSpeedLevel 0-2). */
+int gk_probes_bake_frame(gk_ctx* ctx, uint32_t frame_count, uint32_t temporal_frames);
+/* Copies probes [first, first+count) to / from host memory (either pointer may be NULL);
+ * gk_probes_write is FCPUProbeBaker::UploadGPU (CPUAccelerationStructure.cpp:333-338). */
+int gk_probes_read(gk_ctx* ctx, gk_ambient_cube* cubes, gk_voxel* voxels, uint32_t first, uint32_t count);
+int gk_probes_write(gk_ctx* ctx, const gk_ambient_cube* cubes, const gk_voxel* voxels, uint32_t first,
+                    uint32_t count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -201,7 +201,27 @@ struct Light {              // one emissive triangle (world space)
     uint32_t inst, prim;
 };
 
+// ambient-cube probe grid (gk_probe_grid / AmbientCube / VoxelData layouts)
+struct ProbeGridO {
+    uint32_t size_xy, size_z;
+    float unit, offset[3];
+};
+struct CubeO {            // AmbientCube, BasicTypes.slang:116-138
+    uint32_t pos_z, neg_z, pos_y, neg_y, pos_x, neg_x;
+    uint32_t pos_z_d, neg_z_d, pos_y_d, neg_y_d, pos_x_d, neg_x_d;
+    uint32_t sky0, sky1;  // skyVisibility_pznzpyny, skyVisibility_pxnxs0s1
+};
+struct VoxelO {           // VoxelData, BasicTypes.slang:145-151
+    uint32_t mat_id, age, dist0, dist1;  // dist0: distanceToSolid_gg_z01, dist1: _x01_y01
+};
+struct ProbeRef {
+    ProbeGridO grid{};
+    const CubeO* cubes = nullptr;  // nullptr: no grid
+    const VoxelO* vox = nullptr;
+};
+
 struct Scene {
+    ProbeRef probes;        // REF_V2 termination lookup (arrays owned by the caller)
     std::vector<Mesh> meshes;
     std::vector<Instance> inst;
     std::vector<Material> mats;

@@ -6,8 +6,13 @@
 //                  (Shading.slang:995-1241) with FHardwareRayTracerV2 hit reconstruction
 //                  (Shading.slang:784-826), CalculateMotionVectorV2 (Shading.slang:50-72),
 //                  FHardwareDirectIlluminatorV2 (Shading.slang:886-907) and the sampling
-//                  helpers of Const_Func.slang:8-49,227-340. Probe termination
-//                  (interpolateAmbientCubesV2) returns 0: no probes are baked.
+//                  helpers of Const_Func.slang:8-49,227-340. Path termination
+//                  multiplies by interpolateAmbientCubesV2<Full> over the scene's
+//                  probe grid ((0,0,0,1) without one).
+//   probes         VoxelizeCube (CPUAccelerationStructure.cpp:85-160) and the GPU bake
+//                  FGpuProbeGenerator::RenderAmbientCubeOnly/FaceTask (AmbientCube.slang:
+//                  472-566,650-690), one probe at a time; a bake pass reads the grid as it
+//                  was when the pass started.
 //   PT_MIS         the build's GGX/Lambert path tracer with next-event estimation to
 //                  emissive triangles and power-heuristic MIS, templated on float/double
 //                  (double = the convergence oracle).
@@ -248,6 +253,306 @@ void combined_prev(const float* world, const float* prev, float* out) {
     }
 }
 
+// ---------------------------------------------------------------- ambient-cube probes
+// packRGB10A2 / unpackRGB10A2 (AmbientCube.slang:66-87), MAX_ILLUMINANCE = 512
+uint32_t pack_rgb10a2(const F4& c) {
+    float q[4];
+    for (int i = 0; i < 4; ++i) q[i] = std::fmin(std::fmax(c.v[i] / 512.0f, 0.0f), 1.0f);
+    return uint32_t(q[0] * 1023.0f) | (uint32_t(q[1] * 1023.0f) << 10) | (uint32_t(q[2] * 1023.0f) << 20) |
+           (uint32_t(q[3] * 3.0f) << 30);
+}
+F4 unpack_rgb10a2(uint32_t p) {
+    const float r = float(p & 0x3FF) / 1023.0f;
+    const float g = float((p >> 10) & 0x3FF) / 1023.0f;
+    const float b = float((p >> 20) & 0x3FF) / 1023.0f;
+    return F4{{r * 512.f, g * 512.f, b * 512.f, 0.0f * 512.f}};
+}
+// unpack_bytes / pack_bytes (AmbientCube.slang:90-109)
+void unpack_bytes(uint32_t p, uint32_t b[4]) {
+    b[0] = p & 0xFF; b[1] = (p >> 8) & 0xFF; b[2] = (p >> 16) & 0xFF; b[3] = (p >> 24) & 0xFF;
+}
+uint32_t pack_bytes(const uint32_t b[4]) {
+    return (b[0] & 0xFF) | ((b[1] & 0xFF) << 8) | ((b[2] & 0xFF) << 16) | ((b[3] & 0xFF) << 24);
+}
+F4 add4(const F4& a, const F4& b) { return F4{{a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2], a.v[3] + b.v[3]}}; }
+F4 scale4(float s, const F4& a) { return F4{{s * a.v[0], s * a.v[1], s * a.v[2], s * a.v[3]}}; }
+F4 lerp4(const F4& a, const F4& b, float t) {
+    F4 o;
+    for (int i = 0; i < 4; ++i) o.v[i] = a.v[i] + (b.v[i] - a.v[i]) * t;
+    return o;
+}
+
+// sampleAmbientCubeHL2_DI and sampleAmbientCubeHL2_Full (AmbientCube.slang:117-232)
+F4 sample_cube(const CubeO& cube, F3 n, bool full) {
+    const float wx = std::fmax(n.x, 0.f), wnx = std::fmax(-n.x, 0.f);
+    const float wy = std::fmax(n.y, 0.f), wny = std::fmax(-n.y, 0.f);
+    const float wz = std::fmax(n.z, 0.f), wnz = std::fmax(-n.z, 0.f);
+    const float sum = wx + wnx + wy + wny + wz + wnz;
+    F4 color{{0.f, 0.f, 0.f, 0.f}};
+    float skyvis = 0.f;
+    if (full) {
+        uint32_t s0[4], s1[4];
+        unpack_bytes(cube.sky0, s0);
+        unpack_bytes(cube.sky1, s1);
+        skyvis += float(s0[0]) * wz;
+        skyvis += float(s0[1]) * wnz;
+        skyvis += float(s0[2]) * wy;
+        skyvis += float(s0[3]) * wny;
+        skyvis += float(s1[0]) * wx;
+        skyvis += float(s1[1]) * wnx;
+    }
+    color = add4(color, scale4(wx, unpack_rgb10a2(cube.pos_x_d)));
+    color = add4(color, scale4(wnx, unpack_rgb10a2(cube.neg_x_d)));
+    color = add4(color, scale4(wy, unpack_rgb10a2(cube.pos_y_d)));
+    color = add4(color, scale4(wny, unpack_rgb10a2(cube.neg_y_d)));
+    color = add4(color, scale4(wz, unpack_rgb10a2(cube.pos_z_d)));
+    color = add4(color, scale4(wnz, unpack_rgb10a2(cube.neg_z_d)));
+    const float k = sum > 0.0f ? (1.0f / sum) : 1.0f;
+    if (!full) {
+        for (int i = 0; i < 4; ++i) color.v[i] *= k;
+        return color;
+    }
+    color = add4(color, scale4(wx, unpack_rgb10a2(cube.pos_x)));
+    color = add4(color, scale4(wnx, unpack_rgb10a2(cube.neg_x)));
+    color = add4(color, scale4(wy, unpack_rgb10a2(cube.pos_y)));
+    color = add4(color, scale4(wny, unpack_rgb10a2(cube.neg_y)));
+    color = add4(color, scale4(wz, unpack_rgb10a2(cube.pos_z)));
+    color = add4(color, scale4(wnz, unpack_rgb10a2(cube.neg_z)));
+    for (int i = 0; i < 3; ++i) color.v[i] *= k;
+    skyvis *= k;
+    color.v[3] = skyvis / 255.0f;
+    return color;
+}
+
+// interpolateAmbientCubesV2 (AmbientCube.slang:268-345)
+F4 interpolate_cubes(const ProbeRef& pr, F3 in_pos, F3 normal, bool full) {
+    if (!pr.cubes) return F4{{0.f, 0.f, 0.f, 1.f}};
+    const ProbeGridO& g = pr.grid;
+    const F3 nearpos((in_pos.x - g.offset[0]) / g.unit, (in_pos.y - g.offset[1]) / g.unit,
+                     (in_pos.z - g.offset[2]) / g.unit);
+    const int XY = int(g.size_xy), Z = int(g.size_z);
+    // the reference's early out; a NaN position also counts as outside
+    if (!(nearpos.x >= 0 && nearpos.y >= 0 && nearpos.z >= 0 && nearpos.x <= float(XY - 2) &&
+          nearpos.y <= float(Z - 2) && nearpos.z <= float(XY - 2)))
+        return F4{{0.f, 0.f, 0.f, 1.f}};
+    const F3 fl(std::floor(nearpos.x), std::floor(nearpos.y), std::floor(nearpos.z));
+    const int base[3] = {int(fl.x), int(fl.y), int(fl.z)};
+    const F3 frac = nearpos - fl;
+    float total_weight = 0.0f;
+    F4 result{{0.f, 0.f, 0.f, 0.f}};
+    for (int i = 0; i < 8; i++) {
+        const int off[3] = {i & 1, (i >> 1) & 1, (i >> 2) & 1};
+        const int px = base[0] + off[0], py = base[1] + off[1], pz = base[2] + off[2];
+        const size_t idx = size_t(py) * XY * XY + size_t(pz) * XY + size_t(px);
+        const CubeO& cube = pr.cubes[idx];
+        const VoxelO& voxel = pr.vox[idx];
+        uint32_t u0[4], u1[4];
+        unpack_bytes(voxel.dist0, u0);
+        unpack_bytes(voxel.dist1, u1);
+        float d0[4], d1[4];
+        for (int k = 0; k < 4; ++k) { d0[k] = float(u0[k]) / 255.0f; d1[k] = float(u1[k]) / 255.0f; }
+        if (d0[1] < 0.01f) continue;
+        const F3 to_local = frac - F3(float(off[0]), float(off[1]), float(off[2]));
+        const float dist_to_probe = length(to_local);
+        const F3 dir = normalize(to_local);
+        const float dist_pz = d0[2], dist_nz = d0[3], dist_px = d1[0], dist_nx = d1[1], dist_py = d1[2],
+                    dist_ny = d1[3];
+        const float hit_length = std::sqrt(std::fmax(dir.x, 0.0f)) * dist_px + std::sqrt(std::fmax(-dir.x, 0.0f)) * dist_nx +
+                                 std::sqrt(std::fmax(dir.y, 0.0f)) * dist_py + std::sqrt(std::fmax(-dir.y, 0.0f)) * dist_ny +
+                                 std::sqrt(std::fmax(dir.z, 0.0f)) * dist_pz + std::sqrt(std::fmax(-dir.z, 0.0f)) * dist_nz;
+        if (dist_to_probe > hit_length + 0.05f) continue;
+        const float wx = off[0] == 0 ? (1.0f - frac.x) : frac.x;
+        const float wy = off[1] == 0 ? (1.0f - frac.y) : frac.y;
+        const float wz = off[2] == 0 ? (1.0f - frac.z) : frac.z;
+        const float weight = wx * wy * wz;
+        const F4 sc = sample_cube(cube, normal, full);
+        for (int k = 0; k < 4; ++k) result.v[k] = result.v[k] + sc.v[k] * weight;
+        total_weight += weight;
+    }
+    if (!(total_weight > 0.0f)) return F4{{0.f, 0.f, 0.f, 0.f}};
+    for (int k = 0; k < 4; ++k) result.v[k] = result.v[k] / total_weight;
+    return result;
+}
+
+F3 probe_origin(const ProbeGridO& g, uint32_t idx) {
+    const uint32_t y = idx / (g.size_xy * g.size_xy);
+    const uint32_t z = (idx - y * g.size_xy * g.size_xy) / g.size_xy;
+    const uint32_t x = idx - y * g.size_xy * g.size_xy - z * g.size_xy;
+    return F3(float(x) * g.unit + g.offset[0], float(y) * g.unit + g.offset[1], float(z) * g.unit + g.offset[2]);
+}
+
+// InsideGeometry (CPUAccelerationStructure.cpp:85-110); the hit normal is the triangle's
+// geometric normal taken to world space by the instance transform (:50-58)
+bool inside_geometry(const Scene& sc, float unit, F3 origin, F3 ray_dir, VoxelO& cube, float& distance) {
+    Hit h;
+    if (trace_closest(sc, origin, ray_dir, 0.0f, unit * 64.f, h)) {
+        distance = h.t;
+        if (distance <= unit) {
+            const uint32_t mat = material_of(sc, h.inst, h.prim);
+            cube.mat_id = mat;
+            const Instance& I = sc.inst[h.inst];
+            const Mesh& m = sc.meshes[I.mesh_id];
+            const F3 n = xform_vector(I.world, normalize(cross(m.e1[h.prim], m.e2[h.prim])));
+            if (dot(n, ray_dir) > 0.0f || sc.mats[mat].material_model == MAT_DIFFUSE_LIGHT) {
+                distance = 0;
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+// DetectDistance (CPUAccelerationStructure.cpp:72-83)
+float detect_distance(const Scene& sc, float unit, F3 origin, F3 dir) {
+    Hit h;
+    if (trace_closest(sc, origin, normalize(dir), 0.0f, unit * 64.f, h)) return h.t;
+    return 255;
+}
+
+// VoxelizeCube (CPUAccelerationStructure.cpp:112-160)
+void voxelize_cube(const Scene& sc, float unit, VoxelO& cube, F3 origin) {
+    cube.age = 0;
+    cube.mat_id = 0;
+    float dist_py = 255.0f, dist_ny = 255.0f, dist_px = 255.0f, dist_nx = 255.0f, dist_pz = 255.0f, dist_nz = 255.0f;
+    inside_geometry(sc, unit, origin, F3(0, 1, 0), cube, dist_py);
+    inside_geometry(sc, unit, origin, F3(0, -1, 0), cube, dist_ny);
+    inside_geometry(sc, unit, origin, F3(1, 0, 0), cube, dist_px);
+    inside_geometry(sc, unit, origin, F3(-1, 0, 0), cube, dist_nx);
+    inside_geometry(sc, unit, origin, F3(0, 0, 1), cube, dist_pz);
+    inside_geometry(sc, unit, origin, F3(0, 0, -1), cube, dist_nz);
+    float min_dist = std::min({dist_py, dist_ny, dist_px, dist_nx, dist_pz, dist_nz});
+    if (min_dist > 254.0f) {
+        min_dist = std::min(min_dist, detect_distance(sc, unit, origin, F3(1, 1, 1)));
+        min_dist = std::min(min_dist, detect_distance(sc, unit, origin, F3(-1, 1, 1)));
+        min_dist = std::min(min_dist, detect_distance(sc, unit, origin, F3(-1, -1, 1)));
+        min_dist = std::min(min_dist, detect_distance(sc, unit, origin, F3(-1, 1, 1)));
+        min_dist = std::min(min_dist, detect_distance(sc, unit, origin, F3(1, 1, -1)));
+        min_dist = std::min(min_dist, detect_distance(sc, unit, origin, F3(-1, 1, -1)));
+        min_dist = std::min(min_dist, detect_distance(sc, unit, origin, F3(-1, -1, -1)));
+        min_dist = std::min(min_dist, detect_distance(sc, unit, origin, F3(-1, 1, -1)));
+    }
+    auto fclamp = [](float x) { return std::fmin(std::fmax(x, 0.0f), 1.0f); };
+    dist_py = fclamp(dist_py / unit);
+    dist_ny = fclamp(dist_ny / unit);
+    dist_px = fclamp(dist_px / unit);
+    dist_nx = fclamp(dist_nx / unit);
+    dist_pz = fclamp(dist_pz / unit);
+    dist_nz = fclamp(dist_nz / unit);
+    const float inside = dist_py * dist_ny * dist_px * dist_nx * dist_pz * dist_nz;
+    const uint32_t b0[4] = {uint32_t(min_dist / unit), uint32_t(inside * 255.0f), uint32_t(dist_pz * 255.0f),
+                            uint32_t(dist_nz * 255.0f)};
+    const uint32_t b1[4] = {uint32_t(dist_px * 255.0f), uint32_t(dist_nx * 255.0f), uint32_t(dist_py * 255.0f),
+                            uint32_t(dist_ny * 255.0f)};
+    cube.dist0 = pack_bytes(b0);
+    cube.dist1 = pack_bytes(b1);
+}
+
+struct LightABI {  // gk_light / LightObject
+    float p0[4], p1[4], p3[4], normal_area[4];
+    uint32_t light_mat_idx, reserved[3];
+};
+
+// FGpuProbeGenerator (AmbientCube.slang:472-690) over a read-only snapshot of the grid
+struct ProbeBaker {
+    const Scene& sc;
+    const FrameParams& P;
+    const LightABI* light;
+    ProbeRef old;  // the grid as it was when the pass started
+
+    // FaceTask (AmbientCube.slang:482-566)
+    void face_task(F3 origin, F3 basis, uint32_t iterate, uint32_t& direct_light, uint32_t& indirect_light,
+                   uint32_t& sky_vis_out, uint32_t& sun_vis_out) const {
+        static const float grid3x3[9][2] = {{-0.667f, -0.667f}, {0.0f, -0.667f}, {0.667f, -0.667f},
+                                            {-0.667f, 0.0f},    {0.0f, 0.0f},    {0.667f, 0.0f},
+                                            {-0.667f, 0.667f},  {0.0f, 0.667f},  {0.667f, 0.667f}};
+        static const float grid4x4[16][2] = {
+            {-0.75f, -0.75f}, {-0.25f, -0.75f}, {0.25f, -0.75f}, {0.75f, -0.75f}, {-0.75f, -0.25f}, {-0.25f, -0.25f},
+            {0.25f, -0.25f},  {0.75f, -0.25f},  {-0.75f, 0.25f}, {-0.25f, 0.25f}, {0.25f, 0.25f},   {0.75f, 0.25f},
+            {-0.75f, 0.75f},  {-0.25f, 0.75f},  {0.25f, 0.75f},  {0.75f, 0.75f}};
+        const float unit = old.grid.unit;
+        origin = origin + (basis * unit) * 0.25f;
+        F4 direct_color{{0, 0, 0, 0}}, bounce_color{{0, 0, 0, 0}};
+        float sky_visibility = 0.0f;
+        const float ox = grid3x3[iterate % 9][0] * 0.25f, oy = grid3x3[iterate % 9][1] * 0.25f;
+        const F4 sky = P.has_sky ? F4{{P.sky_color[0], P.sky_color[1], P.sky_color[2], P.sky_color[3]}}
+                                 : F4{{0.f, 0.f, 0.f, 0.f}};
+        for (uint32_t i = 0; i < 16; i++) {
+            const F3 hemi = normalize(F3(grid4x4[i][0] + ox, grid4x4[i][1] + oy, 1.0f));
+            const F3 ray_dir = align_with_normal(hemi, basis);
+            Hit h;
+            if (trace_closest(sc, origin, ray_dir, EPS, 20.f, h)) {
+                const Vtx v = hit_vertex(sc, h, origin, ray_dir);
+                const Material& hm = sc.mats[v.mat];
+                const F4 albedo{{hm.diffuse[0], hm.diffuse[1], hm.diffuse[2], hm.diffuse[3]}};
+                const F4 li = interpolate_cubes(old, v.pos, v.nrm, false);
+                F4 t = mul4(albedo, li);
+                for (int k = 0; k < 4; ++k) t.v[k] = t.v[k] * 1.25f;
+                bounce_color = add4(bounce_color, t);
+            } else {
+                direct_color = add4(direct_color, sky);
+                sky_visibility += 1.0f;
+            }
+        }
+        for (int k = 0; k < 4; ++k) {
+            direct_color.v[k] = direct_color.v[k] / 16.0f;
+            bounce_color.v[k] = bounce_color.v[k] / 16.0f;
+        }
+        if (light) {
+            const uint32_t nm = uint32_t(sc.mats.size());
+            const Material& lm = sc.mats[light->light_mat_idx > nm - 1 ? nm - 1 : light->light_mat_idx];
+            const F3 p1(light->p1[0], light->p1[1], light->p1[2]), p3(light->p3[0], light->p3[1], light->p3[2]);
+            const F3 light_pos = p1 + (p3 - p1) * 0.5f;
+            // TraceSegment(origin, lightPos, CUBE_UNIT * 0.5f), Shading.slang:759-782
+            const float e = unit * 0.5f;
+            const F3 seg = light_pos - origin;
+            const float len = length(seg);
+            const F3 sd(seg.x / len, seg.y / len, seg.z / len);
+            const float light_atten = trace_any(sc, origin, sd, e, len - e) ? 0.0f : 1.0f;
+            const F3 light_dir = normalize(light_pos - origin);
+            const float ndotl = std::fmin(std::fmax(dot(basis, light_dir), 0.0f), 1.0f);
+            const float distance = length(light_pos - origin);
+            const float attenuation = ndotl * light->normal_area[3] / (distance * distance * 3.14159f);
+            for (int k = 0; k < 4; ++k) direct_color.v[k] = direct_color.v[k] + lm.diffuse[k] * attenuation * light_atten;
+        }
+        if (P.has_sun) {
+            const F3 sun_dir(P.sun_direction[0], P.sun_direction[1], P.sun_direction[2]);
+            const float sun_atten = trace_any(sc, origin, sun_dir, EPS, PT_MAX_TRACE_DISTANCE) ? 0.0f : 1.0f;
+            const float ndotl = std::fmin(std::fmax(dot(basis, sun_dir), 0.0f), 1.0f);
+            sun_vis_out = sun_atten > 0.0f ? 1 : 0;
+            for (int k = 0; k < 4; ++k)
+                direct_color.v[k] = direct_color.v[k] + P.sun_color[k] * sun_atten * ndotl * 1.0f * 0.25f;
+        }
+        const float curr_weight = 0.125f;
+        const float sv_old = float(sky_vis_out);
+        sky_vis_out = uint32_t(sv_old + (255.0f * sky_visibility / 16.0f - sv_old) * curr_weight);
+        direct_light = pack_rgb10a2(lerp4(unpack_rgb10a2(direct_light), direct_color, curr_weight));
+        indirect_light = pack_rgb10a2(lerp4(unpack_rgb10a2(indirect_light), bounce_color, curr_weight));
+    }
+
+    // RenderAmbientCubeOnly (AmbientCube.slang:650-690) writing into out (a copy of the
+    // probe's old cube) and vox
+    void render(uint32_t idx, CubeO& out, VoxelO& vox) const {
+        uint32_t up[4];
+        unpack_bytes(vox.dist0, up);
+        if (up[0] >= 8) return;
+        const uint32_t iterate = vox.age;
+        vox.age = vox.age + 1;
+        uint32_t s0[4], s1[4];
+        unpack_bytes(out.sky0, s0);
+        unpack_bytes(out.sky1, s1);
+        uint32_t sunvis = 0;
+        const F3 origin = probe_origin(old.grid, idx);
+        face_task(origin, F3(0, 1, 0), iterate, out.pos_y_d, out.pos_y, s0[2], s1[2]);
+        face_task(origin, F3(0, -1, 0), iterate, out.neg_y_d, out.neg_y, s0[3], sunvis);
+        face_task(origin, F3(1, 0, 0), iterate, out.pos_x_d, out.pos_x, s1[0], sunvis);
+        face_task(origin, F3(-1, 0, 0), iterate, out.neg_x_d, out.neg_x, s1[1], sunvis);
+        face_task(origin, F3(0, 0, 1), iterate, out.pos_z_d, out.pos_z, s0[0], sunvis);
+        face_task(origin, F3(0, 0, -1), iterate, out.neg_z_d, out.neg_z, s0[1], sunvis);
+        out.sky0 = pack_bytes(s0);
+        out.sky1 = pack_bytes(s1);
+    }
+};
+
 // ---------------------------------------------------------------- REF_V2 per pixel
 struct RefOut {
     float diffuse[4], spec[4], albedo[4], normal[4];
@@ -399,8 +704,7 @@ struct RefV2 {
                     }
                     const bool early = mat.material_model != MAT_DIELECTRIC && rnd1(seed) < P.exit_probability;
                     if (b == maxb - 1 || early) {
-                        // probe termination: no baked probes -> (0,0,0,1)
-                        color.v[0] = 0.f; color.v[1] = 0.f; color.v[2] = 0.f;
+                        color = mul4(color, interpolate_cubes(sc.probes, vs.pos, vs.nrm, true));
                         break;
                     }
                 }
@@ -1016,6 +1320,52 @@ void oracle_primary_rays(const FrameParams* P, uint32_t W, uint32_t H, float* ra
             r[0] = O.x; r[1] = O.y; r[2] = O.z; r[3] = 0.f;
             r[4] = D.x; r[5] = D.y; r[6] = D.z; r[7] = 1e30f;
         }
+}
+
+// ---------------------------------------------------------------- probe exports
+// Voxelizes probes [first, first+count) of grid g into vox (indexed by probe).
+void oracle_probe_voxelize(void* s, const ProbeGridO* g, VoxelO* vox, uint32_t first, uint32_t count,
+                           uint32_t nthreads) {
+    const Scene& sc = *static_cast<Scene*>(s);
+    parallel_rows(count, nthreads, [&](uint32_t i) {
+        const uint32_t idx = first + i;
+        voxelize_cube(sc, g->unit, vox[idx], probe_origin(*g, idx));
+    });
+}
+
+// One bake pass over probes [first, first+count): cubes/vox are the whole grid, updated in
+// place; light may be NULL (no LightObject).
+void oracle_probe_bake(void* s, const ProbeGridO* g, const FrameParams* P, const LightABI* light, CubeO* cubes,
+                       VoxelO* vox, uint32_t first, uint32_t count, uint32_t nthreads) {
+    const Scene& sc = *static_cast<Scene*>(s);
+    const size_t n = size_t(g->size_xy) * g->size_xy * g->size_z;
+    std::vector<CubeO> snapshot(cubes, cubes + n);
+    std::vector<VoxelO> vsnap(vox, vox + n);
+    ProbeBaker B{sc, *P, light, ProbeRef{*g, snapshot.data(), vsnap.data()}};
+    parallel_rows(count, nthreads, [&](uint32_t i) {
+        const uint32_t idx = first + i;
+        B.render(idx, cubes[idx], vox[idx]);
+    });
+}
+
+// interpolateAmbientCubesV2 at n points (pos, nrm: 3 floats each; out: 4 floats each)
+void oracle_probe_lookup(const ProbeGridO* g, const CubeO* cubes, const VoxelO* vox, const float* pos,
+                         const float* nrm, uint32_t n, int full, float* out) {
+    const ProbeRef pr{*g, cubes, vox};
+    for (uint32_t i = 0; i < n; ++i) {
+        const F4 r = interpolate_cubes(pr, F3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]),
+                                       F3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]), full != 0);
+        std::memcpy(out + 4 * i, r.v, 16);
+    }
+}
+
+uint32_t oracle_probe_pack(const float* c) { return pack_rgb10a2(F4{{c[0], c[1], c[2], c[3]}}); }
+void oracle_probe_unpack(uint32_t p, float* c) { const F4 r = unpack_rgb10a2(p); std::memcpy(c, r.v, 16); }
+
+// REF_V2 termination reads this grid (arrays must outlive the renders); cubes NULL detaches.
+void oracle_scene_set_probes(void* s, const ProbeGridO* g, const CubeO* cubes, const VoxelO* vox) {
+    Scene& sc = *static_cast<Scene*>(s);
+    sc.probes = cubes ? ProbeRef{*g, cubes, vox} : ProbeRef{};
 }
 
 // REF_V2 frame over the pixel rectangle [x0,x0+w) x [y0,y0+h) of a W x H image.

@@ -54,6 +54,12 @@ def oracle():
             "oracle_ptmis_spec_pdf": (C.c_double, [C.c_double, _P, _P]),
             "oracle_num_lights": (_U, [_P]),
             "oracle_diff_render": (None, [_P, _P, _P, _U, _U, _U, _U, _U, _U, _U, _U, _P, _U]),
+            "oracle_probe_voxelize": (None, [_P, _P, _P, _U, _U, _U]),
+            "oracle_probe_bake": (None, [_P, _P, _P, _P, _P, _P, _U, _U, _U]),
+            "oracle_probe_lookup": (None, [_P, _P, _P, _P, _P, _U, C.c_int, _P]),
+            "oracle_probe_pack": (_U, [_P]),
+            "oracle_probe_unpack": (None, [_U, _P]),
+            "oracle_scene_set_probes": (None, [_P, _P, _P, _P]),
         }
         optional = {
             "oracle_reproject": (None, [_P, _U, _U] + [_P] * 12),
@@ -183,6 +189,55 @@ class OracleScene:
         rays = oracle().oracle_render_ptmis(self.h, C.addressof(params), W, H, x0, y0, w, h, frame0, frames,
                                             int(fp64), ptr(rgb), ptr(d), ptr(s), threads or os.cpu_count())
         return {"rgb": rgb, "diffuse": d, "spec": s, "rays": rays}
+
+
+# ---------------------------------------------------------------- probes
+class ProbeGridO(C.Structure):
+    _fields_ = [("size_xy", _U), ("size_z", _U), ("unit", C.c_float), ("offset", C.c_float * 3)]
+
+
+class OracleProbes:
+    """Probe grid arrays (numpy records of the product's CUBE_DTYPE / VOXEL_DTYPE layout)
+    with the oracle's voxelize / bake / lookup."""
+
+    def __init__(self, size_xy, size_z, unit, offset, cube_dtype, voxel_dtype):
+        self.g = ProbeGridO(size_xy, size_z, unit, (C.c_float * 3)(*offset))
+        self.n = size_xy * size_xy * size_z
+        self.cubes = np.zeros(self.n, cube_dtype)
+        self.vox = np.zeros(self.n, voxel_dtype)
+
+    def voxelize(self, scene: OracleScene, threads=None):
+        oracle().oracle_probe_voxelize(scene.h, C.addressof(self.g), ptr(self.vox), 0, self.n,
+                                       threads or os.cpu_count())
+
+    def bake(self, scene: OracleScene, params, light=None, first=0, count=None, threads=None):
+        n = self.n - first if count is None else count
+        oracle().oracle_probe_bake(scene.h, C.addressof(self.g), C.addressof(params),
+                                   None if light is None else C.addressof(light), ptr(self.cubes), ptr(self.vox),
+                                   first, n, threads or os.cpu_count())
+
+    def lookup(self, pos, nrm, full=True):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+        out = np.zeros((pos.shape[0], 4), np.float32)
+        oracle().oracle_probe_lookup(C.addressof(self.g), ptr(self.cubes), ptr(self.vox), ptr(pos), ptr(nrm),
+                                     pos.shape[0], int(full), ptr(out))
+        return out
+
+    def attach(self, scene: OracleScene):
+        """REF_V2 renders of scene terminate paths with this grid (keep self alive)."""
+        oracle().oracle_scene_set_probes(scene.h, C.addressof(self.g), ptr(self.cubes), ptr(self.vox))
+
+
+def probe_pack(c):
+    a = np.ascontiguousarray(c, np.float32).reshape(4)
+    return oracle().oracle_probe_pack(ptr(a))
+
+
+def probe_unpack(p):
+    out = np.zeros(4, np.float32)
+    oracle().oracle_probe_unpack(int(p), ptr(out))
+    return out
 
 
 def primary_rays(params, W, H):

@@ -83,3 +83,49 @@ def random_rays(n, lo, hi, seed=0, tmin=0.0, tmax=1e30):
     r[:, 4:7] = d
     r[:, 7] = tmax
     return r
+
+
+# ---------------------------------------------------------------- probe-grid test scenes
+def tri_scene(tris, albedo=(0.5, 0.5, 0.5), extra_materials=(), mat=0):
+    """One mesh of the given triangles ((n,3,3) vertex positions), flat normals along
+    cross(v1-v0, v2-v0), all on material mat; material 0 is Lambertian, extra materials
+    follow."""
+    from gknextengine_amd.scenes import material
+    tris = np.asarray(tris, np.float32).reshape(-1, 3, 3)
+    n = np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    s = gk.SceneData()
+    s.add_mesh(tris.reshape(-1, 3), np.arange(3 * len(tris), dtype=np.uint32).reshape(-1, 3),
+               np.repeat(n, 3, axis=0).astype(np.float32), np.full(len(tris), mat, np.uint32))
+    s.materials = [material(albedo, model=gk.MAT_LAMBERTIAN), *extra_materials]
+    s.add_instance(0)
+    return s
+
+
+def floor_tris(L=100.0, y=0.0):
+    """Two triangles of the plane y = const, geometric normal +y."""
+    return [[(-L, y, -L), (-L, y, L), (L, y, L)], [(-L, y, -L), (L, y, L), (L, y, -L)]]
+
+
+def small_tri(center, normal, r=0.2):
+    """A triangle of circumradius r around center, facing along normal."""
+    c = np.asarray(center, np.float64)
+    n = np.asarray(normal, np.float64)
+    n /= np.linalg.norm(n)
+    a = np.cross(n, (0.0, 1.0, 0.0) if abs(n[1]) < 0.9 else (1.0, 0.0, 0.0))
+    a /= np.linalg.norm(a)
+    b = np.cross(n, a)
+    pts = [c + r * (np.cos(t) * a + np.sin(t) * b) for t in (0.0, 2.0944, 4.1888)]
+    return [pts]
+
+
+def quad_light(center, half, area, mat_idx):
+    """A LightObject quad at center (y = const plane) whose p1/p3 midpoint is center."""
+    L = gk.Light()
+    cx, cy, cz = center
+    L.p0[:] = [cx - half, cy, cz - half, 1.0]
+    L.p1[:] = [cx - half, cy, cz + half, 1.0]
+    L.p3[:] = [cx + half, cy, cz - half, 1.0]
+    L.normal_area[:] = [0.0, -1.0, 0.0, area]
+    L.light_mat_idx = mat_idx
+    return L
