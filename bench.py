@@ -15,7 +15,8 @@ TLAS refit every frame, 4 bounces, reprojection + bilateral, 120 frames) and ``c
 (config 1: 256x256, 16 frames x 1 spp, 4 bounces), ``diff`` (config 4: path-replay adjoint
 at 512x512, 4 spp, with its finite-difference and 4096-spp reference-gradient checks) and
 ``denoise`` (config 5: learned vs bilateral denoiser on 1-spp 1080p frames of the city and
-the Cornell box). ``--relmse N``
+the Cornell box), ``probes`` (the ambient-cube probe grid: voxelize and bake over the
+city). ``--relmse N``
 adds the relMSE of the measured frame against an N-spp FP64-accumulated render of the
 same frame (``denoise`` defaults to N = 4096).
 """
@@ -584,6 +585,88 @@ def run_denoise(args):
                        "resolution": [W, H], **{f"{k}_{m}": v for k, d in res.items() for m, v in d.items()}}}
 
 
+def run_probes(args):
+    """Ambient-cube probe grid on the city: the reference's 192 x 48 x 192 grid (0.25 m),
+    device voxelization of every probe (14 rays each), full bake passes (96 hemisphere rays
+    and one sun ray per face of every probe within 2 m of a surface) and the renderer's
+    per-frame slice (1/120 of the grid, BakeSpeedLevel 1)."""
+    import numpy as np
+
+    import gknextengine_amd as gk
+    from gknextengine_amd import scenes
+    log("building city scene")
+    scene = gk.procgen_city(64, 4, 256, 64, 2024)
+    ctx = gk.Context(64, 64)
+    ctx.upload(scene)
+    cam = scenes.CITY_CAM
+    p = gk.make_frame_params(cam["eye"], cam["target"], 64, 64, fov=cam["fov"])
+    p.estimator = gk.ESTIMATOR_REF_V2
+    p.has_sky = 1
+    p.sky_color[:] = [60.0, 80.0, 120.0, 1.0]
+    p.has_sun = 1
+    p.sun_direction[:] = [0.36, 0.86, -0.36, 0.0]
+    p.sun_color[:] = [400.0, 380.0, 340.0, 1.0]
+    ctx.frame(p)
+    ctx.probes_init()
+    ctx.probes_voxelize()
+    ctx.probes_bake()
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.probes_voxelize()
+    ctx.sync()
+    t_vox = time.perf_counter() - t0
+    _, vox = ctx.probes_read()
+    n = len(vox)
+    active = int(((vox["dist_gg_z01"] & 0xFF) < 8).sum())
+    rays_pass = active * (6 * 16 + 6 * 1)      # hemisphere rays + one sun ray per face
+    for _ in range(args.warmup):
+        ctx.probes_bake()
+    ctx.sync()
+    s0 = ctx.stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.probes_bake()
+    ctx.sync()
+    t_bake = (time.perf_counter() - t0) / args.steps
+    s1 = ctx.stats()
+    trace_ms = (s1.ms_trace - s0.ms_trace) / args.steps
+    rays_counted = (s1.rays_traced - s0.rays_traced) // args.steps
+    t0 = time.perf_counter()
+    for f in range(120):
+        ctx.probes_bake_frame(f, 120)
+    ctx.sync()
+    t_slice = (time.perf_counter() - t0) / 120
+    ctx.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle as po
+        osc = po.OracleScene(scene)
+        G = po.OracleProbes(*gk.REFERENCE_PROBE_GRID, gk.CUBE_DTYPE, gk.VOXEL_DTYPE)
+        G.vox[:] = vox
+        G.vox["age"] = 0
+        act_idx = np.nonzero((vox["dist_gg_z01"] & 0xFF) < 8)[0]
+        first = int(act_idx[len(act_idx) // 2])
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        G.bake(osc, p, None, first=first, count=16384, threads=threads)
+        dt = time.perf_counter() - t0
+        a = int(((vox["dist_gg_z01"][first:first + 16384] & 0xFF) < 8).sum())
+        cpu = {"value": round(a * 102 / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"oracle bake of probes [{first}, {first + 16384}) ({a} baked), city, reference grid"}
+    mrays = rays_pass / t_bake / 1e6
+    return {"metric": METRIC, "value": round(mrays, 1), "unit": "Mrays/s (probe bake)", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_bake * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": "ambient-cube probe grid 192x48x192 @0.25 m over the 0.92M-triangle city",
+                       "probes": n, "baked_probes": active, "rays_per_pass": rays_pass,
+                       "rays_counted_per_pass": int(rays_counted),
+                       "voxelize_ms": round(t_vox * 1e3, 3), "voxelize_mrays_per_s": round(14 * n / t_vox / 1e6, 1),
+                       "bake_pass_ms": round(t_bake * 1e3, 3), "bake_trace_ms": round(trace_ms, 3),
+                       "frame_slice_ms": round(t_slice * 1e3, 3)},
+            "cpu_baseline": cpu}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -592,7 +675,7 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--bounces", type=int, default=8)
-    ap.add_argument("--config", default="city", choices=["city", "dynamic", "cornell", "denoise", "diff"])
+    ap.add_argument("--config", default="city", choices=["city", "dynamic", "cornell", "denoise", "diff", "probes"])
     ap.add_argument("--relmse", type=int, default=0, help="reference spp for relMSE (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sort", default="off", choices=["on", "off"], help="material-ordered shading")
@@ -605,7 +688,7 @@ def main():
               "shard).", file=sys.stderr)
         sys.exit(2)
     run = {"city": run_city, "dynamic": run_dynamic, "cornell": run_cornell, "denoise": run_denoise,
-           "diff": run_diff}[args.config]
+           "diff": run_diff, "probes": run_probes}[args.config]
     print(json.dumps(run(args)), flush=True)
 
 

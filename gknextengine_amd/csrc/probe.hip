@@ -35,7 +35,7 @@ int collect_timing(Ctx& c);
 
 namespace {
 
-constexpr uint32_t kChunk = 65536;          // probes per chunk
+constexpr uint32_t kChunk = 262144;         // probes per chunk (bake: 25 M rays)
 constexpr uint32_t kVoxRays = 14;            // 6 axis + 8 diagonal rays per probe
 constexpr uint32_t kFaceRays = 16;           // FACE_TRACING (AmbientCube.slang:27,57)
 constexpr uint32_t kProbeRays = 6 * kFaceRays;
@@ -275,24 +275,25 @@ __device__ inline void face_sky_slot(uint32_t f, uint32_t& word, uint32_t& shift
     }
 }
 
-// FaceTask's shading and accumulation (AmbientCube.slang:497-566) for one (probe, face)
-__global__ void k_bake_face(DScene s, BakeK K, ProbeDev P, const gk_voxel* __restrict__ vox,
-                            const uint32_t* __restrict__ list, const uint32_t* __restrict__ counters,
-                            const float4* __restrict__ ro, const float4* __restrict__ rd,
-                            const float4* __restrict__ hit, const uint32_t* __restrict__ hit_inst,
-                            const float4* __restrict__ shit, gk_ambient_cube* __restrict__ next,
-                            uint32_t* __restrict__ face_sky) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= counters[0] * 6u) return;
-    const uint32_t sidx = t / 6u, f = t - sidx * 6u;
-    const uint32_t idx = list[sidx];
-    const f3 basis = face_basis(f);
-    const f3 o = face_origin(K.g, idx, basis);
-    float4 direct = make_float4(0.f, 0.f, 0.f, 0.f), bounce = make_float4(0.f, 0.f, 0.f, 0.f);
-    float skyv = 0.f;
-    const float4 sky = K.has_sky ? make_float4(K.sky[0], K.sky[1], K.sky[2], K.sky[3]) : make_float4(0.f, 0.f, 0.f, 0.f);
-    for (uint32_t i = 0; i < kFaceRays; ++i) {
-        const size_t r = size_t(sidx) * kProbeRays + f * kFaceRays + i;
+// FaceTask's shading and accumulation (AmbientCube.slang:497-566). One thread per ray
+// shades its hit (albedo x direct-light lookup x 1.25) or miss (sky); the first thread of
+// each 16-ray face then sums the face's terms in ray order, as the reference's loop does,
+// adds the light and sun terms and blends the face into the new grid.
+__global__ __launch_bounds__(256) void k_bake_face(DScene s, BakeK K, ProbeDev P, const gk_voxel* __restrict__ vox,
+                                                   const uint32_t* __restrict__ list,
+                                                   const uint32_t* __restrict__ counters,
+                                                   const float4* __restrict__ ro, const float4* __restrict__ rd,
+                                                   const float4* __restrict__ hit,
+                                                   const uint32_t* __restrict__ hit_inst,
+                                                   const float4* __restrict__ shit, gk_ambient_cube* __restrict__ next,
+                                                   uint32_t* __restrict__ face_sky) {
+    __shared__ float4 s_bounce[256], s_direct[256];
+    __shared__ float s_miss[256];
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = r < counters[0] * kProbeRays;
+    float4 b = make_float4(0.f, 0.f, 0.f, 0.f), d = make_float4(0.f, 0.f, 0.f, 0.f);
+    float miss = 0.f;
+    if (live) {
         const float4 h = hit[r];
         if (__float_as_uint(h.w) != kMiss) {
             const uint32_t inst = s.two_level ? hit_inst[r] : 0u;
@@ -301,16 +302,32 @@ __global__ void k_bake_face(DScene s, BakeK K, ProbeDev P, const gk_voxel* __res
             hit_vertex(s, inst, prim, h.x, h.y, h.z, ld3(ro[r]), ld3(rd[r]), pos, nrm);
             const gk_material& m = s.mats[material_of(s, inst, prim)];
             const float4 li = probe_lookup<false>(P, pos, nrm);
-            bounce.x = bounce.x + (m.diffuse[0] * li.x) * 1.25f;
-            bounce.y = bounce.y + (m.diffuse[1] * li.y) * 1.25f;
-            bounce.z = bounce.z + (m.diffuse[2] * li.z) * 1.25f;
-            bounce.w = bounce.w + (m.diffuse[3] * li.w) * 1.25f;
+            b = make_float4((m.diffuse[0] * li.x) * 1.25f, (m.diffuse[1] * li.y) * 1.25f,
+                            (m.diffuse[2] * li.z) * 1.25f, (m.diffuse[3] * li.w) * 1.25f);
         } else {
-            direct.x = direct.x + sky.x; direct.y = direct.y + sky.y;
-            direct.z = direct.z + sky.z; direct.w = direct.w + sky.w;
-            skyv = skyv + 1.0f;
+            if (K.has_sky) d = make_float4(K.sky[0], K.sky[1], K.sky[2], K.sky[3]);
+            miss = 1.f;
         }
     }
+    s_bounce[threadIdx.x] = b;
+    s_direct[threadIdx.x] = d;
+    s_miss[threadIdx.x] = miss;
+    __syncthreads();
+    if (!live || (threadIdx.x & (kFaceRays - 1u)) != 0u) return;
+    // x + 0 = x: adding the other kind's zero keeps the reference's conditional sums
+    float4 direct = make_float4(0.f, 0.f, 0.f, 0.f), bounce = make_float4(0.f, 0.f, 0.f, 0.f);
+    float skyv = 0.f;
+    for (uint32_t i = 0; i < kFaceRays; ++i) {
+        const float4 bi = s_bounce[threadIdx.x + i], di = s_direct[threadIdx.x + i];
+        bounce = make_float4(bounce.x + bi.x, bounce.y + bi.y, bounce.z + bi.z, bounce.w + bi.w);
+        direct = make_float4(direct.x + di.x, direct.y + di.y, direct.z + di.z, direct.w + di.w);
+        skyv = skyv + s_miss[threadIdx.x + i];
+    }
+    const uint32_t t = r / kFaceRays;  // (probe, face)
+    const uint32_t sidx = t / 6u, f = t - sidx * 6u;
+    const uint32_t idx = list[sidx];
+    const f3 basis = face_basis(f);
+    const f3 o = face_origin(K.g, idx, basis);
     const float nf = float(kFaceRays);
     direct = make_float4(direct.x / nf, direct.y / nf, direct.z / nf, direct.w / nf);
     bounce = make_float4(bounce.x / nf, bounce.y / nf, bounce.z / nf, bounce.w / nf);
@@ -450,15 +467,16 @@ int probes_voxelize(Ctx& c) {
     if (P.count == 0 || !c.scene_ready) return GK_ERR_STATE;
     int st = reserve_trace_events(c);
     if (st != GK_OK) return st;
-    const size_t nr = size_t(kChunk) * kVoxRays;
+    const uint32_t chunk = std::min(kChunk, P.count);
+    const size_t nr = size_t(chunk) * kVoxRays;
     GK_HIP(P.ro.resize(std::max(P.ro.count, nr)));
     GK_HIP(P.rd.resize(std::max(P.rd.count, nr)));
     GK_HIP(P.hit.resize(std::max(P.hit.count, nr)));
     GK_HIP(P.hit_inst.resize(std::max(P.hit_inst.count, nr)));
     const GridK g = grid_k(c);
     const DScene s = make_dscene(c);
-    for (uint32_t first = 0; first < P.count; first += kChunk) {
-        const uint32_t n = std::min(kChunk, P.count - first);
+    for (uint32_t first = 0; first < P.count; first += chunk) {
+        const uint32_t n = std::min(chunk, P.count - first);
         k_vox_rays<<<nblk(size_t(n) * kVoxRays), 256, 0, c.stream>>>(g, first, n, P.ro.ptr, P.rd.ptr);
         GK_HIP(hipGetLastError());
         st = launch_trace(c, P.ro.ptr, P.rd.ptr, nullptr, n * kVoxRays, P.hit.ptr, P.hit_inst.ptr, false, nullptr);
@@ -497,22 +515,23 @@ int probes_bake(Ctx& c, uint32_t first, uint32_t count) {
         K.light_mat = L.light_mat_idx > nm - 1u ? nm - 1u : L.light_mat_idx;
     }
     K.nany = K.has_light + K.has_sun;
-    GK_HIP(P.list.resize(kChunk));
+    const uint32_t chunk = std::min(kChunk, count);
+    GK_HIP(P.list.resize(std::max<size_t>(P.list.count, chunk)));
     GK_HIP(P.counters.resize(4));
-    GK_HIP(P.face_sky.resize(size_t(kChunk) * 6));
-    const size_t nr = size_t(kChunk) * kProbeRays;
+    GK_HIP(P.face_sky.resize(std::max<size_t>(P.face_sky.count, size_t(chunk) * 6)));
+    const size_t nr = size_t(chunk) * kProbeRays;
     GK_HIP(P.ro.resize(std::max(P.ro.count, nr)));
     GK_HIP(P.rd.resize(std::max(P.rd.count, nr)));
     GK_HIP(P.hit.resize(std::max(P.hit.count, nr)));
     GK_HIP(P.hit_inst.resize(std::max(P.hit_inst.count, nr)));
-    const size_t ns = size_t(kChunk) * 6u * std::max(1u, K.nany);
-    GK_HIP(P.so.resize(ns));
-    GK_HIP(P.sd.resize(ns));
-    GK_HIP(P.shit.resize(ns));
+    const size_t ns = size_t(chunk) * 6u * std::max(1u, K.nany);
+    GK_HIP(P.so.resize(std::max(P.so.count, ns)));
+    GK_HIP(P.sd.resize(std::max(P.sd.count, ns)));
+    GK_HIP(P.shit.resize(std::max(P.shit.count, ns)));
     const DScene s = make_dscene(c);
     const ProbeDev PD = probe_dev(c);
-    for (uint32_t c0 = first; c0 < first + count; c0 += kChunk) {
-        const uint32_t n = std::min(kChunk, first + count - c0);
+    for (uint32_t c0 = first; c0 < first + count; c0 += chunk) {
+        const uint32_t n = std::min(chunk, first + count - c0);
         k_zero_u32<<<1, 64, 0, c.stream>>>(P.counters.ptr, 4);
         k_bake_select<<<nblk(n), 256, 0, c.stream>>>(P.voxels.ptr, c0, n, P.list.ptr, P.counters.ptr);
         k_bake_counts<<<1, 1, 0, c.stream>>>(P.counters.ptr, K.nany);
@@ -527,7 +546,7 @@ int probes_bake(Ctx& c, uint32_t first, uint32_t count) {
             st = launch_trace(c, P.ro.ptr, P.rd.ptr, P.counters.ptr + 1, n * kProbeRays, P.hit.ptr, P.hit_inst.ptr,
                               false, nullptr);
         if (st != GK_OK) return st;
-        k_bake_face<<<nblk(size_t(n) * 6u), 256, 0, c.stream>>>(s, K, PD, P.voxels.ptr, P.list.ptr, P.counters.ptr,
+        k_bake_face<<<nblk(size_t(n) * kProbeRays), 256, 0, c.stream>>>(s, K, PD, P.voxels.ptr, P.list.ptr, P.counters.ptr,
                                                                  P.ro.ptr, P.rd.ptr, P.hit.ptr, P.hit_inst.ptr,
                                                                  P.shit.ptr, P.next.ptr, P.face_sky.ptr);
         k_bake_commit<<<nblk(n), 256, 0, c.stream>>>(P.list.ptr, P.counters.ptr, P.cubes.ptr, P.face_sky.ptr,
