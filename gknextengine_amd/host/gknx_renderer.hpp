@@ -12,6 +12,10 @@
 //   UpdateUniformBuffer                 NextEngine::GetUniformBufferObject (src/Runtime/Engine.cpp:736-849)
 //   Render                              PathTracingRenderer::Render (PathTracingRenderer.cpp:104-218)
 //   BeforeNextFrame                     the frame fence wait (VulkanBaseRenderer.cpp:832)
+//   VoxelizeProbes                      FCPUAccelerationStructure::AsyncProcessFull + UploadGPU
+//                                       (src/Assets/CPUAccelerationStructure.cpp:333-431)
+//   BakeProbes                          RayTraceBaseRenderer::PostRender's hw-lightbake slice
+//                                       (src/Rendering/RayTraceBaseRenderer.cpp:225-274)
 //   RayCast                             NextEngine::RayCastGPU / RayCastInCPU (Engine.cpp:684-690)
 // Errors are thrown as std::runtime_error, as the reference's Check(VkResult) does
 // (src/Vulkan/Vulkan.hpp:39). Header-only; link libgknx_rt.so.
@@ -92,6 +96,17 @@ public:
         check(gk_render(ctx()), "gk_render");
     }
     void BeforeNextFrame() { check(gk_sync(ctx()), "gk_sync"); }
+
+    // Ambient-cube probes: grid nullptr = the reference's 192 x 48 x 192 grid.
+    void VoxelizeProbes(const gk_probe_grid* grid = nullptr) {
+        check(gk_probes_init(ctx(), grid), "gk_probes_init");
+        check(gk_probes_voxelize(ctx()), "gk_probes_voxelize");
+    }
+    // BakeSpeedLevel 0/1/2 -> temporal_frames 30/120/300; level 2 does not bake
+    void BakeProbes(uint32_t frame_count, int bake_speed_level = 1) {
+        if (bake_speed_level == 2) return;
+        check(gk_probes_bake_frame(ctx(), frame_count, bake_speed_level == 0 ? 30u : 120u), "gk_probes_bake_frame");
+    }
     void ClearAccumulation() { check(gk_reset_accumulation(ctx()), "gk_reset_accumulation"); }
 
     template <class T>
