@@ -48,7 +48,12 @@ TARGET_FORMAT = {
 class Mesh(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("normals", C.c_void_p), ("texcoords", C.c_void_p),
                 ("indices", C.c_void_p), ("tri_material", C.c_void_p),
-                ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32)]
+                ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("tangents", C.c_void_p)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("srgb", C.c_uint32), ("reserved", C.c_uint32),
+                ("rgba8", C.c_void_p)]
 
 
 class Material(C.Structure):
@@ -122,7 +127,7 @@ class UNetLayer(C.Structure):
 
 
 assert C.sizeof(Material) == 64 and C.sizeof(Instance) == 224 and C.sizeof(FrameParams) == 656
-assert C.sizeof(Light) == 80 and C.sizeof(Mesh) == 48
+assert C.sizeof(Light) == 80 and C.sizeof(Mesh) == 56 and C.sizeof(Texture) == 24
 
 # every entry point declared in include/gknx_rt.h: (name, restype, argtypes)
 _P = C.c_void_p
@@ -136,6 +141,7 @@ SIGNATURES = [
     ("gk_scene_upload", C.c_int, [_P, C.POINTER(Mesh), _U, C.POINTER(Material), _U, C.POINTER(Light), _U,
                                   C.POINTER(Instance), _U]),
     ("gk_materials_update", C.c_int, [_P, C.POINTER(Material), _U, _U]),
+    ("gk_textures_upload", C.c_int, [_P, C.POINTER(Texture), _U]),
     ("gk_instances_update", C.c_int, [_P, _P, _U, _U]),
     ("gk_instances_update_device", C.c_int, [_P, _P, _U, _U]),
     ("gk_get_bvh_stats", C.c_int, [_P, C.POINTER(BvhStats)]),
@@ -243,12 +249,16 @@ class SceneData:
         self.materials: list[Material] = []
         self.instances: list[Instance] = []
         self.lights: list[Light] = []    # LightObject quads (the probe bake uses the first)
+        self.textures: list[dict] = []   # {"rgba": (h, w, 4) uint8, "srgb": bool}
 
-    def add_mesh(self, positions, indices, normals=None, tri_material=None) -> int:
-        m = {"positions": np.ascontiguousarray(positions, np.float32).reshape(-1, 3),
+    def add_mesh(self, positions, indices, normals=None, tri_material=None, texcoords=None, tangents=None) -> int:
+        def arr(a, k):
+            return None if a is None else np.ascontiguousarray(a, np.float32).reshape(-1, k)
+        m = {"positions": arr(positions, 3),
              "indices": np.ascontiguousarray(indices, np.uint32).reshape(-1, 3),
-             "normals": None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3),
-             "tri_material": None if tri_material is None else np.ascontiguousarray(tri_material, np.uint32)}
+             "normals": arr(normals, 3),
+             "tri_material": None if tri_material is None else np.ascontiguousarray(tri_material, np.uint32),
+             "texcoords": arr(texcoords, 2), "tangents": arr(tangents, 4)}
         self.meshes.append(m)
         return len(self.meshes) - 1
 
@@ -273,7 +283,8 @@ class SceneData:
         for i, m in enumerate(self.meshes):
             arr[i].positions = ptr(m["positions"])
             arr[i].normals = ptr(m["normals"])
-            arr[i].texcoords = None
+            arr[i].texcoords = ptr(m.get("texcoords"))
+            arr[i].tangents = ptr(m.get("tangents"))
             arr[i].indices = ptr(m["indices"])
             arr[i].tri_material = ptr(m["tri_material"])
             arr[i].num_vertices = m["positions"].shape[0]
@@ -290,6 +301,21 @@ class SceneData:
         arr = (Instance * len(self.instances))()
         for i, m in enumerate(self.instances):
             arr[i] = m
+        return arr
+
+    def add_texture(self, rgba, srgb=True) -> int:
+        """An RGBA8 texture ((h, w, 4) uint8, rows top to bottom); returns its id."""
+        a = np.ascontiguousarray(rgba, np.uint8)
+        assert a.ndim == 3 and a.shape[2] == 4
+        self.textures.append({"rgba": a, "srgb": bool(srgb)})
+        return len(self.textures) - 1
+
+    def c_textures(self):
+        arr = (Texture * max(1, len(self.textures)))()
+        for i, t in enumerate(self.textures):
+            arr[i].width, arr[i].height = t["rgba"].shape[1], t["rgba"].shape[0]
+            arr[i].srgb = 1 if t["srgb"] else 0
+            arr[i].rgba8 = t["rgba"].ctypes.data
         return arr
 
     def c_lights(self):
@@ -394,9 +420,11 @@ class Context:
     def upload(self, scene: SceneData):
         meshes, mats, insts = scene.c_meshes(), scene.c_materials(), scene.c_instances()
         lights = scene.c_lights()
-        self._keep = (scene, meshes, mats, insts, lights)
+        textures = scene.c_textures()
+        self._keep = (scene, meshes, mats, insts, lights, textures)
         check(lib().gk_scene_upload(self.h, meshes, len(scene.meshes), mats, len(scene.materials), lights,
                                     len(scene.lights), insts, len(scene.instances)), "gk_scene_upload")
+        check(lib().gk_textures_upload(self.h, textures, len(scene.textures)), "gk_textures_upload")
 
     def update_instances(self, worlds: np.ndarray, first: int = 0):
         w = np.ascontiguousarray(np.transpose(np.asarray(worlds, np.float32), (0, 2, 1)))  # column-major

@@ -12,6 +12,11 @@
 
 namespace gk {
 
+// layouts the ABI fixes (the reference's byte-identical records, and the ctypes binding)
+static_assert(sizeof(gk_mesh) == 56 && sizeof(gk_texture) == 24, "mesh / texture records");
+static_assert(sizeof(gk_material) == 64 && sizeof(gk_light) == 80 && sizeof(gk_instance) == 224, "scene records");
+static_assert(sizeof(gk_ambient_cube) == 56 && sizeof(gk_voxel) == 16, "probe records");
+
 namespace {
 thread_local char g_last_error[256] = {0};
 }
@@ -326,6 +331,49 @@ int gk_materials_update(gk_ctx* ctx, const gk_material* materials, uint32_t firs
     std::copy(materials, materials + count, c.h_materials.begin() + first);
     GK_HIP(hipMemcpyAsync(c.materials.ptr + first, materials, sizeof(gk_material) * count, hipMemcpyHostToDevice, c.stream));
     GK_HIP(hipStreamSynchronize(c.stream));
+    return GK_OK;
+}
+
+int gk_textures_upload(gk_ctx* ctx, const gk_texture* textures, uint32_t count) {
+    GK_CHECK_CTX(ctx);
+    if (count && !textures) return GK_ERR_ARG;
+    uint64_t texels = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const gk_texture& t = textures[i];
+        if (t.width == 0 || t.height == 0 || !t.rgba8) return GK_ERR_ARG;
+        if (t.width > 16384 || t.height > 16384) return GK_ERR_LIMIT;
+        texels += uint64_t(t.width) * t.height;
+    }
+    if (texels > (uint64_t(1) << 31)) return GK_ERR_LIMIT;
+    Ctx& c = ctx->c;
+    GK_HIP(hipSetDevice(c.device));
+    GK_HIP(hipStreamSynchronize(c.stream));
+    c.num_tex = 0;
+    if (count == 0) return GK_OK;
+    std::vector<TexInfo> info(count);
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        info[i] = TexInfo{off, textures[i].width, textures[i].height, textures[i].srgb ? 1u : 0u, 0u};
+        off += uint64_t(textures[i].width) * textures[i].height;
+    }
+    GK_HIP(c.texels.resize(std::max<uint64_t>(texels, 1)));
+    GK_HIP(c.tex.resize(count));
+    for (uint32_t i = 0; i < count; ++i)
+        GK_HIP(hipMemcpyAsync(c.texels.ptr + info[i].offset, textures[i].rgba8, 4 * uint64_t(info[i].w) * info[i].h,
+                              hipMemcpyHostToDevice, c.stream));
+    GK_HIP(hipMemcpyAsync(c.tex.ptr, info.data(), sizeof(TexInfo) * count, hipMemcpyHostToDevice, c.stream));
+    // byte -> linear: the sRGB transfer function (as an R8G8B8A8_SRGB view decodes), and
+    // byte / 255 for UNORM; evaluated in double on the host so every device reads the same floats
+    float lut[512];
+    for (int b = 0; b < 256; ++b) {
+        const double x = b / 255.0;
+        lut[b] = float(x <= 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4));
+        lut[256 + b] = float(x);
+    }
+    GK_HIP(c.tex_lut.resize(512));
+    GK_HIP(hipMemcpyAsync(c.tex_lut.ptr, lut, sizeof(lut), hipMemcpyHostToDevice, c.stream));
+    GK_HIP(hipStreamSynchronize(c.stream));
+    c.num_tex = count;
     return GK_OK;
 }
 

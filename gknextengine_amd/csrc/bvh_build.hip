@@ -91,6 +91,24 @@ __global__ void k_prep_tris(const float* __restrict__ pos, const uint32_t* __res
     tri_mat[tri_base + t] = m;
 }
 
+// texture coordinates and tangents of the triangle's vertices (original order); a mesh
+// without them gets zeros
+__global__ void k_prep_tex(const uint32_t* __restrict__ idx, const float* __restrict__ uv,
+                           const float* __restrict__ tan, uint32_t nv, uint32_t nt, uint32_t tri_base,
+                           float2* __restrict__ tri_uv, float4* __restrict__ tri_tan) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    for (int k = 0; k < 3; ++k) {
+        uint32_t a = idx[3 * t + k];
+        if (a >= nv) a = 0;  // reported by k_prep_tris
+        const size_t o = 3 * size_t(tri_base + t) + k;
+        if (tri_uv) tri_uv[o] = uv ? make_float2(uv[2 * a], uv[2 * a + 1]) : make_float2(0.f, 0.f);
+        if (tri_tan)
+            tri_tan[o] = tan ? make_float4(tan[4 * a], tan[4 * a + 1], tan[4 * a + 2], tan[4 * a + 3])
+                             : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
 // ---------------------------------------------------------------- centroid bounds
 __global__ void k_init_bounds(uint32_t* enc) {
     if (threadIdx.x < 3) enc[threadIdx.x] = 0xffffffffu;       // min
@@ -667,13 +685,22 @@ int build_blas(Ctx& c, const gk_mesh* meshes, uint32_t num_meshes) {
     GK_HIP(c.tri_nrm.resize(3 * size_t(std::max<uint32_t>(total, 1))));
     GK_HIP(c.tri_mat.resize(std::max<uint32_t>(total, 1)));
     GK_HIP(c.tri_rec.resize(std::max<uint32_t>(total, 1)));
+    bool any_uv = false, any_tan = false;
+    for (uint32_t m = 0; m < num_meshes; ++m) {
+        any_uv |= meshes[m].texcoords != nullptr;
+        any_tan |= meshes[m].tangents != nullptr;
+    }
+    if (any_uv) GK_HIP(c.tri_uv.resize(3 * size_t(std::max<uint32_t>(total, 1))));
+    else c.tri_uv.release();
+    if (any_tan) GK_HIP(c.tri_tan.resize(3 * size_t(std::max<uint32_t>(total, 1))));
+    else c.tri_tan.release();
     c.max_mesh_tris = 0;
     for (uint32_t m = 0; m < num_meshes; ++m) c.max_mesh_tris = std::max(c.max_mesh_tris, meshes[m].num_triangles);
     size_t node_cap = 0;
     for (uint32_t m = 0; m < num_meshes; ++m) node_cap += meshes[m].num_triangles + 1;
     GK_HIP(c.blas_nodes.resize(std::max<size_t>(node_cap, 1)));
     c.h_meshes.assign(num_meshes, MeshInfo{});
-    DevBuf<float> d_pos, d_nrm;
+    DevBuf<float> d_pos, d_nrm, d_uv, d_tan;
     DevBuf<uint32_t> d_idx, d_mat, d_err;
     DevBuf<TriRec> tmp;
     GK_HIP(d_err.resize(1));
@@ -713,6 +740,20 @@ int build_blas(Ctx& c, const gk_mesh* meshes, uint32_t num_meshes) {
         if (M.tri_material) {
             GK_HIP(d_mat.resize(nt));
             GK_HIP(hipMemcpyAsync(d_mat.ptr, M.tri_material, 4 * size_t(nt), hipMemcpyHostToDevice, st));
+        }
+        if (M.texcoords) {
+            GK_HIP(d_uv.resize(2 * size_t(nv)));
+            GK_HIP(hipMemcpyAsync(d_uv.ptr, M.texcoords, 8 * size_t(nv), hipMemcpyHostToDevice, st));
+        }
+        if (M.tangents) {
+            GK_HIP(d_tan.resize(4 * size_t(nv)));
+            GK_HIP(hipMemcpyAsync(d_tan.ptr, M.tangents, 16 * size_t(nv), hipMemcpyHostToDevice, st));
+        }
+        if (any_uv || any_tan) {
+            k_prep_tex<<<nblk(nt), 256, 0, st>>>(d_idx.ptr, M.texcoords ? d_uv.ptr : nullptr,
+                                                M.tangents ? d_tan.ptr : nullptr, nv, nt, tri_base,
+                                                any_uv ? c.tri_uv.ptr : nullptr, any_tan ? c.tri_tan.ptr : nullptr);
+            GK_HIP(hipGetLastError());
         }
         GK_HIP(tmp.resize(nt));
         GK_HIP(c.scratch.prim_lo.resize(nt));

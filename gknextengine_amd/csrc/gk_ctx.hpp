@@ -67,6 +67,12 @@ struct InstDev {         // per instance, 256 bytes
 };
 static_assert(sizeof(InstDev) == 288, "InstDev size");
 
+struct TexInfo {         // one texture of the pool
+    uint64_t offset;     // first texel in Ctx::texels
+    uint32_t w, h;
+    uint32_t srgb, pad;
+};
+
 struct LightTri {        // emissive triangle in world space, 64 bytes
     float4 p0;           // w = area
     float4 e1;           // w = luminance-weighted selection pdf
@@ -113,6 +119,13 @@ struct Ctx {
     DevBuf<float4> tri_nrm;         // 3 per triangle, original order: n0, n1, n2 (object space)
     DevBuf<uint32_t> tri_mat;       // local material slot per triangle, original order
     DevBuf<uint32_t> tri_rec;       // original order -> leaf-order record index (within the mesh)
+    DevBuf<float2> tri_uv;          // 3 per triangle, original order (empty: no mesh has texcoords)
+    DevBuf<float4> tri_tan;         // 3 per triangle, original order (empty: no mesh has tangents)
+    // textures (gk_textures_upload)
+    DevBuf<uint32_t> texels;        // RGBA8, all textures back to back
+    DevBuf<TexInfo> tex;
+    DevBuf<float> tex_lut;          // [0,256): sRGB byte -> linear, [256,512): byte / 255
+    uint32_t num_tex = 0;
     uint32_t max_mesh_tris = 0;     // largest mesh (visibility-buffer packing limit)
     DevBuf<Bvh4Node> blas_nodes;
     DevBuf<MeshInfo> meshes;
@@ -186,7 +199,7 @@ struct Ctx {
         // REF_V2 extra state
         DevBuf<float4> s_pos, s_nrm;          // current vertex (w: material bits / misc)
         DevBuf<float4> s_dir;                 // current ray direction
-        DevBuf<float4> s_ppos, s_pnrm;        // primary vertex
+        DevBuf<float4> s_ppos, s_pnrm, s_palb;  // primary vertex, its (textured) albedo
         DevBuf<ushort4> hist_d, hist_s, hist_a;  // previous accumulation targets (RGBA16F)
         DevBuf<uint32_t> spill;               // traversal stack overflow
         uint32_t spill_threads = 0;

@@ -22,6 +22,13 @@ struct DScene {
     uint32_t nlights;
     uint32_t blas_root;
     uint32_t two_level;
+    // textures
+    const float2* tri_uv;     // null: no texture coordinates
+    const float4* tri_tan;    // null: no tangents
+    const TexInfo* tex;
+    const uint32_t* texels;
+    const float* tex_lut;
+    uint32_t ntex;
 };
 
 inline DScene make_dscene(const Ctx& c) {
@@ -40,6 +47,12 @@ inline DScene make_dscene(const Ctx& c) {
     s.nlights = c.num_lights;
     s.blas_root = c.h_meshes.empty() ? 0u : c.h_meshes[0].root;
     s.two_level = c.blas_only ? 0u : 1u;
+    s.tri_uv = c.tri_uv.count ? c.tri_uv.ptr : nullptr;
+    s.tri_tan = c.tri_tan.count ? c.tri_tan.ptr : nullptr;
+    s.tex = c.tex.ptr;
+    s.texels = c.texels.ptr;
+    s.tex_lut = c.tex_lut.ptr;
+    s.ntex = c.num_tex;
     return s;
 }
 
@@ -87,6 +100,95 @@ __device__ inline void hit_vertex(const DScene& s, uint32_t inst, uint32_t prim,
     const f3 mix = (n0 + (n1 - n0) * u) + (n2 - n0) * v;
     nrm = normalize(inv_transpose(I.inv, mix));
     pos = O + D * t;
+}
+
+// ---------------------------------------------------------------- textures
+// SampleLevel(uv, 0) with the default sampler (Sampler.hpp:9-26): REPEAT addressing,
+// bilinear between the four texels around u * w - 0.5, v * h - 0.5, each decoded to
+// linear first (sRGB textures through the sRGB transfer function).
+__device__ inline float4 tex_texel(const DScene& s, const TexInfo& t, uint32_t x, uint32_t y) {
+    const uint32_t p = s.texels[t.offset + size_t(y) * t.w + x];
+    const float* lut = s.tex_lut + (t.srgb ? 0 : 256);
+    return make_float4(lut[p & 0xffu], lut[(p >> 8) & 0xffu], lut[(p >> 16) & 0xffu], s.tex_lut[256 + (p >> 24)]);
+}
+__device__ inline void tex_axis(float u, uint32_t n, uint32_t& i0, uint32_t& i1, float& a) {
+    float w = u - floorf(u);
+    if (!(w >= 0.f && w < 1.f)) w = 0.f;  // NaN / infinite coordinates
+    const float x = w * float(n) - 0.5f;
+    const float fx = floorf(x);
+    a = x - fx;
+    const int k = int(fx);
+    i0 = k < 0 ? n - 1u : (uint32_t(k) >= n ? uint32_t(k) - n : uint32_t(k));
+    i1 = i0 + 1u == n ? 0u : i0 + 1u;
+}
+__device__ inline float4 tex_sample(const DScene& s, uint32_t id, float2 uv) {
+    const TexInfo t = s.tex[id];
+    uint32_t x0, x1, y0, y1;
+    float ax, ay;
+    tex_axis(uv.x, t.w, x0, x1, ax);
+    tex_axis(uv.y, t.h, y0, y1, ay);
+    const float4 a = tex_texel(s, t, x0, y0), b = tex_texel(s, t, x1, y0);
+    const float4 c = tex_texel(s, t, x0, y1), d = tex_texel(s, t, x1, y1);
+    const float bx = 1.f - ax, by = 1.f - ay;
+    const float4 top = make_float4(a.x * bx + b.x * ax, a.y * bx + b.y * ax, a.z * bx + b.z * ax, a.w * bx + b.w * ax);
+    const float4 bot = make_float4(c.x * bx + d.x * ax, c.y * bx + d.y * ax, c.z * bx + d.z * ax, c.w * bx + d.w * ax);
+    return make_float4(top.x * by + bot.x * ay, top.y * by + bot.y * ay, top.z * by + bot.z * ay, top.w * by + bot.w * ay);
+}
+__device__ inline bool has_tex(const DScene& s, int32_t id) { return id >= 0 && uint32_t(id) < s.ntex; }
+
+// texture coordinate at a traced hit: Mix(t0, t1, t2, bary) = t0 + (t1-t0) u + (t2-t0) v
+// (FHardwareRayTracerV2::TraceRay, Shading.slang:820)
+__device__ inline float2 hit_uv(const DScene& s, uint32_t inst, uint32_t prim, float u, float v) {
+    if (!s.tri_uv) return make_float2(0.f, 0.f);
+    const size_t b = 3 * size_t(s.inst[inst].tri_base + prim);
+    const float2 t0 = s.tri_uv[b], t1 = s.tri_uv[b + 1], t2 = s.tri_uv[b + 2];
+    return make_float2((t0.x + (t1.x - t0.x) * u) + (t2.x - t0.x) * v, (t0.y + (t1.y - t0.y) * u) + (t2.y - t0.y) * v);
+}
+
+// albedo = Diffuse * base colour texture (FetchGBufferV2 / GetRayColor, Shading.slang:84-90,1047-1052)
+__device__ inline float4 mat_albedo(const DScene& s, const gk_material& m, float2 uv) {
+    float4 a = make_float4(m.diffuse[0], m.diffuse[1], m.diffuse[2], m.diffuse[3]);
+    if (has_tex(s, m.diffuse_texture_id)) {
+        const float4 t = tex_sample(s, uint32_t(m.diffuse_texture_id), uv);
+        a = make_float4(a.x * t.x, a.y * t.y, a.z * t.z, a.w * t.w);
+    }
+    return a;
+}
+
+// FetchGBufferV2 at the primary hit (Shading.slang:82-112) with get_material_dataV2's
+// attributes (GeneralFunc.slang:48-53: bx*a0 + (by*a1 + bz*a2)): albedo, roughness and
+// the shading normal, which a normal map replaces by
+// normalize(world * normalize(mikkTSpace(tex, local normal, local tangent))).
+__device__ inline void fetch_gbuffer(const DScene& s, uint32_t inst, uint32_t prim, float u, float v,
+                                     const gk_material& m, f3& nrm, float4& albedo, float& rough) {
+    const float by = u, bz = v, bx = 1.0f - (by + bz);
+    const size_t b = 3 * size_t(s.inst[inst].tri_base + prim);
+    float2 uv = make_float2(0.f, 0.f);
+    if (s.tri_uv) {
+        const float2 t0 = s.tri_uv[b], t1 = s.tri_uv[b + 1], t2 = s.tri_uv[b + 2];
+        uv = make_float2(bx * t0.x + (by * t1.x + bz * t2.x), bx * t0.y + (by * t1.y + bz * t2.y));
+    }
+    albedo = mat_albedo(s, m, uv);
+    rough = m.fuzziness;
+    if (has_tex(s, m.mra_texture_id)) rough = rough * tex_sample(s, uint32_t(m.mra_texture_id), uv).y;
+    if (has_tex(s, m.normal_texture_id)) {
+        const float4 nt = tex_sample(s, uint32_t(m.normal_texture_id), uv);
+        const f3 n0 = ld3(s.tri_nrm[b]), n1 = ld3(s.tri_nrm[b + 1]), n2 = ld3(s.tri_nrm[b + 2]);
+        const f3 ln = n0 * bx + (n1 * by + n2 * bz);
+        float4 lt = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (s.tri_tan) {
+            const float4 a0 = s.tri_tan[b], a1 = s.tri_tan[b + 1], a2 = s.tri_tan[b + 2];
+            lt = make_float4(bx * a0.x + (by * a1.x + bz * a2.x), bx * a0.y + (by * a1.y + bz * a2.y),
+                             bx * a0.z + (by * a1.z + bz * a2.z), bx * a0.w + (by * a1.w + bz * a2.w));
+        }
+        const f3 vn = f3{nt.x * 2.0f - 1.0f, nt.y * 2.0f - 1.0f, nt.z * 2.0f - 1.0f};
+        const f3 t3 = f3{lt.x, lt.y, lt.z};
+        const f3 vb = cross(ln, t3) * lt.w;
+        const f3 mapped = normalize((t3 * vn.x + vb * vn.y) + ln * vn.z);
+        float w4[4];
+        mul44v(s.inst[inst].world, mapped.x, mapped.y, mapped.z, 0.f, w4);
+        nrm = normalize(f3{w4[0], w4[1], w4[2]});
+    }
 }
 
 __device__ inline ushort4 to_h4(float a, float b, float c, float d) {

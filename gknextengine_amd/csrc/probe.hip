@@ -301,9 +301,10 @@ __global__ __launch_bounds__(256) void k_bake_face(DScene s, BakeK K, ProbeDev P
             f3 pos, nrm;
             hit_vertex(s, inst, prim, h.x, h.y, h.z, ld3(ro[r]), ld3(rd[r]), pos, nrm);
             const gk_material& m = s.mats[material_of(s, inst, prim)];
+            const float4 alb = mat_albedo(s, m, hit_uv(s, inst, prim, h.y, h.z));
             const float4 li = probe_lookup<false>(P, pos, nrm);
-            b = make_float4((m.diffuse[0] * li.x) * 1.25f, (m.diffuse[1] * li.y) * 1.25f,
-                            (m.diffuse[2] * li.z) * 1.25f, (m.diffuse[3] * li.w) * 1.25f);
+            b = make_float4((alb.x * li.x) * 1.25f, (alb.y * li.y) * 1.25f, (alb.z * li.z) * 1.25f,
+                            (alb.w * li.w) * 1.25f);
         } else {
             if (K.has_sky) d = make_float4(K.sky[0], K.sky[1], K.sky[2], K.sky[3]);
             miss = 1.f;

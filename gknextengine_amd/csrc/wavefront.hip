@@ -106,6 +106,7 @@ struct PathState {
     float4* dir;    // current incoming direction
     float4* ppos;   // REF_V2 primary vertex (pulled back), w = primary material bits
     float4* pnrm;   // REF_V2 primary normal
+    float4* palb;   // REF_V2 primary albedo (FetchGBufferV2: textured)
 };
 
 __device__ inline uint32_t pixel_of(const FrameK& K, uint32_t p) {
@@ -256,7 +257,7 @@ __device__ inline void mul44m(const float* a, const float* b, float* o) {
 
 // PrimaryHit G-buffer (Shading.slang:1204-1215) + CalculateMotionVectorV2 (:50-72)
 __device__ inline void write_gbuffer(const DScene& s, const FrameK& K, const Targets& T, uint32_t px, uint32_t inst,
-                                     uint32_t prim, float4 hit, f3 pos, f3 nrm, const gk_material& m,
+                                     uint32_t prim, float4 hit, f3 pos, f3 nrm, float4 albedo, float rough,
                                      uint32_t& moment_out) {
     float clip[4];
     mul44v(K.vp, pos.x, pos.y, pos.z, 1.f, clip);
@@ -276,8 +277,8 @@ __device__ inline void write_gbuffer(const DScene& s, const FrameK& K, const Tar
     T.moment[px] = uint16_t(mo);
     moment_out = mo;
     T.motion[px] = make_float2((pfx - cfx) * float(K.W), (pfy - cfy) * float(K.H));
-    T.albedo[px] = to_h4(m.diffuse[0], m.diffuse[1], m.diffuse[2], m.diffuse[3]);
-    T.normal[px] = to_h4(nrm.x, nrm.y, nrm.z, m.fuzziness);
+    T.albedo[px] = to_h4(albedo.x, albedo.y, albedo.z, albedo.w);
+    T.normal[px] = to_h4(nrm.x, nrm.y, nrm.z, rough);
     T.objid[px] = I.instance_id;
     if (!K.vis_primary) T.minigbuffer[px] = ((inst + 1u) << 17) | (prim & 0x1ffffu);
     T.phit[px] = hit;
@@ -422,10 +423,10 @@ __device__ inline void ref_end_sample(const DScene& s, const FrameK& K, const Ta
                                       rng4& seed, bool& emit, float4& ro, float4& rd) {
     const float4 pp = S.ppos[p];
     const uint32_t pmat = __float_as_uint(pp.w);
-    const gk_material& pm = s.mats[pmat];
     if (flags & 16u) {
-        color.x = color.x * pm.diffuse[0]; color.y = color.y * pm.diffuse[1];
-        color.z = color.z * pm.diffuse[2]; color.w = color.w * pm.diffuse[3];
+        const float4 pa = S.palb[p];
+        color.x = color.x * pa.x; color.y = color.y * pa.y;
+        color.z = color.z * pa.z; color.w = color.w * pa.w;
     }
     float4 acc = (flags & 8u) ? S.ls[p] : S.ld[p];
     acc.x = acc.x + color.x; acc.y = acc.y + color.y; acc.z = acc.z + color.z; acc.w = acc.w + color.w;
@@ -501,9 +502,12 @@ __global__ __launch_bounds__(256) void k_ref_primary(DScene s, FrameK K, Targets
             const uint32_t mi = material_of(s, inst, prim);
             const gk_material& m = s.mats[mi];
             f3 pos = O + D * h.x;
-            const f3 nrm = primary_normal(s, inst, prim, h.y, h.z);
+            f3 nrm = primary_normal(s, inst, prim, h.y, h.z);
+            float4 alb;
+            float rough;
+            fetch_gbuffer(s, inst, prim, h.y, h.z, m, nrm, alb, rough);
             uint32_t moment = 0;
-            write_gbuffer(s, K, T, px, inst, prim, h, pos, nrm, m, moment);
+            write_gbuffer(s, K, T, px, inst, prim, h, pos, nrm, alb, rough, moment);
             if (m.material_model != GK_MATERIAL_DIELECTRIC) pos = pos - D * 0.03f;
             if (m.material_model == GK_MATERIAL_DIFFUSE_LIGHT) {
                 T.diffuse[px] = to_h4(m.diffuse[0], m.diffuse[1], m.diffuse[2], m.diffuse[3]);
@@ -517,6 +521,7 @@ __global__ __launch_bounds__(256) void k_ref_primary(DScene s, FrameK K, Targets
                 if (nsamp < 1u) nsamp = 1u;
                 S.ppos[p] = make_float4(pos.x, pos.y, pos.z, __uint_as_float(mi));
                 S.pnrm[p] = make_float4(nrm.x, nrm.y, nrm.z, 0.f);
+                S.palb[p] = alb;
                 S.ld[p] = make_float4(0.f, 0.f, 0.f, 0.f);
                 S.ls[p] = make_float4(0.f, 0.f, 0.f, 0.f);
                 S.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
@@ -570,8 +575,9 @@ __global__ __launch_bounds__(256) void k_ref_shade(DScene s, FrameK K, Targets T
             const gk_material& hm = s.mats[mi];
             const bool hit_light = hm.material_model == GK_MATERIAL_DIFFUSE_LIGHT;
             if (hit_light || !chance_reflect) {
-                color.x = color.x * hm.diffuse[0]; color.y = color.y * hm.diffuse[1];
-                color.z = color.z * hm.diffuse[2]; color.w = color.w * hm.diffuse[3];
+                const float4 ha = mat_albedo(s, hm, hit_uv(s, inst, prim, h.y, h.z));
+                color.x = color.x * ha.x; color.y = color.y * ha.y;
+                color.z = color.z * ha.z; color.w = color.w * ha.w;
             }
             if (back && !origin_diel) {
                 color = make_float4(0.5f, 0.5f, 0.5f, 0.f);
@@ -689,9 +695,12 @@ __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T,
             } else {
                 const uint32_t prim = __float_as_uint(h.w);
                 const gk_material& m = s.mats[material_of(s, inst, prim)];
+                f3 nrm = primary_normal(s, inst, prim, h.y, h.z);
+                float4 alb;
+                float rough;
+                fetch_gbuffer(s, inst, prim, h.y, h.z, m, nrm, alb, rough);
                 uint32_t moment = 0;
-                write_gbuffer(s, K, T, px, inst, prim, h, O + D * h.x, primary_normal(s, inst, prim, h.y, h.z), m,
-                              moment);
+                write_gbuffer(s, K, T, px, inst, prim, h, O + D * h.x, nrm, alb, rough, moment);
             }
         }
         if (__float_as_uint(h.w) == kMiss) {
@@ -700,7 +709,27 @@ __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T,
             const uint32_t prim = __float_as_uint(h.w);
             const InstDev& I = s.inst[inst];
             const uint32_t mi = material_of(s, inst, prim);
-            const gk_material& mat = s.mats[mi];
+            gk_material mat = s.mats[mi];
+            f3 mapped_n = f3{0.f, 0.f, 0.f};
+            bool mapped = false;
+            if (mat.material_model != GK_MATERIAL_DIFFUSE_LIGHT &&
+                (has_tex(s, mat.diffuse_texture_id) || has_tex(s, mat.mra_texture_id) ||
+                 (depth == 0u && has_tex(s, mat.normal_texture_id)))) {
+                // textured albedo and roughness at every vertex, the normal map at the primary hit
+                float4 alb;
+                float rough;
+                if (depth == 0u) {
+                    fetch_gbuffer(s, inst, prim, h.y, h.z, mat, mapped_n, alb, rough);
+                    mapped = has_tex(s, mat.normal_texture_id);
+                } else {
+                    const float2 uv = hit_uv(s, inst, prim, h.y, h.z);
+                    alb = mat_albedo(s, mat, uv);
+                    rough = mat.fuzziness;
+                    if (has_tex(s, mat.mra_texture_id)) rough = rough * tex_sample(s, uint32_t(mat.mra_texture_id), uv).y;
+                }
+                mat.diffuse[0] = alb.x; mat.diffuse[1] = alb.y; mat.diffuse[2] = alb.z; mat.diffuse[3] = alb.w;
+                mat.fuzziness = rough;
+            }
             const f3 P = O + D * h.x;
             // shading normal: interpolated vertex normals; geometric normal: the object-space
             // face normal e1 x e2 kept in the w lanes of tri_nrm; both by the inverse transpose
@@ -710,6 +739,7 @@ __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T,
             f3 Ns = normalize(inv_transpose(I.inv, (ld3(n0) * bw + ld3(n1) * bu) + ld3(n2) * bv));
             const f3 face = f3{n0.w, n1.w, n2.w};
             f3 Ng = normalize(inv_transpose(I.inv, face));
+            if (mapped) Ns = mapped_n;
             if (dot(Ng, D) > 0.f) Ng = -Ng;
             if (dot(Ns, Ng) < 0.f) Ns = -Ns;
             if (mat.material_model == GK_MATERIAL_DIFFUSE_LIGHT) {
@@ -1011,7 +1041,7 @@ int ensure_wavefront(Ctx& c) {
     GK_HIP(w.s_thr.resize(n)); GK_HIP(w.s_ld.resize(n)); GK_HIP(w.s_ls.resize(n));
     GK_HIP(w.s_rng.resize(n)); GK_HIP(w.s_misc.resize(n));
     GK_HIP(w.s_pos.resize(n)); GK_HIP(w.s_nrm.resize(n)); GK_HIP(w.s_dir.resize(n));
-    GK_HIP(w.s_ppos.resize(n)); GK_HIP(w.s_pnrm.resize(n));
+    GK_HIP(w.s_ppos.resize(n)); GK_HIP(w.s_pnrm.resize(n)); GK_HIP(w.s_palb.resize(n));
     w.capacity = n;
     return GK_OK;
 }
@@ -1102,6 +1132,7 @@ int render_frame(Ctx& c) {
     PathState P;
     P.thr = w.s_thr.ptr; P.ld = w.s_ld.ptr; P.ls = w.s_ls.ptr; P.rng = w.s_rng.ptr; P.misc = w.s_misc.ptr;
     P.pos = w.s_pos.ptr; P.nrm = w.s_nrm.ptr; P.dir = w.s_dir.ptr; P.ppos = w.s_ppos.ptr; P.pnrm = w.s_pnrm.ptr;
+    P.palb = w.s_palb.ptr;
 
     // previous object ids for reprojection (copy pass, PathTracingRenderer.cpp:160-216)
     k_copy_objid<<<nblk(n), 256, 0, s>>>(T.objid, static_cast<uint32_t*>(c.target_ptr[GK_RT_OBJECTID_1]), n);

@@ -108,17 +108,19 @@ typedef struct gk_device_desc {
 typedef struct gk_mesh {
     const float* positions;        /* 3 * num_vertices                            */
     const float* normals;          /* 3 * num_vertices, or NULL (face normals)    */
-    const float* texcoords;        /* 2 * num_vertices, or NULL                   */
+    const float* texcoords;        /* 2 * num_vertices (Vertex::TexCoord), or NULL (0, 0) */
     const uint32_t* indices;       /* 3 * num_triangles                           */
     const uint32_t* tri_material;  /* num_triangles local material slots, or NULL */
     uint32_t num_vertices;
     uint32_t num_triangles;
+    const float* tangents;         /* 4 * num_vertices (Vertex::Tangent: xyz, handedness w),
+                                      or NULL (zero: normal maps then keep the vertex normal) */
 } gk_mesh;
 
 /* Material, byte-identical to Material (BasicTypes.slang:218-236). 64 bytes. */
 typedef struct gk_material {
     float diffuse[4];               /* albedo, or radiance for DIFFUSE_LIGHT */
-    int32_t diffuse_texture_id;     /* -1: none (textures are not supported) */
+    int32_t diffuse_texture_id;     /* -1: none; else an index of gk_textures_upload */
     int32_t mra_texture_id;
     int32_t normal_texture_id;
     float fuzziness;                /* roughness                             */
@@ -129,6 +131,18 @@ typedef struct gk_material {
     float normal_texture_scale;
     float reserved[3];
 } gk_material;
+
+/* An RGBA8 texture (GlobalTexturePool's LDR path, Texture.cpp:600-612). srgb: the texels
+ * are sRGB-encoded (base colour; VK_FORMAT_R8G8B8A8_SRGB) and are decoded to linear before
+ * filtering; otherwise UNORM (metallic-roughness, normal maps). Sampled as the reference's
+ * SampleLevel(uv, 0) with its default sampler (Sampler.hpp:9-26): bilinear at level 0,
+ * REPEAT in u and v. */
+typedef struct gk_texture {
+    uint32_t width, height;
+    uint32_t srgb;
+    uint32_t reserved;
+    const uint8_t* rgba8;           /* width * height * 4 bytes, rows top to bottom */
+} gk_texture;
 
 /* Quad light, byte-identical to LightObject (BasicTypes.slang:171-182). */
 typedef struct gk_light {
@@ -303,6 +317,17 @@ int gk_scene_upload(gk_ctx* ctx,
 
 /* Replaces Scene::UpdateAllMaterials (Scene.cpp:393-408). */
 int gk_materials_update(gk_ctx* ctx, const gk_material* materials, uint32_t first, uint32_t count);
+
+/* Replaces the context's textures (GlobalTexturePool::LoadTexture for each image of a
+ * scene, Texture.cpp:260-273,369-700; FSceneLoader.cpp:253-291): texture i is what
+ * gk_material texture ids i refer to. Materials sample them as FetchGBufferV2 and
+ * GetRayColor do (Shading.slang:82-112,1046-1052): albedo = Diffuse * base colour texture
+ * at every hit; at the primary hit also roughness = Fuzziness * MRA.g (the NORMAL
+ * target's w) and the normal map (mikkTSpace with the interpolated object-space normal
+ * and tangent, then the instance transform). A texture id outside [0, count) counts as
+ * none. count 0 removes all textures. Texels are stored uncompressed (the reference
+ * transcodes LDR images to BC7). */
+int gk_textures_upload(gk_ctx* ctx, const gk_texture* textures, uint32_t count);
 
 /* Replaces Scene::UpdateNodesGpuDriven + RayTraceBaseRenderer::AfterUpdateScene /
  * PreRender TLAS update (Scene.cpp:443-492; RayTraceBaseRenderer.cpp:165-218).

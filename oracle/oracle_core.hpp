@@ -178,6 +178,8 @@ struct Mesh {
     std::vector<F3> v0, e1, e2;     // per triangle: v0, v1-v0, v2-v0 (object space)
     std::vector<F3> n0, n1, n2;     // per-triangle vertex normals (object space)
     std::vector<uint32_t> mat;      // local material slot per triangle
+    std::vector<float> uv;          // 6 per triangle (vertex texture coordinates), or empty
+    std::vector<float> tan;         // 12 per triangle (vertex tangents xyzw), or empty
     // BVH2 (binned SAH, restating tinybvh BVH::Build)
     struct Node { F3 bmin; uint32_t left_first; F3 bmax; uint32_t count; };
     std::vector<Node> nodes;
@@ -220,8 +222,15 @@ struct ProbeRef {
     const VoxelO* vox = nullptr;
 };
 
+struct TextureO {          // an RGBA8 texture (gk_texture)
+    uint32_t w = 0, h = 0, srgb = 0;
+    std::vector<uint32_t> texels;
+};
+
 struct Scene {
     ProbeRef probes;        // REF_V2 termination lookup (arrays owned by the caller)
+    std::vector<TextureO> tex;
+    float tex_lut[512] = {};  // sRGB byte -> linear, then byte / 255
     std::vector<Mesh> meshes;
     std::vector<Instance> inst;
     std::vector<Material> mats;

@@ -253,6 +253,91 @@ void combined_prev(const float* world, const float* prev, float* out) {
     }
 }
 
+// ---------------------------------------------------------------- textures
+// SampleLevel(uv, 0) on the reference's default sampler (Sampler.hpp:9-26: linear
+// filter, REPEAT): the coordinate is wrapped into [0, 1), the four texels around
+// u * w - 0.5 are decoded to linear (sRGB textures by the sRGB transfer function) and
+// blended, first along u, then along v.
+F4 texel_lin(const Scene& sc, const TextureO& t, uint32_t x, uint32_t y) {
+    const uint32_t p = t.texels[size_t(y) * t.w + x];
+    const float* lut = t.srgb ? sc.tex_lut : sc.tex_lut + 256;
+    return F4{{lut[p & 0xFF], lut[(p >> 8) & 0xFF], lut[(p >> 16) & 0xFF], sc.tex_lut[256 + (p >> 24)]}};
+}
+void wrap_axis(float u, uint32_t n, uint32_t& i0, uint32_t& i1, float& a) {
+    float w = u - std::floor(u);
+    if (!(w >= 0.f && w < 1.f)) w = 0.f;
+    const float x = w * float(n) - 0.5f;
+    const float fx = std::floor(x);
+    a = x - fx;
+    const int k = int(fx);
+    i0 = k < 0 ? n - 1 : (uint32_t(k) >= n ? uint32_t(k) - n : uint32_t(k));
+    i1 = (i0 + 1 == n) ? 0 : i0 + 1;
+}
+F4 tex_sample(const Scene& sc, uint32_t id, float u, float v) {
+    const TextureO& t = sc.tex[id];
+    uint32_t x0, x1, y0, y1;
+    float ax, ay;
+    wrap_axis(u, t.w, x0, x1, ax);
+    wrap_axis(v, t.h, y0, y1, ay);
+    const F4 a = texel_lin(sc, t, x0, y0), b = texel_lin(sc, t, x1, y0);
+    const F4 c = texel_lin(sc, t, x0, y1), d = texel_lin(sc, t, x1, y1);
+    F4 o;
+    for (int k = 0; k < 4; ++k) {
+        const float top = a.v[k] * (1.f - ax) + b.v[k] * ax;
+        const float bot = c.v[k] * (1.f - ax) + d.v[k] * ax;
+        o.v[k] = top * (1.f - ay) + bot * ay;
+    }
+    return o;
+}
+bool has_tex(const Scene& sc, int32_t id) { return id >= 0 && uint32_t(id) < sc.tex.size(); }
+
+// Diffuse * base colour texture (GetRayColor, Shading.slang:1047-1052)
+F4 mat_albedo(const Scene& sc, const Material& m, float u, float v) {
+    F4 a{{m.diffuse[0], m.diffuse[1], m.diffuse[2], m.diffuse[3]}};
+    if (has_tex(sc, m.diffuse_texture_id)) a = mul4(a, tex_sample(sc, uint32_t(m.diffuse_texture_id), u, v));
+    return a;
+}
+
+// texture coordinate at a traced hit: Mix(t0, t1, t2, bary) (Shading.slang:820, PreProcessor.slang:24)
+void hit_uv(const Scene& sc, const Hit& h, float& u, float& v) {
+    const Mesh& m = sc.meshes[sc.inst[h.inst].mesh_id];
+    u = v = 0.f;
+    if (m.uv.empty()) return;
+    const float* t = &m.uv[6 * size_t(h.prim)];
+    u = (t[0] + (t[2] - t[0]) * h.u) + (t[4] - t[0]) * h.v;
+    v = (t[1] + (t[3] - t[1]) * h.u) + (t[5] - t[1]) * h.v;
+}
+
+// FetchGBufferV2 (Shading.slang:82-112) with get_material_dataV2's interpolation
+// bx*a0 + (by*a1 + bz*a2) (GeneralFunc.slang:48-53) and mikkTSpace (Shading.slang:74-80)
+void fetch_gbuffer(const Scene& sc, const Hit& h, const Material& mat, F3& nrm, F4& albedo, float& rough) {
+    const Instance& I = sc.inst[h.inst];
+    const Mesh& m = sc.meshes[I.mesh_id];
+    const float by = h.u, bz = h.v, bx = 1.0f - (by + bz);
+    float u = 0.f, v = 0.f;
+    if (!m.uv.empty()) {
+        const float* t = &m.uv[6 * size_t(h.prim)];
+        u = bx * t[0] + (by * t[2] + bz * t[4]);
+        v = bx * t[1] + (by * t[3] + bz * t[5]);
+    }
+    albedo = mat_albedo(sc, mat, u, v);
+    rough = mat.fuzziness;
+    if (has_tex(sc, mat.mra_texture_id)) rough = rough * tex_sample(sc, uint32_t(mat.mra_texture_id), u, v).v[1];
+    if (!has_tex(sc, mat.normal_texture_id)) return;
+    const F4 nt = tex_sample(sc, uint32_t(mat.normal_texture_id), u, v);
+    const F3 ln = m.n0[h.prim] * bx + (m.n1[h.prim] * by + m.n2[h.prim] * bz);
+    float lt[4] = {0.f, 0.f, 0.f, 0.f};
+    if (!m.tan.empty()) {
+        const float* a = &m.tan[12 * size_t(h.prim)];
+        for (int k = 0; k < 4; ++k) lt[k] = bx * a[k] + (by * a[4 + k] + bz * a[8 + k]);
+    }
+    const F3 vnt(nt.v[0] * 2.0f - 1.0f, nt.v[1] * 2.0f - 1.0f, nt.v[2] * 2.0f - 1.0f);
+    const F3 tg(lt[0], lt[1], lt[2]);
+    const F3 vb = lt[3] * cross(ln, tg);
+    const F3 n = normalize(vnt.x * tg + vnt.y * vb + vnt.z * ln);
+    nrm = normalize(xform_vector(I.world, n));
+}
+
 // ---------------------------------------------------------------- ambient-cube probes
 // packRGB10A2 / unpackRGB10A2 (AmbientCube.slang:66-87), MAX_ILLUMINANCE = 512
 uint32_t pack_rgb10a2(const F4& c) {
@@ -483,7 +568,9 @@ struct ProbeBaker {
             if (trace_closest(sc, origin, ray_dir, EPS, 20.f, h)) {
                 const Vtx v = hit_vertex(sc, h, origin, ray_dir);
                 const Material& hm = sc.mats[v.mat];
-                const F4 albedo{{hm.diffuse[0], hm.diffuse[1], hm.diffuse[2], hm.diffuse[3]}};
+                float tu, tv;
+                hit_uv(sc, h, tu, tv);
+                const F4 albedo = mat_albedo(sc, hm, tu, tv);
                 const F4 li = interpolate_cubes(old, v.pos, v.nrm, false);
                 F4 t = mul4(albedo, li);
                 for (int k = 0; k < 4; ++k) t.v[k] = t.v[k] * 1.25f;
@@ -605,7 +692,9 @@ struct RefV2 {
         if (trace(org, trace_dir, PT_MAX_TRACE_DISTANCE, h, rays)) {
             v = hit_vertex(sc, h, org, trace_dir);
             const Material& hm = mat_at(sc, v.mat);
-            const F4 alb{{hm.diffuse[0], hm.diffuse[1], hm.diffuse[2], hm.diffuse[3]}};
+            float tu, tv;
+            hit_uv(sc, h, tu, tv);
+            const F4 alb = mat_albedo(sc, hm, tu, tv);
             if (hm.material_model == MAT_DIFFUSE_LIGHT || !chance_reflect) color = mul4(color, alb);
             if (back && mat.material_model != MAT_DIELECTRIC) {
                 color = F4{{0.5f, 0.5f, 0.5f, 0.f}};
@@ -642,6 +731,9 @@ struct RefV2 {
         Vtx pv = primary_vertex(sc, h, O, D);
         const Instance& I = sc.inst[h.inst];
         const Material& mat = mat_at(sc, pv.mat);
+        F4 palb;
+        float prough;
+        fetch_gbuffer(sc, h, mat, pv.nrm, palb, prough);
         // PrimaryHit, Shading.slang:1204-1220
         {
             const float p4[4] = {pv.pos.x, pv.pos.y, pv.pos.z, 1.f};
@@ -661,13 +753,13 @@ struct RefV2 {
             else moment = moment > 0 ? uint16_t(moment - 1) : 0;
             out.motion[0] = (pfx - cfx) * float(W);
             out.motion[1] = (pfy - cfy) * float(H);
-            for (int i = 0; i < 4; ++i) out.albedo[i] = mat.diffuse[i];
+            for (int i = 0; i < 4; ++i) out.albedo[i] = palb.v[i];
             out.normal[0] = pv.nrm.x; out.normal[1] = pv.nrm.y; out.normal[2] = pv.nrm.z;
-            out.normal[3] = mat.fuzziness;
+            out.normal[3] = prough;
             out.objid = I.instance_id;
             if (mat.material_model != MAT_DIELECTRIC) pv.pos = pv.pos - D * 0.03f;
         }
-        const F4 primary_albedo{{mat.diffuse[0], mat.diffuse[1], mat.diffuse[2], mat.diffuse[3]}};
+        const F4 primary_albedo = palb;
         // Render, Shading.slang:1085-1184
         if (mat.material_model == MAT_DIFFUSE_LIGHT) {
             for (int i = 0; i < 4; ++i) out.diffuse[i] = mat.diffuse[i];
@@ -853,7 +945,29 @@ struct PtMis {
             const Instance& I = sc.inst[h.inst];
             const Mesh& m = sc.meshes[I.mesh_id];
             const uint32_t mi = material_of(sc, h.inst, h.prim);
-            const Material& mat = sc.mats[mi];
+            Material mat = sc.mats[mi];
+            // textures: albedo and roughness at every vertex, the normal map at the primary hit
+            bool mapped = false;
+            F3 mapped_n;
+            if (mat.material_model != MAT_DIFFUSE_LIGHT &&
+                (has_tex(sc, mat.diffuse_texture_id) || has_tex(sc, mat.mra_texture_id) ||
+                 (depth == 0 && has_tex(sc, mat.normal_texture_id)))) {
+                const Hit hf{float(h.t), float(h.u), float(h.v), h.prim, h.inst};
+                F4 alb;
+                float rough;
+                if (depth == 0) {
+                    fetch_gbuffer(sc, hf, mat, mapped_n, alb, rough);
+                    mapped = has_tex(sc, mat.normal_texture_id);
+                } else {
+                    float tu, tv;
+                    hit_uv(sc, hf, tu, tv);
+                    alb = mat_albedo(sc, mat, tu, tv);
+                    rough = mat.fuzziness;
+                    if (has_tex(sc, mat.mra_texture_id)) rough = rough * tex_sample(sc, uint32_t(mat.mra_texture_id), tu, tv).v[1];
+                }
+                for (int k = 0; k < 4; ++k) mat.diffuse[k] = alb.v[k];
+                mat.fuzziness = rough;
+            }
             const V P_ = O + D * h.t;
             // world normals: geometric (inverse transpose of e1 x e2) and shading
             const float* Wi = I.inv;
@@ -872,6 +986,7 @@ struct PtMis {
             for (int j = 0; j < 3; ++j)
                 Ns[j] = T(Wr[0 * 4 + j]) * nsum.x + T(Wr[1 * 4 + j]) * nsum.y + T(Wr[2 * 4 + j]) * nsum.z;
             Ns = normalize(Ns);
+            if (mapped) Ns = cvt(mapped_n);
             if (dot(Ng, D) > T(0)) Ng = -Ng;
             if (dot(Ns, Ng) < T(0)) Ns = -Ns;
             if (mat.material_model == MAT_DIFFUSE_LIGHT) {
@@ -1225,6 +1340,59 @@ int oracle_scene_add_mesh(void* s, const float* pos, const float* nrm, const uin
     }
     sc.meshes.push_back(std::move(m));
     return int(sc.meshes.size() - 1);
+}
+
+// per-vertex texture coordinates (2 floats) and tangents (4 floats) of mesh `mesh`
+// (either may be NULL), gathered per triangle through idx
+int oracle_scene_set_mesh_attribs(void* s, uint32_t mesh, const uint32_t* idx, uint32_t nv, uint32_t nt,
+                                  const float* uv, const float* tan) {
+    Scene& sc = *static_cast<Scene*>(s);
+    if (mesh >= sc.meshes.size()) return -1;
+    Mesh& m = sc.meshes[mesh];
+    m.uv.assign(uv ? 6 * size_t(nt) : 0, 0.f);
+    m.tan.assign(tan ? 12 * size_t(nt) : 0, 0.f);
+    for (uint32_t t = 0; t < nt; ++t)
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t a = idx[3 * t + k];
+            if (a >= nv) return -1;
+            if (uv) { m.uv[6 * t + 2 * k] = uv[2 * a]; m.uv[6 * t + 2 * k + 1] = uv[2 * a + 1]; }
+            if (tan)
+                for (int j = 0; j < 4; ++j) m.tan[12 * t + 4 * k + j] = tan[4 * a + j];
+        }
+    return 0;
+}
+
+// tex_sample at n coordinates (uv: 2 floats each, out: 4 floats each)
+void oracle_tex_sample(void* s, uint32_t id, const float* uv, uint32_t n, float* out) {
+    const Scene& sc = *static_cast<Scene*>(s);
+    for (uint32_t i = 0; i < n; ++i) {
+        const F4 r = tex_sample(sc, id, uv[2 * i], uv[2 * i + 1]);
+        std::memcpy(out + 4 * i, r.v, 16);
+    }
+}
+
+struct TexABI {  // gk_texture
+    uint32_t w, h, srgb, reserved;
+    const uint8_t* rgba8;
+};
+
+// the scene's textures; the byte -> linear tables as the sRGB transfer function and
+// byte / 255, evaluated in double
+int oracle_scene_set_textures(void* s, const TexABI* t, uint32_t n) {
+    Scene& sc = *static_cast<Scene*>(s);
+    sc.tex.assign(n, TextureO{});
+    for (uint32_t i = 0; i < n; ++i) {
+        TextureO& o = sc.tex[i];
+        o.w = t[i].w; o.h = t[i].h; o.srgb = t[i].srgb ? 1 : 0;
+        o.texels.resize(size_t(o.w) * o.h);
+        std::memcpy(o.texels.data(), t[i].rgba8, 4 * o.texels.size());
+    }
+    for (int b = 0; b < 256; ++b) {
+        const double x = b / 255.0;
+        sc.tex_lut[b] = float(x <= 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4));
+        sc.tex_lut[256 + b] = float(x);
+    }
+    return 0;
 }
 
 int oracle_scene_set_materials(void* s, const Material* mats, uint32_t n) {

@@ -60,6 +60,9 @@ def oracle():
             "oracle_probe_pack": (_U, [_P]),
             "oracle_probe_unpack": (None, [_U, _P]),
             "oracle_scene_set_probes": (None, [_P, _P, _P, _P]),
+            "oracle_scene_set_mesh_attribs": (C.c_int, [_P, _U, _P, _U, _U, _P, _P]),
+            "oracle_scene_set_textures": (C.c_int, [_P, _P, _U]),
+            "oracle_tex_sample": (None, [_P, _U, _P, _U, _P]),
         }
         optional = {
             "oracle_reproject": (None, [_P, _U, _U] + [_P] * 12),
@@ -120,6 +123,13 @@ class OracleScene:
             r = L.oracle_scene_add_mesh(self.h, ptr(m["positions"]), ptr(m["normals"]), ptr(m["indices"]),
                                         ptr(m["tri_material"]), m["positions"].shape[0], m["indices"].shape[0])
             assert r >= 0
+            if m.get("texcoords") is not None or m.get("tangents") is not None:
+                assert L.oracle_scene_set_mesh_attribs(self.h, r, ptr(m["indices"]), m["positions"].shape[0],
+                                                       m["indices"].shape[0], ptr(m.get("texcoords")),
+                                                       ptr(m.get("tangents"))) == 0
+        if getattr(scene, "textures", None):
+            self._tex = scene.c_textures()
+            L.oracle_scene_set_textures(self.h, C.addressof(self._tex), len(scene.textures))
         mats = scene.c_materials()
         L.oracle_scene_set_materials(self.h, C.addressof(mats), len(scene.materials))
         for inst in scene.instances:
@@ -148,6 +158,12 @@ class OracleScene:
         inst = np.zeros(n, np.uint32)
         oracle().oracle_trace(self.h, ptr(rays), n, ptr(hits), ptr(inst), int(any_hit), threads or os.cpu_count())
         return hits, inst
+
+    def tex_sample(self, tex_id, uv):
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.zeros((uv.shape[0], 4), np.float32)
+        oracle().oracle_tex_sample(self.h, tex_id, ptr(uv), uv.shape[0], ptr(out))
+        return out
 
     def num_lights(self):
         return oracle().oracle_num_lights(self.h)

@@ -129,3 +129,53 @@ def quad_light(center, half, area, mat_idx):
     L.normal_area[:] = [0.0, -1.0, 0.0, area]
     L.light_mat_idx = mat_idx
     return L
+
+
+# ---------------------------------------------------------------- textured test scenes
+def planar_uv(positions, normals, scale=3.0):
+    """Texture coordinates by projection along each vertex normal's dominant axis, and the
+    matching object-space tangent (the u axis, handedness +1)."""
+    p = np.asarray(positions, np.float64)
+    n = np.asarray(normals, np.float64)
+    ax = np.argmax(np.abs(n), axis=1)
+    uv = np.zeros((len(p), 2))
+    tan = np.zeros((len(p), 4))
+    tan[:, 3] = 1.0
+    for a, (iu, iv) in enumerate(((2, 1), (0, 2), (0, 1))):
+        sel = ax == a
+        uv[sel, 0] = p[sel, iu] * scale
+        uv[sel, 1] = p[sel, iv] * scale
+        tan[sel, iu] = 1.0
+    return uv.astype(np.float32), tan.astype(np.float32)
+
+
+def checker(n=8, cells=2, seed=0):
+    rng = np.random.default_rng(seed)
+    cols = rng.integers(40, 255, (cells * cells, 4)).astype(np.uint8)
+    cols[:, 3] = 255
+    y, x = np.mgrid[0:n, 0:n]
+    k = (y * cells // n) * cells + (x * cells // n)
+    return cols[k]
+
+
+def textured_cornell():
+    """The Cornell box with planar texture coordinates: a checker base colour and a
+    metallic-roughness map on the white material, a normal map on the red one."""
+    s = gk.procgen_cornell()
+    m = s.meshes[0]
+    uv, tan = planar_uv(m["positions"], m["normals"])
+    m["texcoords"], m["tangents"] = uv, tan
+    rng = np.random.default_rng(3)
+    t_col = s.add_texture(checker(8, 4, seed=1), srgb=True)
+    mra = rng.integers(0, 256, (4, 4, 4)).astype(np.uint8)
+    t_mra = s.add_texture(mra, srgb=False)
+    nm = np.zeros((4, 4, 4), np.uint8)
+    nm[..., 0] = rng.integers(60, 196, (4, 4))
+    nm[..., 1] = rng.integers(60, 196, (4, 4))
+    nm[..., 2] = 230
+    nm[..., 3] = 255
+    t_nrm = s.add_texture(nm, srgb=False)
+    s.materials[2].diffuse_texture_id = t_col
+    s.materials[2].mra_texture_id = t_mra
+    s.materials[0].normal_texture_id = t_nrm
+    return s

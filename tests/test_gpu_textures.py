@@ -1,0 +1,116 @@
+"""Parity of textured shading (gk_textures_upload; FetchGBufferV2 / GetRayColor texture
+reads) with the CPU oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import gknextengine_amd as gk
+from oracle import pyoracle as po
+from tests import scenes
+from tests.test_gpu_render import cornell_params, relmse
+
+pytestmark = pytest.mark.gpu
+
+
+def test_ref_v2_textured_cornell_matches_oracle():
+    W = H = 128
+    s = scenes.textured_cornell()
+    osc = po.OracleScene(s)
+    moment = np.zeros((H, W), np.uint16)
+    exact, acc_g, acc_c = [], 0.0, 0.0
+    with gk.Context(W, H) as ctx:
+        ctx.upload(s)
+        for f in range(8):
+            p = cornell_params(W, H, frame=f)
+            ctx.frame(p)
+            ctx.render()
+            ctx.sync()
+            o = osc.render_refv2(p, W, H, moment=moment)
+            gd, gs = ctx.read("SINGLE_DIFFUSE"), ctx.read("SINGLE_SPECULAR")
+            od, os_ = o["diffuse"].astype(np.float16), o["spec"].astype(np.float16)
+            same = np.all(gd.view(np.uint16) == od.view(np.uint16), axis=2) & \
+                np.all(gs.view(np.uint16) == os_.view(np.uint16), axis=2)
+            exact.append(same.mean())
+            acc_g = acc_g + gd[..., :3].astype(np.float64) + gs[..., :3]
+            acc_c = acc_c + o["diffuse"][..., :3] + o["spec"][..., :3]
+            if f == 0:
+                assert np.array_equal(ctx.read("ALBEDO").view(np.uint16),
+                                      o["albedo"].astype(np.float16).view(np.uint16))
+                assert np.array_equal(ctx.read("NORMAL").view(np.uint16),
+                                      o["normal"].astype(np.float16).view(np.uint16))
+                # the textures did change the G-buffer
+                base = po.OracleScene(gk.procgen_cornell()).render_refv2(p, W, H)
+                assert not np.array_equal(base["albedo"], o["albedo"])
+                assert not np.array_equal(base["normal"], o["normal"])
+    assert min(exact) > 0.98, exact
+    assert relmse(acc_g / 8, acc_c / 8) < 1e-4
+
+
+def test_pt_mis_textured_cornell_matches_oracle():
+    W = H = 128
+    s = scenes.textured_cornell()
+    osc = po.OracleScene(s)
+    frames = 8
+    with gk.Context(W, H) as ctx:
+        ctx.upload(s)
+        for f in range(frames):
+            ctx.frame(cornell_params(W, H, frame=f, estimator=gk.ESTIMATOR_PT_MIS, bounces=4))
+            ctx.render()
+        ctx.sync()
+        rad = ctx.read("RADIANCE")
+        alb = ctx.read("ALBEDO")
+    gpu = rad[..., :3] / rad[..., 3:4]
+    o = osc.render_ptmis(cornell_params(W, H, estimator=gk.ESTIMATOR_PT_MIS, bounces=4), W, H, frame0=0,
+                         frames=frames)
+    cpu = o["rgb"]
+    close = np.all(np.abs(gpu - cpu) <= 1e-4 * (1 + np.abs(cpu)), axis=2)
+    assert close.mean() > 0.97, close.mean()
+    assert relmse(gpu, cpu) < 1e-3
+    ref = osc.render_refv2(cornell_params(W, H), W, H)
+    assert np.array_equal(alb.view(np.uint16), ref["albedo"].astype(np.float16).view(np.uint16))
+
+
+def test_textured_probe_bake_matches_oracle():
+    from tests.test_gpu_probes import CORNELL_GRID, bright_params, oracle_grid
+    s = scenes.textured_cornell()
+    s.materials[3].diffuse[:] = [2000.0, 2000.0, 2000.0, 1.0]
+    osc = po.OracleScene(s)
+    G = oracle_grid(CORNELL_GRID)
+    G.voxelize(osc)
+    p = bright_params()
+    with gk.Context(32, 32) as ctx:
+        ctx.upload(s)
+        ctx.frame(p)
+        ctx.probes_init(*CORNELL_GRID[:3], CORNELL_GRID[3])
+        ctx.probes_voxelize()
+        for _ in range(3):
+            ctx.probes_bake()
+            G.bake(osc, p, s.lights[0])
+        cubes, _ = ctx.probes_read()
+    for f in gk.CUBE_FIELDS:
+        assert np.array_equal(cubes[f], G.cubes[f]), f
+
+
+def test_texture_upload_arguments_and_unknown_ids():
+    L = gk.lib()
+    W = H = 32
+    base = gk.procgen_cornell()
+    s = gk.procgen_cornell()
+    s.materials[2].diffuse_texture_id = 7          # no texture 7: counts as none
+    outs = []
+    for scene in (base, s):
+        with gk.Context(W, H) as ctx:
+            ctx.upload(scene)
+            ctx.frame(cornell_params(W, H))
+            ctx.render()
+            outs.append(ctx.read("SINGLE_DIFFUSE"))
+            bad = (gk.Texture * 1)()
+            bad[0].width, bad[0].height, bad[0].rgba8 = 0, 4, None
+            assert L.gk_textures_upload(ctx.h, bad, 1) == gk.GK_ERR_ARG
+            big = (gk.Texture * 1)()
+            buf = np.zeros(16, np.uint8)
+            big[0].width, big[0].height, big[0].rgba8 = 20000, 1, buf.ctypes.data
+            assert L.gk_textures_upload(ctx.h, big, 1) == gk.GK_ERR_LIMIT
+            assert L.gk_textures_upload(ctx.h, None, 0) == 0
+    assert np.array_equal(outs[0].view(np.uint16), outs[1].view(np.uint16))
