@@ -121,6 +121,16 @@ VOXEL_DTYPE = np.dtype([(n, np.uint32) for n in ("mat_id", "age", "dist_gg_z01",
 REFERENCE_PROBE_GRID = (192, 48, 0.25, (-24.0, -0.34375, -24.0))
 
 
+class GltfView(C.Structure):
+    _fields_ = [("meshes", C.c_void_p), ("num_meshes", C.c_uint32), ("materials", C.c_void_p),
+                ("num_materials", C.c_uint32), ("instances", C.c_void_p), ("num_instances", C.c_uint32),
+                ("lights", C.c_void_p), ("num_lights", C.c_uint32), ("textures", C.c_void_p),
+                ("num_textures", C.c_uint32), ("num_cameras", C.c_uint32), ("camera", CameraDesc),
+                ("has_sky", C.c_uint32), ("has_sun", C.c_uint32), ("sky_idx", C.c_int32),
+                ("sky_intensity", C.c_float), ("sky_rotation", C.c_float), ("sun_intensity", C.c_float),
+                ("sun_rotation", C.c_float)]
+
+
 class UNetLayer(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("cin_a", "cin_b", "cout", "cout_real", "kpad", "up", "relu", "level")] + [
         ("offset", C.c_uint64)]
@@ -176,6 +186,12 @@ SIGNATURES = [
     ("gk_unet_train_step", C.c_int, [_P, _P, _P, _U, _U, _U, C.c_float, C.POINTER(C.c_float)]),
     ("gk_unet_infer", C.c_int, [_P, _P, _P, _U, _U]),
     ("gk_denoise_input", C.c_int, [_P, _P]),
+    ("gk_gltf_load", C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    ("gk_gltf_load_memory", C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    ("gk_gltf_get_view", C.c_int, [_P, C.POINTER(GltfView)]),
+    ("gk_gltf_upload", C.c_int, [_P, _P]),
+    ("gk_gltf_free", C.c_int, [_P]),
+    ("gk_gltf_error", C.c_char_p, []),
     ("gk_probes_init", C.c_int, [_P, C.POINTER(ProbeGrid)]),
     ("gk_probes_voxelize", C.c_int, [_P]),
     ("gk_probes_bake", C.c_int, [_P, _U, _U]),
@@ -325,6 +341,71 @@ class SceneData:
         for i, m in enumerate(self.lights):
             arr[i] = m
         return arr
+
+
+class GltfScene:
+    """A parsed glTF file (gk_gltf_load): the C-side arrays, copied into ``scene``
+    (a SceneData) plus the first camera and the environment extras in ``info``."""
+
+    def __init__(self, source):
+        L = lib()
+        h = C.c_void_p()
+        if isinstance(source, (bytes, bytearray)):
+            buf = C.create_string_buffer(bytes(source), len(source))
+            st = L.gk_gltf_load_memory(buf, len(source), C.byref(h))
+        else:
+            st = L.gk_gltf_load(str(source).encode(), C.byref(h))
+        if st != 0:
+            raise GkError(st, "gk_gltf_load: " + L.gk_gltf_error().decode())
+        self.h = h
+        v = GltfView()
+        check(L.gk_gltf_get_view(h, C.byref(v)), "gk_gltf_get_view")
+        s = SceneData()
+        meshes = C.cast(v.meshes, C.POINTER(Mesh))
+        for i in range(v.num_meshes):
+            m = meshes[i]
+            nv, nt = m.num_vertices, m.num_triangles
+
+            def grab(p, n, k, dt=np.float32):
+                if not p or n == 0:
+                    return np.zeros((0, k), dt)
+                return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float if dt == np.float32 else C.c_uint32)),
+                                             (n * k,)).reshape(n, k).copy()
+            s.add_mesh(grab(m.positions, nv, 3), grab(m.indices, nt, 3, np.uint32), grab(m.normals, nv, 3),
+                       grab(m.tri_material, nt, 1, np.uint32).reshape(-1), texcoords=grab(m.texcoords, nv, 2),
+                       tangents=grab(m.tangents, nv, 4))
+        mats = C.cast(v.materials, C.POINTER(Material))
+        s.materials = [Material.from_buffer_copy(mats[i]) for i in range(v.num_materials)]
+        insts = C.cast(v.instances, C.POINTER(Instance))
+        s.instances = [Instance.from_buffer_copy(insts[i]) for i in range(v.num_instances)]
+        lights = C.cast(v.lights, C.POINTER(Light))
+        s.lights = [Light.from_buffer_copy(lights[i]) for i in range(v.num_lights)]
+        texs = C.cast(v.textures, C.POINTER(Texture))
+        for i in range(v.num_textures):
+            t = texs[i]
+            a = np.ctypeslib.as_array(C.cast(t.rgba8, C.POINTER(C.c_uint8)), (t.height * t.width * 4,))
+            s.add_texture(a.reshape(t.height, t.width, 4).copy(), bool(t.srgb))
+        self.scene = s
+        cam = v.camera
+        self.info = {"num_cameras": v.num_cameras, "eye": tuple(cam.eye), "target": tuple(cam.target),
+                     "fov": cam.fov_y_degrees, "has_sky": v.has_sky, "has_sun": v.has_sun, "sky_idx": v.sky_idx,
+                     "sky_intensity": v.sky_intensity, "sky_rotation": v.sky_rotation,
+                     "sun_intensity": v.sun_intensity, "sun_rotation": v.sun_rotation}
+
+    def upload(self, ctx: "Context"):
+        """gk_gltf_upload: the loader's arrays straight into the context."""
+        check(lib().gk_gltf_upload(ctx.h, self.h), "gk_gltf_upload")
+
+    def close(self):
+        if self.h:
+            lib().gk_gltf_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def procgen_cornell() -> SceneData:

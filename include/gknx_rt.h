@@ -555,6 +555,53 @@ int gk_probes_read(gk_ctx* ctx, gk_ambient_cube* cubes, gk_voxel* voxels, uint32
 int gk_probes_write(gk_ctx* ctx, const gk_ambient_cube* cubes, const gk_voxel* voxels, uint32_t first,
                     uint32_t count);
 
+/* ---------------------------------------------------------------- glTF scenes
+ * FSceneLoader::LoadGLTFScene (src/Assets/FSceneLoader.cpp:209-700) for .glb and .gltf
+ * files (buffers in the GLB chunk, in files next to the .gltf, or base64 data URIs;
+ * images PNG). The result is the host arrays gk_scene_upload / gk_textures_upload take:
+ *   meshes     one per glTF mesh: its TRIANGLES primitives concatenated, tri_material =
+ *              the primitive's section index (FSceneLoader.cpp:398-510); tangents from
+ *              TANGENT, else generated from the texture coordinates
+ *   materials  the reference's PBR mapping (FSceneLoader.cpp:308-388): Diffuse =
+ *              (sqrt(baseColorFactor.rgb), 1), Fuzziness = roughnessFactor, Metalness =
+ *              metallicFactor, Mixture by default, Metallic / Lambertian by the same
+ *              thresholds, KHR_materials_ior, KHR_materials_transmission > 0.8 ->
+ *              Dielectric, extras.ior2, KHR_materials_emissive_strength -> DiffuseLight
+ *              of emissive * strength * 100
+ *   textures   base colour (sRGB), metallic-roughness and normal images, numbered in
+ *              the reference's load order (per material: base colour, MR, normal)
+ *   instances  one per node with a mesh, depth-first from scene 0's roots, world =
+ *              parent * T * R * S, the node's primitive materials in mat_id[] (use_remap)
+ *   lights     nodes with extras.arealight (FSceneLoader.cpp:152-172)
+ * plus the first camera node and scene 0's environment extras. Node matrices (as
+ * opposed to TRS), animations and sparse accessors are not read. */
+typedef struct gk_gltf gk_gltf;
+
+typedef struct gk_gltf_view {
+    const gk_mesh* meshes;           uint32_t num_meshes;
+    const gk_material* materials;    uint32_t num_materials;
+    const gk_instance* instances;    uint32_t num_instances;
+    const gk_light* lights;          uint32_t num_lights;
+    const gk_texture* textures;      uint32_t num_textures;
+    uint32_t num_cameras;            /* camera nodes; the first one is in camera        */
+    gk_camera_desc camera;           /* eye, eye + forward, up (0,1,0), fov 40          */
+    uint32_t has_sky, has_sun;       /* scene 0 extras (FSceneLoader.cpp:512-550)       */
+    int32_t sky_idx;
+    float sky_intensity, sky_rotation, sun_intensity, sun_rotation;
+} gk_gltf_view;
+
+/* Parse a .glb / .gltf file, or .glb bytes in memory. GK_ERR_IO: unreadable file;
+ * GK_ERR_ARG: malformed or unsupported content (gk_gltf_error() says what). */
+int gk_gltf_load(const char* path, gk_gltf** out);
+int gk_gltf_load_memory(const void* glb, size_t bytes, gk_gltf** out);
+/* Pointers stay valid until gk_gltf_free. */
+int gk_gltf_get_view(gk_gltf* scene, gk_gltf_view* out);
+/* gk_scene_upload + gk_textures_upload of the loaded arrays. */
+int gk_gltf_upload(gk_ctx* ctx, gk_gltf* scene);
+int gk_gltf_free(gk_gltf* scene);
+/* The calling thread's last glTF loader message ("" if none). */
+const char* gk_gltf_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,3 +114,48 @@ def test_texture_upload_arguments_and_unknown_ids():
             assert L.gk_textures_upload(ctx.h, big, 1) == gk.GK_ERR_LIMIT
             assert L.gk_textures_upload(ctx.h, None, 0) == 0
     assert np.array_equal(outs[0].view(np.uint16), outs[1].view(np.uint16))
+
+
+def test_gltf_scene_renders_like_the_oracle():
+    """The loader's scene (textures, node transforms, section-indexed materials, an emissive
+    primitive) uploaded by gk_gltf_upload renders bit for bit as the same arrays uploaded by
+    gk_scene_upload, and REF_V2 matches the oracle on it."""
+    from tests.test_gltf import make_scene
+    B, _ = make_scene()
+    g = gk.GltfScene(B.glb())
+    s = g.scene
+    pts = []
+    for inst in s.instances[:2]:  # frame the quad and the parts (the light node sits far above)
+        M = np.array(list(inst.world), np.float64).reshape(4, 4).T
+        P = s.meshes[inst.mesh_id]["positions"].astype(np.float64)
+        pts.append(P @ M[:3, :3].T + M[:3, 3])
+    pts = np.concatenate(pts)
+    c, ext = pts.mean(0), np.ptp(pts, 0).max()
+    W = H = 96
+    p = gk.make_frame_params(tuple(c + np.array([0.3, 0.4, 0.7]) * ext), tuple(c), W, H, fov=50.0)
+    p.estimator = gk.ESTIMATOR_REF_V2
+    p.bounces = p.max_bounces = 4
+    p.samples = 1
+    p.has_sky = 1
+    p.sky_color[:] = [0.6, 0.7, 0.9, 1.0]
+    outs = []
+    for via_loader in (True, False):
+        with gk.Context(W, H) as ctx:
+            if via_loader:
+                g.upload(ctx)
+            else:
+                ctx.upload(s)
+            ctx.frame(p)
+            ctx.render()
+            ctx.sync()
+            outs.append((ctx.read("SINGLE_DIFFUSE"), ctx.read("SINGLE_SPECULAR"), ctx.read("ALBEDO")))
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint16), b.view(np.uint16))
+    o = po.OracleScene(s).render_refv2(p, W, H)
+    gd, gs, ga = outs[0]
+    same = np.all(gd.view(np.uint16) == o["diffuse"].astype(np.float16).view(np.uint16), axis=2) & \
+        np.all(gs.view(np.uint16) == o["spec"].astype(np.float16).view(np.uint16), axis=2)
+    assert same.mean() > 0.98, same.mean()
+    assert np.array_equal(ga.view(np.uint16), o["albedo"].astype(np.float16).view(np.uint16))
+    hit = o["objid"] != 65535
+    assert hit.mean() > 0.05
