@@ -20,6 +20,7 @@ LIB_PATH = PKG / "_lib" / "libgknx_rt.so"
 GK_OK = 0
 GK_ERR_ARG, GK_ERR_HIP, GK_ERR_OOM, GK_ERR_STATE, GK_ERR_LIMIT, GK_ERR_NO_DEVICE, GK_ERR_IO = -1, -2, -3, -4, -5, -6, -7
 
+UPSCALE_LINEAR, UPSCALE_FSR_EASU = 0, 1
 MAT_LAMBERTIAN, MAT_METALLIC, MAT_DIELECTRIC, MAT_ISOTROPIC, MAT_DIFFUSE_LIGHT, MAT_MIXTURE = range(6)
 ESTIMATOR_REF_V2, ESTIMATOR_PT_MIS = 0, 1
 DENOISER_NONE, DENOISER_BILATERAL, DENOISER_UNET = 0, 1, 2
@@ -186,6 +187,9 @@ SIGNATURES = [
     ("gk_unet_train_step", C.c_int, [_P, _P, _P, _U, _U, _U, C.c_float, C.POINTER(C.c_float)]),
     ("gk_unet_infer", C.c_int, [_P, _P, _P, _U, _U]),
     ("gk_denoise_input", C.c_int, [_P, _P]),
+    ("gk_render_extent", C.c_int, [_U, _U, _U, C.POINTER(_U), C.POINTER(_U)]),
+    ("gk_upscale", C.c_int, [_P, C.c_int, _U, _U, _P]),
+    ("gk_screenshot", C.c_int, [_P, _P, _U, _U, C.c_char_p]),
     ("gk_gltf_load", C.c_int, [C.c_char_p, C.POINTER(_P)]),
     ("gk_gltf_load_memory", C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     ("gk_gltf_get_view", C.c_int, [_P, C.POINTER(GltfView)]),
@@ -408,6 +412,12 @@ class GltfScene:
             pass
 
 
+def render_extent(out_w: int, out_h: int, super_resolution: int) -> tuple[int, int]:
+    w, h = C.c_uint32(0), C.c_uint32(0)
+    check(lib().gk_render_extent(out_w, out_h, super_resolution, C.byref(w), C.byref(h)), "gk_render_extent")
+    return w.value, h.value
+
+
 def procgen_cornell() -> SceneData:
     L = lib()
     nv, nt = C.c_uint32(0), C.c_uint32(0)
@@ -543,6 +553,25 @@ class Context:
         n, p = C.c_size_t(0), C.c_void_p()
         check(lib().gk_target_info(self.h, RT[name], C.byref(n), C.byref(p)), "gk_target_info")
         return p.value
+
+    # output stage
+    def upscale(self, out_w: int, out_h: int, mode: int = UPSCALE_LINEAR, d_out: int | None = None):
+        """GK_RT_DENOISED resolved to out_w x out_h RGBA16F: into the device buffer d_out, or
+        read back to the host as a float16 array when d_out is None."""
+        import torch
+        buf = None
+        if d_out is None:
+            buf = torch.empty((out_h, out_w, 4), dtype=torch.float16, device="cuda")
+            torch.cuda.synchronize()
+            d_out = buf.data_ptr()
+        check(lib().gk_upscale(self.h, mode, out_w, out_h, d_out), "gk_upscale")
+        if buf is not None:
+            self.sync()
+            return buf.cpu().numpy()
+        return None
+
+    def screenshot(self, path, d_img: int | None = None, w: int = 0, h: int = 0):
+        check(lib().gk_screenshot(self.h, d_img, w, h, str(path).encode()), "gk_screenshot")
 
     # ambient-cube probe grid
     def probes_init(self, size_xy=None, size_z=None, unit=None, offset=None):

@@ -31,4 +31,22 @@ __device__ inline float gt_tonemap(float x) {
     return T_x * w0 + L_x * w1 + S_x * w2;
 }
 
+// LinearToST2084UE (Const_Func.slang:51-67): SMPTE ST 2084 (PQ) of luminance in nits
+__device__ inline float st2084(float lin) {
+    const float m1 = 0.1593017578125f, m2 = 78.84375f, c1 = 0.8359375f, c2 = 18.8515625f, c3 = 18.6875f;
+    const float L = lin / 10000.f;
+    const float Lm = powf(L, m1);
+    const float N1 = c1 + c2 * Lm, N2 = 1.0f + c3 * Lm;
+    return powf(N1 * (1.0f / N2), m2);
+}
+
+// the final transform of the compose pass (Process.DenoiseJBF.comp.slang:170-181): SDR
+// GT-tonemaps radiance * PaperWhiteNit / 40000; HDR tonemaps radiance / 2000, scales back
+// by 2000 and encodes radiance * PaperWhiteNit / 230 with ST 2084
+__device__ inline float display_encode(float v, float paper_white, bool hdr) {
+    if (!hdr) return gt_tonemap(v * (paper_white / 40000.0f));
+    const float t = gt_tonemap(v / 2000.f) * 2000.f;
+    return st2084(t * paper_white / 230.0f);
+}
+
 }  // namespace gk

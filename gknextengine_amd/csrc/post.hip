@@ -186,10 +186,11 @@ __global__ __launch_bounds__(64) void k_jbf(PostK K, PostT T) {
         const size_t i = size_t(y) * K.W + x;
         total = ld3(from_h4(T.acc_d[i])) * ld3(from_h4(T.acc_a[i])) + ld3(from_h4(T.acc_s[i]));
     }
-    const float sc = K.paper_white / 40000.0f;
-    T.out[size_t(y) * K.W + x] = K.linear ? to_h4(total.x, total.y, total.z, 1.f)
-                                          : to_h4(gt_tonemap(total.x * sc), gt_tonemap(total.y * sc),
-                                                  gt_tonemap(total.z * sc), 1.f);
+    const bool hdr = K.hdr != 0u;
+    T.out[size_t(y) * K.W + x] =
+        K.linear ? to_h4(total.x, total.y, total.z, 1.f)
+                 : to_h4(display_encode(total.x, K.paper_white, hdr), display_encode(total.y, K.paper_white, hdr),
+                         display_encode(total.z, K.paper_white, hdr), 1.f);
 }
 
 __global__ void k_copy_history(const ushort4* a, const ushort4* b, const ushort4* c, ushort4* pa, ushort4* pb,

@@ -412,14 +412,16 @@ __global__ void k_unpack(const float* Y, float* out, uint32_t nimg, uint32_t h, 
 
 // network output -> tonemapped display image (GK_RT_DENOISED, RGBA16F), as the bilateral
 // path writes it
-__global__ void k_unpack_display(const float* Y, ushort4* out, uint32_t h, uint32_t w, uint32_t wp, float scale,
-                                 uint32_t linear) {
+__global__ void k_unpack_display(const float* Y, ushort4* out, uint32_t h, uint32_t w, uint32_t wp,
+                                 float paper_white, uint32_t hdr, uint32_t linear) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= h * w) return;
     const uint32_t x = t % w, y = t / w;
     const float4 z = reinterpret_cast<const float4*>(Y)[size_t(y) * wp + x];
     const float r = fmaxf(expm1f(z.x), 0.f), g = fmaxf(expm1f(z.y), 0.f), b = fmaxf(expm1f(z.z), 0.f);
-    out[t] = linear ? to_h4(r, g, b, 1.f) : to_h4(gt_tonemap(r * scale), gt_tonemap(g * scale), gt_tonemap(b * scale), 1.f);
+    out[t] = linear ? to_h4(r, g, b, 1.f)
+                    : to_h4(display_encode(r, paper_white, hdr != 0u), display_encode(g, paper_white, hdr != 0u),
+                            display_encode(b, paper_white, hdr != 0u), 1.f);
 }
 
 // loss = sum (z - log(1+t))^2 / (3 n h w); g = d loss / d z as BF16 (16 channels)
@@ -884,7 +886,7 @@ int unet_denoise_frame(Ctx& c) {
     if ((s = pack_input(n, n.feat.ptr, 1, H, W, hp, wp)) != GK_OK) return s;
     if ((s = forward(n, 1, hp, wp)) != GK_OK) return s;
     k_unpack_display<<<nblk(n_px), 256, 0, c.stream>>>(n.y32.ptr, static_cast<ushort4*>(c.target_ptr[GK_RT_DENOISED]),
-                                                       H, W, wp, c.params.paper_white_nit / 40000.0f,
+                                                       H, W, wp, c.params.paper_white_nit, c.params.hdr ? 1u : 0u,
                                                        (c.params.flags & GK_FRAME_LINEAR_OUTPUT) ? 1u : 0u);
     GK_HIP(hipGetLastError());
     return GK_OK;

@@ -555,6 +555,26 @@ int gk_probes_read(gk_ctx* ctx, gk_ambient_cube* cubes, gk_voxel* voxels, uint32
 int gk_probes_write(gk_ctx* ctx, const gk_ambient_cube* cubes, const gk_voxel* voxels, uint32_t first,
                     uint32_t count);
 
+/* ---------------------------------------------------------------- output stage
+ * The compose pass writes GK_RT_DENOISED at the render extent: GT-tonemapped, or with
+ * gk_frame_params.hdr ST 2084 (PQ) of the tonemapped radiance scaled by PaperWhiteNit / 230
+ * (Process.DenoiseJBF.comp.slang:170-181, Const_Func.slang:51-67). */
+enum gk_upscale_mode { GK_UPSCALE_LINEAR = 0, GK_UPSCALE_FSR_EASU = 1 };
+/* The render extent of SuperResolution 0 / 1 / 2 (50 / 66 / 100 %): out * 2 / (4 - sr)
+ * (VulkanBaseRenderer.cpp:474-476). */
+int gk_render_extent(uint32_t out_w, uint32_t out_h, uint32_t super_resolution, uint32_t* w, uint32_t* h);
+/* The resolve pass (VulkanBaseRenderer.cpp:1041-1067): GK_RT_DENOISED to a device RGBA16F
+ * image of out_w x out_h. GK_UPSCALE_LINEAR is the live vkCmdBlitImage with
+ * VK_FILTER_LINEAR (pixel centres mapped into the source, bilinear, clamp to edge);
+ * GK_UPSCALE_FSR_EASU is Process.UpScaleFSR.comp.slang (FSR 1 EASU; a copy at equal
+ * sizes), the compute path the reference keeps compiled out. Enqueued on ctx's stream. */
+int gk_upscale(gk_ctx* ctx, int mode, uint32_t out_w, uint32_t out_h, void* d_out_rgba16f);
+/* ScreenShot::SaveSwapChainToFile (src/Runtime/ScreenShot.cpp:99-283): an RGBA16F device
+ * image of w x h (NULL: GK_RT_DENOISED at the render extent) stored as the UNORM swap chain
+ * stores it (round(clamp(x, 0, 1) * 255)) and written as an 8-bit RGB PNG (the reference
+ * writes JPEG at quality 91). Synchronises ctx's stream. */
+int gk_screenshot(gk_ctx* ctx, const void* d_rgba16f, uint32_t w, uint32_t h, const char* path);
+
 /* ---------------------------------------------------------------- glTF scenes
  * FSceneLoader::LoadGLTFScene (src/Assets/FSceneLoader.cpp:209-700) for .glb and .gltf
  * files (buffers in the GLB chunk, in files next to the .gltf, or base64 data URIs;

@@ -169,13 +169,30 @@ void oracle_reproject(const uint32_t* params, uint32_t W, uint32_t H, const floa
         }
 }
 
-// fparams: {bf_sigma, bf_sigma_lum, paper_white_nit}, bf_size: 0 disables the filter.
-// out: W*H*4 tonemapped colour (alpha 1).
+// LinearToST2084UE, Const_Func.slang:51-67
+float st2084(float lin) {
+    const float m1 = 0.1593017578125f, m2 = 78.84375f, c1 = 0.8359375f, c2 = 18.8515625f, c3 = 18.6875f;
+    const float L = lin / 10000.f;
+    const float Lm = std::pow(L, m1);
+    const float N1 = c1 + c2 * Lm, N2 = 1.0f + c3 * Lm;
+    return std::pow(N1 * (1.0f / N2), m2);
+}
+
+// the compose pass's output transform (Process.DenoiseJBF.comp.slang:170-181)
+float display(float v, float paper_white, bool hdr) {
+    if (!hdr) return gt(v * (paper_white / 40000.0f));
+    const float t = gt(v / 2000.f) * 2000.f;
+    return st2084(t * paper_white / 230.0f);
+}
+
+// fparams: {bf_sigma, bf_sigma_lum, paper_white_nit, hdr (0/1)}, bf_size: 0 disables the
+// filter. out: W*H*4 display colour (alpha 1).
 void oracle_jbf(const float* fparams, uint32_t W, uint32_t H, const float* acc_d, const float* acc_s,
                 const float* acc_a, const uint32_t* bf_size, float* out) {
     const int w = int(W), h = int(H);
     const Img4 D{acc_d, w, h}, S{acc_s, w, h};
-    const float sigma = fparams[0], sigma_l = fparams[1] * 100.0f, sc = fparams[2] / 40000.0f;
+    const float sigma = fparams[0], sigma_l = fparams[1] * 100.0f, pw = fparams[2];
+    const bool hdr = fparams[3] != 0.f;
     const float lc[3] = {0.212671f, 0.715160f, 0.072169f};
     for (int y = 0; y < h; ++y)
         for (int x = 0; x < w; ++x) {
@@ -208,9 +225,129 @@ void oracle_jbf(const float* fparams, uint32_t W, uint32_t H, const float* acc_d
             } else {
                 for (int c = 0; c < 3; ++c) total[c] = acc_d[4 * i + c] * acc_a[4 * i + c] + acc_s[4 * i + c];
             }
-            for (int c = 0; c < 3; ++c) out[4 * i + c] = gt(total[c] * sc);
+            for (int c = 0; c < 3; ++c) out[4 * i + c] = display(total[c], pw, hdr);
             out[4 * i + 3] = 1.f;
         }
 }
+
+// ---------------------------------------------------------------- resolve to the display
+// src, out: float RGBA images (row-major); mode 0: vkCmdBlitImage with VK_FILTER_LINEAR
+// (VulkanBaseRenderer.cpp:1050-1064), mode 1: Process.UpScaleFSR.comp.slang (FSR 1 EASU).
+void oracle_upscale(int mode, const float* src, uint32_t sw_, uint32_t sh_, uint32_t dw_, uint32_t dh_, float* out) {
+    const int sw = int(sw_), sh = int(sh_), dw = int(dw_), dh = int(dh_);
+    auto texel = [&](int x, int y, float c[4]) {
+        x = std::min(std::max(x, 0), sw - 1);
+        y = std::min(std::max(y, 0), sh - 1);
+        for (int k = 0; k < 4; ++k) c[k] = src[(size_t(y) * sw + x) * 4 + k];
+    };
+    for (int y = 0; y < dh; ++y)
+        for (int x = 0; x < dw; ++x) {
+            float* o = out + (size_t(y) * dw + x) * 4;
+            if (mode == 0) {
+                // destination centre in source texels, minus the texel-centre half
+                const float u = (float(x) + 0.5f) * (float(sw) / float(dw)) - 0.5f;
+                const float v = (float(y) + 0.5f) * (float(sh) / float(dh)) - 0.5f;
+                const float fu = std::floor(u), fv = std::floor(v);
+                const float a = u - fu, b = v - fv;
+                float p[4], q[4], r[4], t[4];
+                texel(int(fu), int(fv), p);
+                texel(int(fu) + 1, int(fv), q);
+                texel(int(fu), int(fv) + 1, r);
+                texel(int(fu) + 1, int(fv) + 1, t);
+                for (int k = 0; k < 4; ++k) {
+                    const float top = p[k] * (1.f - a) + q[k] * a;
+                    const float bot = r[k] * (1.f - a) + t[k] * a;
+                    o[k] = top * (1.f - b) + bot * b;
+                }
+                continue;
+            }
+            if (sw == dw && sh == dh) {  // the shader's copy branch
+                texel(x, y, o);
+                continue;
+            }
+            // FsrEasuCon with input viewport = input size
+            const float con0[4] = {float(sw) / float(dw), float(sh) / float(dh), 0.5f * float(sw) / float(dw) - 0.5f,
+                                   0.5f * float(sh) / float(dh) - 0.5f};
+            const float con1[4] = {1.f / float(sw), 1.f / float(sh), 1.f / float(sw), -1.f / float(sh)};
+            const float con2[4] = {-1.f / float(sw), 2.f / float(sh), 1.f / float(sw), 2.f / float(sh)};
+            const float con3[2] = {0.f, 4.f / float(sh)};
+            const float ip[2] = {(float(x) / float(dw)) * float(dw), (float(y) / float(dh)) * float(dh)};
+            float pp[2] = {ip[0] * con0[0] + con0[2], ip[1] * con0[1] + con0[3]};
+            const float fp[2] = {std::floor(pp[0]), std::floor(pp[1])};
+            pp[0] -= fp[0];
+            pp[1] -= fp[1];
+            const float p0[2] = {fp[0] * con1[0] + con1[2], fp[1] * con1[1] + con1[3]};
+            const float base[4][2] = {{p0[0], p0[1]}, {p0[0] + con2[0], p0[1] + con2[1]},
+                                      {p0[0] + con2[2], p0[1] + con2[3]}, {p0[0] + con3[0], p0[1] + con3[1]}};
+            const float offx[2] = {-.5f * con1[0], .5f * con1[0]}, offy[2] = {-.5f * con1[1], .5f * con1[1]};
+            // taps b c i j f e k l h g o n: (base, x offset -/+, y offset -/+)
+            static const int tb[12][3] = {{0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}, {1, 1, 0}, {1, 0, 0},
+                                          {2, 0, 1}, {2, 1, 1}, {2, 1, 0}, {2, 0, 0}, {3, 1, 0}, {3, 0, 0}};
+            float C[12][4], Lm[12];
+            for (int t = 0; t < 12; ++t) {
+                const float px = base[tb[t][0]][0] + offx[tb[t][1]], py = base[tb[t][0]][1] + offy[tb[t][2]];
+                texel(int(px * float(sw)), int(py * float(sh)), C[t]);
+                Lm[t] = C[t][1] + 0.5f * (C[t][0] + C[t][2]);
+            }
+            enum { B, Cc, I, J, F, E, K, L, Hh, G, O, N };
+            float dir[2] = {0.f, 0.f}, len = 0.f;
+            auto setf = [&](float w, float lA, float lB, float lC, float lD, float lE) {
+                float lenX = std::max(std::fabs(lD - lC), std::fabs(lC - lB));
+                const float dirX = lD - lB;
+                dir[0] += dirX * w;
+                // clamp as the GPU does: a flat neighbourhood's 0 / 0 becomes 0 (maxNum)
+                lenX = std::fmin(std::fmax(std::fabs(dirX) / lenX, 0.f), 1.f);
+                lenX *= lenX;
+                len += lenX * w;
+                float lenY = std::max(std::fabs(lE - lC), std::fabs(lC - lA));
+                const float dirY = lE - lA;
+                dir[1] += dirY * w;
+                lenY = std::fmin(std::fmax(std::fabs(dirY) / lenY, 0.f), 1.f);
+                lenY *= lenY;
+                len += lenY * w;
+            };
+            setf((1.f - pp[0]) * (1.f - pp[1]), Lm[B], Lm[E], Lm[F], Lm[G], Lm[J]);
+            setf(pp[0] * (1.f - pp[1]), Lm[Cc], Lm[F], Lm[G], Lm[Hh], Lm[K]);
+            setf((1.f - pp[0]) * pp[1], Lm[F], Lm[I], Lm[J], Lm[K], Lm[N]);
+            setf(pp[0] * pp[1], Lm[G], Lm[J], Lm[K], Lm[L], Lm[O]);
+            float dirR = dir[0] * dir[0] + dir[1] * dir[1];
+            const bool zro = dirR < (1.0f / 32768.0f);
+            dirR = 1.0f / std::sqrt(dirR);
+            if (zro) { dirR = 1.0f; dir[0] = 1.0f; }
+            dir[0] *= dirR;
+            dir[1] *= dirR;
+            len = len * 0.5f;
+            len *= len;
+            const float stretch = (dir[0] * dir[0] + dir[1] * dir[1]) / std::max(std::fabs(dir[0]), std::fabs(dir[1]));
+            const float len2[2] = {1.f + (stretch - 1.0f) * len, 1.f - .5f * len};
+            const float lob = .5f - .29f * len, clp = 1.f / lob;
+            static const float toff[12][2] = {{0, -1}, {1, -1}, {-1, 1}, {0, 1}, {0, 0}, {-1, 0},
+                                              {1, 1}, {2, 1}, {2, 0}, {1, 0}, {1, 2}, {0, 2}};
+            float aC[4] = {0, 0, 0, 0}, aW = 0.f;
+            for (int t = 0; t < 12; ++t) {
+                const float ox = toff[t][0] - pp[0], oy = toff[t][1] - pp[1];
+                float vx = ox * dir[0] + oy * dir[1];
+                float vy = ox * -dir[1] + oy * dir[0];
+                vx *= len2[0];
+                vy *= len2[1];
+                const float d2 = std::min(vx * vx + vy * vy, clp);
+                float wB = .4f * d2 - 1.f, wA = lob * d2 - 1.f;
+                wB *= wB;
+                wA *= wA;
+                wB = 1.5625f * wB - .5625f;
+                const float w = wB * wA;
+                for (int k = 0; k < 4; ++k) aC[k] += C[t][k] * w;
+                aW += w;
+            }
+            for (int k = 0; k < 4; ++k) {
+                const float mn = std::min(std::min(C[F][k], C[G][k]), std::min(C[J][k], C[K][k]));
+                const float mx = std::max(std::max(C[F][k], C[G][k]), std::max(C[J][k], C[K][k]));
+                o[k] = std::min(mx, std::max(mn, aC[k] / aW));
+            }
+        }
+}
+
+float oracle_st2084(float x) { return st2084(x); }
+float oracle_display(float v, float paper_white, int hdr) { return display(v, paper_white, hdr != 0); }
 
 }  // extern "C"

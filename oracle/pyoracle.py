@@ -68,6 +68,9 @@ def oracle():
             "oracle_reproject": (None, [_P, _U, _U] + [_P] * 12),
             "oracle_jbf": (None, [_P, _U, _U] + [_P] * 5),
             "oracle_gt_tonemap": (C.c_float, [C.c_float]),
+            "oracle_upscale": (None, [C.c_int, _P, _U, _U, _U, _U, _P]),
+            "oracle_st2084": (C.c_float, [C.c_float]),
+            "oracle_display": (C.c_float, [C.c_float, C.c_float, C.c_int]),
         }
         for name, (res, args) in {**sig, **optional}.items():
             if not hasattr(L, name):
@@ -378,11 +381,20 @@ def reproject(src_d, src_s, src_a, normal, prev_d, prev_s, prev_a, obj0, obj1, m
     return acc[0], acc[1], acc[2]
 
 
-def jbf(acc_d, acc_s, acc_a, sigma=2.0, sigma_lum=3.0, paper_white=600.0, bf_size=5):
+def jbf(acc_d, acc_s, acc_a, sigma=2.0, sigma_lum=3.0, paper_white=600.0, bf_size=5, hdr=False):
     H, W = acc_d.shape[:2]
-    fp = np.array([sigma, sigma_lum, paper_white], np.float32)
+    fp = np.array([sigma, sigma_lum, paper_white, 1.0 if hdr else 0.0], np.float32)
     bs = np.array([bf_size], np.uint32)
     out = np.zeros((H, W, 4), np.float32)
     d, s, a = _f4(acc_d), _f4(acc_s), _f4(acc_a)
     oracle().oracle_jbf(ptr(fp), W, H, ptr(d), ptr(s), ptr(a), ptr(bs), ptr(out))
+    return out
+
+
+def upscale(img, out_w, out_h, mode=0):
+    """The resolve pass on an (H, W, 4) float image (mode 0 linear blit, 1 FSR EASU)."""
+    a = np.ascontiguousarray(img, np.float32)
+    H, W = a.shape[:2]
+    out = np.zeros((out_h, out_w, 4), np.float32)
+    oracle().oracle_upscale(int(mode), ptr(a), W, H, out_w, out_h, ptr(out))
     return out
