@@ -16,7 +16,7 @@ TLAS refit every frame, 4 bounces, reprojection + bilateral, 120 frames) and ``c
 at 512x512, 4 spp, with its finite-difference and 4096-spp reference-gradient checks) and
 ``denoise`` (config 5: learned vs bilateral denoiser on 1-spp 1080p frames of the city and
 the Cornell box), ``probes`` (the ambient-cube probe grid: voxelize and bake over the
-city). ``--relmse N``
+city), ``output`` (SuperResolution 1 render, HDR compose, linear and FSR EASU resolves). ``--relmse N``
 adds the relMSE of the measured frame against an N-spp FP64-accumulated render of the
 same frame (``denoise`` defaults to N = 4096).
 """
@@ -667,6 +667,59 @@ def run_probes(args):
             "cpu_baseline": cpu}
 
 
+def run_output(args):
+    """The output stage on the city at SuperResolution 1 (render 1280x720, display
+    1920x1080): HDR compose, then the linear-blit and FSR EASU resolves, per frame."""
+    import torch
+
+    import gknextengine_amd as gk
+    from gknextengine_amd import scenes
+    ow, oh = args.width, args.height
+    W, H = gk.render_extent(ow, oh, 1)
+    ctx = gk.Context(W, H)
+    ctx.upload(gk.procgen_city(64, 4, 256, 64, 2024))
+    cam = scenes.CITY_CAM
+    p = gk.make_frame_params(cam["eye"], cam["target"], W, H, fov=cam["fov"])
+    p.estimator = gk.ESTIMATOR_PT_MIS
+    p.bounces = p.max_bounces = args.bounces
+    p.denoiser = gk.DENOISER_BILATERAL
+    p.hdr = 1
+    p.flags |= gk.FRAME_NO_MATERIAL_SORT
+    out = torch.empty((oh, ow, 4), dtype=torch.float16, device="cuda")
+    torch.cuda.synchronize()
+    res = {}
+    for name, mode in (("linear", gk.UPSCALE_LINEAR), ("fsr_easu", gk.UPSCALE_FSR_EASU)):
+        for f in range(args.warmup):
+            p.total_frames = f
+            ctx.frame(p)
+            ctx.render()
+            check = gk.lib().gk_upscale(ctx.h, mode, ow, oh, out.data_ptr())
+            assert check == 0
+        ctx.sync()
+        t0 = time.perf_counter()
+        for f in range(args.steps):
+            p.total_frames = args.warmup + f
+            ctx.frame(p)
+            ctx.render()
+        ctx.sync()
+        t_frame = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            gk.lib().gk_upscale(ctx.h, mode, ow, oh, out.data_ptr())
+        ctx.sync()
+        t_up = (time.perf_counter() - t0) / args.steps
+        res[name] = {"frame_ms": round(t_frame * 1e3, 3), "upscale_ms": round(t_up * 1e3, 4)}
+    ctx.close()
+    px = ow * oh
+    bytes_px = 8 + 8  # RGBA16F out + at least one RGBA16F source texel read per output pixel
+    return {"metric": METRIC, "value": res["fsr_easu"]["upscale_ms"], "unit": "ms (FSR EASU 1280x720 -> 1920x1080)",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["fsr_easu"]["frame_ms"],
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "city PT_MIS at SuperResolution 1 (1280x720) + HDR compose + resolve to 1920x1080",
+                       "render": [W, H], "display": [ow, oh], **{f"{k}_{m}": v for k, d in res.items() for m, v in d.items()},
+                       "resolve_min_bytes_per_launch": px * bytes_px}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -675,7 +728,7 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--bounces", type=int, default=8)
-    ap.add_argument("--config", default="city", choices=["city", "dynamic", "cornell", "denoise", "diff", "probes"])
+    ap.add_argument("--config", default="city", choices=["city", "dynamic", "cornell", "denoise", "diff", "probes", "output"])
     ap.add_argument("--relmse", type=int, default=0, help="reference spp for relMSE (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sort", default="off", choices=["on", "off"], help="material-ordered shading")
@@ -688,7 +741,7 @@ def main():
               "shard).", file=sys.stderr)
         sys.exit(2)
     run = {"city": run_city, "dynamic": run_dynamic, "cornell": run_cornell, "denoise": run_denoise,
-           "diff": run_diff, "probes": run_probes}[args.config]
+           "diff": run_diff, "probes": run_probes, "output": run_output}[args.config]
     print(json.dumps(run(args)), flush=True)
 
 
