@@ -70,12 +70,21 @@ __device__ inline f3 filter_history(const ushort4* prev, float sx, float sy, int
     return h;
 }
 
+// The 5x5 neighbourhood (object ids, normals, colours) and the stride-2 motion samples of
+// an 8x8 block are staged in LDS first (zero outside the image, as the loads there return).
+constexpr int kRP = 2, kTileP = 8 + 2 * kRP, kTileM = 8 + 4 * kRP;
+
 __global__ __launch_bounds__(64) void k_reproject(PostK K, PostT T) {
-    const int x = int(blockIdx.x * 8 + threadIdx.x), y = int(blockIdx.y * 8 + threadIdx.y);
-    if (x >= int(K.W) || y >= int(K.H)) return;
-    const size_t i = size_t(y) * K.W + x;
-    const float4 cd = from_h4(T.src_d[i]), cs = from_h4(T.src_s[i]), ca = from_h4(T.src_a[i]);
+    __shared__ uint32_t s_obj[kTileP][kTileP];
+    __shared__ ushort4 s_nrm[kTileP][kTileP], s_d[kTileP][kTileP], s_s[kTileP][kTileP];
+    __shared__ float2 s_mv[kTileM][kTileM];
+    const int tx = int(threadIdx.x), ty = int(threadIdx.y);
+    const int gx = int(blockIdx.x) * 8, gy = int(blockIdx.y) * 8;
+    const int x = gx + tx, y = gy + ty;
     if (K.progressive) {
+        if (x >= int(K.W) || y >= int(K.H)) return;
+        const size_t i = size_t(y) * K.W + x;
+        const float4 cd = from_h4(T.src_d[i]), cs = from_h4(T.src_s[i]), ca = from_h4(T.src_a[i]);
         const float t = 1.0f / 512.0f;
         const float4 pd = from_h4(T.prev_d[i]), ps = from_h4(T.prev_s[i]), pa = from_h4(T.prev_a[i]);
         T.acc_d[i] = to_h4(lerpf(pd.x, cd.x, t), lerpf(pd.y, cd.y, t), lerpf(pd.z, cd.z, t), lerpf(pd.w, cd.w, t));
@@ -83,6 +92,26 @@ __global__ __launch_bounds__(64) void k_reproject(PostK K, PostT T) {
         T.acc_a[i] = to_h4(lerpf(pa.x, ca.x, t), lerpf(pa.y, ca.y, t), lerpf(pa.z, ca.z, t), lerpf(pa.w, ca.w, t));
         return;
     }
+    const ushort4 z4 = make_ushort4(0, 0, 0, 0);
+    for (int k = ty * 8 + tx; k < kTileP * kTileP; k += 64) {
+        const int r = k / kTileP, cc = k % kTileP;
+        const int sx = gx + cc - kRP, sy = gy + r - kRP;
+        const bool in = inb(sx, sy, K);
+        const size_t j = in ? size_t(sy) * K.W + sx : 0;
+        s_obj[r][cc] = in ? T.obj0[j] : 0u;
+        s_nrm[r][cc] = in ? T.normal[j] : z4;
+        s_d[r][cc] = in ? T.src_d[j] : z4;
+        s_s[r][cc] = in ? T.src_s[j] : z4;
+    }
+    for (int k = ty * 8 + tx; k < kTileM * kTileM; k += 64) {
+        const int r = k / kTileM, cc = k % kTileM;
+        const int mx = gx + cc - 2 * kRP, my = gy + r - 2 * kRP;
+        s_mv[r][cc] = inb(mx, my, K) ? T.motion[size_t(my) * K.W + mx] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    if (x >= int(K.W) || y >= int(K.H)) return;
+    const size_t i = size_t(y) * K.W + x;
+    const float4 cd = from_h4(T.src_d[i]), cs = from_h4(T.src_s[i]), ca = from_h4(T.src_a[i]);
     const float2 mv = T.motion[i];
     const float fx = float(x) + mv.x, fy = float(y) + mv.y;
     const int ppx = int(floorf(fx)), ppy = int(floorf(fy));
@@ -102,21 +131,19 @@ __global__ __launch_bounds__(64) void k_reproject(PostK K, PostT T) {
     float tw = 0.f, motion_in_range = 0.f;
     for (int oy = -2; oy <= 2; ++oy)
         for (int ox = -2; ox <= 2; ++ox) {
-            const int sx = x + ox, sy = y + oy;
-            const uint32_t pid = ld_u32(T.obj0, sx, sy, K);
-            const f3 wn = ld3(ld_h4(T.normal, sx, sy, K));
+            const int r = ty + kRP + oy, cc = tx + kRP + ox;
+            const uint32_t pid = s_obj[r][cc];
+            const f3 wn = ld3(from_h4(s_nrm[r][cc]));
             const bool center = ox == 0 && oy == 0;
             const float dist = sqrtf(float(ox * ox + oy * oy));
             const float w = calc_weight(dist, pid == cur, center, wn, cn);
-            const float4 a = ld_h4(T.src_d, sx, sy, K), b = ld_h4(T.src_s, sx, sy, K);
+            const float4 a = from_h4(s_d[r][cc]), b = from_h4(s_s[r][cc]);
             sd.x += a.x * w; sd.y += a.y * w; sd.z += a.z * w; sd.w += a.w * w;
             ss.x += b.x * w; ss.y += b.y * w; ss.z += b.z * w; ss.w += b.w * w;
             tw += w;
-            const int mx = x + ox * 2, my = y + oy * 2;
-            if (inb(mx, my, K)) {
-                const float2 m2 = T.motion[size_t(my) * K.W + mx];
-                motion_in_range += sqrtf(m2.x * m2.x + m2.y * m2.y);
-            }
+            // out-of-image samples are zero in the tile and add sqrt(0) = 0
+            const float2 m2 = s_mv[ty + 2 * kRP + 2 * oy][tx + 2 * kRP + 2 * ox];
+            motion_in_range += sqrtf(m2.x * m2.x + m2.y * m2.y);
         }
     sd.x /= tw; sd.y /= tw; sd.z /= tw; sd.w /= tw;
     ss.x /= tw; ss.y /= tw; ss.z /= tw; ss.w /= tw;
