@@ -301,7 +301,7 @@ __device__ inline void write_miss_gbuffer(const FrameK& K, const Targets& T, uin
 // REF_V2 in PathState
 __global__ void k_raygen(FrameK K, Queues Q, PathState S, uint32_t n, uint32_t queue_state) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) { Q.cnt[0] = n; Q.cnt[1] = 0u; Q.cnt[2] = 0u; }
+    if (p == 0) { Q.cnt[0] = n; Q.cnt[1] = 0u; Q.cnt[2] = 0u; Q.cnt[3] = 0u; }
     if (p >= n) return;
     const uint32_t px = pixel_of(K, p);
     const uint32_t x = px % K.W, y = px / K.W;
@@ -654,7 +654,9 @@ __global__ __launch_bounds__(256) void k_ref_shade(DScene s, FrameK K, Targets T
 
 // PT_MIS flags (misc.y): bits 0-7 depth, bit8 lobe0 decided, bit9 lobe0 = diffuse,
 // bit10 last vertex was delta (or camera), bits 16-27 sample index
-__global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T, Queues Q, PathState S, int cur) {
+// sh: the shadow queue's counter (2 and 3 alternate between bounces)
+__global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T, Queues Q, PathState S, int cur,
+                                                  int sh) {
     const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = Q.cnt[cur];
     const int nxt = cur ^ 1;
@@ -909,7 +911,7 @@ __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T,
         nrng = make_uint4(seed.x, seed.y, seed.z, seed.w);
     }
     uint32_t slot, ss;
-    block_append2(&Q.cnt[nxt], emit, slot, &Q.cnt[2], emit_sh, ss);
+    block_append2(&Q.cnt[nxt], emit, slot, &Q.cnt[sh], emit_sh, ss);
     if (emit) {
         Q.o[nxt][slot] = ro;
         Q.d[nxt][slot] = rd;
@@ -975,9 +977,15 @@ __global__ __launch_bounds__(256) void k_sort_scatter(Queues Q, int cur) {
 }
 
 // shadow rays: add the NEE contribution of unoccluded rays
-__global__ void k_pt_shadow_resolve(Queues Q, PathState S) {
+// Also clears the counters the next bounce appends to: the queue this bounce shaded (cur)
+// and the other shadow counter; neither is read after this bounce's trace.
+__global__ void k_pt_shadow_resolve(Queues Q, PathState S, int cur, int sh) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Q.cnt[2]) return;
+    if (i == 0) {
+        Q.cnt[cur] = 0u;
+        Q.cnt[sh ^ 1] = 0u;
+    }
+    if (i >= Q.cnt[sh]) return;
     if (__float_as_uint(Q.shit[i].w) != 0u) return;  // occluded
     const uint32_t p = Q.spath[i];
     const float4 a = Q.sc_d[i], b = Q.sc_s[i];
@@ -1176,23 +1184,24 @@ int render_frame(Ctx& c) {
         // rays (occlusion), whose contributions are resolved before the next shade.
         const uint32_t iters = (K.bounces + 1u) * K.samples;
         for (uint32_t it = 0; it < iters; ++it) {
-            k_reset_counter<<<1, 1, 0, s>>>(Q.cnt + (cur ^ 1));
-            k_reset_counter<<<1, 1, 0, s>>>(Q.cnt + 2);
+            // queue counters: cur (size of this bounce's hits), cur ^ 1 (next queue, zero),
+            // sh (this bounce's shadow rays, zero); k_raygen and the previous resolve cleared them
+            const int sh = 2 + int(it & 1u);
             if (sort) {
                 k_sort_count<<<nblk(n), 256, 0, s>>>(S0, Q, cur);
                 k_sort_scan<<<1, 64, 0, s>>>(Q.sort_cnt);
                 k_sort_scatter<<<nblk(n), 256, 0, s>>>(Q, cur);
             }
-            k_pt_shade<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, cur);
+            k_pt_shade<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, cur, sh);
             GK_HIP(hipGetLastError());
             const int nxt = cur ^ 1;
             if (it + 1 < iters)
-                st = launch_trace_pair(c, Q.o[nxt], Q.d[nxt], Q.cnt + nxt, n, Q.hit, Q.hit_inst, Q.so, Q.sd, Q.cnt + 2,
+                st = launch_trace_pair(c, Q.o[nxt], Q.d[nxt], Q.cnt + nxt, n, Q.hit, Q.hit_inst, Q.so, Q.sd, Q.cnt + sh,
                                        n, Q.shit);
             else
-                st = launch_trace(c, Q.so, Q.sd, Q.cnt + 2, n, Q.shit, nullptr, true, nullptr);
+                st = launch_trace(c, Q.so, Q.sd, Q.cnt + sh, n, Q.shit, nullptr, true, nullptr);
             if (st != GK_OK) return st;
-            k_pt_shadow_resolve<<<nblk(n), 256, 0, s>>>(Q, P);
+            k_pt_shadow_resolve<<<nblk(n), 256, 0, s>>>(Q, P, cur, sh);
             GK_HIP(hipGetLastError());
             cur = nxt;
         }
