@@ -16,7 +16,8 @@ TLAS refit every frame, 4 bounces, reprojection + bilateral, 120 frames) and ``c
 at 512x512, 4 spp, with its finite-difference and 4096-spp reference-gradient checks) and
 ``denoise`` (config 5: learned vs bilateral denoiser on 1-spp 1080p frames of the city and
 the Cornell box), ``probes`` (the ambient-cube probe grid: voxelize and bake over the
-city), ``output`` (SuperResolution 1 render, HDR compose, linear and FSR EASU resolves). ``--relmse N``
+city), ``output`` (SuperResolution 1 render, HDR compose, linear and FSR EASU resolves), ``copy`` (a
+256 MiB device copy: the measured HBM ceiling and the counter calibration kernel). ``--relmse N``
 adds the relMSE of the measured frame against an N-spp FP64-accumulated render of the
 same frame (``denoise`` defaults to N = 4096).
 """
@@ -720,6 +721,32 @@ def run_output(args):
                        "resolve_min_bytes_per_launch": px * bytes_px}}
 
 
+def run_copy(args):
+    """A plain 256 MiB device-to-device copy (torch ``copy_``, kernel __amd_rocclr_copyBuffer):
+    the measured HBM ceiling beside the 8 TB/s peak, and the known-size kernel whose
+    FETCH_SIZE / WRITE_SIZE counters calibrate the traffic figures of the other kernels."""
+    import torch
+    n = 256 * 1024 * 1024 // 4
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    assert bool((y == 1).all())
+    gbs = 2 * 4 * n / dt / 1e9
+    return {"metric": "HBM copy bandwidth", "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "256 MiB device-to-device copy (read + write)", "bytes_per_step": 2 * 4 * n},
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -728,7 +755,7 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--bounces", type=int, default=8)
-    ap.add_argument("--config", default="city", choices=["city", "dynamic", "cornell", "denoise", "diff", "probes", "output"])
+    ap.add_argument("--config", default="city", choices=["city", "dynamic", "cornell", "denoise", "diff", "probes", "output", "copy"])
     ap.add_argument("--relmse", type=int, default=0, help="reference spp for relMSE (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sort", default="off", choices=["on", "off"], help="material-ordered shading")
@@ -741,7 +768,8 @@ def main():
               "shard).", file=sys.stderr)
         sys.exit(2)
     run = {"city": run_city, "dynamic": run_dynamic, "cornell": run_cornell, "denoise": run_denoise,
-           "diff": run_diff, "probes": run_probes, "output": run_output}[args.config]
+           "diff": run_diff, "probes": run_probes, "output": run_output,
+           "copy": run_copy}[args.config]
     print(json.dumps(run(args)), flush=True)
 
 
