@@ -118,9 +118,12 @@ __device__ inline f3 tvector(f3 v, const float* M) {
               M[8] * v.x + M[9] * v.y + M[10] * v.z};
 }
 
-// widen an interval end by 2^-20 relative (also well defined for +-inf)
-__device__ inline float widen_hi(float x) { return x > 0.f ? x * 1.00000095367431640625f : x * 0.99999904632568359375f; }
-__device__ inline float widen_lo(float x) { return x > 0.f ? x * 0.99999904632568359375f : x * 1.00000095367431640625f; }
+// widen an interval end by 2^-20 relative: x +- |x| * 2^-20 is exactly x * (1 +- 2^-20), and
+// the fma rounds it once, so this equals the rounded product; one instruction (|x| is a
+// source modifier). An end at +-infinity becomes NaN and the box is culled: a box whose
+// entry or exit lies at infinity holds no hit at a finite distance.
+__device__ inline float widen_hi(float x) { return __builtin_fmaf(__builtin_fabsf(x), 0x1p-20f, x); }
+__device__ inline float widen_lo(float x) { return __builtin_fmaf(-__builtin_fabsf(x), 0x1p-20f, x); }
 
 // slab test of the 4 children; returns entry distances (or +inf when missed)
 __device__ inline void slab4(const Bvh4Node& n, const Lane& L, float limit, float tl[4]) {
@@ -142,6 +145,17 @@ __device__ inline void slab4(const Bvh4Node& n, const Lane& L, float limit, floa
         hi = widen_hi(hi);
         tl[i] = (hi >= lo && hi >= L.tmin && lo <= limit) ? lo : __builtin_inff();
     }
+}
+
+// compare-exchange of two (distance, entry) pairs, nearer first
+__device__ inline void cswap(float (&t)[4], uint32_t (&c)[4], int a, int b) {
+    const bool s = t[b] < t[a];
+    const float ta = t[a], tb = t[b];
+    const uint32_t ca = c[a], cb = c[b];
+    t[a] = s ? tb : ta;
+    t[b] = s ? ta : tb;
+    c[a] = s ? cb : ca;
+    c[b] = s ? ca : cb;
 }
 
 // Moeller-Trumbore, tinybvh IntersectTri operation order (tiny_bvh.h:6818-6833)
@@ -306,37 +320,36 @@ __global__ __launch_bounds__(kBlock) void k_trace(
         const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
         const uint32_t mt4[4] = {n.meta.x, n.meta.y, n.meta.z, n.meta.w};
         // TLAS leaves push their instances; BLAS leaves join the internal children as
-        // entries (flag 0x40000000, count << 27, first triangle), all near to far
+        // entries (flag 0x40000000, count << 27, first triangle), all near to far. The
+        // four (distance, entry) pairs go through a fixed 5-exchange sorting network with
+        // rejected children at +inf, so every index is static (a variable-length sort
+        // compiles to chains of selects on a dynamic index); the accepted children are
+        // then a prefix of the sorted four. The order of equal distances does not matter:
+        // the closest hit is canonical.
         float kt[4];
         uint32_t kc[4];
-        int nk = 0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            if (!(tl[i] <= L.bt)) continue;
-            if (mt4[i] != 0u && tl_level) {
+            bool ok = tl[i] <= L.bt;
+            if (ok && mt4[i] != 0u && tl_level) {
                 for (uint32_t k = 0; k < mt4[i]; ++k)
                     push(L, ring, spill, gtid, stride, kInstFlag | tlas_prims[ch[i] + k]);
-                continue;
+                ok = false;
             }
-            kt[nk] = tl[i];
-            kc[nk] = mt4[i] == 0u ? ch[i] : (0x40000000u | (mt4[i] << 27) | ch[i]);
-            nk++;
+            kt[i] = ok ? tl[i] : __builtin_inff();
+            kc[i] = mt4[i] == 0u ? ch[i] : (0x40000000u | (mt4[i] << 27) | ch[i]);
         }
-        // insertion sort (<= 4 entries)
-        for (int i = 1; i < nk; ++i) {
-            const float t = kt[i];
-            const uint32_t c = kc[i];
-            int j = i - 1;
-            while (j >= 0 && kt[j] > t) { kt[j + 1] = kt[j]; kc[j + 1] = kc[j]; --j; }
-            kt[j + 1] = t;
-            kc[j + 1] = c;
-        }
-        if (nk == 0) {
-            L.node = kNone;
-        } else {
-            for (int i = nk - 1; i >= 1; --i) push(L, ring, spill, gtid, stride, kc[i]);
-            L.node = kc[0];
-        }
+        const int nk = int(kt[0] < __builtin_inff()) + int(kt[1] < __builtin_inff()) +
+                       int(kt[2] < __builtin_inff()) + int(kt[3] < __builtin_inff());
+        cswap(kt, kc, 0, 1);
+        cswap(kt, kc, 2, 3);
+        cswap(kt, kc, 0, 2);
+        cswap(kt, kc, 1, 3);
+        cswap(kt, kc, 1, 2);
+        if (nk > 3) push(L, ring, spill, gtid, stride, kc[3]);
+        if (nk > 2) push(L, ring, spill, gtid, stride, kc[2]);
+        if (nk > 1) push(L, ring, spill, gtid, stride, kc[1]);
+        L.node = nk > 0 ? kc[0] : kNone;
     }
 }
 
