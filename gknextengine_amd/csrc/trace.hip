@@ -135,6 +135,11 @@ __device__ inline void slab4(const Bvh4Node& n, const Lane& L, float limit, floa
     const float hz[4] = {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
+        // tinybvh's (plane - O) * rD, not one fma with a precomputed O * rD: for a ray
+        // whose origin lies on a box face and whose direction is parallel to it, the
+        // difference is exactly 0, so the box is culled as the reference culls it (its
+        // voxelize misses such grazing hits); the fma form is off by the rounding of O * rD
+        // there and enters the box.
         float a = (lx[i] - L.O.x) * L.rD.x, b = (hx[i] - L.O.x) * L.rD.x;
         float lo = fminf(a, b), hi = fmaxf(a, b);
         a = (ly[i] - L.O.y) * L.rD.y; b = (hy[i] - L.O.y) * L.rD.y;
@@ -324,32 +329,30 @@ __global__ __launch_bounds__(kBlock) void k_trace(
         // four (distance, entry) pairs go through a fixed 5-exchange sorting network with
         // rejected children at +inf, so every index is static (a variable-length sort
         // compiles to chains of selects on a dynamic index); the accepted children are
-        // then a prefix of the sorted four. The order of equal distances does not matter:
-        // the closest hit is canonical.
+        // then the finite prefix of the sorted four (slab4 returns +inf for a child it
+        // rejects, and accepts only entries <= L.bt). The order of equal distances does
+        // not matter: the closest hit is canonical.
         float kt[4];
         uint32_t kc[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            bool ok = tl[i] <= L.bt;
-            if (ok && mt4[i] != 0u && tl_level) {
+            kt[i] = tl[i];
+            if (tl_level && mt4[i] != 0u && tl[i] <= L.bt) {
                 for (uint32_t k = 0; k < mt4[i]; ++k)
                     push(L, ring, spill, gtid, stride, kInstFlag | tlas_prims[ch[i] + k]);
-                ok = false;
+                kt[i] = __builtin_inff();
             }
-            kt[i] = ok ? tl[i] : __builtin_inff();
             kc[i] = mt4[i] == 0u ? ch[i] : (0x40000000u | (mt4[i] << 27) | ch[i]);
         }
-        const int nk = int(kt[0] < __builtin_inff()) + int(kt[1] < __builtin_inff()) +
-                       int(kt[2] < __builtin_inff()) + int(kt[3] < __builtin_inff());
         cswap(kt, kc, 0, 1);
         cswap(kt, kc, 2, 3);
         cswap(kt, kc, 0, 2);
         cswap(kt, kc, 1, 3);
         cswap(kt, kc, 1, 2);
-        if (nk > 3) push(L, ring, spill, gtid, stride, kc[3]);
-        if (nk > 2) push(L, ring, spill, gtid, stride, kc[2]);
-        if (nk > 1) push(L, ring, spill, gtid, stride, kc[1]);
-        L.node = nk > 0 ? kc[0] : kNone;
+        if (kt[3] < __builtin_inff()) push(L, ring, spill, gtid, stride, kc[3]);
+        if (kt[2] < __builtin_inff()) push(L, ring, spill, gtid, stride, kc[2]);
+        if (kt[1] < __builtin_inff()) push(L, ring, spill, gtid, stride, kc[1]);
+        L.node = kt[0] < __builtin_inff() ? kc[0] : kNone;
     }
 }
 

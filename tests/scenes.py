@@ -179,3 +179,28 @@ def textured_cornell():
     s.materials[2].mra_texture_id = t_mra
     s.materials[0].normal_texture_id = t_nrm
     return s
+
+
+def face_plane_rays():
+    """Rays of the unit Cornell box whose origin lies exactly on a face plane (x, y or z in
+    {0, 1}) at grid points just outside, on and inside the box, each fired along the two
+    other axes (parallel to that plane) both ways: tinybvh's slab distance to that plane
+    is exactly 0 for them, which decides whether a box that ends on the plane is entered
+    (the probe voxelize casts such rays from probes that sit on wall planes)."""
+    coords = np.array([-0.0625, 0.0, 0.25, 0.5, 1.0, 1.0625], np.float32)
+    rays = []
+    for ax in range(3):
+        others = [a for a in range(3) if a != ax]
+        for plane in (0.0, 1.0):
+            for c0 in coords:
+                for c1 in coords:
+                    O = np.zeros(3, np.float32)
+                    O[ax] = plane
+                    O[others[0]] = c0
+                    O[others[1]] = c1
+                    for d_ax in others:
+                        for sgn in (1.0, -1.0):
+                            D = np.zeros(3, np.float32)
+                            D[d_ax] = sgn
+                            rays.append([*O, 0.0, *D, 4.0])
+    return np.asarray(rays, np.float32)

@@ -110,6 +110,18 @@ def test_tlas_bricks_bit_exact(device):
     assert np.array_equal(i2[hit2], oi2[hit2])
 
 
+def test_face_plane_rays_bit_exact(device):
+    """Origins on a face plane, directions parallel to it: the kernel's box test must
+    cull a box that ends on that plane exactly as tinybvh's (plane - O) * rD does."""
+    s = gk.procgen_cornell()
+    rays = scenes.face_plane_rays()
+    with gk.Context(8, 8) as ctx:
+        ctx.upload(s)
+        h, _ = gpu_trace(ctx, rays, device)
+    o, _ = po.OracleScene(s).trace(rays)
+    assert np.array_equal(_u32(h), _u32(o))
+
+
 def test_empty_and_edge_inputs(device):
     s = gk.procgen_cornell()
     with gk.Context(8, 8) as ctx:

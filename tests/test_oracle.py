@@ -163,6 +163,27 @@ def test_oracle_full_frame_vs_live_tinybvh():
     assert ties < 400
 
 
+def test_oracle_face_plane_rays_follow_tinybvh_culling():
+    """Rays that start on a face plane and run parallel to it: the oracle's closest hits
+    equal the reference's tinybvh, including the misses where Moeller-Trumbore alone would
+    report a hit on a triangle edge but the box ending on that plane is culled (its slab
+    exit is exactly t = 0); the traversal kernel must reproduce exactly this."""
+    s = gk.procgen_cornell()
+    rays = scenes.face_plane_rays()
+    tris = po.scene_triangles(s.meshes[0])
+    h, _ = po.OracleScene(s).trace(rays)
+    bf = brute_force(tris, rays)
+    culled = (_u32(h[:, 3]) == 0xFFFFFFFF) & (_u32(bf[:, 3]) != 0xFFFFFFFF)
+    # e.g. (1, -1/16, 0) fired along +y meets the floor's corner, which the box test culls
+    assert culled.sum() > 0
+    i = np.nonzero(np.all(rays[:, :3] == [1.0, -0.0625, 0.0], 1) & np.all(rays[:, 4:7] == [0, 1, 0], 1))[0]
+    assert len(i) >= 1 and culled[i].all()
+    if po.ref_available():
+        g, _ = po.TinyBVH(tris).intersect(rays)
+        assert np.array_equal(_u32(h[:, 3]) == 0xFFFFFFFF, _u32(g[:, 3]) == 0xFFFFFFFF)
+        _compare_hits(h, g, "face-plane rays")
+
+
 # ------------------------------------------------------------------ RNG and samplers
 def test_pcg4d_known_answer_and_uniformity():
     # one step from (0,0,0,0), evaluated by hand: v = 1013904223 in every lane, then
