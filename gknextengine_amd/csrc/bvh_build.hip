@@ -288,6 +288,7 @@ struct CollapseArgs {
     Bvh4Node* out;
     uint32_t node_base;
     uint32_t leaf_base;
+    uint32_t blas;    // 1: leaf children hold their traversal stack entry (blas_leaf_entry)
     uint32_t* wide_count;
     uint32_t* depth;  // per wide node (may be null)
     uint32_t level;
@@ -358,10 +359,12 @@ __global__ void k_collapse(CollapseArgs a, const uint2* __restrict__ qin, const 
         if (e & kLeafBit) {
             child[j] = a.leaf_base + (e & ~kLeafBit);
             meta[j] = 1u;
+            if (a.blas) child[j] = blas_leaf_entry(child[j], 1u);
         } else if (a.bhi[e].w != 0.f) {
             const uint2 r = a.brange[e];
             child[j] = a.leaf_base + r.x;
             meta[j] = r.y - r.x + 1u;
+            if (a.blas) child[j] = blas_leaf_entry(child[j], meta[j]);
         } else {
             const uint32_t idx = atomicAdd(a.wide_count, 1u);
             child[j] = a.node_base + idx;
@@ -417,7 +420,7 @@ struct WideResult {
 // LBVH + SAH collapse + 4-wide collapse of n primitive boxes. Wide nodes are written
 // to out[node_base ...], leaf children point at leaf_base + sorted position.
 int build_wide(Ctx& c, const float4* plo, const float4* phi, uint32_t n, uint32_t max_leaf, Bvh4Node* out,
-               uint32_t node_base, uint32_t leaf_base, uint32_t* depth_out, WideResult& res) {
+               uint32_t node_base, uint32_t leaf_base, uint32_t* depth_out, WideResult& res, bool blas) {
     BuildScratch& s = c.scratch;
     hipStream_t st = c.stream;
     GK_HIP(s.keys.resize(n)); GK_HIP(s.keys_alt.resize(n));
@@ -459,7 +462,7 @@ int build_wide(Ctx& c, const float4* plo, const float4* phi, uint32_t n, uint32_
     uint2* qb = reinterpret_cast<uint2*>(s.queue_b.ptr);
     k_set_root<<<1, 1, 0, st>>>(qa, s.counters.ptr, root_elem);
     CollapseArgs a{s.bchild.ptr, s.brange.ptr, s.bnode_lo.ptr, s.bnode_hi.ptr, plo, phi, vals,
-                   out, node_base, leaf_base, s.counters.ptr + 0, depth_out, 0};
+                   out, node_base, leaf_base, blas ? 1u : 0u, s.counters.ptr + 0, depth_out, 0};
     uint32_t qcount = 1, level = 0;
     uint32_t* cin = s.counters.ptr + 1;
     uint32_t* cout = s.counters.ptr + 2;
@@ -765,7 +768,7 @@ int build_blas(Ctx& c, const gk_mesh* meshes, uint32_t num_meshes) {
         GK_HIP(hipGetLastError());
         WideResult r;
         int s = build_wide(c, c.scratch.prim_lo.ptr, c.scratch.prim_hi.ptr, nt, 4, c.blas_nodes.ptr, node_base,
-                           tri_base, nullptr, r);
+                           tri_base, nullptr, r, true);
         if (s != GK_OK) return s;
         k_reorder_tris<<<nblk(nt), 256, 0, st>>>(tmp.ptr, r.vals, nt, c.tris.ptr + tri_base, c.tri_rec.ptr + tri_base);
         GK_HIP(hipGetLastError());
@@ -851,7 +854,7 @@ int build_tlas(Ctx& c) {
     GK_HIP(hipEventCreate(&e0));
     GK_HIP(hipEventCreate(&e1));
     GK_HIP(hipEventRecord(e0, st));
-    int s = build_wide(c, c.inst_lo.ptr, c.inst_hi.ptr, n, 2, c.tlas_nodes.ptr, 0, 0, depth.ptr, r);
+    int s = build_wide(c, c.inst_lo.ptr, c.inst_hi.ptr, n, 2, c.tlas_nodes.ptr, 0, 0, depth.ptr, r, false);
     if (s != GK_OK) return s;
     k_copy_u32<<<nblk(n), 256, 0, st>>>(r.vals, c.tlas_prims.ptr, n);
     GK_HIP(hipEventRecord(e1, st));

@@ -103,8 +103,15 @@ __device__ inline void sincos_2pi(T u, T& s, T& c) {
 // ---------------------------------------------------------------- acceleration structures
 // 4-wide BVH node, one 128-byte cache line. Child i is
 //   internal: meta[i] == 0, child[i] = node index
-//   leaf:     meta[i] = primitive count (> 0), child[i] = first primitive record
-//   empty:    lo > hi (never hit)
+//   leaf:     meta[i] = primitive count (> 0); child[i] = the first primitive record in a
+//             TLAS node, and in a BLAS node the traversal stack entry of the leaf,
+//             kBlasLeaf | count << 27 | first triangle record (blas_leaf_entry)
+//   empty:    a point box far outside the scene (never hit), child and meta 0
+constexpr uint32_t kBlasLeaf = 0x40000000u;
+constexpr uint32_t kLeafFirstMask = 0x07ffffffu;
+__host__ __device__ inline uint32_t blas_leaf_entry(uint32_t first, uint32_t count) {
+    return kBlasLeaf | (count << 27) | first;
+}
 struct alignas(128) Bvh4Node {
     float4 lox, hix, loy, hiy, loz, hiz;
     uint4 child;

@@ -269,7 +269,7 @@ __device__ inline bool trace_closest_single(const DScene& s, f3 O, f3 D, float t
                 continue;
             }
             for (uint32_t k = 0; k < me[i]; ++k) {
-                const TriRec tr = s.tris[ch[i] + k];
+                const TriRec tr = s.tris[(ch[i] & kLeafFirstMask) + k];
                 const f3 v0 = ld3(tr.v0), e1 = ld3(tr.e1), e2 = ld3(tr.e2);
                 const f3 h = cross(D, e2);
                 const float aa = dot(e1, h);
@@ -350,7 +350,7 @@ __device__ inline bool occluded_single(const DScene& s, f3 O, f3 D, float tmin, 
                 continue;
             }
             for (uint32_t k = 0; k < me[i]; ++k) {
-                const TriRec tr = s.tris[ch[i] + k];
+                const TriRec tr = s.tris[(ch[i] & kLeafFirstMask) + k];
                 const f3 v0 = ld3(tr.v0), e1 = ld3(tr.e1), e2 = ld3(tr.e2);
                 const f3 h = cross(D, e2);
                 const float aa = dot(e1, h);

@@ -35,8 +35,12 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kRing = 16;            // LDS stack entries per lane
 constexpr int kSpill = 64;           // global spill entries per thread
-constexpr int kMaxBlocksPerCU = 8;   // spill-area capacity; launches use the kernel's occupancy
-constexpr uint32_t kRefill = 48;     // idle lanes that trigger a wave refill (<= 64)
+// Resident work-groups per CU of a trace launch (the spill area is sized for it): at most the
+// kernel's occupancy, capped at 4, i.e. 4 waves per SIMD. Fewer waves share each CU's caches;
+// measured on the 1080p city (trace ms per frame): 6 -> 3.13, 5 -> 3.03, 4 -> 3.00,
+// 3 -> 3.09, 2 -> 3.57.
+constexpr int kMaxBlocksPerCU = 4;
+constexpr uint32_t kRefill = 56;     // idle lanes that trigger a wave refill (<= 64)
 
 struct Lane {
     f3 O, D, rD;
@@ -211,7 +215,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(
     for (;;) {
         // ---- refill idle lanes: one atomic per wave, ranks from the ballot. A refill
         // stalls the whole wave on the ray loads, so it waits until kRefill lanes are idle
-        // (measured on the 1080p city: 16 -> 9.3 ms/frame, 48 -> 8.4 ms, 64 -> 8.5 ms).
+        // (measured on the 1080p city, trace ms per frame at 4 waves per SIMD: 32 -> 3.26,
+        // 40 -> 3.09, 48 -> 3.00, 52 -> 2.96, 56 -> 2.95, 60 -> 2.95, 64 -> 3.00).
         if (!exhausted) {
             const uint64_t need = __ballot(!L.active);
             const uint32_t n_need = __popcll(need);
@@ -298,8 +303,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(
         }
 
         const bool tl_level = TWO && L.level == 0;
-        if (!tl_level && (L.node & 0xc0000000u) == 0x40000000u) {  // a postponed BLAS leaf
-            const uint32_t first = L.node & 0x07ffffffu, cnt = (L.node >> 27) & 7u;
+        if (!tl_level && (L.node & 0xc0000000u) == kBlasLeaf) {  // a postponed BLAS leaf
+            const uint32_t first = L.node & kLeafFirstMask, cnt = (L.node >> 27) & 7u;
             bool term = false;
             for (uint32_t k = 0; k < cnt; ++k) {
                 const TriRec tr = tris[first + k];
@@ -325,7 +330,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(
         const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
         const uint32_t mt4[4] = {n.meta.x, n.meta.y, n.meta.z, n.meta.w};
         // TLAS leaves push their instances; BLAS leaves join the internal children as
-        // entries (flag 0x40000000, count << 27, first triangle), all near to far. The
+        // entries (a BLAS node stores a leaf child as its stack entry, kBlasLeaf | count
+        // << 27 | first triangle), all near to far. The
         // four (distance, entry) pairs go through a fixed 5-exchange sorting network with
         // rejected children at +inf, so every index is static (a variable-length sort
         // compiles to chains of selects on a dynamic index); the accepted children are
@@ -342,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(
                     push(L, ring, spill, gtid, stride, kInstFlag | tlas_prims[ch[i] + k]);
                 kt[i] = __builtin_inff();
             }
-            kc[i] = mt4[i] == 0u ? ch[i] : (0x40000000u | (mt4[i] << 27) | ch[i]);
+            kc[i] = ch[i];
         }
         cswap(kt, kc, 0, 1);
         cswap(kt, kc, 2, 3);
