@@ -767,7 +767,7 @@ int build_blas(Ctx& c, const gk_mesh* meshes, uint32_t num_meshes) {
                                              c.tri_mat.ptr, d_err.ptr);
         GK_HIP(hipGetLastError());
         WideResult r;
-        int s = build_wide(c, c.scratch.prim_lo.ptr, c.scratch.prim_hi.ptr, nt, 4, c.blas_nodes.ptr, node_base,
+        int s = build_wide(c, c.scratch.prim_lo.ptr, c.scratch.prim_hi.ptr, nt, kMaxLeafTris, c.blas_nodes.ptr, node_base,
                            tri_base, nullptr, r, true);
         if (s != GK_OK) return s;
         k_reorder_tris<<<nblk(nt), 256, 0, st>>>(tmp.ptr, r.vals, nt, c.tris.ptr + tri_base, c.tri_rec.ptr + tri_base);

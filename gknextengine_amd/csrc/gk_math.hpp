@@ -108,6 +108,7 @@ __device__ inline void sincos_2pi(T u, T& s, T& c) {
 //             kBlasLeaf | count << 27 | first triangle record (blas_leaf_entry)
 //   empty:    a point box far outside the scene (never hit), child and meta 0
 constexpr uint32_t kBlasLeaf = 0x40000000u;
+constexpr uint32_t kMaxLeafTris = 4;  // BLAS leaf size cap of the builder (<= 7: the count has 3 bits)
 constexpr uint32_t kLeafFirstMask = 0x07ffffffu;
 __host__ __device__ inline uint32_t blas_leaf_entry(uint32_t first, uint32_t count) {
     return kBlasLeaf | (count << 27) | first;

@@ -305,9 +305,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(
         const bool tl_level = TWO && L.level == 0;
         if (!tl_level && (L.node & 0xc0000000u) == kBlasLeaf) {  // a postponed BLAS leaf
             const uint32_t first = L.node & kLeafFirstMask, cnt = (L.node >> 27) & 7u;
+            // the next record is loaded while the current one is tested
+            TriRec nxt = tris[first];
             bool term = false;
+#pragma unroll 1
             for (uint32_t k = 0; k < cnt; ++k) {
-                const TriRec tr = tris[first + k];
+                const TriRec tr = nxt;
+                if (k + 1 < cnt) nxt = tris[first + k + 1];
                 float t, u, v;
                 if (!mt(tr, L.O, L.D, t, u, v)) continue;
                 if (!(t > L.tmin && t < L.tmax)) continue;
