@@ -1,10 +1,12 @@
 // post.hip - temporal reprojection, joint bilateral filter + tonemap, history copy.
 //
 //   k_reproject   Process.ReProject.comp.slang:71-223 (8x8 groups, one thread per pixel)
-//   k_jbf         Process.DenoiseJBF.comp.slang:75-181: an 18x18 LDS tile of
-//                 {diffuse, spec, normal, object id}, 6x6 taps at stride 2 without the
-//                 centre, weights exp(-d^2/2s^2) * exp(-dlum^2/2sL^2); GT tonemap
-//                 (Const_Func.slang:84-126) of Total * PaperWhiteNit / 40000
+//   k_jbf         Process.DenoiseJBF.comp.slang:75-181: 16x16 pixels per work-group over
+//                 a 26x26 LDS tile of {diffuse, spec}, 6x6 taps at stride 2 without the
+//                 centre, weights exp(-d^2/2s^2) * exp(-dlum^2/2sL^2) (the spatial factor
+//                 takes four values; the luminance factor uses the hardware exp2, as a
+//                 shader's exp does); GT tonemap (Const_Func.slang:84-126) of
+//                 Total * PaperWhiteNit / 40000
 //   history copy  the "copy pass" accum -> prev of PathTracingRenderer.cpp:160-216
 // Storage images read out of bounds return zero, as robust Vulkan image loads do.
 #include <hip/hip_runtime.h>
@@ -164,17 +166,17 @@ __global__ __launch_bounds__(64) void k_reproject(PostK K, PostT T) {
     T.acc_a[i] = to_h4(lerpf(ha.x, ca.x, keep), lerpf(ha.y, ca.y, keep), lerpf(ha.z, ca.z, keep), oa.w);
 }
 
-constexpr int kR = 5, kTile = 8 + 2 * kR;
+constexpr int kR = 5, kJB = 16, kTile = kJB + 2 * kR;  // kJB x kJB pixels per work-group
 
-__global__ __launch_bounds__(64) void k_jbf(PostK K, PostT T) {
+__global__ __launch_bounds__(kJB * kJB) void k_jbf(PostK K, PostT T) {
     __shared__ float3 sd[kTile][kTile];
     __shared__ float3 ssp[kTile][kTile];
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int gx = blockIdx.x * 8, gy = blockIdx.y * 8;
+    const int gx = blockIdx.x * kJB, gy = blockIdx.y * kJB;
     const int x = gx + tx, y = gy + ty;
     f3 total;
     if (K.bf_size > 0u) {
-        for (int idx = ty * 8 + tx; idx < kTile * kTile; idx += 64) {
+        for (int idx = ty * kJB + tx; idx < kTile * kTile; idx += kJB * kJB) {
             const int row = idx / kTile, col = idx % kTile;
             const int cx = gx + col - kR, cy = gy + row - kR;
             const float4 d = ld_h4(T.acc_d, cx, cy, K), s = ld_h4(T.acc_s, cx, cy, K);
@@ -190,18 +192,27 @@ __global__ __launch_bounds__(64) void k_jbf(PostK K, PostT T) {
         const f3 lc = f3{0.212671f, 0.715160f, 0.072169f};
         const float clum = dot(cc, lc);
         const float sigma = K.bf_sigma, sigma_l = K.bf_sigma_lum * 100.0f;
+        // the spatial weight depends on i*i + j*j only, which is 2, 10, 18 or (clamped to
+        // distance 1) at least 26: four values, computed once with the per-tap expression
+        auto spatial = [&](int n) {
+            float dist = float(n) / float(kR * kR);
+            dist = fminf(fmaxf(dist, 0.f), 1.f);
+            return expf(-dist * dist / (2.0f * sigma * sigma));
+        };
+        const float F2 = spatial(2), F10 = spatial(10), F18 = spatial(18), Ffar = spatial(2 * kR * kR);
         f3 acc = f3{0.f, 0.f, 0.f};
         float wsum = 0.f;
+#pragma unroll
         for (int i = -kR; i <= kR; i += 2)
+#pragma unroll
             for (int j = -kR; j <= kR; j += 2) {
                 const float3 q = sd[ty + kR + i][tx + kR + j];
                 const f3 ci = f3{q.x + 0.001f, q.y + 0.001f, q.z + 0.001f};
                 const float lumi = dot(ci, lc);
-                float dist = float(i * i + j * j) / float(kR * kR);
-                dist = fminf(fmaxf(dist, 0.f), 1.f);
+                const int n = i * i + j * j;
                 const float dl = (clum - lumi) * (clum - lumi);
-                const float Fi = expf(-dist * dist / (2.0f * sigma * sigma));
-                const float Li = expf(-dl * dl / (2.0f * sigma_l * sigma_l));
+                const float Fi = n == 2 ? F2 : n == 10 ? F10 : n == 18 ? F18 : Ffar;
+                const float Li = __expf(-dl * dl / (2.0f * sigma_l * sigma_l));
                 acc = acc + ci * (Fi * Li);
                 wsum += Fi * Li;
             }
@@ -282,7 +293,8 @@ int post_process(Ctx& c) {
         const int s = unet_denoise_frame(c);
         if (s != GK_OK) return s;
     } else {
-        k_jbf<<<grid, blk, 0, c.stream>>>(K, T);
+        const dim3 jgrid((c.width + kJB - 1) / kJB, (c.height + kJB - 1) / kJB), jblk(kJB, kJB);
+        k_jbf<<<jgrid, jblk, 0, c.stream>>>(K, T);
         GK_HIP(hipGetLastError());
     }
     hipEventRecord(c.ev[6], c.stream);
