@@ -238,6 +238,40 @@ def run_city(args):
     }
 
 
+def dynamic_cpu_baseline(ctx, b, p, W, H, worlds):
+    """The reference's CPU side of the dynamic scene on host cores: tinybvh's TLAS rebuild
+    per frame over the 4096 brick instances (single thread, as UpdateBVH), its TLAS
+    Intersect on the frame's camera rays (all threads), and the restated reprojection +
+    bilateral filter on the last frame's own 1080p inputs (one thread)."""
+    import numpy as np
+
+    from oracle import pyoracle as po
+    if not po.ref_available():
+        return None
+    threads = min(os.cpu_count() or 1, 16)
+    bb = po.TinyBVH(po.scene_triangles(b.scene.meshes[0]))
+    tl = po.TinyTLAS([bb], worlds[0].reshape(-1, 16), np.zeros(b.n, np.uint32))
+    rebuild_ms = [tl.rebuild(worlds[f].reshape(-1, 16)) for f in range(1, 17)]
+    rays = po.primary_rays(p, W, H)
+    _, _, ms_trace = tl.intersect(rays, threads=threads)
+    h2f = lambda a: np.asarray(a).astype(np.float32)
+    src = [h2f(ctx.read(k)) for k in ("SINGLE_DIFFUSE", "SINGLE_SPECULAR", "ALBEDO", "NORMAL")]
+    acc = [h2f(ctx.read(k)) for k in ("ACCUM_DIFFUSE", "ACCUM_SPECULAR", "ACCUM_ALBEDO")]
+    t0 = time.perf_counter()
+    out = po.reproject(*src, *acc, ctx.read("OBJECTID_0"), ctx.read("OBJECTID_1"), ctx.read("MOTIONVECTOR"),
+                       ctx.read("MOTIONMOMENT"), acc, temporal_frames=16)
+    t1 = time.perf_counter()
+    po.jbf(*out)
+    t2 = time.perf_counter()
+    return {"value": round(rays.shape[0] / (ms_trace * 1e-3) / 1e6, 2), "unit": "Mrays/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"tinybvh 1.3.8 TLAS (4096 instances) Intersect on the {rays.shape[0]} camera rays of the last "
+                      f"frame; TLAS rebuild per frame (1 thread, 16 frames) and the restated reprojection + bilateral "
+                      f"of that frame (1 thread)",
+            "tlas_rebuild_ms": round(float(np.median(rebuild_ms)), 3),
+            "reproject_ms": round((t1 - t0) * 1e3, 1), "bilateral_ms": round((t2 - t1) * 1e3, 1)}
+
+
 def run_dynamic(args):
     """Config 3: 120 frames of 4096 moving bricks at 1080p, 4 bounces, TLAS refit per
     frame from device-resident matrices, reprojection + bilateral."""
@@ -281,6 +315,7 @@ def run_dynamic(args):
     s1 = ctx.stats()
     rays = s1.rays_traced - stats0.rays_traced
     ms = np.array(times[9:]) * 1e3
+    cpu = None if args.no_cpu_baseline else dynamic_cpu_baseline(ctx, b, p, W, H, worlds)
     extra = {}
     if args.relmse:
         den = ctx.read("DENOISED")[..., :3].astype(np.float64)
@@ -297,7 +332,8 @@ def run_dynamic(args):
                                    "PT_MIS 1 spp 4 bounces, reproject + bilateral", "resolution": [W, H],
                        "ms_frame_mean": round(float(ms.mean()), 3), "ms_frame_p95": round(float(np.percentile(ms, 95)), 3),
                        "tlas_ms": round(s1.ms_tlas, 3), "reproject_ms": round(s1.ms_reproject, 3),
-                       "denoise_ms": round(s1.ms_denoise, 3), **extra}}
+                       "denoise_ms": round(s1.ms_denoise, 3), **extra},
+            "cpu_baseline": cpu}
 
 
 def run_cornell(args):
@@ -340,12 +376,31 @@ def run_cornell(args):
         if args.relmse and est == gk.ESTIMATOR_PT_MIS:
             ref = reference_frame(ctx, p, args.relmse)
             res["relmse_16spp_vs_ref"] = relmse(img, ref)
+        if est == gk.ESTIMATOR_REF_V2:
+            refv2_params = gk.FrameParams.from_buffer_copy(p)
         out[name] = res
         ctx.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the REF_V2 restatement (FPathTracingRendererV2) on all host threads, 4 frames
+        from oracle import pyoracle as po
+        threads = min(os.cpu_count() or 1, 16)
+        osc = po.OracleScene(s)
+        nrays = 0
+        t0 = time.perf_counter()
+        for f in range(4):
+            refv2_params.total_frames = f
+            nrays += int(osc.render_refv2(refv2_params, W, H, threads=threads)["rays"])
+        dt = (time.perf_counter() - t0) / 4
+        cpu = {"value": round(nrays / 4 / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"oracle REF_V2 frames 0-3 of the Cornell box at {W}x{H}, 4 bounces "
+                         f"({dt * 1e3:.1f} ms/frame against {out['REF_V2']['ms_per_frame']} ms on the GPU)",
+               "ms_per_frame": round(dt * 1e3, 2)}
     return {"metric": METRIC, "value": out["PT_MIS"]["mrays_per_s"], "unit": "Mrays/s", "n_gpus": 1, "steps": 16,
             "warmup": 3, "ms_per_step": out["PT_MIS"]["ms_per_frame"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "Cornell box (config 1), 256x256, 16 frames x 1 spp, 4 bounces", **out}}
+            "config": {"workload": "Cornell box (config 1), 256x256, 16 frames x 1 spp, 4 bounces", **out},
+            "cpu_baseline": cpu}
 
 
 def denoise_compare(ctx, net, p, ref_spp, reps=20):
