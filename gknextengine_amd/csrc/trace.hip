@@ -129,6 +129,24 @@ __device__ inline f3 tvector(f3 v, const float* M) {
 __device__ inline float widen_hi(float x) { return __builtin_fmaf(__builtin_fabsf(x), 0x1p-20f, x); }
 __device__ inline float widen_lo(float x) { return __builtin_fmaf(-__builtin_fabsf(x), 0x1p-20f, x); }
 
+// slab test of one child box; returns its entry distance, or +inf when missed
+__device__ inline float slab1(float lx, float hx, float ly, float hy, float lz, float hz, f3 O, f3 rD,
+                              float tmin, float limit) {
+    // tinybvh's (plane - O) * rD, not one fma with a precomputed O * rD: for a ray whose
+    // origin lies on a box face and whose direction is parallel to it, the difference is
+    // exactly 0, so the box is culled as the reference culls it (its voxelize misses such
+    // grazing hits); the fma form is off by the rounding of O * rD there and enters the box.
+    float a = (lx - O.x) * rD.x, b = (hx - O.x) * rD.x;
+    float lo = fminf(a, b), hi = fmaxf(a, b);
+    a = (ly - O.y) * rD.y; b = (hy - O.y) * rD.y;
+    lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
+    a = (lz - O.z) * rD.z; b = (hz - O.z) * rD.z;
+    lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
+    lo = widen_lo(lo);
+    hi = widen_hi(hi);
+    return (hi >= lo && hi >= tmin && lo <= limit) ? lo : __builtin_inff();
+}
+
 // slab test of the 4 children; returns entry distances (or +inf when missed)
 __device__ inline void slab4(const Bvh4Node& n, const Lane& L, float limit, float tl[4]) {
     const float lx[4] = {n.lox.x, n.lox.y, n.lox.z, n.lox.w};
@@ -138,22 +156,7 @@ __device__ inline void slab4(const Bvh4Node& n, const Lane& L, float limit, floa
     const float lz[4] = {n.loz.x, n.loz.y, n.loz.z, n.loz.w};
     const float hz[4] = {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w};
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        // tinybvh's (plane - O) * rD, not one fma with a precomputed O * rD: for a ray
-        // whose origin lies on a box face and whose direction is parallel to it, the
-        // difference is exactly 0, so the box is culled as the reference culls it (its
-        // voxelize misses such grazing hits); the fma form is off by the rounding of O * rD
-        // there and enters the box.
-        float a = (lx[i] - L.O.x) * L.rD.x, b = (hx[i] - L.O.x) * L.rD.x;
-        float lo = fminf(a, b), hi = fmaxf(a, b);
-        a = (ly[i] - L.O.y) * L.rD.y; b = (hy[i] - L.O.y) * L.rD.y;
-        lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
-        a = (lz[i] - L.O.z) * L.rD.z; b = (hz[i] - L.O.z) * L.rD.z;
-        lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
-        lo = widen_lo(lo);
-        hi = widen_hi(hi);
-        tl[i] = (hi >= lo && hi >= L.tmin && lo <= limit) ? lo : __builtin_inff();
-    }
+    for (int i = 0; i < 4; ++i) tl[i] = slab1(lx[i], hx[i], ly[i], hy[i], lz[i], hz[i], L.O, L.rD, L.tmin, limit);
 }
 
 // compare-exchange of two (distance, entry) pairs, nearer first
@@ -184,6 +187,125 @@ __device__ inline bool mt(const TriRec& tr, f3 O, f3 D, float& t, float& u, floa
     return true;
 }
 
+// ---- quad traversal: four lanes per ray, for launches with few rays. The last bounces of
+// a frame trace a few thousand rays, so each lane of the per-lane traversal holds one ray
+// and a step runs the node and the leaf code in turn for a few waves: the launch lasts as
+// long as its slowest ray's chain of dependent steps. Here lane q of a quad tests child q
+// of a node, or triangle q of a leaf, and the quad shares one stack in LDS, which shortens
+// each step about fourfold. The per-child box test and the triangle test are the same
+// functions as in the per-lane traversal, and the closest hit is the smallest
+// (t, primitive), so the results are the same. Every branch below is uniform across a
+// quad, so all four lanes of a quad are active at each quad exchange.
+constexpr uint32_t kQuadMax = 65536;                 // launches of at most this many rays
+constexpr int kQuads = kBlock / 4;                   // quads per work-group
+constexpr int kQStack = kRing * kBlock / kQuads;     // stack entries per quad (the ring's LDS)
+
+// the value of lane j of this lane's quad (DPP quad permute, a VALU move)
+template <int J>
+__device__ inline uint32_t quad_get(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(v), J * 0x55, 0xf, 0xf, false));
+}
+template <int J>
+__device__ inline float quad_getf(float v) { return __uint_as_float(quad_get<J>(__float_as_uint(v))); }
+// the value of lane q ^ M of this lane's quad
+template <int M>
+__device__ inline uint32_t quad_xor(uint32_t v) {
+    constexpr int ctrl = (0 ^ M) | ((1 ^ M) << 2) | ((2 ^ M) << 4) | ((3 ^ M) << 6);
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(v), ctrl, 0xf, 0xf, false));
+}
+
+template <int M>
+__device__ inline void quad_min_hit(float& t, float& u, float& v, uint32_t& prim) {
+    const float ot = __uint_as_float(quad_xor<M>(__float_as_uint(t)));
+    const float ou = __uint_as_float(quad_xor<M>(__float_as_uint(u)));
+    const float ov = __uint_as_float(quad_xor<M>(__float_as_uint(v)));
+    const uint32_t op = quad_xor<M>(prim);
+    if (ot < t || (ot == t && op < prim)) { t = ot; u = ou; v = ov; prim = op; }
+}
+
+template <int MODE>
+__device__ void trace_quads(const Bvh4Node* __restrict__ blas, const TriRec* __restrict__ tris, uint32_t blas_root,
+                            const TraceJob& j0, const TraceJob& j1, uint32_t c0, uint32_t count,
+                            uint32_t (*stack)[kQuads], uint32_t* __restrict__ overflow_flag) {
+    const uint32_t q = __lane_id() & 3u;
+    const uint32_t qi = threadIdx.x >> 2;
+    const uint32_t nquads = gridDim.x * kQuads;
+    for (uint32_t r = blockIdx.x * kQuads + qi; r < count; r += nquads) {
+        const bool second = MODE == 2 && r >= c0;
+        const uint32_t ri = second ? r - c0 : r;
+        const bool any = MODE == 1 || second;
+        const float4 o = second ? j1.o[ri] : j0.o[ri];
+        const float4 d = second ? j1.d[ri] : j0.d[ri];
+        const f3 O = ld3(o), D = ld3(d);
+        const f3 rD = f3{safercp(D.x), safercp(D.y), safercp(D.z)};
+        const float tmin = o.w, tmax = d.w;
+        float bt = tmax, bu = 0.f, bv = 0.f;
+        uint32_t bprim = 0xffffffffu;
+        bool found = false, overflow = false;
+        uint32_t node = blas_root;
+        int sp = 0;
+        for (;;) {
+            if (node == kNone) {
+                if (sp == 0) break;
+                node = stack[--sp][qi];
+            }
+            if ((node & 0xc0000000u) == kBlasLeaf) {
+                const uint32_t first = node & kLeafFirstMask, cnt = (node >> 27) & 7u;
+                float ht = __builtin_inff(), hu = 0.f, hv = 0.f;
+                uint32_t hp = 0xffffffffu;
+                for (uint32_t k = q; k < cnt; k += 4u) {
+                    const TriRec tr = tris[first + k];
+                    float t, u, v;
+                    if (!mt(tr, O, D, t, u, v)) continue;
+                    if (!(t > tmin && t < tmax)) continue;
+                    const uint32_t prim = __float_as_uint(tr.v0.w);
+                    if (t < ht || (t == ht && prim < hp)) { ht = t; hu = u; hv = v; hp = prim; }
+                }
+                quad_min_hit<1>(ht, hu, hv, hp);
+                quad_min_hit<2>(ht, hu, hv, hp);
+                node = kNone;
+                if (hp != 0xffffffffu) {
+                    if (any) { found = true; break; }
+                    if (!found || ht < bt || (ht == bt && hp < bprim)) {
+                        bt = ht; bu = hu; bv = hv; bprim = hp; found = true;
+                    }
+                }
+                continue;
+            }
+            const Bvh4Node& N = blas[node];
+            const float* f = reinterpret_cast<const float*>(&N);
+            const float tq = slab1(f[q], f[4 + q], f[8 + q], f[12 + q], f[16 + q], f[20 + q], O, rD, tmin, bt);
+            const uint32_t cq = reinterpret_cast<const uint32_t*>(&N.child)[q];
+            float kt[4] = {quad_getf<0>(tq), quad_getf<1>(tq), quad_getf<2>(tq), quad_getf<3>(tq)};
+            uint32_t kc[4] = {quad_get<0>(cq), quad_get<1>(cq), quad_get<2>(cq), quad_get<3>(cq)};
+            cswap(kt, kc, 0, 1);
+            cswap(kt, kc, 2, 3);
+            cswap(kt, kc, 0, 2);
+            cswap(kt, kc, 1, 3);
+            cswap(kt, kc, 1, 2);
+#pragma unroll
+            for (int k = 3; k >= 1; --k) {
+                if (kt[k] < __builtin_inff()) {
+                    if (sp < kQStack) stack[sp++][qi] = kc[k];
+                    else overflow = true;
+                }
+            }
+            node = kt[0] < __builtin_inff() ? kc[0] : kNone;
+        }
+        if (q == 0u) {
+            if (any) {
+                float4* out = second ? j1.hit : j0.hit;
+                out[ri] = make_float4(0.f, 0.f, 0.f, __uint_as_float(found ? 1u : 0u));
+            } else {
+                j0.hit[ri] = found ? make_float4(bt, bu, bv, __uint_as_float(bprim))
+                                   : make_float4(__builtin_inff(), 0.f, 0.f, __uint_as_float(0xffffffffu));
+                if (j0.inst) j0.inst[ri] = found ? 0u : 0xffffffffu;
+            }
+            if (overflow) atomicOr(overflow_flag, 1u);
+        }
+    }
+}
+
 template <int MODE, bool TWO>
 __global__ __launch_bounds__(kBlock) void k_trace(
     const Bvh4Node* __restrict__ blas, const TriRec* __restrict__ tris,
@@ -212,6 +334,11 @@ __global__ __launch_bounds__(kBlock) void k_trace(
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(&ray_stats[MODE == 1 ? 1 : 0], (unsigned long long)c0);
         if (MODE == 2) atomicAdd(&ray_stats[1], (unsigned long long)c1);
+    }
+    if (!TWO && count <= kQuadMax) {
+        trace_quads<MODE>(blas, tris, blas_root, j0, j1, c0, count, reinterpret_cast<uint32_t (*)[kQuads]>(&ring[0][0]),
+                          overflow_flag);
+        return;
     }
 
     Lane L;

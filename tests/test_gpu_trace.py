@@ -83,6 +83,32 @@ def test_city_camera_and_random_rays(device):
     _compare_hits(h[: len(g["rays"])], g["hits"], "city gpu vs tinybvh")
 
 
+def test_small_launches_match_large(device):
+    """A launch of at most 65536 rays runs the four-lanes-per-ray traversal, a larger one the
+    one-lane-per-ray traversal: the same rays give the same bits either way, closest and any
+    hit, and the small launches equal the oracle."""
+    c = gk.procgen_city()
+    rr = scenes.random_rays(150000, -30.0, 30.0, seed=31, tmin=1e-3, tmax=1000.0)
+    rr[:, 1] = np.abs(rr[:, 1]) * 0.3
+    occ = rr.copy()
+    occ[:, 7] = 4.0
+    with gk.Context(64, 64) as ctx:
+        ctx.upload(c)
+        big, ibig = gpu_trace(ctx, rr, device)
+        small, ismall = gpu_trace(ctx, rr[:40000], device)
+        one, _ = gpu_trace(ctx, rr[7:8], device)
+        obig, _ = gpu_trace(ctx, occ, device, any_hit=True)
+        osmall, _ = gpu_trace(ctx, occ[:40000], device, any_hit=True)
+    assert np.array_equal(_u32(small), _u32(big[:40000]))
+    assert np.array_equal(ismall, ibig[:40000])
+    assert np.array_equal(_u32(one), _u32(big[7:8]))
+    assert np.array_equal(_u32(osmall[:, 3]), _u32(obig[:40000, 3]))
+    hit = _u32(small[:, 3]) != 0xFFFFFFFF
+    assert 0.2 < hit.mean() < 0.999 and 0.05 < (_u32(osmall[:, 3]) == 1).mean() < 0.95
+    o, _ = po.OracleScene(c).trace(rr[:40000])
+    assert np.array_equal(_u32(small), _u32(o))
+
+
 def test_tlas_bricks_bit_exact(device):
     b = scenes.Bricks()
     g = np.load(GOLDEN / "bricks_tinybvh.npz")
