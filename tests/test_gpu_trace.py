@@ -84,11 +84,12 @@ def test_city_camera_and_random_rays(device):
 
 
 def test_small_launches_match_large(device):
-    """A launch of at most 65536 rays runs the four-lanes-per-ray traversal, a larger one the
-    one-lane-per-ray traversal: the same rays give the same bits either way, closest and any
-    hit, and the small launches equal the oracle."""
+    """A launch of at most 8 rays per quad of the grid (524288 rays on 256 CUs) runs the
+    four-lanes-per-ray traversal, a larger one the one-lane-per-ray traversal: the same rays
+    give the same bits either way, closest and any hit, and the small launches equal the
+    oracle."""
     c = gk.procgen_city()
-    rr = scenes.random_rays(150000, -30.0, 30.0, seed=31, tmin=1e-3, tmax=1000.0)
+    rr = scenes.random_rays(1200000, -30.0, 30.0, seed=31, tmin=1e-3, tmax=1000.0)
     rr[:, 1] = np.abs(rr[:, 1]) * 0.3
     occ = rr.copy()
     occ[:, 7] = 4.0
