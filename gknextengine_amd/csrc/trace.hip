@@ -327,10 +327,11 @@ __global__ __launch_bounds__(kBlock) void k_trace(
     }
     const uint32_t count = c0 + c1;
     // rays a wave holds at once: its share of the launch, but at least kMinShare, so that a
-    // small launch spreads over more waves whose fewer rays diverge less (measured on the
-    // 1080p city, whose last bounces trace 3-50 K rays: trace 2.81 -> 2.79 ms per frame;
-    // shares of 1 or 2 rays cost more issue slots than they save: 2.87 ms). At least 1: a
-    // wave learns that the queue is drained only from a fetch.
+    // small launch that the quad traversal does not take (a two-level scene's) spreads over
+    // more waves whose fewer rays diverge less (measured with the city's last bounces before
+    // the quad traversal existed: trace 2.81 -> 2.79 ms per frame; shares of 1 or 2 rays cost
+    // more issue slots than they save: 2.87 ms). At least 1: a wave learns that the queue is
+    // drained only from a fetch.
     constexpr uint32_t kMinShare = 16u;
     const uint32_t waves = gridDim.x * (kBlock / 64u);
     const uint32_t quota = count > waves * kMinShare ? (count + waves - 1u) / waves : kMinShare;
