@@ -201,7 +201,11 @@ __device__ inline bool mt(const TriRec& tr, f3 O, f3 D, float& t, float& u, floa
 // the one-lane traversal alone 2.79)
 constexpr uint32_t kQuadRays = 8;
 constexpr int kQuads = kBlock / 4;                   // quads per work-group
-constexpr int kQStack = kRing * kBlock / kQuads;     // stack entries per quad (the ring's LDS)
+constexpr int kQStack = 128;                         // stack entries per quad (deeper than a lane's
+                                                     // 16 in LDS + kSpill in memory)
+static_assert(kQStack >= kRing + kSpill, "a quad holds at least a lane's stack depth");
+// LDS words of a work-group: the lanes' rings or the quads' stacks
+constexpr int kLdsWords = kRing * kBlock > kQStack * kQuads ? kRing * kBlock : kQStack * kQuads;
 
 // the value of lane j of this lane's quad (DPP quad permute, a VALU move)
 template <int J>
@@ -316,7 +320,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(
     const InstDev* __restrict__ inst, uint32_t blas_root, TraceJob j0, TraceJob j1,
     uint32_t* __restrict__ next_ray, uint32_t* __restrict__ spill,
     uint32_t* __restrict__ overflow_flag, unsigned long long* __restrict__ ray_stats) {
-    __shared__ uint32_t ring[kRing][kBlock];
+    __shared__ uint32_t lds[kLdsWords];
+    auto ring = reinterpret_cast<uint32_t (*)[kBlock]>(lds);
     const uint32_t stride = gridDim.x * kBlock;
     const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
     uint32_t c0 = j0.max_count, c1 = 0;
@@ -340,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(
         if (MODE == 2) atomicAdd(&ray_stats[1], (unsigned long long)c1);
     }
     if (!TWO && count <= kQuadRays * gridDim.x * kQuads) {
-        trace_quads<MODE>(blas, tris, blas_root, j0, j1, c0, count, reinterpret_cast<uint32_t (*)[kQuads]>(&ring[0][0]),
+        trace_quads<MODE>(blas, tris, blas_root, j0, j1, c0, count, reinterpret_cast<uint32_t (*)[kQuads]>(lds),
                           overflow_flag);
         return;
     }
