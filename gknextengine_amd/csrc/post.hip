@@ -1,6 +1,6 @@
 // post.hip - temporal reprojection, joint bilateral filter + tonemap, history copy.
 //
-//   k_reproject   Process.ReProject.comp.slang:71-223 (8x8 groups, one thread per pixel)
+//   k_reproject   Process.ReProject.comp.slang:71-223 (16x16 groups, one thread per pixel)
 //   k_jbf         Process.DenoiseJBF.comp.slang:75-181: 16x16 pixels per work-group over
 //                 a 26x26 LDS tile of {diffuse, spec}, 6x6 taps at stride 2 without the
 //                 centre, weights exp(-d^2/2s^2) * exp(-dlum^2/2sL^2) (the spatial factor
@@ -74,14 +74,16 @@ __device__ inline f3 filter_history(const ushort4* prev, float sx, float sy, int
 
 // The 5x5 neighbourhood (object ids, normals, colours) and the stride-2 motion samples of
 // an 8x8 block are staged in LDS first (zero outside the image, as the loads there return).
-constexpr int kRP = 2, kTileP = 8 + 2 * kRP, kTileM = 8 + 4 * kRP;
+// kRB x kRB pixels per work-group; tiles of the 5x5 neighbourhood (kTileP) and of the
+// motion samples two pixels apart (kTileM)
+constexpr int kRB = 16, kRP = 2, kTileP = kRB + 2 * kRP, kTileM = kRB + 4 * kRP;
 
-__global__ __launch_bounds__(64) void k_reproject(PostK K, PostT T) {
+__global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
     __shared__ uint32_t s_obj[kTileP][kTileP];
     __shared__ ushort4 s_nrm[kTileP][kTileP], s_d[kTileP][kTileP], s_s[kTileP][kTileP];
     __shared__ float2 s_mv[kTileM][kTileM];
     const int tx = int(threadIdx.x), ty = int(threadIdx.y);
-    const int gx = int(blockIdx.x) * 8, gy = int(blockIdx.y) * 8;
+    const int gx = int(blockIdx.x) * kRB, gy = int(blockIdx.y) * kRB;
     const int x = gx + tx, y = gy + ty;
     if (K.progressive) {
         if (x >= int(K.W) || y >= int(K.H)) return;
@@ -95,7 +97,7 @@ __global__ __launch_bounds__(64) void k_reproject(PostK K, PostT T) {
         return;
     }
     const ushort4 z4 = make_ushort4(0, 0, 0, 0);
-    for (int k = ty * 8 + tx; k < kTileP * kTileP; k += 64) {
+    for (int k = ty * kRB + tx; k < kTileP * kTileP; k += kRB * kRB) {
         const int r = k / kTileP, cc = k % kTileP;
         const int sx = gx + cc - kRP, sy = gy + r - kRP;
         const bool in = inb(sx, sy, K);
@@ -105,7 +107,7 @@ __global__ __launch_bounds__(64) void k_reproject(PostK K, PostT T) {
         s_d[r][cc] = in ? T.src_d[j] : z4;
         s_s[r][cc] = in ? T.src_s[j] : z4;
     }
-    for (int k = ty * 8 + tx; k < kTileM * kTileM; k += 64) {
+    for (int k = ty * kRB + tx; k < kTileM * kTileM; k += kRB * kRB) {
         const int r = k / kTileM, cc = k % kTileM;
         const int mx = gx + cc - 2 * kRP, my = gy + r - 2 * kRP;
         s_mv[r][cc] = inb(mx, my, K) ? T.motion[size_t(my) * K.W + mx] : make_float2(0.f, 0.f);
@@ -284,7 +286,7 @@ int post_process(Ctx& c) {
     T.motion = static_cast<const float2*>(c.target_ptr[GK_RT_MOTIONVECTOR]);
     T.moment = static_cast<const uint16_t*>(c.target_ptr[GK_RT_MOTIONMOMENT]);
     T.out = static_cast<ushort4*>(c.target_ptr[GK_RT_DENOISED]);
-    const dim3 grid((c.width + 7) / 8, (c.height + 7) / 8), blk(8, 8);
+    const dim3 grid((c.width + kRB - 1) / kRB, (c.height + kRB - 1) / kRB), blk(kRB, kRB);
     hipEventRecord(c.ev[4], c.stream);
     k_reproject<<<grid, blk, 0, c.stream>>>(K, T);
     GK_HIP(hipGetLastError());
