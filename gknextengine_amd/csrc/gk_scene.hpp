@@ -253,6 +253,10 @@ __device__ inline bool trace_closest_single(const DScene& s, f3 O, f3 D, float t
         const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
         const uint32_t me[4] = {n.meta.x, n.meta.y, n.meta.z, n.meta.w};
         node = 0xfffffffeu;
+        // internal children are visited near to far (pushed far first), so that a near
+        // hit culls the far subtrees; leaves are tested as they are met
+        float kt[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
+        uint32_t kc[4] = {0u, 0u, 0u, 0u};
         for (int i = 0; i < 4; ++i) {
             float a = (lx[i] - O.x) * rD.x, b = (hx[i] - O.x) * rD.x;
             float lo = fminf(a, b), hi = fmaxf(a, b);
@@ -263,7 +267,7 @@ __device__ inline bool trace_closest_single(const DScene& s, f3 O, f3 D, float t
             lo = lo > 0.f ? lo * 0.99999904632568359375f : lo * 1.00000095367431640625f;
             hi = hi > 0.f ? hi * 1.00000095367431640625f : hi * 0.99999904632568359375f;
             if (!(hi >= lo && hi >= tmin && lo <= best.t)) continue;
-            if (me[i] == 0u) { if (sp < 96) stack[sp++] = ch[i]; continue; }
+            if (me[i] == 0u) { kt[i] = lo; kc[i] = ch[i]; continue; }
             if (level == 0) {
                 for (uint32_t k = 0; k < me[i]; ++k) if (sp < 96) stack[sp++] = kInstFlag | s.tlas_prims[ch[i] + k];
                 continue;
@@ -291,6 +295,19 @@ __device__ inline bool trace_closest_single(const DScene& s, f3 O, f3 D, float t
                 }
             }
         }
+        // a fixed 5-exchange sorting network (static indices); the closest hit does not
+        // depend on the order of equal distances
+        auto cx = [&](int a, int b) {
+            const bool sw = kt[b] < kt[a];
+            const float ta = kt[a], tb = kt[b];
+            const uint32_t ca = kc[a], cb = kc[b];
+            kt[a] = sw ? tb : ta; kt[b] = sw ? ta : tb;
+            kc[a] = sw ? cb : ca; kc[b] = sw ? ca : cb;
+        };
+        cx(0, 1); cx(2, 3); cx(0, 2); cx(1, 3); cx(1, 2);
+#pragma unroll
+        for (int k = 3; k >= 0; --k)
+            if (kt[k] < __builtin_inff() && sp < 96) stack[sp++] = kc[k];
     }
 }
 
