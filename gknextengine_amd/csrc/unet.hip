@@ -80,9 +80,17 @@ struct ConvK {
 };
 
 // One work-group = 4 waves = a 16 x 16 pixel tile of one image and MT*16 output channels;
-// a wave owns 4 rows of 16 pixels (4 B tiles) and MT A tiles.
-template <int MT>
+// a wave owns 4 rows of 16 pixels (4 B tiles) and MT A tiles. STAGE: the work-group first
+// copies its 18 x 18 input tile (both inputs, upsampled, zero outside the image) into LDS,
+// a pixel's channels in cin / 8 16-byte units plus one unit of padding (an odd stride, so
+// the 16 lanes that read neighbouring pixels spread over the banks), and the 9 taps read
+// their operands from there instead of 9 gathers from memory.
+constexpr int kHalo = 18;
+__host__ __device__ inline uint32_t conv_tile_bytes(uint32_t cin) { return kHalo * kHalo * (cin / 8u + 1u) * 16u; }
+
+template <int MT, bool STAGE>
 __global__ __launch_bounds__(256) void k_conv(ConvK K) {
+    extern __shared__ uint4 tile[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int img = blockIdx.y / K.tiles_y, ty = blockIdx.y - img * K.tiles_y;
     const int W = int(K.W), H = int(K.H);
@@ -99,6 +107,27 @@ __global__ __launch_bounds__(256) void k_conv(ConvK K) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const uint16_t* wrow = K.w + size_t(co0 + (lane & 15)) * K.kpad + kg;
+    const int pstride = cin / 8 + 1;
+    if (STAGE) {
+        const int bx0 = int(blockIdx.x) * 16, by0 = ty * 16, cg = cin / 8;
+        for (int u = int(threadIdx.x); u < kHalo * kHalo * cg; u += 256) {
+            const int pix = u / cg, g = u - pix * cg;
+            const int r = pix / kHalo, c = pix - r * kHalo;
+            const int sy = by0 - 1 + r, sx = bx0 - 1 + c, c8 = g * 8;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (sx >= 0 && sx < W && sy >= 0 && sy < H) {
+                const uint16_t* src;
+                if (c8 < ca)
+                    src = K.a_up ? K.a + (pxa0 + size_t(sy >> 1) * (W >> 1) + (sx >> 1)) * ca + c8
+                                 : K.a + (px0 + size_t(sy) * W + sx) * ca + c8;
+                else
+                    src = K.b + (px0 + size_t(sy) * W + sx) * cb + (c8 - ca);
+                v = *reinterpret_cast<const uint4*>(src);
+            }
+            tile[pix * pstride + g] = v;
+        }
+        __syncthreads();
+    }
     int tap = kg / cin, ci = kg - tap * cin;
     for (uint32_t kc = 0; kc < K.kpad; kc += 32) {
         bf16x8 af[MT];
@@ -109,20 +138,27 @@ __global__ __launch_bounds__(256) void k_conv(ConvK K) {
         if (kk < K.ktap) {
             const int ty3 = tap / 3;
             const int dy = ty3 - 1, dx = tap - ty3 * 3 - 1;
-            const int sx = x + dx;
-            const bool xin = sx >= 0 && sx < W;
+            if (STAGE) {
+                const int lx = (lane & 15) + dx + 1;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int sy = y0 + j + dy;
-                bfr[j] = zero8();
-                if (xin && sy >= 0 && sy < H) {
-                    const uint16_t* src;
-                    if (ci < ca)
-                        src = K.a_up ? K.a + (pxa0 + size_t(sy >> 1) * (W >> 1) + (sx >> 1)) * ca + ci
-                                     : K.a + (px0 + size_t(sy) * W + sx) * ca + ci;
-                    else
-                        src = K.b + (px0 + size_t(sy) * W + sx) * cb + (ci - ca);
-                    bfr[j] = ld8(src);
+                for (int j = 0; j < 4; ++j)
+                    bfr[j] = __builtin_bit_cast(bf16x8, tile[((wv * 4 + j + dy + 1) * kHalo + lx) * pstride + (ci >> 3)]);
+            } else {
+                const int sx = x + dx;
+                const bool xin = sx >= 0 && sx < W;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int sy = y0 + j + dy;
+                    bfr[j] = zero8();
+                    if (xin && sy >= 0 && sy < H) {
+                        const uint16_t* src;
+                        if (ci < ca)
+                            src = K.a_up ? K.a + (pxa0 + size_t(sy >> 1) * (W >> 1) + (sx >> 1)) * ca + ci
+                                         : K.a + (px0 + size_t(sy) * W + sx) * ca + ci;
+                        else
+                            src = K.b + (px0 + size_t(sy) * W + sx) * cb + (ci - ca);
+                        bfr[j] = ld8(src);
+                    }
                 }
             }
         } else {
@@ -667,10 +703,14 @@ int ensure_ws(gk_unet& n, uint32_t nimg, uint32_t h, uint32_t w, bool grad) {
     return GK_OK;
 }
 
+// the input tile goes through LDS when it fits the default 64 KB of dynamic LDS (cin <= 80:
+// every full-resolution layer)
 template <int MT>
 void launch_conv_t(const ConvK& K, uint32_t nimg, hipStream_t st) {
     const dim3 grid((K.W + 15) / 16, K.tiles_y * nimg, K.cout / (MT * 16));
-    k_conv<MT><<<grid, 256, 0, st>>>(K);
+    const uint32_t bytes = conv_tile_bytes(K.cin);
+    if (bytes <= 65536u) k_conv<MT, true><<<grid, 256, bytes, st>>>(K);
+    else k_conv<MT, false><<<grid, 256, 0, st>>>(K);
 }
 
 int launch_conv(ConvK K, uint32_t nimg, hipStream_t st) {
