@@ -81,12 +81,17 @@ struct ConvK {
 
 // One work-group = 4 waves = a 16 x 16 pixel tile of one image and MT*16 output channels;
 // a wave owns 4 rows of 16 pixels (4 B tiles) and MT A tiles. STAGE: the work-group first
-// copies its 18 x 18 input tile (both inputs, upsampled, zero outside the image) into LDS,
-// a pixel's channels in cin / 8 16-byte units, and the 9 taps read their operands from
-// there instead of 9 gathers from memory. (One unit of padding per pixel, an odd stride
+// copies its input tile into LDS (zero outside the image) and the 9 taps read their
+// operands from there instead of 9 gathers from memory: 18 x 18 pixels with a pixel's
+// channels in 16-byte units, and for an upsampled first input its 10 x 10 half-resolution
+// pixels in a region of their own (W and H are even, so a pixel lies in the image exactly
+// when its half-resolution pixel does). (One unit of padding per pixel, an odd stride
 // across the banks, cost more through the larger tile: 1080p network 1.26 -> 1.31 ms.)
-constexpr int kHalo = 18;
-__host__ __device__ inline uint32_t conv_tile_bytes(uint32_t cin) { return kHalo * kHalo * (cin / 8u) * 16u; }
+constexpr int kHalo = 18, kHaloUp = 10;
+__host__ __device__ inline uint32_t conv_tile_bytes(uint32_t ca, uint32_t cb, uint32_t a_up) {
+    return a_up ? (kHalo * kHalo * (cb / 8u) + kHaloUp * kHaloUp * (ca / 8u)) * 16u
+                : kHalo * kHalo * ((ca + cb) / 8u) * 16u;
+}
 
 template <int MT, bool STAGE>
 __global__ __launch_bounds__(256) void k_conv(ConvK K) {
@@ -107,24 +112,36 @@ __global__ __launch_bounds__(256) void k_conv(ConvK K) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const uint16_t* wrow = K.w + size_t(co0 + (lane & 15)) * K.kpad + kg;
-    const int pstride = cin / 8;
+    // full-resolution region: channels [cfull0, cin) of each of the 18 x 18 pixels
+    const int cfull0 = K.a_up ? ca : 0;
+    const int pstride = (cin - cfull0) / 8;
+    const int ustride = ca / 8;                          // half-resolution region (a_up)
+    const int ubase = kHalo * kHalo * pstride;
+    const int bx0 = int(blockIdx.x) * 16, by0 = ty * 16;
+    const int ux0 = (bx0 - 1) >> 1, uy0 = (by0 - 1) >> 1;
     if (STAGE) {
-        const int bx0 = int(blockIdx.x) * 16, by0 = ty * 16, cg = cin / 8;
-        for (int u = int(threadIdx.x); u < kHalo * kHalo * cg; u += 256) {
-            const int pix = u / cg, g = u - pix * cg;
+        for (int u = int(threadIdx.x); u < kHalo * kHalo * pstride; u += 256) {
+            const int pix = u / pstride, g = u - pix * pstride;
             const int r = pix / kHalo, c = pix - r * kHalo;
-            const int sy = by0 - 1 + r, sx = bx0 - 1 + c, c8 = g * 8;
+            const int sy = by0 - 1 + r, sx = bx0 - 1 + c, c8 = cfull0 + g * 8;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
             if (sx >= 0 && sx < W && sy >= 0 && sy < H) {
-                const uint16_t* src;
-                if (c8 < ca)
-                    src = K.a_up ? K.a + (pxa0 + size_t(sy >> 1) * (W >> 1) + (sx >> 1)) * ca + c8
-                                 : K.a + (px0 + size_t(sy) * W + sx) * ca + c8;
-                else
-                    src = K.b + (px0 + size_t(sy) * W + sx) * cb + (c8 - ca);
+                const uint16_t* src = c8 < ca ? K.a + (px0 + size_t(sy) * W + sx) * ca + c8
+                                              : K.b + (px0 + size_t(sy) * W + sx) * cb + (c8 - ca);
                 v = *reinterpret_cast<const uint4*>(src);
             }
             tile[pix * pstride + g] = v;
+        }
+        if (K.a_up) {
+            for (int u = int(threadIdx.x); u < kHaloUp * kHaloUp * ustride; u += 256) {
+                const int pix = u / ustride, g = u - pix * ustride;
+                const int r = pix / kHaloUp, c = pix - r * kHaloUp;
+                const int uy = uy0 + r, ux = ux0 + c;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (ux >= 0 && ux < (W >> 1) && uy >= 0 && uy < (H >> 1))
+                    v = *reinterpret_cast<const uint4*>(K.a + (pxa0 + size_t(uy) * (W >> 1) + ux) * ca + g * 8);
+                tile[ubase + pix * ustride + g] = v;
+            }
         }
         __syncthreads();
     }
@@ -140,9 +157,19 @@ __global__ __launch_bounds__(256) void k_conv(ConvK K) {
             const int dy = ty3 - 1, dx = tap - ty3 * 3 - 1;
             if (STAGE) {
                 const int lx = (lane & 15) + dx + 1;
+                if (ci < cfull0) {  // upsampled input: its half-resolution pixel
+                    const int ucx = ((bx0 - 1 + lx) >> 1) - ux0;
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    bfr[j] = __builtin_bit_cast(bf16x8, tile[((wv * 4 + j + dy + 1) * kHalo + lx) * pstride + (ci >> 3)]);
+                    for (int j = 0; j < 4; ++j) {
+                        const int ucy = ((by0 + wv * 4 + j + dy) >> 1) - uy0;
+                        bfr[j] = __builtin_bit_cast(bf16x8, tile[ubase + (ucy * kHaloUp + ucx) * ustride + (ci >> 3)]);
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        bfr[j] = __builtin_bit_cast(
+                            bf16x8, tile[((wv * 4 + j + dy + 1) * kHalo + lx) * pstride + ((ci - cfull0) >> 3)]);
+                }
             } else {
                 const int sx = x + dx;
                 const bool xin = sx >= 0 && sx < W;
@@ -703,12 +730,12 @@ int ensure_ws(gk_unet& n, uint32_t nimg, uint32_t h, uint32_t w, bool grad) {
     return GK_OK;
 }
 
-// the input tile goes through LDS when it fits the default 64 KB of dynamic LDS (cin <= 96)
+// the input tile goes through LDS when it fits the default 64 KB of dynamic LDS
 template <int MT>
 void launch_conv_t(const ConvK& K, uint32_t nimg, hipStream_t st) {
     const dim3 grid((K.W + 15) / 16, K.tiles_y * nimg, K.cout / (MT * 16));
-    const uint32_t bytes = conv_tile_bytes(K.cin);
-    if (bytes <= 65536u) k_conv<MT, true><<<grid, 256, bytes, st>>>(K);
+    const uint32_t bytes = conv_tile_bytes(K.ca, K.cb, K.a_up);
+    if (bytes <= 65536u && (K.W % 2u) == 0u && (K.H % 2u) == 0u) k_conv<MT, true><<<grid, 256, bytes, st>>>(K);
     else k_conv<MT, false><<<grid, 256, 0, st>>>(K);
 }
 
