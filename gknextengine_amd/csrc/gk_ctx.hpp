@@ -211,6 +211,7 @@ struct Ctx {
     uint32_t tev_used = 0;
     uint64_t trace_launches = 0;
     bool stack_overflow = false;
+    bool trace_counter_zeroed = false;  // the next trace launch's counter is already 0
     gk_stats stats{};
 };
 
@@ -242,6 +243,9 @@ int launch_trace_pair(Ctx& c, const float4* o, const float4* d, const uint32_t* 
                       float4* hit, uint32_t* hit_inst, const float4* so, const float4* sd,
                       const uint32_t* scount_dev, uint32_t smax_count, float4* shit);
 int launch_trace_user(Ctx& c, const gk_ray* rays, uint32_t n, gk_hit* hits, uint32_t* inst, uint32_t flags);
+// the ray counter a trace launch starts from (allocates the trace's spill area); a caller
+// whose kernel zeroed it sets Ctx::trace_counter_zeroed, and the next launch skips its reset
+uint32_t* trace_counter(Ctx& c);
 int ensure_spill(Ctx& c, uint32_t threads);
 int collect_timing(Ctx& c);
 

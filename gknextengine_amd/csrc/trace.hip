@@ -617,16 +617,21 @@ int ensure_spill(Ctx& c, uint32_t threads) {
 
 // Launches one persistent trace over up to two jobs (mode as k_trace). Each launch is
 // bracketed by HIP events on the context stream (collected by collect_timing).
-static int launch_jobs(Ctx& c, int mode, const TraceJob& j0, const TraceJob& j1) {
+uint32_t* trace_counter(Ctx& c) {
     // the spill area is laid out for the largest grid, so its tail (counters) never moves
     const uint32_t cap_threads = cu_count(c.device) * kMaxBlocksPerCU * kBlock;
-    int st = ensure_spill(c, cap_threads);
-    if (st != GK_OK) return st;
-    uint32_t* tail = c.wf.spill.ptr + size_t(cap_threads) * kSpill;  // [next ray, overflow, stats u64 x2]
+    if (ensure_spill(c, cap_threads) != GK_OK) return nullptr;
+    return c.wf.spill.ptr + size_t(cap_threads) * kSpill;  // [next ray, overflow, stats u64 x2]
+}
+
+static int launch_jobs(Ctx& c, int mode, const TraceJob& j0, const TraceJob& j1) {
+    uint32_t* tail = trace_counter(c);
+    if (!tail) return GK_ERR_OOM;
     uint32_t* next = tail;
     uint32_t* ovf = tail + 1;
     unsigned long long* rs = reinterpret_cast<unsigned long long*>(tail + 2);
-    k_reset_counter<<<1, 1, 0, c.stream>>>(next);
+    if (!c.trace_counter_zeroed) k_reset_counter<<<1, 1, 0, c.stream>>>(next);
+    c.trace_counter_zeroed = false;
     const bool two = !c.blas_only;
     const uint32_t root = c.h_meshes.empty() ? 0u : c.h_meshes[0].root;
     const Bvh4Node* tl = c.tlas_nodes.ptr;

@@ -656,8 +656,9 @@ __global__ __launch_bounds__(256) void k_ref_shade(DScene s, FrameK K, Targets T
 // bit10 last vertex was delta (or camera), bits 16-27 sample index
 // sh: the shadow queue's counter (2 and 3 alternate between bounces)
 __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T, Queues Q, PathState S, int cur,
-                                                  int sh) {
+                                                  int sh, uint32_t* __restrict__ trace_next) {
     const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 == 0u) *trace_next = 0u;  // the ray counter of the trace launch that follows
     const uint32_t n = Q.cnt[cur];
     const int nxt = cur ^ 1;
     bool emit = false, emit_sh = false;
@@ -1183,6 +1184,8 @@ int render_frame(Ctx& c) {
         // extension rays it produced (closest hit, for the next iteration) and its shadow
         // rays (occlusion), whose contributions are resolved before the next shade.
         const uint32_t iters = (K.bounces + 1u) * K.samples;
+        uint32_t* tnext = trace_counter(c);
+        if (!tnext) return GK_ERR_OOM;
         for (uint32_t it = 0; it < iters; ++it) {
             // queue counters: cur (size of this bounce's hits), cur ^ 1 (next queue, zero),
             // sh (this bounce's shadow rays, zero); k_raygen and the previous resolve cleared them
@@ -1192,8 +1195,9 @@ int render_frame(Ctx& c) {
                 k_sort_scan<<<1, 64, 0, s>>>(Q.sort_cnt);
                 k_sort_scatter<<<nblk(n), 256, 0, s>>>(Q, cur);
             }
-            k_pt_shade<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, cur, sh);
+            k_pt_shade<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, cur, sh, tnext);
             GK_HIP(hipGetLastError());
+            c.trace_counter_zeroed = true;
             const int nxt = cur ^ 1;
             if (it + 1 < iters)
                 st = launch_trace_pair(c, Q.o[nxt], Q.d[nxt], Q.cnt + nxt, n, Q.hit, Q.hit_inst, Q.so, Q.sd, Q.cnt + sh,
