@@ -625,13 +625,14 @@ uint32_t* trace_counter(Ctx& c) {
 }
 
 static int launch_jobs(Ctx& c, int mode, const TraceJob& j0, const TraceJob& j1) {
+    const bool zeroed = c.trace_counter_zeroed;  // applies to this launch only
+    c.trace_counter_zeroed = false;
     uint32_t* tail = trace_counter(c);
     if (!tail) return GK_ERR_OOM;
     uint32_t* next = tail;
     uint32_t* ovf = tail + 1;
     unsigned long long* rs = reinterpret_cast<unsigned long long*>(tail + 2);
-    if (!c.trace_counter_zeroed) k_reset_counter<<<1, 1, 0, c.stream>>>(next);
-    c.trace_counter_zeroed = false;
+    if (!zeroed) k_reset_counter<<<1, 1, 0, c.stream>>>(next);
     const bool two = !c.blas_only;
     const uint32_t root = c.h_meshes.empty() ? 0u : c.h_meshes[0].root;
     const Bvh4Node* tl = c.tlas_nodes.ptr;
