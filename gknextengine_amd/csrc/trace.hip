@@ -546,12 +546,16 @@ __global__ __launch_bounds__(kBlock) void k_trace(
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             kt[i] = tl[i];
-            if (tl_level && mt4[i] != 0u && tl[i] <= L.bt) {
-                for (uint32_t k = 0; k < mt4[i]; ++k)
-                    push(L, ring, spill, gtid, stride, kInstFlag | tlas_prims[ch[i] + k]);
-                kt[i] = __builtin_inff();
-            }
             kc[i] = ch[i];
+            if (tl_level && mt4[i] != 0u && tl[i] <= L.bt) {
+                if (mt4[i] == 1u) {  // a one-instance leaf is ordered with the other children
+                    kc[i] = kInstFlag | tlas_prims[ch[i]];
+                } else {
+                    for (uint32_t k = 0; k < mt4[i]; ++k)
+                        push(L, ring, spill, gtid, stride, kInstFlag | tlas_prims[ch[i] + k]);
+                    kt[i] = __builtin_inff();
+                }
+            }
         }
         cswap(kt, kc, 0, 1);
         cswap(kt, kc, 2, 3);
@@ -562,6 +566,10 @@ __global__ __launch_bounds__(kBlock) void k_trace(
         if (kt[2] < __builtin_inff()) push(L, ring, spill, gtid, stride, kc[2]);
         if (kt[1] < __builtin_inff()) push(L, ring, spill, gtid, stride, kc[1]);
         L.node = kt[0] < __builtin_inff() ? kc[0] : kNone;
+        if (TWO && L.node != kNone && (L.node & kInstFlag)) {  // instances are entered by the pop path
+            push(L, ring, spill, gtid, stride, L.node);
+            L.node = kNone;
+        }
     }
 }
 
