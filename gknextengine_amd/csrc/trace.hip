@@ -208,6 +208,10 @@ __device__ inline bool mt(const TriRec& tr, f3 O, f3 D, float& t, float& u, floa
 // ms per frame, quad traversal from: 1 ray per quad 2.43, 8 -> 2.40, 16 -> 2.56, 64 -> 3.29;
 // the one-lane traversal alone 2.79)
 constexpr uint32_t kQuadRays = 8;
+// the same for a two-level scene, whose quads also enter and leave instances (4096 bricks at
+// 1080p, ms per frame: one-lane traversal alone 5.54, 1 ray per quad 5.48, 2 -> 5.47,
+// 4 -> 5.65, 8 -> 5.61)
+constexpr uint32_t kQuadRaysTwo = 2;
 constexpr int kQuads = kBlock / 4;                   // quads per work-group
 constexpr int kQStack = 128;                         // stack entries per quad (deeper than a lane's
                                                      // 16 in LDS + kSpill in memory)
@@ -238,8 +242,10 @@ __device__ inline void quad_min_hit(float& t, float& u, float& v, uint32_t& prim
     if (ot < t || (ot == t && op < prim)) { t = ot; u = ou; v = ov; prim = op; }
 }
 
-template <int MODE>
-__device__ void trace_quads(const Bvh4Node* __restrict__ blas, const TriRec* __restrict__ tris, uint32_t blas_root,
+template <int MODE, bool TWO>
+__device__ void trace_quads(const Bvh4Node* __restrict__ blas, const TriRec* __restrict__ tris,
+                            const Bvh4Node* __restrict__ tlas, const uint32_t* __restrict__ tlas_prims,
+                            const InstDev* __restrict__ inst, uint32_t blas_root,
                             const TraceJob& j0, const TraceJob& j1, uint32_t c0, uint32_t count,
                             uint32_t (*stack)[kQuads], uint32_t* __restrict__ overflow_flag) {
     const uint32_t q = __lane_id() & 3u;
@@ -251,20 +257,48 @@ __device__ void trace_quads(const Bvh4Node* __restrict__ blas, const TriRec* __r
         const bool any = MODE == 1 || second;
         const float4 o = second ? j1.o[ri] : j0.o[ri];
         const float4 d = second ? j1.d[ri] : j0.d[ri];
-        const f3 O = ld3(o), D = ld3(d);
-        const f3 rD = f3{safercp(D.x), safercp(D.y), safercp(D.z)};
+        const f3 wO = ld3(o), wD = ld3(d);
+        const f3 wrD = f3{safercp(wD.x), safercp(wD.y), safercp(wD.z)};
+        f3 O = wO, D = wD, rD = wrD;
         const float tmin = o.w, tmax = d.w;
         float bt = tmax, bu = 0.f, bv = 0.f;
-        uint32_t bprim = 0xffffffffu;
+        uint32_t bprim = 0xffffffffu, binst = 0xffffffffu;
+        uint32_t cur_inst = 0u, level = TWO ? 0u : 1u;
         bool found = false, overflow = false;
-        uint32_t node = blas_root;
+        uint32_t node = TWO ? 0u : blas_root;
         int sp = 0;
+        // enter instance ii: the ray goes to its object space, and a mark beneath its BLAS
+        // entries restores the world ray when they are done. False when the stack is full:
+        // the ray then ends (a dropped mark would leave TLAS entries read as BLAS nodes).
+        auto enter = [&](uint32_t ii) {
+            if (sp >= kQStack) return false;
+            stack[sp++][qi] = kStackMark;
+            const InstDev& I = inst[ii];
+            O = tpoint(wO, I.inv);
+            D = tvector(wD, I.inv);
+            rD = f3{safercp(D.x), safercp(D.y), safercp(D.z)};
+            cur_inst = ii;
+            level = 1u;
+            node = I.root;
+            return true;
+        };
         for (;;) {
             if (node == kNone) {
                 if (sp == 0) break;
                 node = stack[--sp][qi];
+                if (TWO) {
+                    if (node == kStackMark) {  // leave the instance
+                        O = wO; D = wD; rD = wrD;
+                        level = 0u;
+                        node = kNone;
+                        continue;
+                    }
+                    if (node & kInstFlag) {
+                        if (!enter(node & ~kInstFlag)) { overflow = true; break; }
+                    }
+                }
             }
-            if ((node & 0xc0000000u) == kBlasLeaf) {
+            if (level != 0u && (node & 0xc0000000u) == kBlasLeaf) {
                 const uint32_t first = node & kLeafFirstMask, cnt = (node >> 27) & 7u;
                 float ht = __builtin_inff(), hu = 0.f, hv = 0.f;
                 uint32_t hp = 0xffffffffu;
@@ -281,16 +315,40 @@ __device__ void trace_quads(const Bvh4Node* __restrict__ blas, const TriRec* __r
                 node = kNone;
                 if (hp != 0xffffffffu) {
                     if (any) { found = true; break; }
-                    if (!found || ht < bt || (ht == bt && hp < bprim)) {
-                        bt = ht; bu = hu; bv = hv; bprim = hp; found = true;
+                    // the leaf's hits share one instance: its smallest (t, primitive) against
+                    // the best so far in (t, instance, primitive) order
+                    if (!found || ht < bt ||
+                        (ht == bt && (cur_inst < binst || (cur_inst == binst && hp < bprim)))) {
+                        bt = ht; bu = hu; bv = hv; bprim = hp; binst = cur_inst; found = true;
                     }
                 }
                 continue;
             }
-            const Bvh4Node& N = blas[node];
+            const bool tl_level = TWO && level == 0u;
+            const Bvh4Node& N = tl_level ? tlas[node] : blas[node];
             const float* f = reinterpret_cast<const float*>(&N);
-            const float tq = slab1(f[q], f[4 + q], f[8 + q], f[12 + q], f[16 + q], f[20 + q], O, rD, tmin, bt);
-            const uint32_t cq = reinterpret_cast<const uint32_t*>(&N.child)[q];
+            float tq = slab1(f[q], f[4 + q], f[8 + q], f[12 + q], f[16 + q], f[20 + q], O, rD, tmin, bt);
+            uint32_t cq = reinterpret_cast<const uint32_t*>(&N.child)[q];
+            if (TWO && tl_level) {
+                // a TLAS leaf child: one instance joins the near-to-far order as an entry;
+                // the instances of a larger leaf are pushed beneath the other children
+                const uint32_t mq = reinterpret_cast<const uint32_t*>(&N.meta)[q];
+                const bool hitq = tq < __builtin_inff();
+                const uint32_t many = (hitq && mq > 1u) ? mq : 0u;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t m = j == 0 ? quad_get<0>(many) : j == 1 ? quad_get<1>(many)
+                                     : j == 2 ? quad_get<2>(many) : quad_get<3>(many);
+                    const uint32_t c = j == 0 ? quad_get<0>(cq) : j == 1 ? quad_get<1>(cq)
+                                     : j == 2 ? quad_get<2>(cq) : quad_get<3>(cq);
+                    for (uint32_t k = 0; k < m; ++k) {
+                        if (sp < kQStack) stack[sp++][qi] = kInstFlag | tlas_prims[c + k];
+                        else overflow = true;
+                    }
+                }
+                if (many) tq = __builtin_inff();
+                else if (hitq && mq == 1u) cq = kInstFlag | tlas_prims[cq];
+            }
             float kt[4] = {quad_getf<0>(tq), quad_getf<1>(tq), quad_getf<2>(tq), quad_getf<3>(tq)};
             uint32_t kc[4] = {quad_get<0>(cq), quad_get<1>(cq), quad_get<2>(cq), quad_get<3>(cq)};
             cswap(kt, kc, 0, 1);
@@ -306,6 +364,9 @@ __device__ void trace_quads(const Bvh4Node* __restrict__ blas, const TriRec* __r
                 }
             }
             node = kt[0] < __builtin_inff() ? kc[0] : kNone;
+            if (TWO && node != kNone && (node & kInstFlag)) {
+                if (!enter(node & ~kInstFlag)) { overflow = true; break; }
+            }
         }
         if (q == 0u) {
             if (any) {
@@ -314,7 +375,7 @@ __device__ void trace_quads(const Bvh4Node* __restrict__ blas, const TriRec* __r
             } else {
                 j0.hit[ri] = found ? make_float4(bt, bu, bv, __uint_as_float(bprim))
                                    : make_float4(__builtin_inff(), 0.f, 0.f, __uint_as_float(0xffffffffu));
-                if (j0.inst) j0.inst[ri] = found ? 0u : 0xffffffffu;
+                if (j0.inst) j0.inst[ri] = found ? binst : 0xffffffffu;
             }
             if (overflow) atomicOr(overflow_flag, 1u);
         }
@@ -339,24 +400,21 @@ __global__ __launch_bounds__(kBlock) void k_trace(
         if (j1.count_dev) { const uint32_t c = *j1.count_dev; c1 = c < c1 ? c : c1; }
     }
     const uint32_t count = c0 + c1;
-    // rays a wave holds at once: its share of the launch, but at least kMinShare, so that a
-    // small launch that the quad traversal does not take (a two-level scene's) spreads over
-    // more waves whose fewer rays diverge less (measured with the city's last bounces before
-    // the quad traversal existed: trace 2.81 -> 2.79 ms per frame; shares of 1 or 2 rays cost
-    // more issue slots than they save: 2.87 ms). At least 1: a wave learns that the queue is
-    // drained only from a fetch.
-    constexpr uint32_t kMinShare = 16u;
-    const uint32_t waves = gridDim.x * (kBlock / 64u);
-    const uint32_t quota = count > waves * kMinShare ? (count + waves - 1u) / waves : kMinShare;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(&ray_stats[MODE == 1 ? 1 : 0], (unsigned long long)c0);
         if (MODE == 2) atomicAdd(&ray_stats[1], (unsigned long long)c1);
     }
-    if (!TWO && count <= kQuadRays * gridDim.x * kQuads) {
-        trace_quads<MODE>(blas, tris, blas_root, j0, j1, c0, count, reinterpret_cast<uint32_t (*)[kQuads]>(lds),
-                          overflow_flag);
+    if (count <= (TWO ? kQuadRaysTwo : kQuadRays) * gridDim.x * kQuads) {
+        trace_quads<MODE, TWO>(blas, tris, tlas, tlas_prims, inst, blas_root, j0, j1, c0, count,
+                               reinterpret_cast<uint32_t (*)[kQuads]>(lds), overflow_flag);
         return;
     }
+    // rays a wave holds at once: its share of the launch. The launch holds more rays than the
+    // quad traversal takes, so the share is at least 1 (a wave learns that the queue is
+    // drained only from a fetch, so a share of 0 would never end).
+    static_assert(kQuadRays >= 1 && kQuadRaysTwo >= 1, "an empty launch takes the quad traversal");
+    const uint32_t waves = gridDim.x * (kBlock / 64u);
+    const uint32_t quota = (count + waves - 1u) / waves;
 
     Lane L;
     L.active = false;

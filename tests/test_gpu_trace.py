@@ -110,6 +110,32 @@ def test_small_launches_match_large(device):
     assert np.array_equal(_u32(small), _u32(o))
 
 
+def test_two_level_small_launches_match_large(device):
+    """Two-level scene: a small launch (quad traversal, instances entered per quad) and a
+    large one (one lane per ray) give the same bits, instance ids included, closest and
+    any hit; the small launch equals the oracle."""
+    b = scenes.Bricks()
+    rr = scenes.random_rays(1200000, -14.0, 14.0, seed=37, tmin=1e-3, tmax=1000.0)
+    occ = rr.copy()
+    occ[:, 7] = 3.0
+    with gk.Context(64, 64) as ctx:
+        ctx.upload(b.scene)
+        big, ibig = gpu_trace(ctx, rr, device)
+        small, ismall = gpu_trace(ctx, rr[:30000], device)
+        one, ione = gpu_trace(ctx, rr[5:6], device)
+        obig, _ = gpu_trace(ctx, occ, device, any_hit=True)
+        osmall, _ = gpu_trace(ctx, occ[:30000], device, any_hit=True)
+    assert np.array_equal(_u32(small), _u32(big[:30000]))
+    assert np.array_equal(ismall, ibig[:30000])
+    assert np.array_equal(_u32(one), _u32(big[5:6])) and np.array_equal(ione, ibig[5:6])
+    assert np.array_equal(_u32(osmall[:, 3]), _u32(obig[:30000, 3]))
+    hit = _u32(small[:, 3]) != 0xFFFFFFFF
+    assert 0.2 < hit.mean() < 0.999 and 0.05 < (_u32(osmall[:, 3]) == 1).mean() < 0.95
+    o, oi = po.OracleScene(b.scene).trace(rr[:30000])
+    assert np.array_equal(_u32(small), _u32(o))
+    assert np.array_equal(ismall[hit], oi[hit])
+
+
 def test_tlas_bricks_bit_exact(device):
     b = scenes.Bricks()
     g = np.load(GOLDEN / "bricks_tinybvh.npz")
