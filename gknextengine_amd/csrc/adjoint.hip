@@ -243,7 +243,8 @@ __device__ inline f3 dpath(const DScene& s, const DiffK& K, uint32_t x, uint32_t
         float pdd, pds;
         bsdf_eval(b, wo, wi, fd, fs, pdd, pds);
         const float pdf = spec ? b.ps * pds : (1.f - b.ps) * pdd;
-        const f3 f = spec ? fs : fd;
+        // per-component selects of values (a select of the two f3 objects keeps them in scratch)
+        const f3 f = f3{spec ? +fs.x : +fd.x, spec ? +fs.y : +fd.y, spec ? +fs.z : +fd.z};
         if (!(pdf > 0.f)) break;
         if (ADJ) {
             // d log f_lobe / d theta times the suffix radiance Ltot - P

@@ -101,11 +101,21 @@ __global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
         const int r = k / kTileP, cc = k % kTileP;
         const int sx = gx + cc - kRP, sy = gy + r - kRP;
         const bool in = inb(sx, sy, K);
-        const size_t j = in ? size_t(sy) * K.W + sx : 0;
-        s_obj[r][cc] = in ? T.obj0[j] : 0u;
-        s_nrm[r][cc] = in ? T.normal[j] : z4;
-        s_d[r][cc] = in ? T.src_d[j] : z4;
-        s_s[r][cc] = in ? T.src_s[j] : z4;
+        // loads under a branch: a select of two lvalues (a global element or the local zero)
+        // selects their addresses and reads through a flat pointer into scratch memory
+        uint32_t ov = 0u;
+        ushort4 nv = z4, dv = z4, sv = z4;
+        if (in) {
+            const size_t j = size_t(sy) * K.W + sx;
+            ov = T.obj0[j];
+            nv = T.normal[j];
+            dv = T.src_d[j];
+            sv = T.src_s[j];
+        }
+        s_obj[r][cc] = ov;
+        s_nrm[r][cc] = nv;
+        s_d[r][cc] = dv;
+        s_s[r][cc] = sv;
     }
     for (int k = ty * kRB + tx; k < kTileM * kTileM; k += kRB * kRB) {
         const int r = k / kTileM, cc = k % kTileM;

@@ -860,7 +860,9 @@ __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T,
                             float pdd, pds;
                             bsdf_eval(b, wo, wi, fd, fs, pdd, pds);
                             const float pdf = spec ? b.ps * pds : (1.f - b.ps) * pdd;
-                            const f3 f = spec ? fs : fd;
+                            // per-component selects of values: a select of the two f3 objects
+                            // copies from a selected address, which keeps both in scratch memory
+                            const f3 f = f3{spec ? +fs.x : +fd.x, spec ? +fs.y : +fd.y, spec ? +fs.z : +fd.z};
                             if (pdf > 0.f) {
                                 thr = thr * f * (wi.z / pdf);
                                 if (depth == 0u) flags = (flags & ~768u) | 256u | (spec ? 0u : 512u);
