@@ -206,7 +206,8 @@ __device__ inline bool mt(const TriRec& tr, f3 O, f3 D, float& t, float& u, floa
 // quad, so all four lanes of a quad are active at each quad exchange.
 // launches of at most this many rays per quad of the grid (measured on the 1080p city, trace
 // ms per frame, quad traversal from: 1 ray per quad 2.43, 8 -> 2.40, 16 -> 2.56, 64 -> 3.29;
-// the one-lane traversal alone 2.79)
+// the one-lane traversal alone 2.79; again once the one-lane traversal postponed leaves:
+// 4 -> 2.38, 6 -> 2.35, 8 -> 2.36)
 constexpr uint32_t kQuadRays = 8;
 // the same for a two-level scene, whose quads also enter and leave instances (4096 bricks at
 // 1080p, ms per frame: one-lane traversal alone 5.54, 1 ray per quad 5.48, 2 -> 5.47,
