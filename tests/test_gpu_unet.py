@@ -44,7 +44,9 @@ def test_layer_table_matches_oracle():
 def test_forward_matches_oracle_with_padding():
     import torch
     rng = np.random.default_rng(0)
-    h, w = 40, 56  # padded to 48 x 64 inside
+    # padded to 48 x 64 inside: every level even, so every convolution stages its input tile
+    # in LDS, the level-3 decoder's 86 KB tile (above the default 64 KB) included
+    h, w = 40, 56
     f = features(rng, 1, h, w)
     with gk.Context(16, 16) as ctx:
         net = gk.UNet(ctx, 32, seed=2)
