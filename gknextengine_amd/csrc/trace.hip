@@ -19,11 +19,13 @@
 //     lets near hits cull far subtrees (measured on the 1080p city: trace 4.46 -> 3.94 ms).
 //     In single-level scenes a lane postpones the leaf it reaches and goes on with nodes;
 //     the wave tests the postponed leaves together (kLeafBatch).
-//   * Launches of at most kQuadRays rays per quad of the grid (a frame's last bounces)
-//     run trace_quads instead: four lanes per ray, one child box or triangle each.
-//   * Two-level scenes: TLAS leaves push instance entries; entering an instance pushes a
-//     return mark and continues in the instance's BLAS with the ray transformed by the
-//     same inverse-matrix arithmetic tinybvh uses (tiny_bvh.h:396-409,2309-2316).
+//   * Launches of at most kQuadRays rays per quad of the grid (a frame's last bounces;
+//     kQuadRaysTwo in two-level scenes) run trace_quads instead: four lanes per ray, one
+//     child box or triangle each.
+//   * Two-level scenes: a TLAS leaf holds one instance, whose entry is ordered near to far
+//     with its siblings; entering an instance pushes a return mark and continues in the
+//     instance's BLAS with the ray transformed by the same inverse-matrix arithmetic
+//     tinybvh uses (tiny_bvh.h:396-409,2309-2316).
 //   * Triangle test: Moeller-Trumbore with exactly tinybvh's operations and rejects
 //     (tiny_bvh.h:6815-6844). Closest hit keeps the smallest t, ties broken towards the
 //     lower (instance, primitive), so results do not depend on traversal order. Box
