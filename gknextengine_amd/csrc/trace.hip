@@ -213,7 +213,7 @@ __device__ inline bool mt(const TriRec& tr, f3 O, f3 D, float& t, float& u, floa
 constexpr uint32_t kQuadRays = 8;
 // the same for a two-level scene, whose quads also enter and leave instances (4096 bricks at
 // 1080p, ms per frame: one-lane traversal alone 5.54, 1 ray per quad 5.48, 2 -> 5.47,
-// 4 -> 5.65, 8 -> 5.61)
+// 3 -> 5.47, 4 -> 5.65, 8 -> 5.61)
 constexpr uint32_t kQuadRaysTwo = 2;
 constexpr int kQuads = kBlock / 4;                   // quads per work-group
 constexpr int kQStack = 128;                         // stack entries per quad (deeper than a lane's
