@@ -187,8 +187,8 @@ __device__ inline f3 dpath(const DScene& s, const DiffK& K, uint32_t x, uint32_t
             const f3 wl = dl * (1.f / dist);
             const float cosl = fabsf(dot(normalize(cross(ld3(L.e1), ld3(L.e2))), wl));
             const f3 wil = f3{dot(T1, wl), dot(B1, wl), dot(Ns, wl)};
-            if (cosl > 0.f && wil.z > 0.f && dot(Ng, wl) > 0.f) {
-                const float psel = s.light_cdf[li + 1] - s.light_cdf[li];
+            const float psel = s.light_cdf[li + 1] - s.light_cdf[li];  // 0: a light switched off
+            if (cosl > 0.f && wil.z > 0.f && dot(Ng, wl) > 0.f && psel > 0.f) {
                 const float pl = psel * d2 / (cosl * L.p0.w);
                 if (!occluded_single(s, Po, wl, 0.f, dist * (1.f - 1e-4f) - off)) {
                     f3 fd, fs;

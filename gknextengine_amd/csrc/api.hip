@@ -84,7 +84,7 @@ bool is_identity(const float* m) {
 
 // ---------------------------------------------------------------- light list (host)
 int upload_lights(Ctx& c, const gk_mesh* meshes) {
-    struct L { uint32_t inst, prim; float v0[3], e1[3], e2[3]; float le[3]; };
+    struct L { uint32_t inst, prim, mat; float v0[3], e1[3], e2[3]; float le[3]; };
     std::vector<L> ls;
     const uint32_t nmat = uint32_t(c.h_materials.size());
     for (uint32_t ii = 0; ii < c.h_instances.size(); ++ii) {
@@ -100,6 +100,7 @@ int upload_lights(Ctx& c, const gk_mesh* meshes) {
             L l;
             l.inst = ii;
             l.prim = p;
+            l.mat = mi;
             const uint32_t a = M.indices[3 * p], b = M.indices[3 * p + 1], d = M.indices[3 * p + 2];
             for (int k = 0; k < 3; ++k) {
                 l.v0[k] = M.positions[3 * a + k];
@@ -111,6 +112,8 @@ int upload_lights(Ctx& c, const gk_mesh* meshes) {
         }
     }
     c.num_lights = uint32_t(ls.size());
+    c.h_light_mat.resize(ls.size());
+    c.h_light_area.resize(ls.size());
     GK_HIP(c.lights.resize(std::max<size_t>(ls.size(), 1)));
     GK_HIP(c.light_obj.resize(3 * std::max<size_t>(ls.size(), 1)));
     GK_HIP(c.light_cdf.resize(ls.size() + 1));
@@ -130,6 +133,8 @@ int upload_lights(Ctx& c, const gk_mesh* meshes) {
         mul44v(W, l.e2[0], l.e2[1], l.e2[2], 0.f, e2);
         const f3 cr = cross(f3{e1[0], e1[1], e1[2]}, f3{e2[0], e2[1], e2[2]});
         const float area = 0.5f * length(cr);
+        c.h_light_mat[i] = l.mat;
+        c.h_light_area[i] = area;
         const double lum = 0.2126 * l.le[0] + 0.7152 * l.le[1] + 0.0722 * l.le[2];
         acc += double(area) * lum;
         cdfd[i + 1] = acc;
@@ -150,6 +155,48 @@ int upload_lights(Ctx& c, const gk_mesh* meshes) {
     GK_HIP(hipMemcpyAsync(c.lights.ptr, lt.data(), sizeof(LightTri) * lt.size(), hipMemcpyHostToDevice, c.stream));
     GK_HIP(hipMemcpyAsync(c.light_obj.ptr, obj.data(), sizeof(float4) * obj.size(), hipMemcpyHostToDevice, c.stream));
     GK_HIP(hipMemcpyAsync(c.light_cdf.ptr, cdf.data(), 4 * cdf.size(), hipMemcpyHostToDevice, c.stream));
+    GK_HIP(hipStreamSynchronize(c.stream));
+    return GK_OK;
+}
+
+// Radiance and selection pdfs of the light list from the current materials: a light whose
+// material is no longer an emitter gets zero radiance and zero selection weight. Emitters
+// made by a material update are not in the list: their light arrives by BSDF sampling
+// alone (the emitter-hit weight is 1 where the light-selection pdf is 0) until the next
+// gk_scene_upload rebuilds the list.
+__global__ void k_set_light_power(LightTri* __restrict__ lights, const float4* __restrict__ power, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 q = power[i];
+    LightTri L = lights[i];
+    L.le.x = q.x; L.le.y = q.y; L.le.z = q.z;  // le.w keeps the primitive
+    L.e1.w = q.w;
+    lights[i] = L;
+}
+
+int refresh_light_power(Ctx& c) {
+    const size_t n = c.num_lights;
+    if (n == 0) return GK_OK;
+    std::vector<float4> q(n);
+    std::vector<double> cdfd(n + 1, 0.0);
+    double acc = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const gk_material& m = c.h_materials[c.h_light_mat[i]];
+        const bool emits = m.material_model == GK_MATERIAL_DIFFUSE_LIGHT;
+        const float le[3] = {emits ? m.diffuse[0] : 0.f, emits ? m.diffuse[1] : 0.f, emits ? m.diffuse[2] : 0.f};
+        const double lum = 0.2126 * le[0] + 0.7152 * le[1] + 0.0722 * le[2];
+        acc += double(c.h_light_area[i]) * lum;
+        cdfd[i + 1] = acc;
+        q[i] = make_float4(le[0], le[1], le[2], 0.f);
+    }
+    std::vector<float> cdf(n + 1, 0.f);
+    for (size_t i = 0; i <= n; ++i) cdf[i] = acc > 0.0 ? float(cdfd[i] / acc) : 0.f;
+    for (size_t i = 0; i < n; ++i) q[i].w = acc > 0.0 ? float((cdfd[i + 1] - cdfd[i]) / acc) : 0.f;
+    GK_HIP(c.light_power.resize(n));
+    GK_HIP(hipMemcpyAsync(c.light_power.ptr, q.data(), sizeof(float4) * n, hipMemcpyHostToDevice, c.stream));
+    GK_HIP(hipMemcpyAsync(c.light_cdf.ptr, cdf.data(), 4 * cdf.size(), hipMemcpyHostToDevice, c.stream));
+    k_set_light_power<<<uint32_t((n + 127) / 128), 128, 0, c.stream>>>(c.lights.ptr, c.light_power.ptr, uint32_t(n));
+    GK_HIP(hipGetLastError());
     GK_HIP(hipStreamSynchronize(c.stream));
     return GK_OK;
 }
@@ -331,6 +378,9 @@ int gk_materials_update(gk_ctx* ctx, const gk_material* materials, uint32_t firs
     std::copy(materials, materials + count, c.h_materials.begin() + first);
     GK_HIP(hipMemcpyAsync(c.materials.ptr + first, materials, sizeof(gk_material) * count, hipMemcpyHostToDevice, c.stream));
     GK_HIP(hipStreamSynchronize(c.stream));
+    // next-event estimation reads emitter radiance from the light list
+    for (uint32_t m : c.h_light_mat)
+        if (m >= first && m - first < count) return refresh_light_power(c);
     return GK_OK;
 }
 

@@ -142,6 +142,9 @@ struct Ctx {
     uint32_t num_lights = 0;
     DevBuf<float> light_cdf;        // num_lights + 1
     DevBuf<float4> light_obj;       // 3 per light: object-space v0, e1, e2
+    std::vector<uint32_t> h_light_mat;  // material of each light (refreshed by gk_materials_update)
+    std::vector<float> h_light_area;    // world-space area at upload (rigid motion keeps it)
+    DevBuf<float4> light_power;     // staging for gk_materials_update: per light (le rgb, pdf)
     BuildScratch scratch;
     gk_bvh_stats bvh_stats{};
     DevBuf<float> grad_partial;     // adjoint: per-work-group gradient rows

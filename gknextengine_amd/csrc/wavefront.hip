@@ -378,7 +378,8 @@ __global__ void k_reset_counter(uint32_t* p) { *p = 0u; }
 // ================================================================ REF_V2
 // flags (misc.y): bit0 chanceReflect of the pending GetRayColor, bit1 BackFace,
 // bit2 origin material is dielectric, bit3 hitReflect (first call), bit4 hitMetal (first call)
-// counters (misc.z): bits 0-7 bounce b of the pending call, 8-19 sample, 20-31 sample count
+// counters (misc.z): bits 0-7 bounce b of the pending call, 8-19 sample, 20-31 sample count - 1
+// (gk_frame allows 1024 samples, x4 on moving pixels: 4096 samples, indices 0..4095)
 
 // GetRayColor up to the trace (Shading.slang:1007-1036); emits the extension ray.
 __device__ inline void ref_begin(const DScene& s, const FrameK& K, const Queues& Q, const PathState& S, uint32_t p,
@@ -432,7 +433,7 @@ __device__ inline void ref_end_sample(const DScene& s, const FrameK& K, const Ta
     acc.x = acc.x + color.x; acc.y = acc.y + color.y; acc.z = acc.z + color.z; acc.w = acc.w + color.w;
     if (flags & 8u) S.ls[p] = acc; else S.ld[p] = acc;
     const uint32_t sample = ((counters >> 8) & 0xfffu) + 1u;
-    const uint32_t nsamp = counters >> 20;
+    const uint32_t nsamp = (counters >> 20) + 1u;
     const f3 ppos = ld3(pp), pnrm = ld3(S.pnrm[p]);
     if (sample < nsamp) {
         counters = (counters & 0xfff00000u) | (sample << 8);
@@ -529,7 +530,7 @@ __global__ __launch_bounds__(256) void k_ref_primary(DScene s, FrameK K, Targets
                 mul44v(K.mvi, 0.f, 0.f, 0.f, 1.f, e4);
                 const f3 dir = normalize(pos - f3{e4[0], e4[1], e4[2]});
                 uint32_t flags = 0u;
-                const uint32_t counters = (nsamp << 20);
+                const uint32_t counters = (nsamp - 1u) << 20;
                 ref_begin(s, K, Q, S, p, pos, nrm, mi, dir, seed, flags, 0u, 1, emit, ro, rd);
                 S.misc[p] = make_uint4(px, flags, counters, mi);
                 S.pos[p] = make_float4(pos.x, pos.y, pos.z, 0.f);
@@ -815,8 +816,9 @@ __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T,
                             const f3 ln = normalize(cross(ld3(L.e1), ld3(L.e2)));
                             const float cosl = fabsf(dot(ln, wl));
                             const f3 wil = f3{dot(T1, wl), dot(B1, wl), dot(Ns, wl)};
-                            if (cosl > 0.f && wil.z > 0.f && dot(Ng, wl) > 0.f) {
-                                const float psel = s.light_cdf[li + 1] - s.light_cdf[li];
+                            // psel 0: a light that a material update switched off
+                            const float psel = s.light_cdf[li + 1] - s.light_cdf[li];
+                            if (cosl > 0.f && wil.z > 0.f && dot(Ng, wl) > 0.f && psel > 0.f) {
                                 const float pl = psel * d2 / (cosl * L.p0.w);
                                 f3 fd, fs;
                                 float pdd, pds;
@@ -1167,8 +1169,9 @@ int render_frame(Ctx& c) {
         k_ref_primary<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P);
         GK_HIP(hipGetLastError());
         cur = 1;
-        // worst case: every sample uses all bounces (samples * 4 for moving pixels)
-        const uint32_t maxb = std::max(K.bounces, K.max_bounces);
+        // worst case: every sample uses all bounces (samples * 4 for moving pixels); a sample
+        // traces at least once, also with no bounces
+        const uint32_t maxb = std::max(1u, std::max(K.bounces, K.max_bounces));
         const uint32_t iters = maxb * K.samples * 4u + 4u;
         for (uint32_t it = 0; it < iters; ++it) {
             st = launch_trace(c, Q.o[cur], Q.d[cur], Q.cnt + cur, n, Q.hit, Q.hit_inst, false, nullptr);
@@ -1177,9 +1180,6 @@ int render_frame(Ctx& c) {
             k_ref_shade<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, cur);
             GK_HIP(hipGetLastError());
             cur ^= 1;
-            if (it + 1 >= uint32_t(K.bounces) * K.samples + 1u && !c.params.has_sun && c.h_instances.size() == 1 &&
-                K.bounces == maxb && K.samples == 1 && false)
-                break;
         }
     } else {
         // Each iteration shades the hits of queue cur, then one launch traces both the

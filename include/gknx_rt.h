@@ -315,7 +315,11 @@ int gk_scene_upload(gk_ctx* ctx,
                     const gk_light* lights, uint32_t num_lights,
                     const gk_instance* instances, uint32_t num_instances);
 
-/* Replaces Scene::UpdateAllMaterials (Scene.cpp:393-408). */
+/* Replaces Scene::UpdateAllMaterials (Scene.cpp:393-408). Next-event estimation follows:
+ * the emissive triangles of the last gk_scene_upload take their radiance and selection
+ * probability from the updated materials (zero for one that is no longer DIFFUSE_LIGHT).
+ * A material made DIFFUSE_LIGHT by an update is lit by BSDF sampling alone until the
+ * next gk_scene_upload adds its triangles to the light list (still unbiased). */
 int gk_materials_update(gk_ctx* ctx, const gk_material* materials, uint32_t first, uint32_t count);
 
 /* Replaces the context's textures (GlobalTexturePool::LoadTexture for each image of a

@@ -823,10 +823,19 @@ struct RefV2 {
 };
 
 // ---------------------------------------------------------------- PT_MIS (templated)
+// mode: which strategies carry light to a vertex (test instrumentation of the oracle):
+//   0  MIS: next-event estimation and BSDF sampling, power-heuristic weights (the estimator)
+//   1  NEE only: emitters reached by a non-delta BSDF sample add nothing
+//   2  BSDF only: shadow rays are drawn but add nothing; emitter hits carry weight 1
+//   3-5 planted mistakes, for tests that show their checks fail on them: the emitter-hit
+//      weight uses a light pdf without the emitter's cosine (3), without the light
+//      selection probability (4), or the balance instead of the power heuristic (5), so
+//      the two weights of a direction no longer sum to one
 template <class T>
 struct PtMis {
     const Scene& sc;
     const FrameParams& P;
+    int mode = 0;
     using V = V3<T>;
     static constexpr T PI = T(3.14159265358979323846);
 
@@ -999,6 +1008,11 @@ struct PtMis {
                     const T d2 = h.t * h.t * dot(D, D);
                     const T pl = cosl > T(0) && area > T(0) ? psel * d2 / (cosl * area) : T(0);
                     w = power(last_pdf, pl);
+                    if (mode == 1) w = T(0);
+                    if (mode == 2) w = T(1);
+                    if (mode == 3) w = power(last_pdf, area > T(0) ? psel * d2 / area : T(0));
+                    if (mode == 4) w = power(last_pdf, pl / psel);
+                    if (mode == 5) w = last_pdf / (last_pdf + pl);
                 }
                 if (depth == 0) Ls = Ls + Le;  // emitter seen directly: specular target
                 else (lobe0 == 0 ? Ld : Ls) = (lobe0 == 0 ? Ld : Ls) + thr * Le * w;
@@ -1053,8 +1067,8 @@ struct PtMis {
                     const V wl = dl * (T(1) / dist);
                     const T cosl = std::fabs(dot(cvt(L.n), wl));
                     const V wil = to_loc(wl);
-                    if (cosl > T(0) && wil.z > T(0) && dot(Ng, wl) > T(0)) {
-                        const T psel = T(sc.light_cdf[li + 1] - sc.light_cdf[li]);
+                    const T psel = T(sc.light_cdf[li + 1] - sc.light_cdf[li]);
+                    if (cosl > T(0) && wil.z > T(0) && dot(Ng, wl) > T(0) && psel > T(0)) {
                         const T pl = psel * d2 / (cosl * T(L.area));
                         rays++;
                         if (!trace_any_t<T>(sc, Po, wl, T(0), dist * (T(1) - T(1e-4)) - off)) {
@@ -1062,8 +1076,10 @@ struct PtMis {
                             T pdd, pds;
                             eval(b, wo, wil, fd, fs, pdd, pds);
                             const V Le = cvt(L.le);
-                            const T wd = power(pl, (T(1) - b.ps) * pdd);
-                            const T ws = power(pl, b.ps * pds);
+                            T wd = power(pl, (T(1) - b.ps) * pdd);
+                            T ws = power(pl, b.ps * pds);
+                            if (mode == 1) wd = ws = T(1);
+                            if (mode == 2) wd = ws = T(0);
                             const V cd = thr * Le * fd * (wil.z * wd / pl);
                             const V cs = thr * Le * fs * (wil.z * ws / pl);
                             if (depth == 0) { Ld = Ld + cd; Ls = Ls + cs; }
@@ -1222,8 +1238,8 @@ struct DiffPath {
                 const V wl = dl * (T(1) / dist);
                 const T cosl = std::fabs(dot(PT::cvt(Lt.n), wl));
                 const V wil = to_loc(wl);
-                if (cosl > 0 && wil.z > 0 && dot(Ng, wl) > 0) {
-                    const T psel = T(sc.light_cdf[li + 1] - sc.light_cdf[li]);
+                const T psel = T(sc.light_cdf[li + 1] - sc.light_cdf[li]);
+                if (cosl > 0 && wil.z > 0 && dot(Ng, wl) > 0 && psel > 0) {
                     const T pl = psel * d2 / (cosl * T(Lt.area));
                     if (!trace_any_t<T>(sc, Po, wl, T(0), dist * (T(1) - T(1e-4)) - off)) {
                         V fd, fs, fd_s, fs_s;
@@ -1577,8 +1593,9 @@ uint64_t oracle_render_refv2(void* s, const FrameParams* P, uint32_t W, uint32_t
 template <class T>
 uint64_t render_ptmis(const Scene& sc, const FrameParams* P, uint32_t W, uint32_t H, uint32_t x0,
                       uint32_t y0, uint32_t w, uint32_t h, uint32_t frame0, uint32_t frames,
-                      double* rgb, double* diffuse, double* spec, uint32_t nthreads) {
+                      double* rgb, double* diffuse, double* spec, uint32_t nthreads, int mode = 0) {
     PtMis<T> pt{sc, *P};
+    pt.mode = mode;
     std::atomic<uint64_t> rays{0};
     const uint32_t spp = std::max(1u, P->samples);
     parallel_rows(h, nthreads, [&](uint32_t yy) {
@@ -1615,6 +1632,15 @@ uint64_t oracle_render_ptmis(void* s, const FrameParams* P, uint32_t W, uint32_t
     const Scene& sc = *static_cast<Scene*>(s);
     if (fp64) return render_ptmis<double>(sc, P, W, H, x0, y0, w, h, frame0, frames, rgb, diffuse, spec, nthreads);
     return render_ptmis<float>(sc, P, W, H, x0, y0, w, h, frame0, frames, rgb, diffuse, spec, nthreads);
+}
+
+// The same render with a strategy mode of PtMis (0 MIS, 1 NEE only, 2 BSDF only, 3-5 planted
+// weight mistakes); double precision.
+uint64_t oracle_render_ptmis_mode(void* s, const FrameParams* P, uint32_t W, uint32_t H, uint32_t x0, uint32_t y0,
+                                  uint32_t w, uint32_t h, uint32_t frame0, uint32_t frames, int mode, double* rgb,
+                                  uint32_t nthreads) {
+    const Scene& sc = *static_cast<Scene*>(s);
+    return render_ptmis<double>(sc, P, W, H, x0, y0, w, h, frame0, frames, rgb, nullptr, nullptr, nthreads, mode);
 }
 
 // Detached differentiable PT_MIS (DiffPath) over a rectangle: mean of spp paths seeded

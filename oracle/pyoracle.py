@@ -46,6 +46,7 @@ def oracle():
             "oracle_primary_rays": (None, [_P, _U, _U, _P]),
             "oracle_render_refv2": (C.c_uint64, [_P, _P, _U, _U, _U, _U, _U, _U, _P, _P, _P, _P, _P, _P, _P, _P, _U]),
             "oracle_render_ptmis": (C.c_uint64, [_P, _P, _U, _U, _U, _U, _U, _U, _U, _U, C.c_int, _P, _P, _P, _U]),
+            "oracle_render_ptmis_mode": (C.c_uint64, [_P, _P, _U, _U, _U, _U, _U, _U, _U, _U, C.c_int, _P, _U]),
             "oracle_pcg4d": (None, [_P, _U, _P]),
             "oracle_u2f": (C.c_float, [_U]),
             "oracle_sincos_2pi": (None, [_P, _U, _P, _P]),
@@ -208,6 +209,21 @@ class OracleScene:
         rays = oracle().oracle_render_ptmis(self.h, C.addressof(params), W, H, x0, y0, w, h, frame0, frames,
                                             int(fp64), ptr(rgb), ptr(d), ptr(s), threads or os.cpu_count())
         return {"rgb": rgb, "diffuse": d, "spec": s, "rays": rays}
+
+    # strategy modes of the PT_MIS oracle (double precision); the PLANTED_* modes are wrong
+    # emitter-hit weights, for tests that show their checks fail on them
+    MIS, NEE_ONLY, BSDF_ONLY = 0, 1, 2
+    PLANTED_NO_COSINE, PLANTED_NO_SELECTION_PDF, PLANTED_BALANCE_HEURISTIC = 3, 4, 5
+
+    def render_ptmis_mode(self, params, W, H, mode, rect=None, frame0=0, frames=1, threads=None):
+        """PT_MIS radiance (h, w, 3), double precision, with only some strategies carrying
+        light: MIS (the estimator), NEE_ONLY, BSDF_ONLY, or a PLANTED_* mistake. Every mode
+        draws the same random numbers, so renders of one seed follow the same paths."""
+        x0, y0, w, h = rect or (0, 0, W, H)
+        rgb = np.zeros((h, w, 3), np.float64)
+        oracle().oracle_render_ptmis_mode(self.h, C.addressof(params), W, H, x0, y0, w, h, frame0, frames, int(mode),
+                                          ptr(rgb), threads or os.cpu_count())
+        return rgb
 
 
 # ---------------------------------------------------------------- probes

@@ -5,71 +5,48 @@ import numpy as np
 
 import gknextengine_amd as gk
 
-CORNELL_CAM = dict(eye=(0.5, 0.5, -1.4), target=(0.5, 0.5, 0.0), fov=40.0)
-CITY_CAM = dict(eye=(-20.0, 28.0, -44.0), target=(2.0, 0.0, 4.0), fov=50.0)
-BRICKS_CAM = dict(eye=(0.0, 10.0, -30.0), target=(0.0, 0.0, 0.0), fov=50.0)
+# the benchmark cameras and the config-3 scene are the package's own (one definition)
+from gknextengine_amd.scenes import BRICKS_CAM, CITY_CAM, CORNELL_CAM, Bricks  # noqa: E402,F401
 
 
-def rot(axis, angle):
-    axis = np.asarray(axis, np.float64)
-    axis = axis / np.linalg.norm(axis)
-    x, y, z = axis
-    c, s = np.cos(angle), np.sin(angle)
-    C = 1 - c
-    return np.array([[c + x * x * C, x * y * C - z * s, x * z * C + y * s],
-                     [y * x * C + z * s, c + y * y * C, y * z * C - x * s],
-                     [z * x * C - y * s, z * y * C + x * s, c + z * z * C]])
+def glossy_cornell():
+    """The Cornell box with glossy materials and a second, large emitter: the ceiling
+    (1 x 1, radiance (1, 0.95, 0.9)) besides the small ceiling light. Red wall Mixture
+    (roughness 0.6), green wall Lambertian, white walls Mixture (roughness 0.35, metallic
+    0.25), short box Mixture (0.3, 0.5), tall box Metallic (0.15). Both emitters make BSDF
+    sampling and next-event estimation each carry a large share of the light."""
+    from gknextengine_amd.scenes import material
+    s = gk.procgen_cornell()
+    tm = s.meshes[0]["tri_material"]
+    tm[2:4] = 4          # ceiling
+    tm[10:22] = 5        # short box
+    tm[22:34] = 6        # tall box
+    s.materials = [material((0.7, 0.07, 0.07), gk.MAT_MIXTURE, 0.6, 0.0),
+                   material((0.07, 0.7, 0.07), gk.MAT_LAMBERTIAN, 1.0, 0.0),
+                   material((0.7, 0.7, 0.7), gk.MAT_MIXTURE, 0.35, 0.25),
+                   s.materials[3],
+                   material((1.0, 0.95, 0.9), gk.MAT_DIFFUSE_LIGHT),
+                   material((0.8, 0.6, 0.3), gk.MAT_MIXTURE, 0.3, 0.5),
+                   material((0.9, 0.9, 0.9), gk.MAT_METALLIC, 0.15, 1.0)]
+    return s
 
 
-class Bricks:
-    """Config 3: instanced 24-triangle bricks on a lattice with random rigid motion."""
-
-    def __init__(self, n_side=16, spacing=1.5, n_materials=8, seed=7):
-        rng = np.random.default_rng(seed)
-        pos, nrm, idx, mat = gk.procgen_brick()
-        self.scene = gk.SceneData()
-        self.scene.add_mesh(pos, idx, nrm, mat)
-        mats = []
-        for k in range(n_materials):
-            m = gk.Material()
-            c = rng.uniform(0.2, 0.9, 3)
-            m.diffuse[:] = [*c, 1.0]
-            m.diffuse_texture_id = m.mra_texture_id = m.normal_texture_id = -1
-            m.fuzziness = float(rng.uniform(0.1, 1.0))
-            m.refraction_index = 1.45
-            m.material_model = gk.MAT_MIXTURE
-            m.metalness = float(rng.random() < 0.3)
-            mats.append(m)
-        light = gk.Material()
-        light.diffuse[:] = [8.0, 8.0, 8.0, 1.0]
-        light.diffuse_texture_id = light.mra_texture_id = light.normal_texture_id = -1
-        light.material_model = gk.MAT_DIFFUSE_LIGHT
-        mats.append(light)
-        self.scene.materials = mats
-        n = n_side ** 3
-        g = (np.arange(n_side) - (n_side - 1) / 2.0) * spacing
-        X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
-        self.pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
-        self.axis = rng.normal(size=(n, 3))
-        self.omega = rng.normal(0.0, 0.5, n)
-        self.vel = rng.normal(0.0, 1.0, (n, 3))
-        self.angle0 = rng.uniform(0, 2 * np.pi, n)
-        self.mat_base = rng.integers(0, n_materials - 1, n)
-        self.n = n
-        for i, w in enumerate(self.worlds(0.0)):
-            # each instance uses two consecutive materials (slot 0: body, slot 1: studs);
-            # one in 64 bricks carries the emissive material on its studs
-            off = int(self.mat_base[i]) if i % 64 else n_materials - 1
-            self.scene.add_instance(0, world=w, instance_id=i, mat_offset=off)
-
-    def worlds(self, t: float) -> np.ndarray:
-        out = np.zeros((self.n, 4, 4), np.float32)
-        for i in range(self.n):
-            R = rot(self.axis[i], self.angle0[i] + self.omega[i] * t)
-            out[i, :3, :3] = R
-            out[i, :3, 3] = self.pos[i] + self.vel[i] * t
-            out[i, 3, 3] = 1.0
-        return out
+def all_models_cornell():
+    """The Cornell box with every material model the estimators shade: walls Mixture,
+    Metallic and Lambertian, the short box Dielectric (ior 1.5), the tall box Mixture."""
+    from gknextengine_amd.scenes import material
+    s = gk.procgen_cornell()
+    tm = s.meshes[0]["tri_material"]
+    tm[10:22] = 4
+    tm[22:34] = 5
+    s.materials = [material((0.7, 0.1, 0.1), gk.MAT_MIXTURE, 0.4, 0.3),
+                   material((0.1, 0.7, 0.1), gk.MAT_METALLIC, 0.25, 1.0),
+                   material((0.7, 0.7, 0.7), gk.MAT_LAMBERTIAN, 1.0, 0.0),
+                   s.materials[3],
+                   material((0.95, 0.95, 0.95), gk.MAT_DIELECTRIC, 0.0, 0.0, 1.5),
+                   material((0.6, 0.6, 0.8), gk.MAT_MIXTURE, 0.2, 0.8)]
+    s.materials[4].refraction_index2 = 1.5
+    return s
 
 
 def random_rays(n, lo, hi, seed=0, tmin=0.0, tmax=1e30):

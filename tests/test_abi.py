@@ -120,3 +120,17 @@ def test_city_full_size_matches_config():
     assert 0.25 < metal.mean() < 0.35 and set(np.unique(metal)) <= {0.0, 1.0}
     assert rough.min() >= 0.05 and rough.max() <= 1.0
     assert sum(m.material_model == gk.MAT_DIFFUSE_LIGHT for m in c.materials) == 1
+
+
+def test_diff_render_rejects_host_and_misshapen_parameters():
+    """diff.render checks params before any pointer reaches the library: a host tensor
+    (the kernels read it as device memory) or a wrong shape raises ValueError."""
+    import torch
+
+    from gknextengine_amd import diff
+    base = [gk.Material() for _ in range(3)]
+    with pytest.raises(ValueError, match="CUDA"):
+        diff.render(torch.zeros(3, 5), None, base, None)
+    if torch.cuda.is_available():
+        with pytest.raises(ValueError, match="shape"):
+            diff.render(torch.zeros(4, 5, device="cuda"), None, base, None)

@@ -194,3 +194,39 @@ def test_primary_hits_from_visibility_buffer(which):
         assert np.all(inst[10:20] == 0xFFFFFFFF)
         assert np.array_equal(inst[20:], ref["PRIMARY_INST"][20:])
         assert np.array_equal(ctx.read("PRIMARY_HIT")[20:].view(np.uint8), ref["PRIMARY_HIT"][20:].view(np.uint8))
+
+
+def test_ref_v2_moving_pixels_at_the_sample_limit_match_oracle():
+    """samples = 1024, gk_frame's limit, on pixels whose object moved: Core.PathTracing
+    multiplies their samples by 4 (Core.PathTracing.comp.slang:25-28), so each such pixel
+    averages 4096 samples. The per-path sample counter holds that count (the test failed
+    with Inf/NaN when the count wrapped to 0), and the frame equals the oracle's."""
+    W = H = 16
+    s = gk.procgen_cornell()
+    osc = po.OracleScene(s)
+    moved = np.eye(4, dtype=np.float32)
+    moved[0, 3] = 0.05
+    moment = np.zeros((H, W), np.uint16)
+    with gk.Context(W, H) as ctx:
+        ctx.upload(s)
+        p0 = cornell_params(W, H, frame=0, bounces=2)
+        ctx.frame(p0)
+        ctx.render()
+        osc.render_refv2(p0, W, H, moment=moment)
+        ctx.update_instances(moved[None])
+        osc.update_instances(moved[None])
+        p1 = cornell_params(W, H, frame=1, bounces=2, samples=1024, prev=p0)
+        ctx.frame(p1)
+        ctx.render()
+        ctx.sync()
+        gd, gs = ctx.read("SINGLE_DIFFUSE"), ctx.read("SINGLE_SPECULAR")
+        gm = ctx.read("MOTIONMOMENT")
+    o = osc.render_refv2(p1, W, H, moment=moment)
+    assert np.array_equal(gm, moment)
+    assert (gm > 0).mean() > 0.5  # most pixels moved: 4096 samples each
+    assert np.isfinite(gd.astype(np.float32)).all() and np.isfinite(gs.astype(np.float32)).all()
+    od, os_ = o["diffuse"].astype(np.float16), o["spec"].astype(np.float16)
+    same = np.all(gd.view(np.uint16) == od.view(np.uint16), axis=2) & np.all(gs.view(np.uint16) == os_.view(np.uint16), axis=2)
+    assert same.mean() > 0.9, same.mean()
+    err = relmse(gd[..., :3].astype(np.float64) + gs[..., :3], o["diffuse"][..., :3].astype(np.float64) + o["spec"][..., :3])
+    assert err < 1e-4, err
