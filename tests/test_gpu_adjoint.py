@@ -176,14 +176,19 @@ def test_backward_replays_its_own_forward_after_interleaved_renders():
 
 
 def test_descent_recovers_perturbed_parameters_from_an_independent_target():
-    """Config 4's set-up at reduced size: the target is a 256-spp render with parameters
+    """Config 4's set-up at reduced size: the target is a 2048-spp render with parameters
     perturbed by N(0, 0.05), drawn from seeds the optimisation never uses; plain
     loss.backward() (adjoint replayed with independent seeds) drives Adam back towards the
-    target's albedos."""
+    target's albedos.
+
+    Sizing: a material's albedo is recovered to within the noise of the pixels it covers.
+    At 128x128 the well-lit tiles cover about 60 pixels each; the target's own noise at
+    2048 spp moves their optimum by well under the 0.04 mean perturbation, and 16 spp per
+    step keeps Adam's steps dominated by the signal until the error is a fraction of it."""
     import torch
     from gknextengine_amd import diff
-    W = H = 64
-    scene, nmat, p = setup(W, H, spp=4)
+    W = H = 128
+    scene, nmat, p = setup(W, H, spp=16)
     base = list(scene.materials)
     with gk.Context(W, H) as ctx:
         ctx.upload(scene)
@@ -192,24 +197,27 @@ def test_descent_recovers_perturbed_parameters_from_an_independent_target():
         truth = theta.clone()
         truth[:nmat, :3] = (truth[:nmat, :3] + 0.05 * torch.randn(nmat, 3, generator=gen).cuda()).clamp(0.02, 0.98)
         acc = torch.zeros((H, W, 3), device="cuda")
-        for k in range(4):
+        for k in range(8):
             t = gk.FrameParams.from_buffer_copy(p)
-            t.samples, t.total_frames, t.global_seed = 64, 1_000_000 + 64 * k, 99
-            acc += diff.render(truth, ctx, base, t).detach()
-        target = acc / 4
-        # coverage: materials seen by enough pixels to be recoverable
-        cover = torch.zeros(nmat, device="cuda")
+            t.samples, t.total_frames, t.global_seed = 256, 1_000_000 + 256 * k, 99
+            with torch.no_grad():
+                acc += diff.render(truth, ctx, base, t)
+        target = acc / 8
+        # coverage: how strongly each material's albedo moves the image (64 spp)
         x = theta.clone().requires_grad_(True)
-        diff.render(x, ctx, base, p).sum().backward()
+        c = gk.FrameParams.from_buffer_copy(p)
+        c.samples, c.total_frames = 64, 500_000
+        diff.render(x, ctx, base, c).sum().backward()
         cover = x.grad[:nmat, :3].abs().sum(dim=1)
-        seen = cover > cover.max() * 0.05
+        seen = cover > cover.max() * 0.2
         params = theta.clone().requires_grad_(True)
-        opt = torch.optim.Adam([params], lr=0.01)
+        opt = torch.optim.Adam([params], lr=0.005)
         err0 = (params[:nmat, :3] - truth[:nmat, :3]).abs()[seen].mean().item()
-        for it in range(60):
+        losses = []
+        for it in range(100):
             opt.zero_grad()
             f = gk.FrameParams.from_buffer_copy(p)
-            f.total_frames = 100 + 4 * it
+            f.total_frames = 100 + 16 * it
             loss = ((diff.render(params, ctx, base, f) - target) ** 2).mean()
             loss.backward()
             params.grad[nmat:] = 0
@@ -217,6 +225,8 @@ def test_descent_recovers_perturbed_parameters_from_an_independent_target():
             opt.step()
             with torch.no_grad():
                 params[:nmat, :3].clamp_(0.01, 0.99)
+            losses.append(loss.item())
         err1 = (params[:nmat, :3] - truth[:nmat, :3]).abs()[seen].mean().item()
-    print(f"albedo error over {int(seen.sum())} materials: {err0:.4f} -> {err1:.4f}")
-    assert err1 < 0.6 * err0, (err0, err1)
+    print(f"albedo error over {int(seen.sum())} materials: {err0:.4f} -> {err1:.4f}; "
+          f"loss {np.mean(losses[:5]):.3e} -> {np.mean(losses[-5:]):.3e}")
+    assert err1 < 0.5 * err0, (err0, err1)
