@@ -79,15 +79,37 @@ def relmse_stats(x, ref):
     return relmse(xs, ref), relmse(x[ok], np.asarray(ref)[ok]), int(bad.sum())
 
 
+def host_cpu():
+    """Host CPU model (lscpu's "Model name", else /proc/cpuinfo), the CPUs this process may
+    run on, and the threads the CPU baselines use: the CPU share of one GPU on the GPU hosts
+    (OMP_NUM_THREADS, 16 there), at most the CPUs available."""
+    import subprocess
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        model = next((ln.split(":", 1)[1].strip() for ln in out.splitlines() if ln.startswith("Model name")), "")
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if not model and Path("/proc/cpuinfo").exists():
+        for ln in Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    visible = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return {"cpu_model": model, "host_cpus_available": visible, "threads": max(1, min(visible, share))}
+
+
 def cpu_baseline(scene, params, W, H):
     """tinybvh BVH::Intersect (the reference's CPU ray cast) on the camera rays and the
-    first-bounce diffuse rays of the same frame, all host threads, 64K-ray tasks."""
+    first-bounce diffuse rays of the same frame, on the CPU share of one GPU, 64K-ray tasks."""
     import numpy as np
 
     from oracle import pyoracle as po
     if not po.ref_available():
         return None
-    threads = min(os.cpu_count() or 1, 16)
+    hc = host_cpu()
+    threads = hc["threads"]
     tris = po.scene_triangles(scene.meshes[0])
     t0 = time.time()
     bvh = po.TinyBVH(tris)
@@ -106,11 +128,36 @@ def cpu_baseline(scene, params, W, H):
     b = b[ok]
     _, ms_bounce = bvh.intersect(b, threads=threads)
     n = rays.shape[0] + b.shape[0]
-    return {"value": round(n / ((ms_primary + ms_bounce) * 1e-3) / 1e6, 2), "unit": "Mrays/s", "cores": threads,
+    value = n / ((ms_primary + ms_bounce) * 1e-3) / 1e6
+    return {"value": round(value, 2), "unit": "Mrays/s", "cores": threads,
             "kind": "reference",
             "sample": f"tinybvh 1.3.8 BVH::Intersect (binned SAH, single ray) on {rays.shape[0]} camera + "
                       f"{b.shape[0]} first-bounce rays of the same 1080p city frame; BLAS build {build_ms:.0f} ms "
-                      f"single-threaded"}
+                      f"single-threaded",
+            "cpu_model": hc["cpu_model"], "host_cpus_available": hc["host_cpus_available"],
+            "mrays_per_thread": round(value / threads, 3),
+            "all_available_cpus_extrapolated": round(value / threads * hc["host_cpus_available"], 1)}
+
+
+BYTES_PER_VERTEX = 244  # path state r+w 2 x 48 B, material 64 B, 3 indices 12 B, 3 vertices 3 x 24 B
+BYTES_PER_PIXEL = 32    # output targets written per pixel and frame
+
+
+def frame_bytes_roofline(s1, s0, steps, wall, pixels, bvh):
+    """The frame's compulsory bytes (52 B per traced ray, 244 B per shaded path vertex =
+    closest-hit ray, 32 B of targets per pixel, one read of the BVH and triangles) over the
+    frame's wall time, against the HBM peak."""
+    rays = (s1.rays_traced - s0.rays_traced) / steps
+    vertices = (s1.extension_rays - s0.extension_rays) / steps
+    bvh_bytes = bvh.blas_bytes + bvh.tlas_bytes
+    total = BYTES_PER_RAY * rays + BYTES_PER_VERTEX * vertices + BYTES_PER_PIXEL * pixels + bvh_bytes
+    t = wall / steps
+    achieved = total / t / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "bytes_per_frame": int(total),
+            "terms_per_frame": {"rays": int(rays), "path_vertices": int(vertices), "pixels": int(pixels),
+                                "bvh_and_triangle_bytes": int(bvh_bytes)},
+            "definition": "(52 B x rays + 244 B x path vertices + 32 B x pixels + BVH bytes) / frame wall time"}
 
 
 def reference_frame(ctx, p, spp, first_frame=100000):
@@ -193,6 +240,7 @@ def run_city(args):
     ms_trace = s1.ms_trace - s0.ms_trace
     value = rays / wall / 1e6
     achieved = BYTES_PER_RAY * rays / (ms_trace * 1e-3) / 1e9
+    frame_roofline = frame_bytes_roofline(s1, s0, args.steps, wall, W * H, bvh)
     traffic = None
     tfile = ROOT / "profiles" / "trace_city1080p_traffic.json"
     if tfile.exists():
@@ -234,6 +282,7 @@ def run_city(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "kernel": "k_trace (primary rays; per-bounce extension + shadow rays)",
                      "bytes_per_ray": BYTES_PER_RAY},
+        "frame_roofline": frame_roofline,
         "cpu_baseline": cpu,
     }
 
@@ -248,7 +297,8 @@ def dynamic_cpu_baseline(ctx, b, p, W, H, worlds):
     from oracle import pyoracle as po
     if not po.ref_available():
         return None
-    threads = min(os.cpu_count() or 1, 16)
+    hc = host_cpu()
+    threads = hc["threads"]
     bb = po.TinyBVH(po.scene_triangles(b.scene.meshes[0]))
     tl = po.TinyTLAS([bb], worlds[0].reshape(-1, 16), np.zeros(b.n, np.uint32))
     rebuild_ms = [tl.rebuild(worlds[f].reshape(-1, 16)) for f in range(1, 17)]
@@ -269,7 +319,8 @@ def dynamic_cpu_baseline(ctx, b, p, W, H, worlds):
                       f"frame; TLAS rebuild per frame (1 thread, 16 frames) and the restated reprojection + bilateral "
                       f"of that frame (1 thread)",
             "tlas_rebuild_ms": round(float(np.median(rebuild_ms)), 3),
-            "reproject_ms": round((t1 - t0) * 1e3, 1), "bilateral_ms": round((t2 - t1) * 1e3, 1)}
+            "reproject_ms": round((t1 - t0) * 1e3, 1), "bilateral_ms": round((t2 - t1) * 1e3, 1),
+            "cpu_model": hc["cpu_model"], "host_cpus_available": hc["host_cpus_available"]}
 
 
 def run_dynamic(args):
@@ -373,9 +424,11 @@ def run_cornell(args):
         img = rad[..., :3] / np.maximum(rad[..., 3:4], 1)
         res = {"ms_per_frame": round(dt / 16 * 1e3, 3),
                "mrays_per_s": round((s1.rays_traced - s0.rays_traced) / dt / 1e6, 2)}
-        if args.relmse and est == gk.ESTIMATOR_PT_MIS:
-            ref = reference_frame(ctx, p, args.relmse)
-            res["relmse_16spp_vs_ref"] = relmse(img, ref)
+        if est == gk.ESTIMATOR_PT_MIS:
+            ref_spp = args.relmse or 4096
+            ref = reference_frame(ctx, p, ref_spp)
+            res["relmse_16spp_vs_fp64_ref"] = relmse(img, ref)
+            res["reference_spp"] = ref_spp
         if est == gk.ESTIMATOR_REF_V2:
             refv2_params = gk.FrameParams.from_buffer_copy(p)
         out[name] = res
@@ -384,7 +437,8 @@ def run_cornell(args):
     if not args.no_cpu_baseline:
         # the REF_V2 restatement (FPathTracingRendererV2) on all host threads, 4 frames
         from oracle import pyoracle as po
-        threads = min(os.cpu_count() or 1, 16)
+        hc = host_cpu()
+        threads = hc["threads"]
         osc = po.OracleScene(s)
         nrays = 0
         t0 = time.perf_counter()
@@ -395,7 +449,8 @@ def run_cornell(args):
         cpu = {"value": round(nrays / 4 / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
                "sample": f"oracle REF_V2 frames 0-3 of the Cornell box at {W}x{H}, 4 bounces "
                          f"({dt * 1e3:.1f} ms/frame against {out['REF_V2']['ms_per_frame']} ms on the GPU)",
-               "ms_per_frame": round(dt * 1e3, 2)}
+               "ms_per_frame": round(dt * 1e3, 2), "cpu_model": hc["cpu_model"],
+               "host_cpus_available": hc["host_cpus_available"]}
     return {"metric": METRIC, "value": out["PT_MIS"]["mrays_per_s"], "unit": "Mrays/s", "n_gpus": 1, "steps": 16,
             "warmup": 3, "ms_per_step": out["PT_MIS"]["ms_per_frame"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -621,6 +676,25 @@ def run_diff(args):
                        "bruteforce_128px_4096spp_h0.02": by_kind, "bruteforce_params": len(bf)}}
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16 matrix-core peak of one MI355X
+
+
+def unet_flops(net, W, H, real_in=10):
+    """Algorithmic FLOPs of one network evaluation at W x H: sum over convolutions of
+    2 * 9 * cin * cout * pixels at the layer's level, with the real channel counts (10 input
+    channels, 3 output channels), not the padded ones the kernels compute."""
+    total = 0
+    per_layer = []
+    for i in range(net.LAYERS):
+        L = net.layer(i)
+        cin = (real_in if i == 0 else L.cin_a) + L.cin_b
+        px = ((W + (1 << L.level) - 1) >> L.level) * ((H + (1 << L.level) - 1) >> L.level)
+        f = 2 * 9 * cin * L.cout_real * px
+        per_layer.append(f)
+        total += f
+    return total, per_layer
+
+
 def run_denoise(args):
     """Config 5: the learned denoiser against the bilateral filter on 1-spp frames at 1080p
     (the city at the benchmark camera, the Cornell box), relMSE against FP64 renders."""
@@ -639,14 +713,24 @@ def run_denoise(args):
         p.global_seed = 11
         p.total_frames = 5
         res[name] = denoise_compare(ctx, net, p, args.relmse or 4096)
+        flops, _ = unet_flops(net, W, H)
+        res[name]["unet_gflop_per_frame"] = round(flops / 1e9, 1)
+        res[name]["unet_tflops"] = round(flops / (res[name]["denoise_ms_unet"] * 1e-3) / 1e12, 1)
         log(name, res[name])
         ctx.close()
     c = res["city"]
+    # the network's time includes its feature gather and output conversion (HIP events
+    # around the whole denoiser pass): a lower bound of the convolutions' own MFMA rate
+    tflops = c["unet_tflops"]
     return {"metric": METRIC, "value": c["relmse_unet"], "unit": "relMSE (linear, vs FP64 render)", "n_gpus": 1,
             "steps": 20, "warmup": 0, "ms_per_step": c["denoise_ms_unet"], "higher_is_better": False,
             "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
             "config": {"workload": "learned denoiser (config 5) vs bilateral on 1-spp PT_MIS frames, 1920x1080",
-                       "resolution": [W, H], **{f"{k}_{m}": v for k, d in res.items() for m, v in d.items()}}}
+                       "resolution": [W, H], **{f"{k}_{m}": v for k, d in res.items() for m, v in d.items()}},
+            "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
+                         "kernel": "the denoiser pass (12 k_conv MFMA convolutions, pooling, feature gather)",
+                         "flop_per_frame": c["unet_gflop_per_frame"] * 1e9}}
 
 
 def run_probes(args):

@@ -26,6 +26,8 @@ ESTIMATOR_REF_V2, ESTIMATOR_PT_MIS = 0, 1
 DENOISER_NONE, DENOISER_BILATERAL, DENOISER_UNET = 0, 1, 2
 FRAME_NO_MATERIAL_SORT, FRAME_SKIP_POST, FRAME_ACCUM_FP64, FRAME_LINEAR_OUTPUT, FRAME_MATERIAL_SORT = 1, 2, 4, 8, 16
 FRAME_PRIMARY_FROM_VISBUFFER = 32
+FRAME_POST_ONLY = 64
+FRAME_TLAS_REBUILD = 128
 TRACE_CLOSEST, TRACE_ANY = 0, 1
 
 TARGETS = [
@@ -100,7 +102,8 @@ class DeviceDesc(C.Structure):
 class BvhStats(C.Structure):
     _fields_ = [("blas_count", C.c_uint32), ("blas_nodes", C.c_uint32), ("blas_leaves", C.c_uint32),
                 ("tlas_nodes", C.c_uint32), ("blas_bytes", C.c_uint64), ("tlas_bytes", C.c_uint64),
-                ("sah_cost", C.c_double), ("build_ms", C.c_double), ("tlas_ms", C.c_double)]
+                ("sah_cost", C.c_double), ("build_ms", C.c_double), ("tlas_ms", C.c_double),
+                ("tlas_sah_cost", C.c_double)]
 
 
 class Stats(C.Structure):

@@ -358,6 +358,7 @@ int gk_render_radiance(gk_ctx* ctx, float* d_image) {
     if (!ctx || !d_image) return GK_ERR_ARG;
     Ctx& c = ctx->c;
     if (!c.scene_ready || !c.frame_ready) return GK_ERR_STATE;
+    if (c.params.aperture > 0.f) return GK_ERR_ARG;  // the differentiable camera is a pinhole
     GK_HIP(hipSetDevice(c.device));
     if (c.tlas_dirty) {
         const int st = refit_tlas(c);
@@ -374,6 +375,7 @@ int gk_adjoint(gk_ctx* ctx, const float* d_dl_dimage, float* d_grad) {
     if (!ctx || !d_dl_dimage || !d_grad) return GK_ERR_ARG;
     Ctx& c = ctx->c;
     if (!c.scene_ready || !c.frame_ready) return GK_ERR_STATE;
+    if (c.params.aperture > 0.f) return GK_ERR_ARG;  // the differentiable camera is a pinhole
     const uint32_t nparam = 5u * uint32_t(c.h_materials.size());
     if (nparam * 4u > 64u * 1024u) return GK_ERR_LIMIT;  // LDS accumulation: <= 3276 materials
     GK_HIP(hipSetDevice(c.device));

@@ -457,7 +457,13 @@ int gk_instances_update_device(gk_ctx* ctx, const float* d_world16, uint32_t fir
 int gk_get_bvh_stats(gk_ctx* ctx, gk_bvh_stats* out) {
     GK_CHECK_CTX(ctx);
     if (!out) return GK_ERR_ARG;
-    *out = ctx->c.bvh_stats;
+    Ctx& c = ctx->c;
+    if (c.scene_ready) {
+        GK_HIP(hipSetDevice(c.device));
+        const int s = tlas_sah(c, c.bvh_stats.tlas_sah_cost);
+        if (s != GK_OK) return s;
+    }
+    *out = c.bvh_stats;
     return GK_OK;
 }
 
@@ -468,11 +474,15 @@ int gk_frame(gk_ctx* ctx, const gk_frame_params* p) {
     if (p->samples > 1024 || p->bounces > 64 || p->max_bounces > 64) return GK_ERR_LIMIT;
     for (int k = 0; k < 16; ++k)
         if (!std::isfinite(p->model_view_inverse[k]) || !std::isfinite(p->projection_inverse[k])) return GK_ERR_ARG;
+    // thin lens: a finite aperture >= 0, and a positive focus distance when it is open; the
+    // visibility-buffer input is a pinhole's (its DOF resample is not built)
+    if (!std::isfinite(p->aperture) || p->aperture < 0.f) return GK_ERR_ARG;
+    if (p->aperture > 0.f && (!std::isfinite(p->focus_distance) || p->focus_distance <= 0.f)) return GK_ERR_ARG;
+    if (p->aperture > 0.f && (p->flags & GK_FRAME_PRIMARY_FROM_VISBUFFER)) return GK_ERR_ARG;
     Ctx& c = ctx->c;
-    // the visibility-buffer packing holds 15 bits of instance+1 and 17 bits of triangle
-    if ((p->flags & GK_FRAME_PRIMARY_FROM_VISBUFFER) &&
-        (c.h_instances.size() >= 0x7fffu || c.max_mesh_tris > 0x20000u))
-        return GK_ERR_LIMIT;
+    // the visibility-buffer packing holds 15 bits of proxy+1 (one proxy per instance and
+    // 65535-triangle section of its mesh) and the triangle within the section
+    if ((p->flags & GK_FRAME_PRIMARY_FROM_VISBUFFER) && c.num_proxies > kMaxVisProxies) return GK_ERR_LIMIT;
     c.prev_params = c.params;
     c.params = *p;
     c.frame_ready = true;
@@ -559,8 +569,11 @@ int gk_get_stats(gk_ctx* ctx, gk_stats* out) {
     }
     c.stats.extension_rays = c.stats.rays_traced - c.stats.shadow_rays;
     *out = c.stats;
-    if (c.stack_overflow) return GK_ERR_LIMIT;  // a traversal stack overflowed: results are invalid
-    return GK_OK;
+    // a traversal stack overflowed since the last report (gk_trace or gk_render): those
+    // results are invalid; reported once
+    const bool ovf = c.stack_overflow;
+    c.stack_overflow = false;
+    return ovf ? GK_ERR_LIMIT : GK_OK;
 }
 
 int gk_trace(gk_ctx* ctx, const gk_ray* d_rays, uint32_t n, gk_hit* d_hits, uint32_t* d_inst, uint32_t flags) {

@@ -310,11 +310,10 @@ __device__ inline void elem_box(const CollapseArgs& a, uint32_t e, float3& lo, f
     }
 }
 
-__global__ void k_collapse(CollapseArgs a, const uint2* __restrict__ qin, const uint32_t* __restrict__ qin_count,
-                           uint2* __restrict__ qout, uint32_t* __restrict__ qout_count) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *qin_count) return;
-    const uint2 item = qin[i];
+// One queue item of the top-down collapse: binary subtree root_elem becomes wide node w, its
+// internal children are queued for the next level with fresh wide-node indices.
+__device__ void collapse_item(const CollapseArgs& a, const uint2 item, uint2* __restrict__ qout,
+                              uint32_t* __restrict__ qout_count) {
     const uint32_t root_elem = item.x, w = item.y;
     uint32_t set[4];
     int ns;
@@ -386,6 +385,65 @@ __global__ void k_collapse(CollapseArgs a, const uint2* __restrict__ qin, const 
     if (a.depth) a.depth[w] = a.level;
 }
 
+__global__ void k_collapse(CollapseArgs a, const uint2* __restrict__ qin, const uint32_t* __restrict__ qin_count,
+                           uint2* __restrict__ qout, uint32_t* __restrict__ qout_count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *qin_count) return;
+    collapse_item(a, qin[i], qout, qout_count);
+}
+
+// The whole top-down collapse in one work-group, level after level with barriers in between,
+// so that no level waits for the host (a TLAS rebuild per frame). Wide nodes are numbered
+// level by level (a level's children take consecutive indices), so level lv holds nodes
+// [end[lv-1], end[lv]); the kernel then writes the refit order of k_refit_tlas, deepest level
+// first: level_info[0] = number of levels, level_info[1 + k] = start of the k-th group in
+// order[]. counters as k_set_root left them: [0] wide nodes, [1] queue a size, [2] queue b size.
+constexpr uint32_t kCollapseMaxLevels = 255;
+__global__ __launch_bounds__(1024) void k_collapse_block(CollapseArgs a, uint2* __restrict__ qa,
+                                                         uint2* __restrict__ qb, uint32_t* __restrict__ counters,
+                                                         uint32_t* __restrict__ order,
+                                                         uint32_t* __restrict__ level_info) {
+    __shared__ uint32_t s_count;
+    __shared__ uint32_t s_end[kCollapseMaxLevels + 1];
+    uint32_t L = 0;
+    for (;; ++L) {
+        uint32_t* cin = counters + 1 + (L & 1u);
+        uint32_t* cout = counters + 1 + ((L + 1u) & 1u);
+        if (threadIdx.x == 0) {
+            s_count = L < kCollapseMaxLevels ? __hip_atomic_load(cin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+            s_end[L] = __hip_atomic_load(counters, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // levels 0..L
+            __hip_atomic_store(cout, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        const uint32_t count = s_count;
+        if (count == 0) break;
+        CollapseArgs al = a;
+        al.level = L;
+        const uint2* qin = (L & 1u) ? qb : qa;
+        uint2* qout = (L & 1u) ? qa : qb;
+        for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) collapse_item(al, qin[i], qout, cout);
+        __threadfence();
+        __syncthreads();
+        // the queues alternate: drop any L1 copy of the entries this level wrote
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    // L levels: 0 .. L-1 (L == kCollapseMaxLevels: the tree is deeper than the kernel
+    // follows; 0 levels tell the refit there is nothing valid to refit)
+    const uint32_t levels = L < kCollapseMaxLevels ? L : 0u;
+    uint32_t pos = 0;
+    for (uint32_t k = 0; k < levels; ++k) {
+        const uint32_t lv = levels - 1u - k;
+        const uint32_t b = lv ? s_end[lv - 1u] : 0u, e = s_end[lv];
+        if (threadIdx.x == 0) level_info[1 + k] = pos;
+        for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) order[pos + (i - b)] = i;
+        pos += e - b;
+    }
+    if (threadIdx.x == 0) {
+        level_info[0] = levels;
+        level_info[1 + levels] = pos;
+    }
+}
+
 __global__ void k_set_root(uint2* q, uint32_t* counters, uint32_t root_elem) {
     q[0] = make_uint2(root_elem, 0u);
     counters[0] = 1u;  // wide nodes allocated (root)
@@ -420,7 +478,8 @@ struct WideResult {
 // LBVH + SAH collapse + 4-wide collapse of n primitive boxes. Wide nodes are written
 // to out[node_base ...], leaf children point at leaf_base + sorted position.
 int build_wide(Ctx& c, const float4* plo, const float4* phi, uint32_t n, uint32_t max_leaf, Bvh4Node* out,
-               uint32_t node_base, uint32_t leaf_base, uint32_t* depth_out, WideResult& res, bool blas) {
+               uint32_t node_base, uint32_t leaf_base, uint32_t* depth_out, WideResult& res, bool blas,
+               uint32_t* refit_order, uint32_t* level_info) {
     BuildScratch& s = c.scratch;
     hipStream_t st = c.stream;
     GK_HIP(s.keys.resize(n)); GK_HIP(s.keys_alt.resize(n));
@@ -463,6 +522,13 @@ int build_wide(Ctx& c, const float4* plo, const float4* phi, uint32_t n, uint32_
     k_set_root<<<1, 1, 0, st>>>(qa, s.counters.ptr, root_elem);
     CollapseArgs a{s.bchild.ptr, s.brange.ptr, s.bnode_lo.ptr, s.bnode_hi.ptr, plo, phi, vals,
                    out, node_base, leaf_base, blas ? 1u : 0u, s.counters.ptr + 0, depth_out, 0};
+    if (refit_order) {
+        // device only: one work-group runs every level and writes the refit order; nothing
+        // is read back (res keeps only the sorted primitive order)
+        k_collapse_block<<<1, 1024, 0, st>>>(a, qa, qb, s.counters.ptr, refit_order, level_info);
+        GK_HIP(hipGetLastError());
+        return GK_OK;
+    }
     uint32_t qcount = 1, level = 0;
     uint32_t* cin = s.counters.ptr + 1;
     uint32_t* cout = s.counters.ptr + 2;
@@ -600,10 +666,14 @@ __global__ void k_update_instances(InstDev* __restrict__ inst, const float* __re
 }
 
 // Bottom-up refit of the wide TLAS by levels (deepest first), one work-group.
+// level_info: [0] number of levels, [1 ..] start of each level's nodes in order[] (deepest
+// level first), as build_tlas or k_collapse_block wrote them
 __global__ void k_refit_tlas(Bvh4Node* __restrict__ nodes, const uint32_t* __restrict__ order,
-                             const uint32_t* __restrict__ level_off, uint32_t levels,
+                             const uint32_t* __restrict__ level_info,
                              const uint32_t* __restrict__ prims, const float4* __restrict__ ilo,
                              const float4* __restrict__ ihi) {
+    const uint32_t levels = level_info[0];
+    const uint32_t* level_off = level_info + 1;
     for (uint32_t L = 0; L < levels; ++L) {
         for (uint32_t i = level_off[L] + threadIdx.x; i < level_off[L + 1]; i += blockDim.x) {
             Bvh4Node& n = nodes[order[i]];
@@ -768,7 +838,7 @@ int build_blas(Ctx& c, const gk_mesh* meshes, uint32_t num_meshes) {
         GK_HIP(hipGetLastError());
         WideResult r;
         int s = build_wide(c, c.scratch.prim_lo.ptr, c.scratch.prim_hi.ptr, nt, kMaxLeafTris, c.blas_nodes.ptr, node_base,
-                           tri_base, nullptr, r, true);
+                           tri_base, nullptr, r, true, nullptr, nullptr);
         if (s != GK_OK) return s;
         k_reorder_tris<<<nblk(nt), 256, 0, st>>>(tmp.ptr, r.vals, nt, c.tris.ptr + tri_base, c.tri_rec.ptr + tri_base);
         GK_HIP(hipGetLastError());
@@ -818,6 +888,7 @@ int build_tlas(Ctx& c) {
     // instance records (world, prev from the host structs)
     std::vector<InstDev> h(n);
     std::vector<float> worlds(16 * size_t(n)), prevs(16 * size_t(n));
+    std::vector<uint32_t> proxy_inst;
     for (uint32_t i = 0; i < n; ++i) {
         const gk_instance& g = c.h_instances[i];
         InstDev& d = h[i];
@@ -830,6 +901,10 @@ int build_tlas(Ctx& c) {
         d.mat_offset = g.mat_offset;
         d.tri_base = c.h_meshes[g.mesh_id].tri_base;
         d.root = c.h_meshes[g.mesh_id].root;
+        // one visibility-buffer proxy per section of the mesh, in instance order
+        d.proxy_base = uint32_t(proxy_inst.size());
+        const uint32_t sections = (c.h_meshes[g.mesh_id].tri_count + kSectionTris - 1) / kSectionTris;
+        proxy_inst.insert(proxy_inst.end(), sections, i);
         memcpy(d.mat_id, g.mat_id, 64);
         memcpy(&worlds[16 * size_t(i)], g.world, 64);
         memcpy(&prevs[16 * size_t(i)], g.prev_world, 64);
@@ -838,6 +913,10 @@ int build_tlas(Ctx& c) {
     GK_HIP(c.inst_lo.resize(std::max<uint32_t>(n, 1)));
     GK_HIP(c.inst_hi.resize(std::max<uint32_t>(n, 1)));
     GK_HIP(c.inst_world_in.resize(32 * size_t(std::max<uint32_t>(n, 1))));
+    c.num_proxies = uint32_t(proxy_inst.size());
+    GK_HIP(c.proxy_inst.resize(std::max<size_t>(proxy_inst.size(), 1)));
+    if (!proxy_inst.empty())
+        GK_HIP(hipMemcpyAsync(c.proxy_inst.ptr, proxy_inst.data(), 4 * proxy_inst.size(), hipMemcpyHostToDevice, st));
     if (n == 0) return GK_OK;
     GK_HIP(hipMemcpyAsync(c.inst.ptr, h.data(), sizeof(InstDev) * n, hipMemcpyHostToDevice, st));
     GK_HIP(hipMemcpyAsync(c.inst_world_in.ptr, worlds.data(), 64 * size_t(n), hipMemcpyHostToDevice, st));
@@ -856,7 +935,8 @@ int build_tlas(Ctx& c) {
     GK_HIP(hipEventRecord(e0, st));
     // one instance per TLAS leaf: each instance's box is tested before the ray enters it, which
     // costs less than entering (4096 bricks at 1080p: 5.69 -> 5.54 ms per frame with 2 per leaf)
-    int s = build_wide(c, c.inst_lo.ptr, c.inst_hi.ptr, n, 1, c.tlas_nodes.ptr, 0, 0, depth.ptr, r, false);
+    int s = build_wide(c, c.inst_lo.ptr, c.inst_hi.ptr, n, 1, c.tlas_nodes.ptr, 0, 0, depth.ptr, r, false, nullptr,
+                       nullptr);
     if (s != GK_OK) return s;
     k_copy_u32<<<nblk(n), 256, 0, st>>>(r.vals, c.tlas_prims.ptr, n);
     GK_HIP(hipEventRecord(e1, st));
@@ -879,13 +959,104 @@ int build_tlas(Ctx& c) {
         for (uint32_t i = 0; i < r.nodes; ++i) if (hd[i] == uint32_t(lv)) order.push_back(i);
         c.tlas_level_offsets.push_back(uint32_t(order.size()));
     }
-    GK_HIP(c.tlas_refit_order.resize(order.size() + c.tlas_level_offsets.size()));
+    // sized for any rebuild of the same instances (at most n wide nodes, kCollapseMaxLevels levels)
+    GK_HIP(c.tlas_refit_order.resize(std::max<size_t>(order.size(), n)));
+    GK_HIP(c.tlas_level_info.resize(kCollapseMaxLevels + 2));
     GK_HIP(hipMemcpy(c.tlas_refit_order.ptr, order.data(), 4 * order.size(), hipMemcpyHostToDevice));
-    GK_HIP(hipMemcpy(c.tlas_refit_order.ptr + order.size(), c.tlas_level_offsets.data(),
-                     4 * c.tlas_level_offsets.size(), hipMemcpyHostToDevice));
+    std::vector<uint32_t> info(1, uint32_t(c.tlas_level_offsets.size()) - 1u);
+    info.insert(info.end(), c.tlas_level_offsets.begin(), c.tlas_level_offsets.end());
+    GK_HIP(hipMemcpy(c.tlas_level_info.ptr, info.data(), 4 * info.size(), hipMemcpyHostToDevice));
     c.bvh_stats.tlas_nodes = r.nodes;
     c.bvh_stats.tlas_bytes = uint64_t(r.nodes) * sizeof(Bvh4Node);
     c.bvh_stats.tlas_ms = ms;
+    c.tlas_dirty = false;
+    return GK_OK;
+}
+
+// SAH cost of the TLAS as it stands: sum over wide nodes of the node's box area plus, for
+// leaf children, instance count x child box area, over the root box area (C_TRAV = C_INT = 1).
+// One work-group; result in *out.
+__global__ __launch_bounds__(256) void k_tlas_sah(const Bvh4Node* __restrict__ nodes, const uint32_t* __restrict__ order,
+                                                   const uint32_t* __restrict__ level_info, double* __restrict__ out) {
+    __shared__ double s_sum[256];
+    const uint32_t levels = level_info[0];
+    const uint32_t nn = level_info[1 + levels];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < nn; i += blockDim.x) {
+        const Bvh4Node& n = nodes[order[i]];
+        const float lx[4] = {n.lox.x, n.lox.y, n.lox.z, n.lox.w}, hx[4] = {n.hix.x, n.hix.y, n.hix.z, n.hix.w};
+        const float ly[4] = {n.loy.x, n.loy.y, n.loy.z, n.loy.w}, hy[4] = {n.hiy.x, n.hiy.y, n.hiy.z, n.hiy.w};
+        const float lz[4] = {n.loz.x, n.loz.y, n.loz.z, n.loz.w}, hz[4] = {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w};
+        const uint32_t me[4] = {n.meta.x, n.meta.y, n.meta.z, n.meta.w};
+        float3 ulo = make_float3(3e38f, 3e38f, 3e38f), uhi = make_float3(-3e38f, -3e38f, -3e38f);
+        for (int j = 0; j < 4; ++j) {
+            if (lx[j] == kEmpty) continue;
+            ulo = make_float3(fminf(ulo.x, lx[j]), fminf(ulo.y, ly[j]), fminf(ulo.z, lz[j]));
+            uhi = make_float3(fmaxf(uhi.x, hx[j]), fmaxf(uhi.y, hy[j]), fmaxf(uhi.z, hz[j]));
+            if (me[j] > 0u) {
+                const double dx = hx[j] - lx[j], dy = hy[j] - ly[j], dz = hz[j] - lz[j];
+                acc += double(me[j]) * 2.0 * (dx * dy + dy * dz + dz * dx);
+            }
+        }
+        if (uhi.x >= ulo.x) {
+            const double dx = uhi.x - ulo.x, dy = uhi.y - ulo.y, dz = uhi.z - ulo.z;
+            const double a = 2.0 * (dx * dy + dy * dz + dz * dx);
+            acc += a;
+        }
+    }
+    s_sum[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s_sum[threadIdx.x] += s_sum[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const Bvh4Node& r = nodes[0];
+        const float lx[4] = {r.lox.x, r.lox.y, r.lox.z, r.lox.w}, hx[4] = {r.hix.x, r.hix.y, r.hix.z, r.hix.w};
+        const float ly[4] = {r.loy.x, r.loy.y, r.loy.z, r.loy.w}, hy[4] = {r.hiy.x, r.hiy.y, r.hiy.z, r.hiy.w};
+        const float lz[4] = {r.loz.x, r.loz.y, r.loz.z, r.loz.w}, hz[4] = {r.hiz.x, r.hiz.y, r.hiz.z, r.hiz.w};
+        double l[3] = {3e38, 3e38, 3e38}, h[3] = {-3e38, -3e38, -3e38};
+        for (int j = 0; j < 4; ++j) {
+            if (lx[j] == kEmpty) continue;
+            l[0] = fmin(l[0], lx[j]); l[1] = fmin(l[1], ly[j]); l[2] = fmin(l[2], lz[j]);
+            h[0] = fmax(h[0], hx[j]); h[1] = fmax(h[1], hy[j]); h[2] = fmax(h[2], hz[j]);
+        }
+        const double dx = h[0] - l[0], dy = h[1] - l[1], dz = h[2] - l[2];
+        const double root = 2.0 * (dx * dy + dy * dz + dz * dx);
+        *out = root > 0.0 ? s_sum[0] / root : 0.0;
+    }
+}
+
+int tlas_sah(Ctx& c, double& out) {
+    out = 0.0;
+    if (c.h_instances.empty() || c.tlas_node_count == 0) return GK_OK;
+    DevBuf<double> d;
+    GK_HIP(d.resize(1));
+    k_tlas_sah<<<1, 256, 0, c.stream>>>(c.tlas_nodes.ptr, c.tlas_refit_order.ptr, c.tlas_level_info.ptr, d.ptr);
+    GK_HIP(hipGetLastError());
+    GK_HIP(hipMemcpyAsync(&out, d.ptr, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    GK_HIP(hipStreamSynchronize(c.stream));
+    return GK_OK;
+}
+
+// A new TLAS over the instances' current world boxes, built on the device without waiting
+// for the host (LBVH, SAH leaf collapse, then the single-work-group 4-wide collapse, which
+// also writes the refit order), as the reference rebuilds its TLAS every frame
+// (TopLevelAccelerationStructure.cpp:73-81). Buffers keep the sizes of the scene's build.
+int rebuild_tlas(Ctx& c) {
+    const uint32_t n = uint32_t(c.h_instances.size());
+    if (n == 0 || c.tlas_node_count == 0) return GK_OK;
+    WideResult r;
+    const int s = build_wide(c, c.inst_lo.ptr, c.inst_hi.ptr, n, 1, c.tlas_nodes.ptr, 0, 0, nullptr, r, false,
+                             c.tlas_refit_order.ptr, c.tlas_level_info.ptr);
+    if (s != GK_OK) return s;
+    k_copy_u32<<<nblk(n), 256, 0, c.stream>>>(r.vals, c.tlas_prims.ptr, n);
+    GK_HIP(hipGetLastError());
+    if (c.num_lights) {
+        k_update_lights<<<nblk(c.num_lights, 128), 128, 0, c.stream>>>(c.lights.ptr, c.light_obj.ptr, c.num_lights,
+                                                                  c.inst.ptr);
+        GK_HIP(hipGetLastError());
+    }
     c.tlas_dirty = false;
     return GK_OK;
 }
@@ -910,10 +1081,8 @@ int update_lights(Ctx& c) {
 
 int refit_tlas(Ctx& c) {
     if (c.h_instances.empty() || c.tlas_node_count == 0) return GK_OK;
-    const uint32_t levels = uint32_t(c.tlas_level_offsets.size()) - 1;
-    const uint32_t nn = c.tlas_node_count;
-    k_refit_tlas<<<1, 1024, 0, c.stream>>>(c.tlas_nodes.ptr, c.tlas_refit_order.ptr, c.tlas_refit_order.ptr + nn,
-                                          levels, c.tlas_prims.ptr, c.inst_lo.ptr, c.inst_hi.ptr);
+    k_refit_tlas<<<1, 1024, 0, c.stream>>>(c.tlas_nodes.ptr, c.tlas_refit_order.ptr, c.tlas_level_info.ptr,
+                                          c.tlas_prims.ptr, c.inst_lo.ptr, c.inst_hi.ptr);
     GK_HIP(hipGetLastError());
     if (c.num_lights) {
         k_update_lights<<<nblk(c.num_lights, 128), 128, 0, c.stream>>>(c.lights.ptr, c.light_obj.ptr, c.num_lights,

@@ -62,10 +62,17 @@ struct InstDev {         // per instance, 256 bytes
     uint32_t mat_offset;
     uint32_t tri_base;   // = meshes[mesh].tri_base
     uint32_t root;       // = meshes[mesh].root
-    uint32_t pad;
+    uint32_t proxy_base; // visibility-buffer proxy of the mesh's section 0 (kSectionTris each)
     uint32_t mat_id[16];
 };
 static_assert(sizeof(InstDev) == 288, "InstDev size");
+
+// Visibility-buffer numbering (Scene::RebuildMeshBuffer slices a model into sections of at
+// most 65535 triangles, Scene.cpp:202-256, and UpdateNodesGpuDriven emits one NodeProxy per
+// node and section, Scene.cpp:472-478): proxy = instance's proxy_base + prim / kSectionTris,
+// and the packed id is (proxy + 1) << 17 | prim % kSectionTris.
+constexpr uint32_t kSectionTris = 65535u;
+constexpr uint32_t kMaxVisProxies = 0x7fffu;  // proxies whose proxy + 1 fits the 15-bit field
 
 struct TexInfo {         // one texture of the pool
     uint64_t offset;     // first texel in Ctx::texels
@@ -126,7 +133,9 @@ struct Ctx {
     DevBuf<TexInfo> tex;
     DevBuf<float> tex_lut;          // [0,256): sRGB byte -> linear, [256,512): byte / 255
     uint32_t num_tex = 0;
-    uint32_t max_mesh_tris = 0;     // largest mesh (visibility-buffer packing limit)
+    uint32_t max_mesh_tris = 0;     // largest mesh
+    uint32_t num_proxies = 0;       // visibility-buffer proxies (instance x section)
+    DevBuf<uint32_t> proxy_inst;    // proxy -> instance
     DevBuf<Bvh4Node> blas_nodes;
     DevBuf<MeshInfo> meshes;
     DevBuf<InstDev> inst;
@@ -134,7 +143,8 @@ struct Ctx {
     DevBuf<float4> inst_lo, inst_hi;  // world-space instance boxes
     DevBuf<Bvh4Node> tlas_nodes;
     DevBuf<uint32_t> tlas_prims;    // TLAS leaf -> instance index
-    DevBuf<uint32_t> tlas_refit_order;
+    DevBuf<uint32_t> tlas_refit_order;  // nodes, deepest level first
+    DevBuf<uint32_t> tlas_level_info;   // [levels, start of each level in tlas_refit_order ...]
     std::vector<uint32_t> tlas_level_offsets;
     uint32_t tlas_node_count = 0;
     uint32_t tlas_depth = 0;
@@ -213,7 +223,7 @@ struct Ctx {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tev;
     uint32_t tev_used = 0;
     uint64_t trace_launches = 0;
-    bool stack_overflow = false;
+    bool stack_overflow = false;        // a traversal stack overflowed since gk_get_stats last reported
     bool trace_counter_zeroed = false;  // the next trace launch's counter is already 0
     gk_stats stats{};
 };
@@ -235,6 +245,8 @@ int build_blas(Ctx& c, const gk_mesh* meshes, uint32_t num_meshes);
 int build_tlas(Ctx& c);
 int update_instances_device(Ctx& c, uint32_t first, uint32_t count);  // from inst_world_in
 int refit_tlas(Ctx& c);
+int rebuild_tlas(Ctx& c);  // device-only TLAS build over the current instance boxes
+int tlas_sah(Ctx& c, double& out);  // SAH cost of the TLAS as last refitted or rebuilt (synchronises)
 int update_lights(Ctx& c);  // world-space emissive triangles from the instance matrices
 
 // trace.hip

@@ -100,6 +100,25 @@ __device__ inline void sincos_2pi(T u, T& s, T& c) {
     }
 }
 
+// concentric_sample_disk (Const_Func.slang:260-275) of RandomInUnitDisk (:277-280): offset
+// 2u - 1; zero maps to zero (isZERO, |x| < 1e-35); the larger coordinate is the radius and
+// the angle pi/4 * (smaller / larger), whose sin/cos come from sincos_2pi of |ratio| / 8
+// (exact) with the sign restored; the second branch keeps the reference's form
+// (sin of the angle, then sqrt(1 - s^2)).
+template <class T>
+__device__ inline void concentric_disk(T ua, T ub, T& dx, T& dy) {
+    const T ox = ua + (ua - T(1)), oy = ub + (ub - T(1));
+    const T nz = T(1e-35);
+    if (ox > -nz && ox < nz && oy > -nz && oy < nz) { dx = T(0); dy = T(0); return; }
+    const bool xmaj = fabs(ox) > fabs(oy);
+    const T r = xmaj ? oy / ox : ox / oy;
+    T s, c;
+    sincos_2pi(fabs(r) * T(0.125), s, c);
+    if (r < T(0)) s = -s;
+    if (xmaj) { dx = ox * c; dy = ox * s; }
+    else { dx = oy * s; dy = oy * sqrt(T(1) - s * s); }
+}
+
 // ---------------------------------------------------------------- acceleration structures
 // 4-wide BVH node, one 128-byte cache line. Child i is
 //   internal: meta[i] == 0, child[i] = node index

@@ -129,7 +129,10 @@ __global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
     const float2 mv = T.motion[i];
     const float fx = float(x) + mv.x, fy = float(y) + mv.y;
     const int ppx = int(floorf(fx)), ppy = int(floorf(fy));
-    const bool inside = ppx < int(K.W) && ppy < int(K.H) && ppx >= -1 && ppy >= -1;
+    // all(previpos < isize) && all(previpos >= -1) with int2 previpos and uint2 isize
+    // (Process.ReProject.comp.slang:109-110): the mixed-sign comparison converts previpos to
+    // unsigned, so -1 fails it and history is used only for 0 <= previpos < size
+    const bool inside = uint32_t(ppx) < K.W && uint32_t(ppy) < K.H;
     const uint32_t cur = T.obj0[i];
     uint32_t p0 = ld_u32(T.obj1, ppx, ppy, K);
     const bool miss = cur == 65535u;

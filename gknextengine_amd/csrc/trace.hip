@@ -755,7 +755,11 @@ int collect_timing(Ctx& c) {
         GK_HIP(hipMemcpy(&ovf, tail + 1, 4, hipMemcpyDeviceToHost));
         c.stats.rays_traced = v[0] + v[1];
         c.stats.shadow_rays = v[1];
-        c.stack_overflow = ovf != 0;
+        if (ovf) {  // kept until gk_get_stats reports it; the device flag starts over
+            c.stack_overflow = true;
+            const uint32_t zero = 0;
+            GK_HIP(hipMemcpy(const_cast<uint32_t*>(tail + 1), &zero, 4, hipMemcpyHostToDevice));
+        }
     }
     return GK_OK;
 }

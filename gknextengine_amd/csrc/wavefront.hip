@@ -50,6 +50,7 @@ struct FrameK {
     float mvi[16], pi[16], vp[16], vpu[16], pvpu[16];
     float sun_dir[4], sun_col[4], sky[4];
     float exit_prob, hit_off;
+    float aperture, focus;  // thin lens (aperture 0: pinhole, no random numbers drawn)
     uint32_t W, H, frame, seed, samples, bounces, max_bounces, temporal, has_sky, has_sun, tiled;
     uint32_t vis_primary;  // primary hits come from GK_RT_MINIGBUFFER (left unwritten)
     ProbeDev probes;       // REF_V2 path termination
@@ -127,6 +128,32 @@ __device__ inline void camera_ray(const FrameK& K, uint32_t x, uint32_t y, f3& O
     mul44v(K.pi, ux, uy, 1.f, 1.f, t);
     const f3 tn = normalize(f3{t[0], t[1], t[2]});
     mul44v(K.mvi, tn.x, tn.y, tn.z, 0.f, d);
+    O = f3{o[0], o[1], o[2]};
+    D = normalize(f3{d[0], d[1], d[2]});
+}
+
+// The thin-lens ray of FHardwarePrimaryRayCaster::TracePrimaryRay (Shading.slang:283-300):
+// a lens offset Aperture / 2 * RandomInUnitDisk, origin MVinv (offset, 0, 1), direction
+// MVinv normalize(Pinv (uv, 1, 1) * FocusDistance - (offset, 0)). The disk draw is the
+// path's first random number. With aperture 0 the pinhole ray is taken and nothing is drawn,
+// as the live visibility-buffer caster draws only for a circle of confusion (:372-379).
+__device__ inline void camera_ray_lens(const FrameK& K, uint32_t x, uint32_t y, rng4& seed, f3& O, f3& D) {
+    if (!(K.aperture > 0.f)) {
+        camera_ray(K, x, y, O, D);
+        return;
+    }
+    float a, b, dx, dy;
+    rnd2(seed, a, b);
+    concentric_disk(a, b, dx, dy);
+    const float half = K.aperture / 2.f;
+    const float lx = half * dx, ly = half * dy;
+    const float ux = (float(x) / float(K.W)) * 2.0f - 1.0f;
+    const float uy = (float(y) / float(K.H)) * 2.0f - 1.0f;
+    float o[4], t[4], d[4];
+    mul44v(K.mvi, lx, ly, 0.f, 1.f, o);
+    mul44v(K.pi, ux, uy, 1.f, 1.f, t);
+    const f3 tl = normalize(f3{t[0] * K.focus - lx, t[1] * K.focus - ly, t[2] * K.focus - 0.f});
+    mul44v(K.mvi, tl.x, tl.y, tl.z, 0.f, d);
     O = f3{o[0], o[1], o[2]};
     D = normalize(f3{d[0], d[1], d[2]});
 }
@@ -280,7 +307,12 @@ __device__ inline void write_gbuffer(const DScene& s, const FrameK& K, const Tar
     T.albedo[px] = to_h4(albedo.x, albedo.y, albedo.z, albedo.w);
     T.normal[px] = to_h4(nrm.x, nrm.y, nrm.z, rough);
     T.objid[px] = I.instance_id;
-    if (!K.vis_primary) T.minigbuffer[px] = ((inst + 1u) << 17) | (prim & 0x1ffffu);
+    if (!K.vis_primary) {
+        // the reference's packing of (NodeProxy + 1, triangle of the section); 0 (no geometry)
+        // for a proxy the 15-bit field cannot name
+        const uint32_t proxy = I.proxy_base + prim / kSectionTris;
+        T.minigbuffer[px] = proxy < kMaxVisProxies ? ((proxy + 1u) << 17) | (prim % kSectionTris) : 0u;
+    }
     T.phit[px] = hit;
     T.pinst[px] = inst;
 }
@@ -306,16 +338,17 @@ __global__ void k_raygen(FrameK K, Queues Q, PathState S, uint32_t n, uint32_t q
     const uint32_t px = pixel_of(K, p);
     const uint32_t x = px % K.W, y = px / K.W;
     f3 O, D;
-    camera_ray(K, x, y, O, D);
+    rng4 seed{x, y, K.frame, K.seed};
+    camera_ray_lens(K, x, y, seed, O, D);
     Q.o[0][p] = make_float4(O.x, O.y, O.z, 0.f);
     Q.d[0][p] = make_float4(D.x, D.y, D.z, 1e30f);
     Q.path[0][p] = p;
     if (queue_state) {
-        Q.qrng[0][p] = make_uint4(x, y, K.frame, K.seed);
+        Q.qrng[0][p] = make_uint4(seed.x, seed.y, seed.z, seed.w);
         Q.qmisc[0][p] = make_uint4(px, 0u, 0u, 0u);
         Q.qthr[0][p] = make_float4(1.f, 1.f, 1.f, 0.f);
     } else {
-        S.rng[p] = make_uint4(x, y, K.frame, K.seed);
+        S.rng[p] = make_uint4(seed.x, seed.y, seed.z, seed.w);
         S.misc[p] = make_uint4(px, 0u, 0u, 0u);
         S.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
     }
@@ -324,8 +357,9 @@ __global__ void k_raygen(FrameK K, Queues Q, PathState S, uint32_t n, uint32_t q
 }
 
 // Primary hits from a visibility buffer (GK_FRAME_PRIMARY_FROM_VISBUFFER): the camera ray of
-// path p hits the triangle GK_RT_MINIGBUFFER names for its pixel ((inst+1) << 17 | tri, 0 = no
-// geometry; Rast.VisibilityPass.vert.slang:18-22), as in FVisibilityBufferRayCasterV2::
+// path p hits the triangle GK_RT_MINIGBUFFER names for its pixel ((proxy+1) << 17 | tri, 0 = no
+// geometry; Rast.VisibilityPass.vert.slang:18-22, where proxy numbers the (instance, section)
+// pairs and tri counts within the section's 65535 triangles), as in FVisibilityBufferRayCasterV2::
 // TracePrimaryRay (Shading.slang:336-370). The barycentrics are get_material_dataV2's
 // Moeller-Trumbore form without rejection (GeneralFunc.slang:31-45); t follows the same
 // form. They are evaluated in the triangle's object space with the tracer's arithmetic
@@ -333,17 +367,18 @@ __global__ void k_raygen(FrameK K, Queues Q, PathState S, uint32_t n, uint32_t q
 // frame of this core reproduces that frame's primary hits bit for bit.
 __global__ void k_vis_primary(DScene s, FrameK K, Queues Q, const uint32_t* __restrict__ vis,
                               const uint32_t* __restrict__ tri_rec, const MeshInfo* __restrict__ meshes,
-                              uint32_t ninst, uint32_t n) {
+                              const uint32_t* __restrict__ proxy_inst, uint32_t nproxy, uint32_t n) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const uint32_t px = pixel_of(K, p);
     const uint32_t pk = vis[px];
-    const uint32_t ip1 = (pk >> 17) & 0x7fffu, tri = pk & 0x1ffffu;
+    const uint32_t ip1 = (pk >> 17) & 0x7fffu, stri = pk & 0x1ffffu;
     float4 hit = make_float4(__builtin_inff(), 0.f, 0.f, __uint_as_float(kMiss));
     uint32_t hinst = kMiss;
-    if (ip1 != 0u && ip1 - 1u < ninst) {
-        const uint32_t inst = ip1 - 1u;
+    if (ip1 != 0u && ip1 - 1u < nproxy && stri < kSectionTris) {
+        const uint32_t inst = proxy_inst[ip1 - 1u];
         const InstDev& I = s.inst[inst];
+        const uint32_t tri = (ip1 - 1u - I.proxy_base) * kSectionTris + stri;  // within the mesh
         if (tri < meshes[I.mesh].tri_count) {
             f3 O = ld3(Q.o[0][p]), D = ld3(Q.d[0][p]);
             if (s.two_level) {
@@ -903,7 +938,7 @@ __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T,
             if (sample < K.samples) {
                 const uint32_t x = px % K.W, y = px / K.W;
                 f3 O2, D2;
-                camera_ray(K, x, y, O2, D2);
+                camera_ray_lens(K, x, y, seed, O2, D2);
                 ro = make_float4(O2.x, O2.y, O2.z, 0.f);
                 rd = make_float4(D2.x, D2.y, D2.z, 1e30f);
                 flags = (flags & 0x300u) | (sample << 16) | 1024u;  // keep the primary lobe, depth 0 -> 1 below
@@ -1072,6 +1107,8 @@ static FrameK make_k(const Ctx& c) {
     memcpy(K.sky, P.sky_color, 16);
     K.exit_prob = P.exit_probability;
     K.hit_off = P.hit_normal_offset;
+    K.aperture = P.aperture;
+    K.focus = P.focus_distance;
     K.W = c.width;
     K.H = c.height;
     K.frame = P.total_frames;
@@ -1101,7 +1138,18 @@ int render_frame(Ctx& c) {
         if (st != GK_OK) return st;
     }
     hipEventRecord(c.ev[0], s);
-    if (c.tlas_dirty) { st = refit_tlas(c); if (st != GK_OK) return st; }
+    if (c.params.flags & GK_FRAME_POST_ONLY) {  // the caller wrote the rt pass's targets
+        hipEventRecord(c.ev[1], s);
+        hipEventRecord(c.ev[2], s);
+        st = post_process(c);
+        if (st != GK_OK) return st;
+        hipEventRecord(c.ev[3], s);
+        return GK_OK;
+    }
+    if (c.tlas_dirty) {
+        st = (c.params.flags & GK_FRAME_TLAS_REBUILD) ? rebuild_tlas(c) : refit_tlas(c);
+        if (st != GK_OK) return st;
+    }
     hipEventRecord(c.ev[1], s);
     const FrameK K = make_k(c);
     const DScene S0 = make_dscene(c);
@@ -1158,7 +1206,7 @@ int render_frame(Ctx& c) {
     // primary rays: traced, or their hits taken from the caller's visibility buffer
     if (K.vis_primary) {
         k_vis_primary<<<nblk(n), 256, 0, s>>>(S0, K, Q, T.minigbuffer, c.tri_rec.ptr, c.meshes.ptr,
-                                               uint32_t(c.h_instances.size()), n);
+                                               c.proxy_inst.ptr, c.num_proxies, n);
         GK_HIP(hipGetLastError());
     } else {
         st = launch_trace(c, Q.o[0], Q.d[0], Q.cnt + 0, n, Q.hit, Q.hit_inst, false, nullptr);

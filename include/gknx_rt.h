@@ -85,7 +85,7 @@ enum gk_target {
     GK_RT_ACCUM_SPECULAR = 10,  /* RGBA16F                                        */
     GK_RT_ACCUM_ALBEDO = 11,    /* RGBA16F                                        */
     GK_RT_DENOISED = 12,        /* RGBA16F  tonemapped output (linear radiance with GK_FRAME_LINEAR_OUTPUT) */
-    GK_RT_MINIGBUFFER = 13,     /* R32UI    (inst+1)<<17 | tri, visibility buffer */
+    GK_RT_MINIGBUFFER = 13,     /* R32UI    (proxy+1)<<17 | tri, visibility buffer (below) */
     GK_RT_PRIMARY_HIT = 14,     /* gk_hit   primary-ray hit {t,u,v,prim}         */
     GK_RT_PRIMARY_INST = 15,    /* R32UI    primary-ray hit instance             */
     GK_RT_RADIANCE = 16,        /* RGBA32F  running FP32 sum of diffuse+specular  */
@@ -189,7 +189,10 @@ typedef struct gk_frame_params {
     float sun_direction[4];
     float sun_color[4];
     float sky_color[4];        /* constant sky radiance (the reference samples an HDRI) */
-    float aperture;
+    float aperture;            /* thin-lens diameter (FHardwarePrimaryRayCaster, Shading.slang:
+                                  283-300); 0: pinhole. > 0 needs focus_distance > 0; not
+                                  with GK_FRAME_PRIMARY_FROM_VISBUFFER, gk_render_radiance
+                                  or gk_adjoint (GK_ERR_ARG) */
     float focus_distance;
     float paper_white_nit;
     float bf_sigma;
@@ -219,15 +222,36 @@ typedef struct gk_frame_params {
 #define GK_FRAME_NO_MATERIAL_SORT 1u   /* shade in path order (the default; overrides 16u) */
 #define GK_FRAME_MATERIAL_SORT 16u     /* sort each hit queue by material class (model x
                                           metal, misses last) before shading               */
-/* Primary hits from the caller's visibility buffer instead of traced camera rays: before
+/* Visibility-buffer ids (GK_RT_MINIGBUFFER), numbered as the reference's raster pass numbers
+ * them: Scene::RebuildMeshBuffer slices each mesh into sections of at most 65535 triangles
+ * (Scene.cpp:202-256) and UpdateNodesGpuDriven emits one NodeProxy per node and section
+ * (Scene.cpp:472-478), so a pixel holds (proxy+1) << 17 | triangle-within-section
+ * (Rast.VisibilityPass.vert.slang:18-22), 0 where no geometry covers it. Here the proxies
+ * are the instances in upload order, each taking ceil(triangles / 65535) consecutive
+ * numbers: triangle t of instance i is proxy proxy_base(i) + t / 65535, id t % 65535. Traced
+ * frames write 0 for a proxy above 0x7ffe (the 15-bit field cannot name it). The reference
+ * keeps at most 10 sections per model (it warns and drops the rest); all sections are
+ * numbered here. Its provoking-vertex reorder (meshopt_generateProvokingIndexBuffer)
+ * changes only the labelling of a triangle's corners, not these ids.
+ * Primary hits from the caller's visibility buffer instead of traced camera rays: before
  * gk_render the caller writes GK_RT_MINIGBUFFER (device pointer from gk_target_info) with
- * (instance+1) << 17 | triangle per pixel, 0 where no geometry covers it, as the
- * reference's raster visibility pass does (Rast.VisibilityPass.vert.slang:18-22,
- * FVisibilityBufferRayCasterV2::TracePrimaryRay, Shading.slang:336-370); the frame then
- * leaves that target unwritten. GK_ERR_LIMIT from gk_frame when the scene has 0x7fff
- * instances or more, or a mesh of more than 2^17 triangles (the packing's limits). */
+ * those ids (FVisibilityBufferRayCasterV2::TracePrimaryRay, Shading.slang:336-370); the
+ * frame then leaves that target unwritten. GK_ERR_LIMIT from gk_frame when the scene has
+ * more than 0x7fff proxies. */
 #define GK_FRAME_PRIMARY_FROM_VISBUFFER 32u
 #define GK_FRAME_SKIP_POST 2u          /* stop after the tracer (no reproject/denoise)  */
+/* Run only the post passes (reprojection, denoiser, history copy) on the targets as they
+ * stand: the caller writes SINGLE_DIFFUSE/SPECULAR, ALBEDO, NORMAL, OBJECTID_0/1,
+ * MOTIONVECTOR and MOTIONMOMENT (gk_write_target) as another tracer's rt pass would
+ * (PathTracingRenderer::Render's reproject and denoise dispatches,
+ * PathTracingRenderer.cpp:104-218, without its rt pass). No ray is traced. */
+#define GK_FRAME_POST_ONLY 64u
+/* Rebuild the TLAS from the instances' current boxes when instances moved, instead of
+ * refitting the existing tree (the reference rebuilds every frame,
+ * TopLevelAccelerationStructure.cpp:73-81): a device-only LBVH + 4-wide collapse, no host
+ * round trip. A refit keeps the tree of the last build, whose boxes grow as instances drift
+ * apart. */
+#define GK_FRAME_TLAS_REBUILD 128u
 #define GK_FRAME_ACCUM_FP64 4u         /* also add the frame's radiance to GK_RT_RADIANCE64 */
 #define GK_FRAME_LINEAR_OUTPUT 8u      /* GK_RT_DENOISED holds linear radiance, not the tonemapped image */
 
@@ -273,6 +297,8 @@ typedef struct gk_bvh_stats {
     double sah_cost;           /* BVH2-equivalent SAH cost of mesh 0 (C_TRAV=C_INT=1) */
     double build_ms;           /* last BLAS build, device time               */
     double tlas_ms;            /* last TLAS build/refit, device time         */
+    double tlas_sah_cost;      /* SAH cost of the TLAS as it stands (after refits or
+                                  rebuilds), C_TRAV = C_INT = 1, relative to its root box */
 } gk_bvh_stats;
 
 /* Per-frame statistics (the reference's SCOPED_GPU_TIMER passes). */
@@ -370,6 +396,9 @@ int gk_write_target(gk_ctx* ctx, int target, const void* host, size_t bytes);
 /* Size in bytes and device address of a render target (for zero-copy readers). */
 int gk_target_info(gk_ctx* ctx, int target, size_t* bytes, void** device_ptr);
 
+/* Statistics since the context was created. GK_ERR_LIMIT (with *out filled) when a
+ * traversal stack overflowed since the last call (gk_trace or gk_render): those hits are
+ * invalid. Reported once. */
 int gk_get_stats(gk_ctx* ctx, gk_stats* out);
 
 /* ---------------------------------------------------------------- ray casting */
@@ -384,6 +413,7 @@ int gk_get_stats(gk_ctx* ctx, gk_stats* out);
  * not depend on the traversal order. */
 int gk_trace(gk_ctx* ctx, const gk_ray* d_rays, uint32_t n, gk_hit* d_hits,
              uint32_t* d_inst, uint32_t flags);
+
 
 /* ---------------------------------------------------------------- host helpers
  * No device needed. */

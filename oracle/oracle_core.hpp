@@ -151,6 +151,26 @@ inline void sincos_2pi(T u, T& s, T& c) {
     }
 }
 
+// ------------------------------------------------------------------ RandomInUnitDisk
+// concentric_sample_disk (Const_Func.slang:260-275): offset 2u - 1, zero (|x| < 1e-35 on
+// both axes) maps to zero, the larger coordinate is the radius and the angle is
+// pi/4 * smaller / larger; the x-major branch returns r (cos, sin), the other r (sin,
+// sqrt(1 - sin^2)) as the reference writes it. sin/cos of the angle come from sincos_2pi of
+// |ratio| / 8 (exact), sign restored.
+template <class T>
+inline void concentric_disk(T ua, T ub, T& dx, T& dy) {
+    const T ox = ua + (ua - T(1)), oy = ub + (ub - T(1));
+    const T nz = T(1e-35);
+    if (ox > -nz && ox < nz && oy > -nz && oy < nz) { dx = T(0); dy = T(0); return; }
+    const bool xmaj = std::fabs(ox) > std::fabs(oy);
+    const T r = xmaj ? oy / ox : ox / oy;
+    T s, c;
+    sincos_2pi(std::fabs(r) * T(0.125), s, c);
+    if (r < T(0)) s = -s;
+    if (xmaj) { dx = ox * c; dy = ox * s; }
+    else { dx = oy * s; dy = oy * std::sqrt(T(1) - s * s); }
+}
+
 // ------------------------------------------------------------------ triangle test
 // Moeller-Trumbore exactly as tinybvh IntersectTri (tiny_bvh.h:6815-6844), with the
 // t-window (tmin, tmax) of a RayDesc. Returns true and writes t,u,v on a hit.

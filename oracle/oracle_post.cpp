@@ -121,7 +121,9 @@ void oracle_reproject(const uint32_t* params, uint32_t W, uint32_t H, const floa
             const float mx = motion[2 * i], my = motion[2 * i + 1];
             const float fx = float(x) + mx, fy = float(y) + my;
             const int ppx = int(std::floor(fx)), ppy = int(std::floor(fy));
-            const bool inside = ppx < w && ppy < h && ppx >= -1 && ppy >= -1;
+            // int2 previpos < uint2 isize compares as unsigned (Process.ReProject.comp.slang:
+            // 109-110): -1 is rejected, so the >= -1 clause never decides
+            const bool inside = uint32_t(ppx) < W && uint32_t(ppy) < H;
             const uint32_t cur = obj0[i];
             uint32_t p[4];
             p[0] = O1.at(ppx, ppy);
@@ -185,14 +187,16 @@ float display(float v, float paper_white, bool hdr) {
     return st2084(t * paper_white / 230.0f);
 }
 
-// fparams: {bf_sigma, bf_sigma_lum, paper_white_nit, hdr (0/1)}, bf_size: 0 disables the
-// filter. out: W*H*4 display colour (alpha 1).
+// fparams: {bf_sigma, bf_sigma_lum, paper_white_nit, hdr (0/1), linear (0/1)}, bf_size: 0
+// disables the filter. out: W*H*4 display colour (alpha 1), or with linear the composed
+// radiance before the display transform (the kernel's GK_FRAME_LINEAR_OUTPUT).
 void oracle_jbf(const float* fparams, uint32_t W, uint32_t H, const float* acc_d, const float* acc_s,
                 const float* acc_a, const uint32_t* bf_size, float* out) {
     const int w = int(W), h = int(H);
     const Img4 D{acc_d, w, h}, S{acc_s, w, h};
     const float sigma = fparams[0], sigma_l = fparams[1] * 100.0f, pw = fparams[2];
     const bool hdr = fparams[3] != 0.f;
+    const bool linear = fparams[4] != 0.f;
     const float lc[3] = {0.212671f, 0.715160f, 0.072169f};
     for (int y = 0; y < h; ++y)
         for (int x = 0; x < w; ++x) {
@@ -225,7 +229,7 @@ void oracle_jbf(const float* fparams, uint32_t W, uint32_t H, const float* acc_d
             } else {
                 for (int c = 0; c < 3; ++c) total[c] = acc_d[4 * i + c] * acc_a[4 * i + c] + acc_s[4 * i + c];
             }
-            for (int c = 0; c < 3; ++c) out[4 * i + c] = display(total[c], pw, hdr);
+            for (int c = 0; c < 3; ++c) out[4 * i + c] = linear ? total[c] : display(total[c], pw, hdr);
             out[4 * i + 3] = 1.f;
         }
 }

@@ -1,7 +1,9 @@
 // oracle_render.cpp - CPU restatement of the path-tracing passes (test infrastructure only).
 //
 //   primary rays   FVisibilityBufferRayCasterV2::TracePrimaryRay ray construction
-//                  (Shading.slang:350-356), aperture 0, traced instead of rasterised.
+//                  (Shading.slang:350-356), traced instead of rasterised; with an aperture,
+//                  FHardwarePrimaryRayCaster's thin lens (Shading.slang:283-300), whose
+//                  RandomInUnitDisk is the path's first draw (none at aperture 0).
 //   REF_V2         FPathTracingRendererV2 Init/PrimaryHit/Render/GetRayColor
 //                  (Shading.slang:995-1241) with FHardwareRayTracerV2 hit reconstruction
 //                  (Shading.slang:784-826), CalculateMotionVectorV2 (Shading.slang:50-72),
@@ -161,6 +163,37 @@ void primary_ray_at(const FrameParams& P, float fx, float fy, uint32_t W, uint32
 }
 void primary_ray(const FrameParams& P, uint32_t x, uint32_t y, uint32_t W, uint32_t H, F3& O, F3& D) {
     primary_ray_at(P, float(x), float(y), W, H, O, D);
+}
+// FHardwarePrimaryRayCaster's thin lens (Shading.slang:283-300): offset = Aperture / 2 *
+// RandomInUnitDisk (the path's first draw), origin MVinv (offset, 0, 1), direction
+// MVinv normalize(Pinv (uv, 1, 1) * FocusDistance - (offset, 0)). Aperture 0: the pinhole
+// ray, no draw.
+void primary_ray_lens(const FrameParams& P, uint32_t x, uint32_t y, uint32_t W, uint32_t H, Rng& seed, F3& O,
+                      F3& D) {
+    if (!(P.aperture > 0.f)) {
+        primary_ray(P, x, y, W, H, O, D);
+        return;
+    }
+    float a, b, dx, dy;
+    rnd2(seed, a, b);
+    concentric_disk(a, b, dx, dy);
+    const float half = P.aperture / 2.f;
+    const float lx = half * dx, ly = half * dy;
+    const float ux = (float(x) / float(W)) * 2.0f - 1.0f;
+    const float uy = (float(y) / float(H)) * 2.0f - 1.0f;
+    const float o4[4] = {lx, ly, 0.f, 1.f};
+    float org[4];
+    mat_mul4(P.model_view_inverse, o4, org);
+    const float t4[4] = {ux, uy, 1.f, 1.f};
+    float tgt[4];
+    mat_mul4(P.projection_inverse, t4, tgt);
+    const F3 tl = normalize(F3(tgt[0] * P.focus_distance - lx, tgt[1] * P.focus_distance - ly,
+                               tgt[2] * P.focus_distance - 0.f));
+    const float d4[4] = {tl.x, tl.y, tl.z, 0.f};
+    float dir[4];
+    mat_mul4(P.model_view_inverse, d4, dir);
+    O = F3(org[0], org[1], org[2]);
+    D = normalize(F3(dir[0], dir[1], dir[2]));
 }
 
 // hit reconstruction ----------------------------------------------------------
@@ -713,7 +746,7 @@ struct RefV2 {
         out = RefOut{};
         uint32_t rays = 0;
         F3 O, D;
-        primary_ray(P, x, y, W, H, O, D);
+        primary_ray_lens(P, x, y, W, H, seed, O, D);
         Hit h;
         rays++;
         if (!trace_closest(sc, O, D, 0.0f, 1e30f, h)) {
@@ -932,7 +965,7 @@ struct PtMis {
     // radiance of one path; rgb accumulated into Ld (primary diffuse lobe) and Ls (rest)
     void path(uint32_t x, uint32_t y, uint32_t W, uint32_t H, Rng& seed, V& Ld, V& Ls, uint64_t& rays) const {
         F3 Of, Df;
-        primary_ray(P, x, y, W, H, Of, Df);
+        primary_ray_lens(P, x, y, W, H, seed, Of, Df);
         V O = cvt(Of), D = cvt(Df);
         V thr(T(1), T(1), T(1));
         int lobe0 = -1;          // 0 diffuse, 1 specular/other at the primary vertex
@@ -1494,6 +1527,20 @@ void oracle_trace(void* s, const float* rays, uint32_t n, float* hits, uint32_t*
     });
 }
 
+// The frame's primary rays with the thin lens of P (each pixel's first draw from its
+// pcg4d seed (x, y, total_frames, global_seed)), as gk_ray records (tmin 0, tmax 1e30).
+void oracle_lens_rays(const FrameParams* P, uint32_t W, uint32_t H, float* rays) {
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            Rng seed{{x, y, P->total_frames, P->global_seed}};
+            F3 O, D;
+            primary_ray_lens(*P, x, y, W, H, seed, O, D);
+            float* r = rays + (size_t(y) * W + x) * 8;
+            r[0] = O.x; r[1] = O.y; r[2] = O.z; r[3] = 0.f;
+            r[4] = D.x; r[5] = D.y; r[6] = D.z; r[7] = 1e30f;
+        }
+}
+
 // Primary rays of a W x H frame as gk_ray records (tmin 0, tmax 1e30).
 void oracle_primary_rays(const FrameParams* P, uint32_t W, uint32_t H, float* rays) {
     for (uint32_t y = 0; y < H; ++y)
@@ -1684,7 +1731,12 @@ void oracle_sample(int kind, float param, const float* dir, uint32_t n, uint32_t
         F3 v;
         if (kind == 0) v = random_hemisphere1(s);
         else if (kind == 1) v = random_in_cone(s, param);
-        else {
+        else if (kind == 3) {  // RandomInUnitDisk
+            float a, b, dx, dy;
+            rnd2(s, a, b);
+            concentric_disk(a, b, dx, dy);
+            v = F3(dx, dy, 0.f);
+        } else {
             float a, b;
             rnd2(s, a, b);
             v = ggx_sample_vndf(param, F3(dir[0], dir[1], dir[2]), a, b);
@@ -1712,6 +1764,84 @@ double oracle_ptmis_spec_pdf(double alpha, const double* wo, const double* wi) {
     D3 h = o + i;
     h = normalize(h);
     return PtMis<double>::vndf_pdf(alpha, o, h);
+}
+void oracle_concentric_disk(float ua, float ub, float* out) { concentric_disk(ua, ub, out[0], out[1]); }
+
+// ---------------------------------------------------------------- visibility buffer
+// The raster pass's ids (Rast.VisibilityPass.vert.slang:18-22) for the NodeProxy list of
+// UpdateNodesGpuDriven (Scene.cpp:472-478): one proxy per instance and 65535-triangle
+// section of its mesh (RebuildMeshBuffer, Scene.cpp:202-256), in instance order.
+static std::vector<uint32_t> proxy_bases(const Scene& sc) {
+    std::vector<uint32_t> base(sc.inst.size() + 1, 0u);
+    for (size_t i = 0; i < sc.inst.size(); ++i) {
+        const uint32_t tris = uint32_t(sc.meshes[sc.inst[i].mesh_id].v0.size());
+        base[i + 1] = base[i] + (tris + 65534u) / 65535u;
+    }
+    return base;
+}
+// hits: n x {t, u, v, prim bits}, inst: n instance indices -> packed ids (0 for a miss or a
+// proxy the 15-bit field cannot name)
+void oracle_vis_pack(void* s, const float* hits, const uint32_t* inst, uint32_t n, uint32_t* out) {
+    const Scene& sc = *static_cast<Scene*>(s);
+    const std::vector<uint32_t> base = proxy_bases(sc);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t prim;
+        std::memcpy(&prim, hits + 4 * i + 3, 4);
+        out[i] = 0u;
+        if (prim == 0xffffffffu) continue;
+        const uint32_t proxy = base[inst[i]] + prim / 65535u;
+        if (proxy + 1u <= 0x7fffu) out[i] = ((proxy + 1u) << 17) | (prim % 65535u);
+    }
+}
+// get_material_dataV2 (GeneralFunc.slang:6-56) for the packed id of each ray: the named
+// triangle's corners moved to world space by the instance's worldTS, barycentrics from the
+// world-space ray by its Moeller-Trumbore form without rejection (:31-45), and the
+// interpolated position. out: n x {instance, triangle (uint bits), b.x, b.y, b.z, pos.xyz};
+// instance 0xffffffff where the id names no triangle. Corners in the mesh's own order (the
+// reference's provoking-vertex reorder relabels them; the position is the same point).
+void oracle_vis_decode(void* s, const uint32_t* packed, const float* rays, uint32_t n, float* out) {
+    const Scene& sc = *static_cast<Scene*>(s);
+    const std::vector<uint32_t> base = proxy_bases(sc);
+    const uint32_t nproxy = base.back();
+    for (uint32_t i = 0; i < n; ++i) {
+        float* o = out + 8 * size_t(i);
+        const uint32_t miss = 0xffffffffu;
+        std::memcpy(&o[0], &miss, 4);
+        std::memcpy(&o[1], &miss, 4);
+        for (int k = 2; k < 8; ++k) o[k] = 0.f;
+        const uint32_t pk = packed[i];
+        const uint32_t ip1 = (pk >> 17) & 0x7fffu, stri = pk & 0x1ffffu;
+        if (ip1 == 0u || ip1 - 1u >= nproxy || stri >= 65535u) continue;
+        const uint32_t proxy = ip1 - 1u;
+        const uint32_t ii = uint32_t(std::upper_bound(base.begin(), base.end(), proxy) - base.begin()) - 1u;
+        const Instance& I = sc.inst[ii];
+        const Mesh& m = sc.meshes[I.mesh_id];
+        const uint32_t tri = (proxy - base[ii]) * 65535u + stri;
+        if (tri >= m.v0.size()) continue;
+        const F3 lp[3] = {m.v0[tri], m.v0[tri] + m.e1[tri], m.v0[tri] + m.e2[tri]};
+        F3 p[3];
+        for (int k = 0; k < 3; ++k) {
+            const float v4[4] = {lp[k].x, lp[k].y, lp[k].z, 1.f};
+            float w4[4];
+            mat_mul4(I.world, v4, w4);
+            p[k] = F3(w4[0], w4[1], w4[2]);
+        }
+        const F3 O(rays[8 * size_t(i)], rays[8 * size_t(i) + 1], rays[8 * size_t(i) + 2]);
+        const F3 D(rays[8 * size_t(i) + 4], rays[8 * size_t(i) + 5], rays[8 * size_t(i) + 6]);
+        const F3 e0 = p[1] - p[0], e1 = p[2] - p[0];
+        const F3 rce1 = cross(D, e1);
+        const float rcp = 1.0f / dot(e0, rce1);
+        const F3 r0 = O - p[0];
+        const float by = rcp * dot(r0, rce1);
+        const F3 e0c0 = cross(e0, r0);
+        const float bz = -rcp * dot(D, e0c0);
+        const float bx = 1.0f - (by + bz);
+        const F3 pos = p[0] * bx + (p[1] * by + p[2] * bz);
+        std::memcpy(&o[0], &ii, 4);
+        std::memcpy(&o[1], &tri, 4);
+        o[2] = bx; o[3] = by; o[4] = bz;
+        o[5] = pos.x; o[6] = pos.y; o[7] = pos.z;
+    }
 }
 uint32_t oracle_num_lights(void* s) { return uint32_t(static_cast<Scene*>(s)->lights.size()); }
 

@@ -44,6 +44,10 @@ def oracle():
             "oracle_blas_stats": (None, [_P, _U, C.POINTER(_U), C.POINTER(C.c_float)]),
             "oracle_trace": (None, [_P, _P, _U, _P, _P, C.c_int, _U]),
             "oracle_primary_rays": (None, [_P, _U, _U, _P]),
+            "oracle_lens_rays": (None, [_P, _U, _U, _P]),
+            "oracle_concentric_disk": (None, [C.c_float, C.c_float, _P]),
+            "oracle_vis_pack": (None, [_P, _P, _P, _U, _P]),
+            "oracle_vis_decode": (None, [_P, _P, _P, _U, _P]),
             "oracle_render_refv2": (C.c_uint64, [_P, _P, _U, _U, _U, _U, _U, _U, _P, _P, _P, _P, _P, _P, _P, _P, _U]),
             "oracle_render_ptmis": (C.c_uint64, [_P, _P, _U, _U, _U, _U, _U, _U, _U, _U, C.c_int, _P, _P, _P, _U]),
             "oracle_render_ptmis_mode": (C.c_uint64, [_P, _P, _U, _U, _U, _U, _U, _U, _U, _U, C.c_int, _P, _U]),
@@ -169,6 +173,24 @@ class OracleScene:
         oracle().oracle_tex_sample(self.h, tex_id, ptr(uv), uv.shape[0], ptr(out))
         return out
 
+    def vis_pack(self, hits, inst):
+        """The raster visibility pass's ids of traced hits ((n, 4) {t, u, v, prim bits}, (n,)
+        instances): (proxy + 1) << 17 | triangle within the 65535-triangle section."""
+        h = np.ascontiguousarray(hits, np.float32)
+        i = np.ascontiguousarray(inst, np.uint32)
+        out = np.zeros(h.shape[0], np.uint32)
+        oracle().oracle_vis_pack(self.h, ptr(h), ptr(i), h.shape[0], ptr(out))
+        return out
+
+    def vis_decode(self, packed, rays):
+        """get_material_dataV2 for packed ids and their rays: (instance, triangle, world-space
+        barycentrics (3,), interpolated world position (3,)); instance 0xffffffff for none."""
+        pk = np.ascontiguousarray(packed, np.uint32).ravel()
+        r = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros((pk.size, 8), np.float32)
+        oracle().oracle_vis_decode(self.h, ptr(pk), ptr(r), pk.size, ptr(out))
+        return out[:, 0].view(np.uint32), out[:, 1].view(np.uint32), out[:, 2:5], out[:, 5:8]
+
     def num_lights(self):
         return oracle().oracle_num_lights(self.h)
 
@@ -279,6 +301,20 @@ def primary_rays(params, W, H):
     rays = np.zeros((H * W, 8), np.float32)
     oracle().oracle_primary_rays(C.addressof(params), W, H, ptr(rays))
     return rays
+
+
+def lens_rays(params, W, H):
+    """Primary rays with the frame's thin lens (aperture 0: the pinhole rays)."""
+    rays = np.zeros((W * H, 8), np.float32)
+    oracle().oracle_lens_rays(C.addressof(params), W, H, ptr(rays))
+    return rays
+
+
+def concentric_disk(ua, ub):
+    """RandomInUnitDisk's map of one square sample (ua, ub) in [0, 1)^2."""
+    out = np.zeros(2, np.float32)
+    oracle().oracle_concentric_disk(float(ua), float(ub), ptr(out))
+    return out
 
 
 def pcg4d(seed, n):
@@ -397,9 +433,10 @@ def reproject(src_d, src_s, src_a, normal, prev_d, prev_s, prev_a, obj0, obj1, m
     return acc[0], acc[1], acc[2]
 
 
-def jbf(acc_d, acc_s, acc_a, sigma=2.0, sigma_lum=3.0, paper_white=600.0, bf_size=5, hdr=False):
+def jbf(acc_d, acc_s, acc_a, sigma=2.0, sigma_lum=3.0, paper_white=600.0, bf_size=5, hdr=False, linear=False):
+    """Process.DenoiseJBF restatement: display colour, or with linear the composed radiance."""
     H, W = acc_d.shape[:2]
-    fp = np.array([sigma, sigma_lum, paper_white, 1.0 if hdr else 0.0], np.float32)
+    fp = np.array([sigma, sigma_lum, paper_white, 1.0 if hdr else 0.0, 1.0 if linear else 0.0], np.float32)
     bs = np.array([bf_size], np.uint32)
     out = np.zeros((H, W, 4), np.float32)
     d, s, a = _f4(acc_d), _f4(acc_s), _f4(acc_a)

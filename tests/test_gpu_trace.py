@@ -195,3 +195,51 @@ def test_empty_and_edge_inputs(device):
     o, _ = po.OracleScene(s).trace(rays)
     assert np.array_equal(_u32(h), _u32(o))
     assert _u32(h[5, 3]) == 0xFFFFFFFF
+
+
+def test_tlas_rebuild_and_refit_alternate_bit_exact(device):
+    """GK_FRAME_TLAS_REBUILD builds a new TLAS on the device (LBVH + single-work-group 4-wide
+    collapse that also writes the refit order); primary hits equal the oracle's (whose TLAS
+    is rebuilt) bit for bit, whether frames rebuild or refit, in any order. After the bricks
+    drift apart for two seconds a refitted tree's SAH cost is well above a rebuilt one's."""
+    b = scenes.Bricks()
+    osc = po.OracleScene(b.scene)
+    W, H = 160, 96
+    c = scenes.BRICKS_CAM
+    sah = {}
+    with gk.Context(W, H) as ctx:
+        ctx.upload(b.scene)
+        sah["built"] = ctx.bvh_stats().tlas_sah_cost
+        for k, (t, rebuild) in enumerate([(0.25, True), (0.5, False), (1.0, True), (1.5, True), (2.0, False),
+                                          (2.0, True)]):
+            w = b.worlds(t)
+            ctx.update_instances(w)
+            osc.update_instances(w)
+            p = gk.make_frame_params(c["eye"], c["target"], W, H, fov=c["fov"], frame=k)
+            p.estimator = gk.ESTIMATOR_PT_MIS
+            p.bounces = p.max_bounces = 1
+            p.flags |= gk.FRAME_SKIP_POST | (gk.FRAME_TLAS_REBUILD if rebuild else 0)
+            ctx.frame(p)
+            ctx.render()
+            ctx.sync()
+            hit = ctx.read("PRIMARY_HIT").reshape(-1, 4)
+            inst = ctx.read("PRIMARY_INST").reshape(-1)
+            oh, oi = osc.trace(po.primary_rays(p, W, H))
+            assert np.array_equal(_u32(hit), _u32(oh)), (t, rebuild)
+            ok = _u32(oh[:, 3]) != 0xFFFFFFFF
+            assert np.array_equal(inst[ok], oi[ok])
+            sah[(t, rebuild)] = ctx.bvh_stats().tlas_sah_cost
+        # random rays through gk_trace after a rebuild use the same tree
+        rays = scenes.random_rays(20000, -14.0, 14.0, seed=13, tmin=1e-3, tmax=1000.0)
+        h, i = gpu_trace(ctx, rays, device)
+    o, oi = osc.trace(rays)
+    assert np.array_equal(_u32(h), _u32(o))
+    # the tree of t = 0 refitted to t = 2 s against the tree rebuilt there
+    with gk.Context(W, H) as ctx:
+        ctx.upload(b.scene)
+        ctx.update_instances(b.worlds(2.0))
+        h, _ = gpu_trace(ctx, rays, device)  # refits
+        sah["refit from t=0"] = ctx.bvh_stats().tlas_sah_cost
+    assert np.array_equal(_u32(h), _u32(o))
+    print({str(k): round(v, 3) for k, v in sah.items()})
+    assert sah["refit from t=0"] > 1.3 * sah[(2.0, True)]
