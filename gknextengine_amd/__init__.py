@@ -29,6 +29,7 @@ FRAME_PRIMARY_FROM_VISBUFFER = 32
 FRAME_POST_ONLY = 64
 FRAME_TLAS_REBUILD = 128
 TRACE_CLOSEST, TRACE_ANY = 0, 1
+TRACE_STATIC_MAX_RAYS = 262144
 
 TARGETS = [
     "SINGLE_DIFFUSE", "SINGLE_SPECULAR", "ALBEDO", "NORMAL", "OBJECTID_0", "OBJECTID_1",
@@ -168,6 +169,7 @@ SIGNATURES = [
     ("gk_target_info", C.c_int, [_P, C.c_int, C.POINTER(C.c_size_t), C.POINTER(_P)]),
     ("gk_get_stats", C.c_int, [_P, C.POINTER(Stats)]),
     ("gk_trace", C.c_int, [_P, _P, _U, _P, _P, _U]),
+    ("gk_trace_static", C.c_int, [_P, _P, _U, _P, _P, _U]),
     ("gk_make_frame_params", C.c_int, [C.POINTER(CameraDesc), _U, _U, _U, _U, C.POINTER(FrameParams),
                                        C.POINTER(FrameParams)]),
     ("gk_procgen_cornell", C.c_int, [_P, _P, _P, _P, C.POINTER(_U), C.POINTER(_U)]),
@@ -622,6 +624,12 @@ class Context:
         """Device-pointer ray cast (gk_ray in, gk_hit out); enqueue only."""
         check(lib().gk_trace(self.h, d_rays, n, d_hits, d_inst, TRACE_ANY if any_hit else TRACE_CLOSEST),
               "gk_trace")
+
+    def trace_static(self, d_rays: int, n: int, d_hits: int, d_inst: int | None = None, any_hit: bool = False):
+        """gk_trace without device allocation (the library's static scratch array, at most
+        TRACE_STATIC_MAX_RAYS rays); enqueue only."""
+        check(lib().gk_trace_static(self.h, d_rays, n, d_hits, d_inst, TRACE_ANY if any_hit else TRACE_CLOSEST),
+              "gk_trace_static")
 
 
 class UNet:

@@ -290,6 +290,8 @@ int gk_create(const gk_device_desc* desc, gk_ctx** out) {
         return GK_ERR_HIP;
     }
     for (auto& e : c.ev) (void)hipEventCreate(&e);
+    // gk_trace_static's array on this GPU; a failure is reported by gk_trace_static itself
+    (void)static_scratch_init(c.device);
     const int s = alloc_targets(c);
     if (s != GK_OK) { gk_destroy(ctx); return s; }
     *out = ctx;
@@ -569,9 +571,12 @@ int gk_get_stats(gk_ctx* ctx, gk_stats* out) {
     }
     c.stats.extension_rays = c.stats.rays_traced - c.stats.shadow_rays;
     *out = c.stats;
-    // a traversal stack overflowed since the last report (gk_trace or gk_render): those
-    // results are invalid; reported once
-    const bool ovf = c.stack_overflow;
+    bool static_ovf = false;
+    const int so = static_scratch_overflow(c, static_ovf);
+    if (so != GK_OK) return so;
+    // a traversal stack overflowed since the last report (gk_trace, gk_render, or the last
+    // gk_trace_static on this GPU): those results are invalid; reported once
+    const bool ovf = c.stack_overflow || static_ovf;
     c.stack_overflow = false;
     return ovf ? GK_ERR_LIMIT : GK_OK;
 }

@@ -263,6 +263,10 @@ int launch_trace_user(Ctx& c, const gk_ray* rays, uint32_t n, gk_hit* hits, uint
 uint32_t* trace_counter(Ctx& c);
 int ensure_spill(Ctx& c, uint32_t threads);
 int collect_timing(Ctx& c);
+// gk_trace_static's device array: resolved per GPU by gk_create; the overflow flag of the
+// last call on c's GPU, read and cleared
+int static_scratch_init(int device);
+int static_scratch_overflow(Ctx& c, bool& overflow);
 
 }  // namespace gk
 

@@ -32,6 +32,8 @@
 //     tests are widened by 2^-20 relative so they never cull an accepted triangle.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "gk_ctx.hpp"
 
 namespace gk {
@@ -776,4 +778,104 @@ int launch_trace_user(Ctx& c, const gk_ray* rays, uint32_t n, gk_hit* hits, uint
                         (flags & GK_TRACE_ANY) != 0, nullptr);
 }
 
+// ---------------------------------------------------------------- allocation-free ray cast
+// gk_trace_static: the same launch as gk_trace with everything beyond the caller's rays
+// and hits in `scratch`, a static device array that each GPU holds from the library's load
+// to the process's end. Layout (32-bit words): a 64-word head (next ray, overflow flag,
+// two 64-bit ray counts), ray staging (origin/tmin then direction/tmax, float4 each, for
+// kStaticMaxRays rays), then the spill stacks of a kStaticBlocks-work-group launch
+// ([entry][thread], kSpill entries per thread). 24 MiB + 256 bytes.
+// Host code never passes `scratch` itself to a kernel (on the host it names a shadow
+// variable, not device memory): gk_create resolves its device address once per GPU with
+// hipGetSymbolAddress, with that GPU current, and every use below goes through it.
+constexpr uint32_t kStaticMaxRays = GK_TRACE_STATIC_MAX_RAYS;
+constexpr uint32_t kStaticBlocks = 256;  // work-groups of a launch (the spill area is sized for it)
+constexpr size_t kStaticHead = 64;
+constexpr size_t kStaticWords = kStaticHead + size_t(kStaticMaxRays) * 8 + size_t(kStaticBlocks) * kBlock * kSpill;
+static __device__ __align__(256) uint32_t scratch[kStaticWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch[kMaxDevices] = {};
+
+__global__ void k_clear_words(uint32_t* __restrict__ p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+}  // namespace
+
+// Resolves this GPU's copy of `scratch` (the caller has made `device` current).
+int static_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return GK_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return GK_OK;
+    void* p = nullptr;
+    GK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[device] = static_cast<uint32_t*>(p);
+    return GK_OK;
+}
+
+// Reads and clears the overflow flag of the last gk_trace_static call on c's GPU.
+int static_scratch_overflow(Ctx& c, bool& overflow) {
+    overflow = false;
+    uint32_t* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (c.device >= 0 && c.device < kMaxDevices) base = g_scratch[c.device];
+    }
+    if (!base) return GK_OK;
+    uint32_t v = 0;
+    GK_HIP(hipMemcpy(&v, base + 1, 4, hipMemcpyDeviceToHost));
+    if (v) {
+        const uint32_t zero = 0;
+        GK_HIP(hipMemcpy(base + 1, &zero, 4, hipMemcpyHostToDevice));
+        overflow = true;
+    }
+    return GK_OK;
+}
+
 }  // namespace gk
+
+extern "C" int gk_trace_static(gk_ctx* ctx, const gk_ray* d_rays, uint32_t n, gk_hit* d_hits, uint32_t* d_inst,
+                               uint32_t flags) {
+    using namespace gk;
+    GK_CHECK_CTX(ctx);
+    Ctx& c = ctx->c;
+    if (!c.scene_ready) return GK_ERR_STATE;
+    if ((!d_rays || !d_hits) && n) return GK_ERR_ARG;
+    if (flags > GK_TRACE_ANY) return GK_ERR_ARG;
+    if (n > kStaticMaxRays) return GK_ERR_LIMIT;
+    uint32_t* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (c.device >= 0 && c.device < kMaxDevices) base = g_scratch[c.device];
+    }
+    if (!base) return GK_ERR_HIP;  // gk_create could not resolve this GPU's array
+    GK_HIP(hipSetDevice(c.device));
+    if (c.tlas_dirty) {
+        const int s = refit_tlas(c);  // in place, no allocation
+        if (s != GK_OK) return s;
+    }
+    if (n == 0) return GK_OK;
+    uint32_t* head = base;                                   // next ray, overflow, ray counts
+    float4* so = reinterpret_cast<float4*>(base + kStaticHead);
+    float4* sd = so + kStaticMaxRays;
+    uint32_t* spill = reinterpret_cast<uint32_t*>(sd + kStaticMaxRays);
+    // the previous call's counters are still there: start from zero
+    k_clear_words<<<1, uint32_t(kStaticHead), 0, c.stream>>>(head, uint32_t(kStaticHead));
+    k_unpack_rays<<<(n + 255) / 256, 256, 0, c.stream>>>(d_rays, n, so, sd);
+    GK_HIP(hipGetLastError());
+    const bool any = (flags & GK_TRACE_ANY) != 0;
+    const bool two = !c.blas_only;
+    const uint32_t root = c.h_meshes.empty() ? 0u : c.h_meshes[0].root;
+    auto* kern = any ? (two ? k_trace<1, true> : k_trace<1, false>) : (two ? k_trace<0, true> : k_trace<0, false>);
+    uint32_t blocks = persistent_blocks(c.device, kern);
+    if (blocks > kStaticBlocks) blocks = kStaticBlocks;
+    const TraceJob j0{so, sd, nullptr, n, reinterpret_cast<float4*>(d_hits), any ? nullptr : d_inst};
+    kern<<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, c.tlas_nodes.ptr, c.tlas_prims.ptr, c.inst.ptr,
+                                         root, j0, TraceJob{}, head, spill, head + 1,
+                                         reinterpret_cast<unsigned long long*>(head + 2));
+    GK_HIP(hipGetLastError());
+    return GK_OK;
+}

@@ -397,8 +397,8 @@ int gk_write_target(gk_ctx* ctx, int target, const void* host, size_t bytes);
 int gk_target_info(gk_ctx* ctx, int target, size_t* bytes, void** device_ptr);
 
 /* Statistics since the context was created. GK_ERR_LIMIT (with *out filled) when a
- * traversal stack overflowed since the last call (gk_trace or gk_render): those hits are
- * invalid. Reported once. */
+ * traversal stack overflowed since the last call (gk_trace, gk_render, or the last
+ * gk_trace_static on the context's GPU): those hits are invalid. Reported once. */
 int gk_get_stats(gk_ctx* ctx, gk_stats* out);
 
 /* ---------------------------------------------------------------- ray casting */
@@ -414,6 +414,21 @@ int gk_get_stats(gk_ctx* ctx, gk_stats* out);
 int gk_trace(gk_ctx* ctx, const gk_ray* d_rays, uint32_t n, gk_hit* d_hits,
              uint32_t* d_inst, uint32_t flags);
 
+/* gk_trace without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory during the call (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same arguments and
+ * results as gk_trace (bit-identical hits); the ray staging, traversal spill stacks, ray
+ * counter and overflow flag live in one static device array of the library, about 24 MiB,
+ * that every GPU the library runs on holds until the process ends. At most
+ * GK_TRACE_STATIC_MAX_RAYS rays per call (GK_ERR_LIMIT above, nothing enqueued). All calls
+ * on one GPU share that array, from any context: calls whose execution overlaps on one GPU
+ * (two streams, two contexts, two threads) race and must be serialised by the caller.
+ * Enqueued on the context stream; a traversal stack overflow of the last such call on the
+ * context's GPU is reported by gk_get_stats (GK_ERR_LIMIT). The scene must be uploaded; a
+ * pending instance update is refitted first, as gk_trace does. */
+#define GK_TRACE_STATIC_MAX_RAYS 262144u
+int gk_trace_static(gk_ctx* ctx, const gk_ray* d_rays, uint32_t n, gk_hit* d_hits,
+                    uint32_t* d_inst, uint32_t flags);
 
 /* ---------------------------------------------------------------- host helpers
  * No device needed. */
