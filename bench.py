@@ -343,6 +343,8 @@ def run_dynamic(args):
     times = []
     stats0 = None
     p = None
+    rebuild = args.tlas == "rebuild"
+    trace_ms, sah = [], []
     for f in range(nframes + 1):
         p = gk.make_frame_params(cam["eye"], cam["target"], W, H, fov=cam["fov"], frame=f, prev=prev)
         p.estimator = gk.ESTIMATOR_PT_MIS
@@ -352,6 +354,8 @@ def run_dynamic(args):
         p.sky_color[:] = [0.6, 0.7, 0.9, 1.0]
         p.temporal_frames = 16
         p.denoiser = gk.DENOISER_BILATERAL
+        if rebuild:
+            p.flags |= gk.FRAME_TLAS_REBUILD
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         if f > 0:
@@ -362,10 +366,21 @@ def run_dynamic(args):
         times.append(time.perf_counter() - t0)
         if f == 8:
             stats0 = ctx.stats()
+        if args.per_frame:  # outside the frame's timed interval
+            st = ctx.stats()
+            trace_ms.append(st.ms_trace)
+            sah.append(ctx.bvh_stats().tlas_sah_cost)
         prev = p
     s1 = ctx.stats()
     rays = s1.rays_traced - stats0.rays_traced
     ms = np.array(times[9:]) * 1e3
+    if args.per_frame:
+        tr = np.diff(np.array(trace_ms))
+        extra_pf = {"per_frame_trace_ms": [round(float(x), 3) for x in tr],
+                    "per_frame_tlas_sah": [round(float(x), 3) for x in sah],
+                    "per_frame_ms": [round(float(x) * 1e3, 3) for x in times]}
+    else:
+        extra_pf = {}
     cpu = None if args.no_cpu_baseline else dynamic_cpu_baseline(ctx, b, p, W, H, worlds)
     extra = {}
     if args.relmse:
@@ -379,11 +394,14 @@ def run_dynamic(args):
     return {"metric": METRIC, "value": round(rays / (ms.sum() * 1e-3) / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
             "steps": len(ms), "warmup": 9, "ms_per_step": round(float(np.median(ms)), 3), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "dynamic bricks (config 3): 4096 instances x 24 tris, TLAS refit per frame, "
+            "config": {"workload": f"dynamic bricks (config 3): 4096 instances x 24 tris, TLAS "
+                                   f"{'rebuilt' if rebuild else 'refitted'} per frame, "
                                    "PT_MIS 1 spp 4 bounces, reproject + bilateral", "resolution": [W, H],
+                       "tlas_update": args.tlas,
                        "ms_frame_mean": round(float(ms.mean()), 3), "ms_frame_p95": round(float(np.percentile(ms, 95)), 3),
+                       "ms_frame_max": round(float(ms.max()), 3),
                        "tlas_ms": round(s1.ms_tlas, 3), "reproject_ms": round(s1.ms_reproject, 3),
-                       "denoise_ms": round(s1.ms_denoise, 3), **extra},
+                       "denoise_ms": round(s1.ms_denoise, 3), **extra, **extra_pf},
             "cpu_baseline": cpu}
 
 
@@ -909,6 +927,9 @@ def main():
     ap.add_argument("--mixed-materials", action="store_true",
                     help="city materials spread over Lambertian/Metallic/Dielectric/Mixture")
     ap.add_argument("--denoiser", default="bilateral", choices=["none", "bilateral"])
+    ap.add_argument("--tlas", default="rebuild", choices=["rebuild", "refit"],
+                    help="dynamic config: per-frame TLAS update")
+    ap.add_argument("--per-frame", action="store_true", help="dynamic config: per-frame trace ms and TLAS SAH")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (DESIGN.md describes how it would "
