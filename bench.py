@@ -333,7 +333,7 @@ def run_dynamic(args):
     from gknextengine_amd import scenes
     W, H = args.width, args.height
     b = scenes.Bricks()
-    nframes = 120
+    nframes = args.frames
     worlds = np.stack([b.worlds(f / 60.0) for f in range(nframes + 1)])           # (F, n, 4, 4)
     d_worlds = torch.from_numpy(np.ascontiguousarray(np.transpose(worlds, (0, 1, 3, 2)))).cuda()  # column-major
     ctx = gk.Context(W, H)
@@ -930,6 +930,7 @@ def main():
     ap.add_argument("--tlas", default="rebuild", choices=["rebuild", "refit"],
                     help="dynamic config: per-frame TLAS update")
     ap.add_argument("--per-frame", action="store_true", help="dynamic config: per-frame trace ms and TLAS SAH")
+    ap.add_argument("--frames", type=int, default=120, help="dynamic config: frames after the first (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (DESIGN.md describes how it would "
