@@ -28,6 +28,7 @@ FRAME_NO_MATERIAL_SORT, FRAME_SKIP_POST, FRAME_ACCUM_FP64, FRAME_LINEAR_OUTPUT, 
 FRAME_PRIMARY_FROM_VISBUFFER = 32
 FRAME_POST_ONLY = 64
 FRAME_TLAS_REBUILD = 128
+FRAME_FAST_GATHER = 256
 TRACE_CLOSEST, TRACE_ANY = 0, 1
 TRACE_STATIC_MAX_RAYS = 262144
 

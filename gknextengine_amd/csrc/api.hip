@@ -50,7 +50,11 @@ size_t target_bpp(int t) {
 void free_targets(Ctx& c) {
     for (void* p : c.target_allocs) (void)hipFree(p);
     c.target_allocs.clear();
-    for (int t = 0; t < GK_RT_COUNT; ++t) { c.target_ptr[t] = nullptr; c.target_bytes[t] = 0; }
+    for (int t = 0; t < GK_RT_COUNT; ++t) {
+        if (c.target_ptr[t]) device_bytes() -= c.target_bytes[t];
+        c.target_ptr[t] = nullptr;
+        c.target_bytes[t] = 0;
+    }
 }
 
 __global__ void k_fill_u16(uint16_t* p, size_t n, uint16_t v) {
@@ -64,7 +68,9 @@ int alloc_targets(Ctx& c) {
     for (int t = 0; t < GK_RT_COUNT; ++t) {
         const size_t b = px * target_bpp(t);
         void* p = nullptr;
+        if (device_bytes() + b > device_budget()) return GK_ERR_OOM;
         GK_HIP(hipMalloc(&p, b));
+        device_bytes() += b;
         c.target_allocs.push_back(p);
         c.target_ptr[t] = p;
         c.target_bytes[t] = b;
@@ -75,6 +81,27 @@ int alloc_targets(Ctx& c) {
     GK_HIP(hipGetLastError());
     GK_HIP(hipStreamSynchronize(c.stream));
     return GK_OK;
+}
+
+// After a failed gk_scene_upload: the scene's device buffers and the build scratch are
+// released, so that a failed (for instance out-of-memory) upload leaves no memory behind.
+void release_scene(Ctx& c) {
+    c.scene_ready = false;
+    c.materials.release(); c.tris.release(); c.tri_nrm.release(); c.tri_mat.release(); c.tri_rec.release();
+    c.tri_uv.release(); c.tri_tan.release(); c.proxy_inst.release(); c.blas_nodes.release(); c.meshes.release();
+    c.inst.release(); c.inst_world_in.release(); c.inst_lo.release(); c.inst_hi.release();
+    c.tlas_nodes.release(); c.tlas_prims.release(); c.tlas_refit_order.release(); c.tlas_level_info.release();
+    c.lights.release(); c.light_cdf.release(); c.light_obj.release(); c.light_power.release();
+    BuildScratch& b = c.scratch;
+    b.keys.release(); b.keys_alt.release(); b.vals.release(); b.vals_alt.release(); b.sort_tmp.release();
+    b.prim_lo.release(); b.prim_hi.release(); b.bparent.release(); b.bchild.release(); b.brange.release();
+    b.bnode_lo.release(); b.bnode_hi.release(); b.bflag.release(); b.queue_a.release(); b.queue_b.release();
+    b.counters.release(); b.bounds.release(); b.node_depth.release();
+    c.h_instances.clear();
+    c.h_meshes.clear();
+    c.num_lights = 0;
+    c.num_proxies = 0;
+    c.tlas_node_count = 0;
 }
 
 bool is_identity(const float* m) {
@@ -356,14 +383,17 @@ int gk_scene_upload(gk_ctx* ctx, const gk_mesh* meshes, uint32_t num_meshes, con
     GK_HIP(c.materials.resize(num_materials));
     GK_HIP(hipMemcpyAsync(c.materials.ptr, materials, sizeof(gk_material) * num_materials, hipMemcpyHostToDevice, c.stream));
     int s = build_blas(c, meshes, num_meshes);
-    if (s != GK_OK) return s;
-    c.h_instances.assign(instances, instances + num_instances);
-    s = build_tlas(c);
-    if (s != GK_OK) return s;
-    s = upload_lights(c, meshes);
-    if (s != GK_OK) return s;
-    s = update_lights(c);
-    if (s != GK_OK) return s;
+    if (s == GK_OK) {
+        c.h_instances.assign(instances, instances + num_instances);
+        s = build_tlas(c);
+    }
+    if (s == GK_OK) s = upload_lights(c, meshes);
+    if (s == GK_OK) s = update_lights(c);
+    if (s != GK_OK) {
+        (void)hipStreamSynchronize(c.stream);
+        release_scene(c);
+        return s;
+    }
     c.blas_only = num_instances == 1 && is_identity(instances[0].world) && instances[0].visible && !instances[0].nort &&
                   instances[0].mesh_id == 0;
     c.frames_rendered = 0;

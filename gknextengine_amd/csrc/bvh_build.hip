@@ -63,9 +63,14 @@ __global__ void k_prep_tris(const float* __restrict__ pos, const uint32_t* __res
     if (t >= nt) return;
     uint32_t a = idx[3 * t], b = idx[3 * t + 1], c = idx[3 * t + 2];
     if (a >= nv || b >= nv || c >= nv) { atomicOr(err, 1u); a = b = c = 0; }
-    const f3 pa = f3{pos[3 * a], pos[3 * a + 1], pos[3 * a + 2]};
-    const f3 pb = f3{pos[3 * b], pos[3 * b + 1], pos[3 * b + 2]};
-    const f3 pc = f3{pos[3 * c], pos[3 * c + 1], pos[3 * c + 2]};
+    f3 pa = f3{pos[3 * a], pos[3 * a + 1], pos[3 * a + 2]};
+    f3 pb = f3{pos[3 * b], pos[3 * b + 1], pos[3 * b + 2]};
+    f3 pc = f3{pos[3 * c], pos[3 * c + 1], pos[3 * c + 2]};
+    if (!(isfinite(pa.x) && isfinite(pa.y) && isfinite(pa.z) && isfinite(pb.x) && isfinite(pb.y) && isfinite(pb.z) &&
+          isfinite(pc.x) && isfinite(pc.y) && isfinite(pc.z))) {  // a NaN/Inf vertex: the upload fails
+        atomicOr(err, 2u);
+        pa = pb = pc = f3{0.f, 0.f, 0.f};
+    }
     const f3 e1 = pb - pa, e2 = pc - pa;
     const uint32_t m = tmat ? tmat[t] : 0u;
     tmp[t].v0 = make_float4(pa.x, pa.y, pa.z, __uint_as_float(t));

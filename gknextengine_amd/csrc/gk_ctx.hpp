@@ -5,6 +5,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <cstdint>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -12,6 +15,24 @@
 #include "gk_math.hpp"
 
 namespace gk {
+
+// ------------------------------------------------------------------ device memory budget
+// Device bytes held by the library's buffers (all contexts of the process), and an optional
+// budget: GKNX_DEVICE_MEMORY_LIMIT=<bytes> in the environment when the library first
+// allocates. An allocation that would exceed it fails as hipMalloc does when the device is
+// full (hipErrorOutOfMemory -> GK_ERR_OOM), so callers can bound the library's footprint and
+// exercise their out-of-memory handling.
+inline std::atomic<uint64_t>& device_bytes() {
+    static std::atomic<uint64_t> n{0};
+    return n;
+}
+inline uint64_t device_budget() {
+    static const uint64_t limit = [] {
+        const char* v = std::getenv("GKNX_DEVICE_MEMORY_LIMIT");
+        return v && *v ? uint64_t(std::strtoull(v, nullptr, 10)) : UINT64_MAX;
+    }();
+    return limit;
+}
 
 // ------------------------------------------------------------------ device buffer
 template <class T>
@@ -24,7 +45,10 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) {
+            (void)hipFree(ptr);
+            device_bytes() -= uint64_t(cap) * sizeof(T);
+        }
         ptr = nullptr;
         count = cap = 0;
     }
@@ -33,8 +57,11 @@ struct DevBuf {
         if (n <= cap) { count = n; return hipSuccess; }
         release();
         if (n == 0) return hipSuccess;
+        const uint64_t bytes = uint64_t(n) * sizeof(T);
+        if (device_bytes() + bytes > device_budget()) return hipErrorOutOfMemory;
         hipError_t e = hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T));
         if (e != hipSuccess) { ptr = nullptr; return e; }
+        device_bytes() += bytes;
         count = cap = n;
         return hipSuccess;
     }

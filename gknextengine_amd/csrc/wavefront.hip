@@ -53,6 +53,7 @@ struct FrameK {
     float aperture, focus;  // thin lens (aperture 0: pinhole, no random numbers drawn)
     uint32_t W, H, frame, seed, samples, bounces, max_bounces, temporal, has_sky, has_sun, tiled;
     uint32_t vis_primary;  // primary hits come from GK_RT_MINIGBUFFER (left unwritten)
+    uint32_t fast_gather;  // REF_V2 ExitAfterFirst
     ProbeDev probes;       // REF_V2 path termination
 };
 
@@ -632,7 +633,16 @@ __global__ __launch_bounds__(256) void k_ref_shade(DScene s, FrameK K, Targets T
         const uint32_t maxb = prim_diel ? K.max_bounces : K.bounces;
         bool end_sample = false;
         if (b == 0u) {
-            if (ret || !(1u < maxb)) end_sample = true;
+            if (ret || !(1u < maxb)) {
+                end_sample = true;
+            } else if (K.fast_gather && !prim_diel) {
+                // ExitAfterFirst (Shading.slang:1118-1124): the probe lookup at the first
+                // bounce's vertex ends the sample
+                const float4 pr = probe_lookup<true>(K.probes, pos, nrm);
+                color.x = color.x * pr.x; color.y = color.y * pr.y;
+                color.z = color.z * pr.z; color.w = color.w * pr.w;
+                end_sample = true;
+            }
         } else {
             if (ret) {
                 end_sample = true;
@@ -1121,6 +1131,7 @@ static FrameK make_k(const Ctx& c) {
     K.has_sun = P.has_sun;
     K.tiled = (c.width % 8 == 0 && c.height % 8 == 0) ? 1u : 0u;
     K.vis_primary = (P.flags & GK_FRAME_PRIMARY_FROM_VISBUFFER) ? 1u : 0u;
+    K.fast_gather = (P.flags & GK_FRAME_FAST_GATHER) ? 1u : 0u;
     K.probes = probe_dev(c);
     return K;
 }

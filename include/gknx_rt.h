@@ -252,6 +252,11 @@ typedef struct gk_frame_params {
  * round trip. A refit keeps the tree of the last build, whose boxes grow as instances drift
  * apart. */
 #define GK_FRAME_TLAS_REBUILD 128u
+/* REF_V2: FastGather (UserSettings::FastGather -> UniformBufferObject::FastGather, the
+ * renderer's ExitAfterFirst, Core.PathTracing.comp.slang:17): a sample whose first bounce
+ * neither ends on an emitter/sky nor starts at a dielectric primary ends there, multiplied
+ * by the probe lookup at that bounce's vertex (Shading.slang:1118-1124). */
+#define GK_FRAME_FAST_GATHER 256u
 #define GK_FRAME_ACCUM_FP64 4u         /* also add the frame's radiance to GK_RT_RADIANCE64 */
 #define GK_FRAME_LINEAR_OUTPUT 8u      /* GK_RT_DENOISED holds linear radiance, not the tonemapped image */
 
@@ -334,7 +339,14 @@ int gk_resize(gk_ctx* ctx, uint32_t width, uint32_t height);
 /* Replaces Scene::Reload + RebuildMeshBuffer + OnPostLoadScene's
  * CreateAccelerationStructures (Scene.cpp:92-289; RayTraceBaseRenderer.cpp:106-124,
  * 276-386): uploads geometry, builds one BLAS per mesh on the device and the TLAS.
- * GK_ERR_LIMIT when the meshes hold 2^27 triangles or more in total. */
+ * GK_ERR_LIMIT when the meshes hold 2^27 triangles or more in total; GK_ERR_ARG for no
+ * instance, a mesh id, vertex index or transform that is out of range or not finite, or a
+ * non-finite vertex position. An error found before the device work leaves the previous
+ * scene in place; one found during it (a bad vertex, GK_ERR_OOM) leaves no scene and frees
+ * the scene's device memory (gk_render: GK_ERR_STATE until an upload succeeds).
+ * Material ids past the table are clamped to the last material, as GetMaterial does
+ * (BasicTypes.slang:376-379). GKNX_DEVICE_MEMORY_LIMIT=<bytes> in the environment bounds the
+ * device memory of the library's buffers in the process (GK_ERR_OOM beyond it). */
 int gk_scene_upload(gk_ctx* ctx,
                     const gk_mesh* meshes, uint32_t num_meshes,
                     const gk_material* materials, uint32_t num_materials,

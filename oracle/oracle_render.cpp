@@ -816,7 +816,10 @@ struct RefV2 {
             const uint32_t maxb = mat.material_model == MAT_DIELECTRIC ? P.max_bounces : P.bounces;
             bool hit_reflect = false, hit_metal = false;
             const bool exit_first = get_ray_color(vs, dir, color, seed, hit_reflect, hit_metal, rays);
-            if (!exit_first) {
+            if (!exit_first && (P.flags & 256u) && mat.material_model != MAT_DIELECTRIC) {
+                // ExitAfterFirst (GK_FRAME_FAST_GATHER), Shading.slang:1118-1124
+                color = mul4(color, interpolate_cubes(sc.probes, vs.pos, vs.nrm, true));
+            } else if (!exit_first) {
                 for (uint32_t b = 1; b < maxb; ++b) {
                     bool dc = false;
                     if (get_ray_color(vs, dir, color, seed, dc, dc, rays)) break;

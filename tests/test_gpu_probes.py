@@ -97,9 +97,11 @@ def test_bake_frame_slices_like_the_renderer():
         assert np.array_equal(age, (sl & act).astype(np.uint32))
 
 
-def test_refv2_with_probes_matches_oracle():
+@pytest.mark.parametrize("fast_gather", [False, True])
+def test_refv2_with_probes_matches_oracle(fast_gather):
     """REF_V2 frames whose terminated paths read a baked grid (the device's grid, read back
-    for the oracle)."""
+    for the oracle); with GK_FRAME_FAST_GATHER every non-dielectric sample ends at its first
+    bounce's probe lookup (ExitAfterFirst)."""
     W = H = 64
     s = bright_cornell()
     osc = po.OracleScene(s)
@@ -125,6 +127,8 @@ def test_refv2_with_probes_matches_oracle():
             q.bounces = q.max_bounces = 4
             q.samples = 1
             q.has_sun = 0
+            if fast_gather:
+                q.flags |= gk.FRAME_FAST_GATHER
             ctx.frame(q)
             ctx.render()
             ctx.sync()

@@ -278,3 +278,45 @@ def test_refv2_termination_is_linear_in_the_probe_value():
     mask = outs[0] > 1e-3
     assert mask.mean() > 0.3
     np.testing.assert_allclose(outs[1][mask] / outs[0][mask], 10.0, rtol=1e-4)
+
+
+def test_refv2_fast_gather_ends_every_sample_at_its_first_bounce():
+    """GK_FRAME_FAST_GATHER (the renderer's ExitAfterFirst, Shading.slang:1118-1124): every
+    sample of a non-emitter, non-dielectric primary traces exactly one bounce ray, so a
+    frame traces one primary ray per pixel plus one ray per sample of such pixels, with any
+    seeds; the first bounce's throughput is multiplied by the probe lookup, so with the sky
+    off and a uniform grid the frame is linear in the grid's value, and without a grid
+    only samples whose first bounce reaches the light keep any radiance."""
+    W = H = 24
+    s = gk.procgen_cornell()
+    osc = po.OracleScene(s)
+    p = gk.make_frame_params(scenes.CORNELL_CAM["eye"], scenes.CORNELL_CAM["target"], W, H, fov=40.0)
+    p.estimator = gk.ESTIMATOR_REF_V2
+    p.bounces = p.max_bounces = 4
+    p.samples = 3
+    p.has_sky = p.has_sun = 0
+    p.flags |= gk.FRAME_FAST_GATHER
+    light = next(i for i, m in enumerate(s.materials) if m.material_model == gk.MAT_DIFFUSE_LIGHT)
+    for frame in range(3):
+        p.total_frames = frame
+        o = osc.render_refv2(p, W, H)
+        hit = o["objid"] != 65535
+        on_light = hit & np.all(o["albedo"][..., :3] == np.array(s.materials[light].diffuse[:3], np.float32), axis=-1)
+        assert hit.mean() > 0.5 and on_light.any()  # misses trace their primary ray only
+        assert int(o["rays"]) == W * H + p.samples * int((hit & ~on_light).sum())
+    q = gk.FrameParams.from_buffer_copy(p)
+    q.flags &= ~gk.FRAME_FAST_GATHER
+    assert int(osc.render_refv2(q, W, H)["rays"]) > int(osc.render_refv2(p, W, H)["rays"])
+    base = osc.render_refv2(p, W, H)["diffuse"][..., :3].astype(np.float64)
+    outs = []
+    for r in (64, 640):
+        G = grid(20, 20, 0.0625, (-0.125, -0.125, -0.125))
+        free_voxels(G)
+        for f in gk.CUBE_FIELDS[:12]:
+            G.cubes[f] = pack_ref(r, r, r)
+        G.attach(osc)
+        outs.append(osc.render_refv2(p, W, H)["diffuse"][..., :3].astype(np.float64) - base)
+    po.oracle().oracle_scene_set_probes(osc.h, None, None, None)
+    mask = outs[0] > 1e-3
+    assert mask.mean() > 0.5
+    np.testing.assert_allclose(outs[1][mask] / outs[0][mask], 10.0, rtol=1e-4)
