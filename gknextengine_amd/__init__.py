@@ -89,7 +89,7 @@ class FrameParams(C.Structure):
                                  "bf_sigma_normal", "exit_probability", "hit_normal_offset")] + [
         (n, C.c_uint32) for n in ("total_frames", "global_seed", "samples", "bounces", "max_bounces",
                                   "temporal_frames", "bf_size", "estimator", "denoiser", "has_sky", "has_sun",
-                                  "taa", "progressive", "hdr", "flags", "reserved")]
+                                  "taa", "progressive", "hdr", "flags")] + [("sky_rotation", C.c_float)]
 
 
 class CameraDesc(C.Structure):
@@ -158,6 +158,7 @@ SIGNATURES = [
                                   C.POINTER(Instance), _U]),
     ("gk_materials_update", C.c_int, [_P, C.POINTER(Material), _U, _U]),
     ("gk_textures_upload", C.c_int, [_P, C.POINTER(Texture), _U]),
+    ("gk_sky_upload", C.c_int, [_P, _P, _U, _U]),
     ("gk_instances_update", C.c_int, [_P, _P, _U, _U]),
     ("gk_instances_update_device", C.c_int, [_P, _P, _U, _U]),
     ("gk_get_bvh_stats", C.c_int, [_P, C.POINTER(BvhStats)]),
@@ -522,6 +523,16 @@ class Context:
         check(lib().gk_scene_upload(self.h, meshes, len(scene.meshes), mats, len(scene.materials), lights,
                                     len(scene.lights), insts, len(scene.instances)), "gk_scene_upload")
         check(lib().gk_textures_upload(self.h, textures, len(scene.textures)), "gk_textures_upload")
+
+    def sky_upload(self, rgba=None):
+        """The sky image (equirectangular, H x W x 4 float32, row 0 at +y); None removes it."""
+        if rgba is None:
+            check(lib().gk_sky_upload(self.h, None, 0, 0), "gk_sky_upload")
+            return
+        a = np.ascontiguousarray(rgba, np.float32)
+        if a.ndim != 3 or a.shape[2] != 4:
+            raise ValueError("sky image must be H x W x 4")
+        check(lib().gk_sky_upload(self.h, ptr(a), a.shape[1], a.shape[0]), "gk_sky_upload")
 
     def update_instances(self, worlds: np.ndarray, first: int = 0):
         w = np.ascontiguousarray(np.transpose(np.asarray(worlds, np.float32), (0, 2, 1)))  # column-major

@@ -34,7 +34,7 @@ using namespace ptmis;
 
 struct DiffK {
     float mvi[16], pi[16];
-    float sky[3];
+    float sky[4], sky_rot;
     uint32_t W, H, frame0, seed, spp, bounces, has_sky;
 };
 
@@ -109,7 +109,10 @@ __device__ inline f3 dpath(const DScene& s, const DiffK& K, uint32_t x, uint32_t
     for (uint32_t depth = 0;; ++depth) {
         SHit h;
         if (!trace_closest_single(s, O, D, 0.f, 1e30f, h)) {
-            if (K.has_sky) add(T * f3{K.sky[0], K.sky[1], K.sky[2]});
+            if (K.has_sky) {
+                const float4 sky = sky_radiance(s, K.sky, K.sky_rot, D);
+                add(T * f3{sky.x, sky.y, sky.z});
+            }
             break;
         }
         const InstDev& I = s.inst[h.inst];
@@ -333,9 +336,8 @@ DiffK make_diffk(const Ctx& c) {
     DiffK K;
     memcpy(K.mvi, c.params.model_view_inverse, 64);
     memcpy(K.pi, c.params.projection_inverse, 64);
-    K.sky[0] = c.params.sky_color[0];
-    K.sky[1] = c.params.sky_color[1];
-    K.sky[2] = c.params.sky_color[2];
+    memcpy(K.sky, c.params.sky_color, 16);
+    K.sky_rot = c.params.sky_rotation;
     K.W = c.width;
     K.H = c.height;
     K.frame0 = c.params.total_frames;

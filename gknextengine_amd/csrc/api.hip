@@ -275,6 +275,34 @@ float halton(int index, int base) {
 }
 
 }  // namespace
+
+// Texture.cpp:180-253: the solid-angle weighted sum over pixel centres of each colour times
+// the nine real SH basis functions (the reference's signs), in float, rows then columns.
+void project_sky_sh(const float* rgba, uint32_t w, uint32_t h, float out[27]) {
+    constexpr float kPiF = 3.14159265358979323846f;
+    constexpr float c0 = 0.282095f, c1 = 0.488603f, c2 = 1.092548f, c3 = 0.315392f, c4 = 0.546274f;
+    for (int i = 0; i < 27; ++i) out[i] = 0.f;
+    for (uint32_t y = 0; y < h; ++y) {
+        const float v = (float(y) + 0.5f) / float(h);
+        const float theta = v * kPiF;
+        const float st = std::sin(theta), ct = std::cos(theta);
+        const float weight = st * (kPiF / float(h)) * (2.0f * kPiF / float(w));
+        for (uint32_t x = 0; x < w; ++x) {
+            const float u = (float(x) + 0.5f) / float(w);
+            const float phi = u * 2.0f * kPiF;
+            const float sp = std::sin(phi), cp = std::cos(phi);
+            const float dx = st * cp, dy = ct, dz = st * sp;
+            const float b[9] = {c0, -c1 * dy, c1 * dz, -c1 * dx, c2 * dx * dy, -c2 * dy * dz,
+                                c3 * (3.0f * dy * dy - 1.0f), -c2 * dx * dz, c4 * (dx * dx - dz * dz)};
+            const float* px = rgba + 4 * (size_t(y) * w + x);
+            for (int i = 0; i < 9; ++i) {
+                out[i] += px[0] * b[i] * weight;
+                out[9 + i] += px[1] * b[i] * weight;
+                out[18 + i] += px[2] * b[i] * weight;
+            }
+        }
+    }
+}
 }  // namespace gk
 
 using namespace gk;
@@ -456,6 +484,30 @@ int gk_textures_upload(gk_ctx* ctx, const gk_texture* textures, uint32_t count) 
     GK_HIP(hipMemcpyAsync(c.tex_lut.ptr, lut, sizeof(lut), hipMemcpyHostToDevice, c.stream));
     GK_HIP(hipStreamSynchronize(c.stream));
     c.num_tex = count;
+    return GK_OK;
+}
+
+int gk_sky_upload(gk_ctx* ctx, const float* rgba32f, uint32_t width, uint32_t height) {
+    GK_CHECK_CTX(ctx);
+    Ctx& c = ctx->c;
+    if (width == 0 || height == 0) {
+        c.sky_w = c.sky_h = 0;
+        return GK_OK;
+    }
+    if (!rgba32f) return GK_ERR_ARG;
+    if (width > 16384 || height > 16384) return GK_ERR_LIMIT;
+    const size_t n = size_t(width) * height;
+    for (size_t k = 0; k < 4 * n; ++k)
+        if (!std::isfinite(rgba32f[k])) return GK_ERR_ARG;
+    GK_HIP(hipSetDevice(c.device));
+    GK_HIP(hipStreamSynchronize(c.stream));
+    c.sky_w = c.sky_h = 0;
+    GK_HIP(c.sky.resize(n));
+    GK_HIP(hipMemcpyAsync(c.sky.ptr, rgba32f, 16 * n, hipMemcpyHostToDevice, c.stream));
+    project_sky_sh(rgba32f, width, height, c.sky_sh);
+    GK_HIP(hipStreamSynchronize(c.stream));
+    c.sky_w = width;
+    c.sky_h = height;
     return GK_OK;
 }
 

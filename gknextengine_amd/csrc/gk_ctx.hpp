@@ -160,6 +160,10 @@ struct Ctx {
     DevBuf<TexInfo> tex;
     DevBuf<float> tex_lut;          // [0,256): sRGB byte -> linear, [256,512): byte / 255
     uint32_t num_tex = 0;
+    // sky image (gk_sky_upload): equirectangular RGBA32F, row 0 at +y; none while sky_w == 0
+    DevBuf<float4> sky;
+    uint32_t sky_w = 0, sky_h = 0;
+    float sky_sh[27] = {};          // ProjectHdrToSh of the image: [channel r, g, b][9]
     uint32_t max_mesh_tris = 0;     // largest mesh
     uint32_t num_proxies = 0;       // visibility-buffer proxies (instance x section)
     DevBuf<uint32_t> proxy_inst;    // proxy -> instance
@@ -275,6 +279,9 @@ int refit_tlas(Ctx& c);
 int rebuild_tlas(Ctx& c);  // device-only TLAS build over the current instance boxes
 int tlas_sah(Ctx& c, double& out);  // SAH cost of the TLAS as last refitted or rebuilt (synchronises)
 int update_lights(Ctx& c);  // world-space emissive triangles from the instance matrices
+// ProjectHdrToSh (Texture.cpp:180-253): 9 spherical-harmonic coefficients per channel of an
+// equirectangular RGBA32F image, out[channel * 9 + i]
+void project_sky_sh(const float* rgba, uint32_t w, uint32_t h, float out[27]);
 
 // trace.hip
 int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev,

@@ -100,6 +100,35 @@ __device__ inline void sincos_2pi(T u, T& s, T& c) {
     }
 }
 
+// atan2(y, x) / (2 pi), in (-1/2, 1/2]; 0 for (0, 0). The ratio t = min/max of |x|, |y|
+// is reduced with atan t = pi/4 + atan((t - 1)/(t + 1)) when t > tan(pi/8), leaving
+// |s| <= tan(pi/8), where the Maclaurin series of atan (terms to s^19, truncation below
+// 1e-8) is summed by Horner's rule in s^2; the octant and quadrant follow from
+// atan(1/t) = pi/2 - atan t and atan2's symmetries.
+__device__ inline float atan2_turns(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    if (mx == 0.f) return 0.f;
+    float t = mn / mx;
+    float base = 0.f;
+    if (t > 0.41421356f) { t = (t - 1.f) / (t + 1.f); base = 0.125f; }
+    const float t2 = t * t;
+    float p = 1.f / 19.f;
+    p = p * t2 - 1.f / 17.f;
+    p = p * t2 + 1.f / 15.f;
+    p = p * t2 - 1.f / 13.f;
+    p = p * t2 + 1.f / 11.f;
+    p = p * t2 - 1.f / 9.f;
+    p = p * t2 + 1.f / 7.f;
+    p = p * t2 - 1.f / 5.f;
+    p = p * t2 + 1.f / 3.f;
+    p = p * t2 - 1.f;
+    float a = base - (p * t) * 0.15915494f;   // -p t = atan t; 0.15915494 = 1 / (2 pi)
+    if (ay > ax) a = 0.25f - a;
+    if (x < 0.f) a = 0.5f - a;
+    return y < 0.f ? -a : a;
+}
+
 // concentric_sample_disk (Const_Func.slang:260-275) of RandomInUnitDisk (:277-280): offset
 // 2u - 1; zero maps to zero (isZERO, |x| < 1e-35); the larger coordinate is the radius and
 // the angle pi/4 * (smaller / larger), whose sin/cos come from sincos_2pi of |ratio| / 8

@@ -29,6 +29,9 @@ struct DScene {
     const uint32_t* texels;
     const float* tex_lut;
     uint32_t ntex;
+    // sky image (null: constant sky)
+    const float4* sky;
+    uint32_t sky_w, sky_h;
 };
 
 inline DScene make_dscene(const Ctx& c) {
@@ -53,6 +56,9 @@ inline DScene make_dscene(const Ctx& c) {
     s.texels = c.texels.ptr;
     s.tex_lut = c.tex_lut.ptr;
     s.ntex = c.num_tex;
+    s.sky = c.sky_w ? c.sky.ptr : nullptr;
+    s.sky_w = c.sky_w;
+    s.sky_h = c.sky_h;
     return s;
 }
 
@@ -135,6 +141,55 @@ __device__ inline float4 tex_sample(const DScene& s, uint32_t id, float2 uv) {
     return make_float4(top.x * by + bot.x * ay, top.y * by + bot.y * ay, top.z * by + bot.z * ay, top.w * by + bot.w * ay);
 }
 __device__ inline bool has_tex(const DScene& s, int32_t id) { return id >= 0 && uint32_t(id) < s.ntex; }
+
+// ---------------------------------------------------------------- sky
+// SampleIBLV2(SkyIdx, dir, SkyRotation, 0) * SkyIntensity (Shading.slang:194-199 at its
+// callers 1077 and 1193): u = (atan2(d.x, d.z) + pi rotation) / (2 pi), v = acos(d.y) / pi
+// (as atan2(sqrt(1 - y^2), y) / pi) on the equirectangular image, bilinear with REPEAT on
+// both axes like tex_sample, clamped to (10, 10, 10, 1), times col (the frame's sky_color).
+// Without an image the sky is col.
+__device__ inline float4 sky_radiance(const DScene& s, const float* col, float rot, f3 dir) {
+    if (!s.sky) return make_float4(col[0], col[1], col[2], col[3]);
+    const f3 d = normalize(dir);
+    const float u = atan2_turns(d.x, d.z) + rot * 0.5f;
+    const float y = d.y < -1.f ? -1.f : (d.y > 1.f ? 1.f : d.y);
+    const float v = 2.f * atan2_turns(sqrtf((1.f - y) * (1.f + y)), y);
+    uint32_t x0, x1, y0, y1;
+    float ax, ay;
+    tex_axis(u, s.sky_w, x0, x1, ax);
+    tex_axis(v, s.sky_h, y0, y1, ay);
+    const float4 a = s.sky[size_t(y0) * s.sky_w + x0], b = s.sky[size_t(y0) * s.sky_w + x1];
+    const float4 c = s.sky[size_t(y1) * s.sky_w + x0], e = s.sky[size_t(y1) * s.sky_w + x1];
+    const float bx = 1.f - ax, by = 1.f - ay;
+    const float4 top = make_float4(a.x * bx + b.x * ax, a.y * bx + b.y * ax, a.z * bx + b.z * ax, a.w * bx + b.w * ax);
+    const float4 bot = make_float4(c.x * bx + e.x * ax, c.y * bx + e.y * ax, c.z * bx + e.z * ax, c.w * bx + e.w * ax);
+    const float4 t = make_float4(top.x * by + bot.x * ay, top.y * by + bot.y * ay, top.z * by + bot.z * ay,
+                                 top.w * by + bot.w * ay);
+    return make_float4(fminf(t.x, 10.f) * col[0], fminf(t.y, 10.f) * col[1], fminf(t.z, 10.f) * col[2],
+                       fminf(t.w, 1.f) * col[3]);
+}
+
+// SampleIBLRough(SkyIdx, dir, SkyRotation) (Shading.slang:146-192): EvaluateSH of the image's
+// coefficients at dir turned about +y by pi * (1 - rotation), alpha 1, times col; without an
+// image col. sh: [channel][9].
+__device__ inline float4 sky_rough(const float* sh, bool has_image, const float* col, float rot, f3 n) {
+    if (!has_image) return make_float4(col[0], col[1], col[2], col[3]);
+    float turns = (1.f - rot) * 0.5f;
+    turns = turns - floorf(turns);
+    float sa, ca;
+    sincos_2pi(turns, sa, ca);
+    const float rx = n.x * ca + n.z * sa, ry = n.y, rz = -n.x * sa + n.z * ca;
+    const float b[9] = {0.282095f, -0.488603f * ry, 0.488603f * rz, -0.488603f * rx, 1.092548f * rx * ry,
+                        -1.092548f * ry * rz, 0.315392f * (3.f * ry * ry - 1.0f), -1.092548f * rx * rz,
+                        0.546274f * (rx * rx - rz * rz)};
+    float r = 0.f, g = 0.f, bl = 0.f;
+    for (int i = 0; i < 9; ++i) {
+        r += sh[i] * b[i];
+        g += sh[9 + i] * b[i];
+        bl += sh[18 + i] * b[i];
+    }
+    return make_float4(r * col[0], g * col[1], bl * col[2], col[3]);
+}
 
 // texture coordinate at a traced hit: Mix(t0, t1, t2, bary) = t0 + (t1-t0) u + (t2-t0) v
 // (FHardwareRayTracerV2::TraceRay, Shading.slang:820)

@@ -8,7 +8,8 @@
 //             FGpuProbeGenerator::RenderAmbientCubeOnly / FaceTask (AmbientCube.slang:
 //             472-566,650-690): per probe within 8 units of a surface and per face, 16
 //             hemisphere rays on a jittered 4x4 grid, a segment to the first quad light and
-//             a sun ray; hits add albedo x the direct-light lookup x 1.25, misses add sky.
+//             a sun ray; hits add albedo x the direct-light lookup x 1.25, misses add the
+//             sky (its spherical harmonics when a sky image is set).
 //             The new face value is lerp(old, new, 1/8) in RGB10.
 //
 // Work is done in chunks of kChunk probes: rays of a chunk are written to queues, traced,
@@ -163,7 +164,8 @@ __global__ void k_vox_finish(DScene s, GridK g, uint32_t first, uint32_t n, cons
 struct BakeK {
     GridK g;
     float sky[4], sun_dir[4], sun_col[4];
-    uint32_t has_sky, has_sun, has_light, nany;  // nany: occlusion rays per face (light, sun)
+    float sky_sh[27], sky_rot;  // SH of the sky image (sky_image != 0)
+    uint32_t has_sky, sky_image, has_sun, has_light, nany;  // nany: occlusion rays per face (light, sun)
     float light_pos[3], light_area;
     uint32_t light_mat;
 };
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(256) void k_bake_face(DScene s, BakeK K, ProbeDev P
             b = make_float4((alb.x * li.x) * 1.25f, (alb.y * li.y) * 1.25f, (alb.z * li.z) * 1.25f,
                             (alb.w * li.w) * 1.25f);
         } else {
-            if (K.has_sky) d = make_float4(K.sky[0], K.sky[1], K.sky[2], K.sky[3]);
+            if (K.has_sky) d = sky_rough(K.sky_sh, K.sky_image != 0u, K.sky, K.sky_rot, ld3(rd[r]));
             miss = 1.f;
         }
     }
@@ -502,6 +504,9 @@ int probes_bake(Ctx& c, uint32_t first, uint32_t count) {
     BakeK K{};
     K.g = grid_k(c);
     memcpy(K.sky, fp.sky_color, 16);
+    memcpy(K.sky_sh, c.sky_sh, sizeof(K.sky_sh));
+    K.sky_rot = fp.sky_rotation;
+    K.sky_image = c.sky_w ? 1u : 0u;
     memcpy(K.sun_dir, fp.sun_direction, 16);
     memcpy(K.sun_col, fp.sun_color, 16);
     K.has_sky = fp.has_sky ? 1u : 0u;

@@ -51,6 +51,7 @@ struct FrameK {
     float sun_dir[4], sun_col[4], sky[4];
     float exit_prob, hit_off;
     float aperture, focus;  // thin lens (aperture 0: pinhole, no random numbers drawn)
+    float sky_rot;          // SkyRotation
     uint32_t W, H, frame, seed, samples, bounces, max_bounces, temporal, has_sky, has_sun, tiled;
     uint32_t vis_primary;  // primary hits come from GK_RT_MINIGBUFFER (left unwritten)
     uint32_t fast_gather;  // REF_V2 ExitAfterFirst
@@ -526,7 +527,7 @@ __global__ __launch_bounds__(256) void k_ref_primary(DScene s, FrameK K, Targets
         seed = rng4{sr.x, sr.y, sr.z, sr.w};
         if (__float_as_uint(h.w) == kMiss) {
             write_miss_gbuffer(K, T, px, h);
-            const float4 sky = K.has_sky ? make_float4(K.sky[0], K.sky[1], K.sky[2], K.sky[3]) : make_float4(0, 0, 0, 0);
+            const float4 sky = K.has_sky ? sky_radiance(s, K.sky, K.sky_rot, ld3(Q.d[0][i])) : make_float4(0, 0, 0, 0);
             T.diffuse[px] = to_h4(sky.x, sky.y, sky.z, sky.w);
             T.spec[px] = to_h4(0.f, 0.f, 0.f, 0.f);
             float4 r = T.radiance[px];
@@ -623,7 +624,7 @@ __global__ __launch_bounds__(256) void k_ref_shade(DScene s, FrameK K, Targets T
                 ret = hit_light;
             }
         } else {
-            const float4 sky = K.has_sky ? make_float4(K.sky[0], K.sky[1], K.sky[2], K.sky[3]) : make_float4(0, 0, 0, 0);
+            const float4 sky = K.has_sky ? sky_radiance(s, K.sky, K.sky_rot, dir) : make_float4(0, 0, 0, 0);
             color.x = color.x * sky.x; color.y = color.y * sky.y; color.z = color.z * sky.z; color.w = color.w * sky.w;
             ret = true;
         }
@@ -753,7 +754,10 @@ __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T,
             }
         }
         if (__float_as_uint(h.w) == kMiss) {
-            if (K.has_sky) add(thr * f3{K.sky[0], K.sky[1], K.sky[2]}, lobe0_diffuse);
+            if (K.has_sky) {
+                const float4 sky = sky_radiance(s, K.sky, K.sky_rot, D);
+                add(thr * f3{sky.x, sky.y, sky.z}, lobe0_diffuse);
+            }
         } else {
             const uint32_t prim = __float_as_uint(h.w);
             const InstDev& I = s.inst[inst];
@@ -1115,6 +1119,7 @@ static FrameK make_k(const Ctx& c) {
     memcpy(K.sun_dir, P.sun_direction, 16);
     memcpy(K.sun_col, P.sun_color, 16);
     memcpy(K.sky, P.sky_color, 16);
+    K.sky_rot = P.sky_rotation;
     K.exit_prob = P.exit_probability;
     K.hit_off = P.hit_normal_offset;
     K.aperture = P.aperture;

@@ -188,7 +188,9 @@ typedef struct gk_frame_params {
     float prev_view_projection_unjit[16];
     float sun_direction[4];
     float sun_color[4];
-    float sky_color[4];        /* constant sky radiance (the reference samples an HDRI) */
+    float sky_color[4];        /* sky radiance without a sky image; with one (gk_sky_upload)
+                                  the image's value times this, component-wise: all four
+                                  set to SkyIntensity is the reference's sky        */
     float aperture;            /* thin-lens diameter (FHardwarePrimaryRayCaster, Shading.slang:
                                   283-300); 0: pinhole. > 0 needs focus_distance > 0; not
                                   with GK_FRAME_PRIMARY_FROM_VISBUFFER, gk_render_radiance
@@ -215,7 +217,8 @@ typedef struct gk_frame_params {
     uint32_t progressive;
     uint32_t hdr;
     uint32_t flags;            /* GK_FRAME_* bits                               */
-    uint32_t reserved;
+    float sky_rotation;        /* SkyRotation: turns the sky image by pi * this about +y
+                                  (Shading.slang:194-199); 0 with no sky image       */
 } gk_frame_params;
 
 /* gk_frame_params.flags */
@@ -370,6 +373,20 @@ int gk_materials_update(gk_ctx* ctx, const gk_material* materials, uint32_t firs
  * none. count 0 removes all textures. Texels are stored uncompressed (the reference
  * transcodes LDR images to BC7). */
 int gk_textures_upload(gk_ctx* ctx, const gk_texture* textures, uint32_t count);
+
+/* Replaces the context's sky image (an HDRI of GlobalTexturePool's hdr branch,
+ * Texture.cpp:436-457, lit as Camera.SkyIdx): an equirectangular RGBA32F image, row 0 the
+ * +y pole, width * height * 4 floats. Every ray that leaves the scene (REF_V2 primary and
+ * GetRayColor misses, PT_MIS and gk_adjoint misses) sees SampleIBLV2(dir, sky_rotation, 0)
+ * (Shading.slang:194-199): u = (atan2(d.x, d.z) + pi * sky_rotation) / (2 pi),
+ * v = acos(d.y) / pi, bilinear with REPEAT on both axes, clamped to (10, 10, 10, 1), times
+ * sky_color; the probe bake's misses see SampleIBLRough (Shading.slang:146-192,
+ * AmbientCube.slang:515), the 9-coefficient spherical harmonics that ProjectHdrToSh
+ * (Texture.cpp:180-253) computes here from the image at upload, times sky_color. Both only
+ * where has_sky is set. width or height 0 (pixels may be null) removes the image: the sky
+ * is then sky_color everywhere. GK_ERR_ARG for a non-finite pixel, GK_ERR_LIMIT for a side
+ * above 16384; the image survives gk_scene_upload. */
+int gk_sky_upload(gk_ctx* ctx, const float* rgba32f, uint32_t width, uint32_t height);
 
 /* Replaces Scene::UpdateNodesGpuDriven + RayTraceBaseRenderer::AfterUpdateScene /
  * PreRender TLAS update (Scene.cpp:443-492; RayTraceBaseRenderer.cpp:165-218).

@@ -35,7 +35,8 @@ struct FrameParams {  // gk_frame_params (UniformBufferObject subset, BasicTypes
     float aperture, focus_distance, paper_white_nit, bf_sigma, bf_sigma_lum, bf_sigma_normal,
         exit_probability, hit_normal_offset;
     uint32_t total_frames, global_seed, samples, bounces, max_bounces, temporal_frames, bf_size,
-        estimator, denoiser, has_sky, has_sun, taa, progressive, hdr, flags, reserved;
+        estimator, denoiser, has_sky, has_sun, taa, progressive, hdr, flags;
+    float sky_rotation;  // SkyRotation
 };
 static_assert(sizeof(FrameParams) == 656, "FrameParams layout");
 
@@ -151,6 +152,36 @@ inline void sincos_2pi(T u, T& s, T& c) {
     }
 }
 
+// ------------------------------------------------------------------ atan2 in turns
+// atan2(y, x) / (2 pi) in (-1/2, 1/2], 0 at the origin, from the Maclaurin series of atan:
+// the ratio t = min/max of |x|, |y| in [0, 1] is moved to |s| <= tan(pi/8) with
+// atan t = pi/4 + atan((t - 1)/(t + 1)) above tan(pi/8); atan s = s (1 - s^2/3 + ... - s^18/19)
+// (remainder below s^21/21 < 1e-8) by Horner's rule; then atan(1/t) = pi/2 - atan t and the
+// reflections into the other octants. Used for the sky image's longitude and colatitude,
+// so that the oracle and the device produce identical bits.
+inline float atan2_turns(float y, float x) {
+    const float ax = std::fabs(x), ay = std::fabs(y);
+    const bool steep = ay > ax;
+    const float hi = steep ? ay : ax, lo = steep ? ax : ay;
+    if (hi == 0.f) return 0.f;
+    float t = lo / hi;
+    float quarter_eighths = 0.f;  // 1/8 turn = pi/4
+    if (t > 0.41421356f) {
+        t = (t - 1.f) / (t + 1.f);
+        quarter_eighths = 0.125f;
+    }
+    const float t2 = t * t;
+    static const float inv_odd[10] = {1.f / 19.f, 1.f / 17.f, 1.f / 15.f, 1.f / 13.f, 1.f / 11.f,
+                                      1.f / 9.f,  1.f / 7.f,  1.f / 5.f,  1.f / 3.f,  1.f};
+    // q = -(1 - t2/3 + t2^2/5 - ...): coefficient of t2^k is (-1)^(k+1) / (2k + 1)
+    float q = inv_odd[0];
+    for (int k = 1; k < 10; ++k) q = (k % 2) ? q * t2 - inv_odd[k] : q * t2 + inv_odd[k];
+    float a = quarter_eighths - (q * t) * 0.15915494f;
+    if (steep) a = 0.25f - a;
+    if (x < 0.f) a = 0.5f - a;
+    return y < 0.f ? -a : a;
+}
+
 // ------------------------------------------------------------------ RandomInUnitDisk
 // concentric_sample_disk (Const_Func.slang:260-275): offset 2u - 1, zero (|x| < 1e-35 on
 // both axes) maps to zero, the larger coordinate is the radius and the angle is
@@ -250,6 +281,11 @@ struct TextureO {          // an RGBA8 texture (gk_texture)
 struct Scene {
     ProbeRef probes;        // REF_V2 termination lookup (arrays owned by the caller)
     std::vector<TextureO> tex;
+    // sky image: equirectangular RGBA32F, row 0 at +y (empty: constant sky), and its
+    // ProjectHdrToSh coefficients [channel][9]
+    std::vector<float> sky;
+    uint32_t sky_w = 0, sky_h = 0;
+    float sky_sh[27] = {};
     float tex_lut[512] = {};  // sRGB byte -> linear, then byte / 255
     std::vector<Mesh> meshes;
     std::vector<Instance> inst;

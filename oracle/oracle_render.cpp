@@ -324,6 +324,80 @@ F4 tex_sample(const Scene& sc, uint32_t id, float u, float v) {
 }
 bool has_tex(const Scene& sc, int32_t id) { return id >= 0 && uint32_t(id) < sc.tex.size(); }
 
+// ---------------------------------------------------------------- sky
+// SampleIBLV2(SkyIdx, dir, SkyRotation, 0) * SkyIntensity (Shading.slang:194-199, called at
+// 1077 and 1193): the direction normalised, longitude u = (atan2(d.x, d.z) + pi rotation) /
+// (2 pi), colatitude v = acos(d.y) / pi, here atan2(sqrt((1 - y)(1 + y)), y) / pi; the float
+// image sampled with the default sampler (REPEAT on both axes, bilinear as tex_sample), then
+// min((10, 10, 10, 1), texel) times the frame's sky_color. Without an image, sky_color.
+F4 sky_radiance(const Scene& sc, const FrameParams& P, F3 dir) {
+    const F4 col{{P.sky_color[0], P.sky_color[1], P.sky_color[2], P.sky_color[3]}};
+    if (sc.sky_w == 0) return col;
+    const F3 d = normalize(dir);
+    const float u = atan2_turns(d.x, d.z) + P.sky_rotation * 0.5f;
+    const float y = std::min(1.f, std::max(-1.f, d.y));
+    const float v = 2.f * atan2_turns(std::sqrt((1.f - y) * (1.f + y)), y);
+    uint32_t x0, x1, y0, y1;
+    float ax, ay;
+    wrap_axis(u, sc.sky_w, x0, x1, ax);
+    wrap_axis(v, sc.sky_h, y0, y1, ay);
+    auto px = [&](uint32_t x, uint32_t yy, int k) { return sc.sky[4 * (size_t(yy) * sc.sky_w + x) + k]; };
+    static const float cap[4] = {10.f, 10.f, 10.f, 1.f};
+    F4 o;
+    for (int k = 0; k < 4; ++k) {
+        const float top = px(x0, y0, k) * (1.f - ax) + px(x1, y0, k) * ax;
+        const float bot = px(x0, y1, k) * (1.f - ax) + px(x1, y1, k) * ax;
+        const float t = top * (1.f - ay) + bot * ay;
+        o.v[k] = std::min(cap[k], t) * col.v[k];
+    }
+    return o;
+}
+
+// SampleIBLRough(SkyIdx, dir, SkyRotation) * SkyIntensity (Shading.slang:146-192 as used by
+// the probe bake, AmbientCube.slang:515): EvaluateSH with rotate = 1 - SkyRotation, i.e. the
+// direction turned about +y by pi (1 - SkyRotation), summed over the nine basis functions in
+// order; alpha 1. Times the frame's sky_color; without an image, sky_color.
+F4 sky_rough(const Scene& sc, const FrameParams& P, F3 n) {
+    const F4 col{{P.sky_color[0], P.sky_color[1], P.sky_color[2], P.sky_color[3]}};
+    if (sc.sky_w == 0) return col;
+    float turns = (1.f - P.sky_rotation) * 0.5f;
+    turns = turns - std::floor(turns);
+    float sa, ca;
+    sincos_2pi(turns, sa, ca);
+    const float rx = n.x * ca + n.z * sa, ry = n.y, rz = -n.x * sa + n.z * ca;
+    const float C0 = 0.282095f, C1 = 0.488603f, C2 = 1.092548f, C3 = 0.315392f, C4 = 0.546274f;
+    const float basis[9] = {C0, -C1 * ry, C1 * rz, -C1 * rx, C2 * rx * ry, -C2 * ry * rz, C3 * (3.f * ry * ry - 1.0f),
+                            -C2 * rx * rz, C4 * (rx * rx - rz * rz)};
+    float rgb[3] = {0.f, 0.f, 0.f};
+    for (int i = 0; i < 9; ++i)
+        for (int ch = 0; ch < 3; ++ch) rgb[ch] += sc.sky_sh[9 * ch + i] * basis[i];
+    return F4{{rgb[0] * col.v[0], rgb[1] * col.v[1], rgb[2] * col.v[2], col.v[3]}};
+}
+
+// ProjectHdrToSh (Texture.cpp:180-253): for each pixel centre, theta = pi (y + 0.5) / h,
+// phi = 2 pi (x + 0.5) / w, direction (sin theta cos phi, cos theta, sin theta sin phi),
+// solid-angle weight sin theta (pi / h)(2 pi / w); coefficient i of channel c accumulates
+// colour_c * basis_i * weight in float over rows, then columns.
+void project_sh(const float* rgba, uint32_t w, uint32_t h, float out[27]) {
+    const float PI = 3.14159265358979323846f;
+    const float C0 = 0.282095f, C1 = 0.488603f, C2 = 1.092548f, C3 = 0.315392f, C4 = 0.546274f;
+    std::fill(out, out + 27, 0.f);
+    for (uint32_t y = 0; y < h; ++y) {
+        const float theta = ((float(y) + 0.5f) / float(h)) * PI;
+        const float sin_t = std::sin(theta), cos_t = std::cos(theta);
+        const float weight = sin_t * (PI / float(h)) * (2.0f * PI / float(w));
+        for (uint32_t x = 0; x < w; ++x) {
+            const float phi = ((float(x) + 0.5f) / float(w)) * 2.0f * PI;
+            const float dx = sin_t * std::cos(phi), dy = cos_t, dz = sin_t * std::sin(phi);
+            const float basis[9] = {C0, -C1 * dy, C1 * dz, -C1 * dx, C2 * dx * dy, -C2 * dy * dz,
+                                    C3 * (3.0f * dy * dy - 1.0f), -C2 * dx * dz, C4 * (dx * dx - dz * dz)};
+            const float* c = rgba + 4 * (size_t(y) * w + x);
+            for (int ch = 0; ch < 3; ++ch)
+                for (int i = 0; i < 9; ++i) out[9 * ch + i] += c[ch] * basis[i] * weight;
+        }
+    }
+}
+
 // Diffuse * base colour texture (GetRayColor, Shading.slang:1047-1052)
 F4 mat_albedo(const Scene& sc, const Material& m, float u, float v) {
     F4 a{{m.diffuse[0], m.diffuse[1], m.diffuse[2], m.diffuse[3]}};
@@ -592,8 +666,6 @@ struct ProbeBaker {
         F4 direct_color{{0, 0, 0, 0}}, bounce_color{{0, 0, 0, 0}};
         float sky_visibility = 0.0f;
         const float ox = grid3x3[iterate % 9][0] * 0.25f, oy = grid3x3[iterate % 9][1] * 0.25f;
-        const F4 sky = P.has_sky ? F4{{P.sky_color[0], P.sky_color[1], P.sky_color[2], P.sky_color[3]}}
-                                 : F4{{0.f, 0.f, 0.f, 0.f}};
         for (uint32_t i = 0; i < 16; i++) {
             const F3 hemi = normalize(F3(grid4x4[i][0] + ox, grid4x4[i][1] + oy, 1.0f));
             const F3 ray_dir = align_with_normal(hemi, basis);
@@ -609,7 +681,7 @@ struct ProbeBaker {
                 for (int k = 0; k < 4; ++k) t.v[k] = t.v[k] * 1.25f;
                 bounce_color = add4(bounce_color, t);
             } else {
-                direct_color = add4(direct_color, sky);
+                if (P.has_sky) direct_color = add4(direct_color, sky_rough(sc, P, ray_dir));
                 sky_visibility += 1.0f;
             }
         }
@@ -735,8 +807,7 @@ struct RefV2 {
             }
             return hm.material_model == MAT_DIFFUSE_LIGHT;
         }
-        const F4 sky = P.has_sky ? F4{{P.sky_color[0], P.sky_color[1], P.sky_color[2], P.sky_color[3]}}
-                                 : F4{{0.f, 0.f, 0.f, 0.f}};
+        const F4 sky = P.has_sky ? sky_radiance(sc, P, trace_dir) : F4{{0.f, 0.f, 0.f, 0.f}};
         color = mul4(color, sky);
         return true;
     }
@@ -750,8 +821,7 @@ struct RefV2 {
         Hit h;
         rays++;
         if (!trace_closest(sc, O, D, 0.0f, 1e30f, h)) {
-            const F4 sky = P.has_sky ? F4{{P.sky_color[0], P.sky_color[1], P.sky_color[2], P.sky_color[3]}}
-                                     : F4{{0.f, 0.f, 0.f, 0.f}};
+            const F4 sky = P.has_sky ? sky_radiance(sc, P, D) : F4{{0.f, 0.f, 0.f, 0.f}};
             for (int i = 0; i < 4; ++i) out.diffuse[i] = sky.v[i];
             out.albedo[0] = out.albedo[1] = out.albedo[2] = out.albedo[3] = 1.f;
             out.normal[1] = 1.f;
@@ -982,7 +1052,8 @@ struct PtMis {
             const T tmin = T(0);
             if (!trace_closest_t<T>(sc, O, D, tmin, T(1e30), h)) {
                 if (P.has_sky) {
-                    const V sky(T(P.sky_color[0]), T(P.sky_color[1]), T(P.sky_color[2]));
+                    const F4 s4 = sky_radiance(sc, P, F3(float(D.x), float(D.y), float(D.z)));
+                    const V sky(T(s4.v[0]), T(s4.v[1]), T(s4.v[2]));
                     (lobe0 == 0 ? Ld : Ls) = (lobe0 == 0 ? Ld : Ls) + thr * sky;
                 }
                 break;
@@ -1219,7 +1290,10 @@ struct DiffPath {
         for (uint32_t depth = 0;; ++depth) {
             HitT<T> h;
             if (!trace_closest_t<T>(sc, O, D, T(0), T(1e30), h)) {
-                if (P.has_sky) L = L + thr * V(P.sky_color[0], P.sky_color[1], P.sky_color[2]);
+                if (P.has_sky) {
+                    const F4 s4 = sky_radiance(sc, P, F3(float(D.x), float(D.y), float(D.z)));
+                    L = L + thr * V(s4.v[0], s4.v[1], s4.v[2]);
+                }
                 break;
             }
             const Instance& I = sc.inst[h.inst];
@@ -1421,6 +1495,41 @@ void oracle_tex_sample(void* s, uint32_t id, const float* uv, uint32_t n, float*
         const F4 r = tex_sample(sc, id, uv[2 * i], uv[2 * i + 1]);
         std::memcpy(out + 4 * i, r.v, 16);
     }
+}
+
+// the sky image (rgba: w * h * 4 floats; w or h 0 removes it) and its SH coefficients
+int oracle_scene_set_sky(void* s, const float* rgba, uint32_t w, uint32_t h) {
+    Scene& sc = *static_cast<Scene*>(s);
+    if (w == 0 || h == 0) {
+        sc.sky.clear();
+        sc.sky_w = sc.sky_h = 0;
+        std::fill(sc.sky_sh, sc.sky_sh + 27, 0.f);
+        return 0;
+    }
+    sc.sky.assign(rgba, rgba + 4 * size_t(w) * h);
+    sc.sky_w = w;
+    sc.sky_h = h;
+    project_sh(rgba, w, h, sc.sky_sh);
+    return 0;
+}
+void oracle_sky_sh(void* s, float* out27) { std::memcpy(out27, static_cast<Scene*>(s)->sky_sh, 27 * 4); }
+// sky_radiance / sky_rough of the frame P at n directions (3 floats each; out 4 each)
+void oracle_sky_radiance(void* s, const FrameParams* P, const float* dirs, uint32_t n, float* out) {
+    const Scene& sc = *static_cast<Scene*>(s);
+    for (uint32_t i = 0; i < n; ++i) {
+        const F4 r = sky_radiance(sc, *P, F3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+        std::memcpy(out + 4 * i, r.v, 16);
+    }
+}
+void oracle_sky_rough(void* s, const FrameParams* P, const float* dirs, uint32_t n, float* out) {
+    const Scene& sc = *static_cast<Scene*>(s);
+    for (uint32_t i = 0; i < n; ++i) {
+        const F4 r = sky_rough(sc, *P, F3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+        std::memcpy(out + 4 * i, r.v, 16);
+    }
+}
+void oracle_atan2_turns(const float* y, const float* x, uint32_t n, float* out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = atan2_turns(y[i], x[i]);
 }
 
 struct TexABI {  // gk_texture

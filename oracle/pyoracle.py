@@ -68,6 +68,11 @@ def oracle():
             "oracle_scene_set_mesh_attribs": (C.c_int, [_P, _U, _P, _U, _U, _P, _P]),
             "oracle_scene_set_textures": (C.c_int, [_P, _P, _U]),
             "oracle_tex_sample": (None, [_P, _U, _P, _U, _P]),
+            "oracle_scene_set_sky": (C.c_int, [_P, _P, _U, _U]),
+            "oracle_sky_sh": (None, [_P, _P]),
+            "oracle_sky_radiance": (None, [_P, _P, _P, _U, _P]),
+            "oracle_sky_rough": (None, [_P, _P, _P, _U, _P]),
+            "oracle_atan2_turns": (None, [_P, _P, _U, _P]),
         }
         optional = {
             "oracle_reproject": (None, [_P, _U, _U] + [_P] * 12),
@@ -117,6 +122,15 @@ def ref():
             fn.argtypes = args
         _r = L
     return _r
+
+
+def atan2_turns(y, x):
+    """The oracle's atan2(y, x) / (2 pi) (Maclaurin series, bit-identical to the device's)."""
+    y = np.ascontiguousarray(y, np.float32).ravel()
+    x = np.ascontiguousarray(x, np.float32).ravel()
+    out = np.zeros(y.size, np.float32)
+    oracle().oracle_atan2_turns(ptr(y), ptr(x), y.size, ptr(out))
+    return out
 
 
 # ---------------------------------------------------------------- oracle scene
@@ -171,6 +185,29 @@ class OracleScene:
         uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
         out = np.zeros((uv.shape[0], 4), np.float32)
         oracle().oracle_tex_sample(self.h, tex_id, ptr(uv), uv.shape[0], ptr(out))
+        return out
+
+    def set_sky(self, rgba=None):
+        """The sky image (H x W x 4 float32, row 0 at +y); None removes it."""
+        if rgba is None:
+            oracle().oracle_scene_set_sky(self.h, None, 0, 0)
+            self._sky = None
+            return
+        self._sky = np.ascontiguousarray(rgba, np.float32)
+        assert oracle().oracle_scene_set_sky(self.h, ptr(self._sky), self._sky.shape[1], self._sky.shape[0]) == 0
+
+    def sky_sh(self):
+        out = np.zeros(27, np.float32)
+        oracle().oracle_sky_sh(self.h, ptr(out))
+        return out.reshape(3, 9)
+
+    def sky_radiance(self, params, dirs, rough=False):
+        """SampleIBLV2 (rough=False) or SampleIBLRough (rough=True) of the frame's sky at
+        (n, 3) directions, times sky_color."""
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros((d.shape[0], 4), np.float32)
+        fn = oracle().oracle_sky_rough if rough else oracle().oracle_sky_radiance
+        fn(self.h, C.byref(params), ptr(d), d.shape[0], ptr(out))
         return out
 
     def vis_pack(self, hits, inst):

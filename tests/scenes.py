@@ -181,3 +181,49 @@ def face_plane_rays():
                             D[d_ax] = sgn
                             rays.append([*O, 0.0, *D, 4.0])
     return np.asarray(rays, np.float32)
+
+
+# ---------------------------------------------------------------- outdoor scene under a sky image
+SKY_CAM = {"eye": (0.0, 1.4, 4.5), "target": (0.0, 0.6, 0.0), "fov": 60.0}
+
+
+def sky_image(w=128, h=64, seed=0):
+    """An equirectangular RGBA32F test sky (row 0 at +y): a zenith-to-horizon gradient, a
+    colour that varies with longitude, a sun disc brighter than the sampler's clamp of 10
+    and noise, so that a wrong longitude, latitude or filter shows."""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(h)[:, None] + 0.5) / h
+    u = (np.arange(w)[None, :] + 0.5) / w
+    img = np.zeros((h, w, 4), np.float32)
+    img[..., 0] = 0.4 + 0.6 * v + 0.3 * np.cos(2 * np.pi * u)
+    img[..., 1] = 0.6 + 0.4 * np.sin(np.pi * v) + 0.2 * np.sin(2 * np.pi * u)
+    img[..., 2] = 1.2 - 0.8 * v + 0.1 * np.cos(6 * np.pi * u)
+    img[..., :3] *= 1.0 + 0.2 * rng.random((h, w, 3))
+    sun = (np.abs(v - 0.3) < 0.05) & (np.abs(u - 0.6) < 0.03)
+    img[sun, :3] = [40.0, 35.0, 30.0]
+    img[..., 3] = 1.0 + 0.5 * rng.random((h, w))   # alpha above 1: the clamp shows
+    return img
+
+
+def sky_scene():
+    """A Lambertian floor and four bricks (Lambertian, metal, glass, mixture) in the open."""
+    from gknextengine_amd.scenes import material
+    tris = np.asarray(floor_tris(L=20.0), np.float32)
+    s = gk.SceneData()
+    s.add_mesh(tris.reshape(-1, 3), np.arange(6, dtype=np.uint32).reshape(-1, 3),
+               np.tile(np.float32([0, 1, 0]), (6, 1)), np.zeros(2, np.uint32))
+    bp, bn, bi, bm = gk.procgen_brick()
+    s.add_mesh(bp, bi, bn, bm)
+    s.materials = [material((0.45, 0.45, 0.4), gk.MAT_LAMBERTIAN),
+                   material((0.8, 0.3, 0.2), gk.MAT_LAMBERTIAN),
+                   material((0.9, 0.9, 0.9), gk.MAT_METALLIC, rough=0.05),
+                   material((1.0, 1.0, 1.0), gk.MAT_DIELECTRIC, rough=0.0, ior=1.5),
+                   material((0.3, 0.6, 0.8), gk.MAT_MIXTURE, rough=0.3, metal=0.5)]
+    s.add_instance(0)
+    for k, (x, z, ang) in enumerate(((-1.6, 0.0, 0.3), (-0.5, -0.6, -0.4), (0.6, 0.4, 0.8), (1.7, -0.3, 0.1))):
+        w = np.eye(4, dtype=np.float32)
+        c, sn = np.cos(ang), np.sin(ang)
+        w[:3, :3] = np.float32([[c, 0, sn], [0, 1, 0], [-sn, 0, c]]) * 1.2
+        w[0, 3], w[2, 3] = x, z
+        s.add_instance(1, world=w, mat_offset=1 + k)
+    return s
