@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -80,21 +81,28 @@ struct ConvK {
 };
 
 // One work-group = 4 waves = a 16 x 16 pixel tile of one image and MT*16 output channels;
-// a wave owns 4 rows of 16 pixels (4 B tiles) and MT A tiles. STAGE: the work-group first
-// copies its input tile into LDS (zero outside the image) and the 9 taps read their
-// operands from there instead of 9 gathers from memory: 18 x 18 pixels with a pixel's
-// channels in 16-byte units, and for an upsampled first input its 10 x 10 half-resolution
-// pixels in a region of their own (W and H are even, so a pixel lies in the image exactly
-// when its half-resolution pixel does). (One unit of padding per pixel, an odd stride
-// across the banks, cost more through the larger tile: 1080p network 1.26 -> 1.31 ms.)
+// a wave owns 4 rows of 16 pixels (4 B tiles) and MT A tiles. MODE 0 gathers operands from
+// memory. MODE >= 1 (staged): the work-group first copies its input tile into LDS (zero
+// outside the image) and the 9 taps read their operands from there instead of 9 gathers
+// from memory: 18 x 18 pixels, stored as one plane of 16-byte units per 8-channel group
+// (a B operand's 8 lanes of one LDS cycle then read 128 contiguous bytes: no bank
+// conflicts), and for an upsampled first input its 10 x 10 half-resolution pixels in planes
+// of their own (W and H are even, so a pixel lies in the image exactly when its
+// half-resolution pixel does). MODE 2 also stages the work-group's weight rows once, each
+// row padded by 16 bytes so that the 16 rows of an A operand fall in distinct banks; the K
+// loop then waits on no global load.
 constexpr int kHalo = 18, kHaloUp = 10;
+constexpr int kPix = kHalo * kHalo, kPixUp = kHaloUp * kHaloUp;
 __host__ __device__ inline uint32_t conv_tile_bytes(uint32_t ca, uint32_t cb, uint32_t a_up) {
-    return a_up ? (kHalo * kHalo * (cb / 8u) + kHaloUp * kHaloUp * (ca / 8u)) * 16u
-                : kHalo * kHalo * ((ca + cb) / 8u) * 16u;
+    return a_up ? (kPix * (cb / 8u) + kPixUp * (ca / 8u)) * 16u : kPix * ((ca + cb) / 8u) * 16u;
+}
+__host__ __device__ inline uint32_t conv_weight_bytes(uint32_t mt, uint32_t kpad) {
+    return mt * 16u * (kpad / 8u + 1u) * 16u;
 }
 
-template <int MT, bool STAGE>
+template <int MT, int MODE>
 __global__ __launch_bounds__(256) void k_conv(ConvK K) {
+    constexpr bool STAGE = MODE >= 1;
     extern __shared__ uint4 tile[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int img = blockIdx.y / K.tiles_y, ty = blockIdx.y - img * K.tiles_y;
@@ -119,8 +127,10 @@ __global__ __launch_bounds__(256) void k_conv(ConvK K) {
     const int ubase = kHalo * kHalo * pstride;
     const int bx0 = int(blockIdx.x) * 16, by0 = ty * 16;
     const int ux0 = (bx0 - 1) >> 1, uy0 = (by0 - 1) >> 1;
+    const int wbase = ubase + (K.a_up ? kPixUp * ustride : 0);  // MODE 2: weight rows
+    const int kq = int(K.kpad) >> 3, wst = kq + 1;
     if (STAGE) {
-        for (int u = int(threadIdx.x); u < kHalo * kHalo * pstride; u += 256) {
+        for (int u = int(threadIdx.x); u < kPix * pstride; u += 256) {
             const int pix = u / pstride, g = u - pix * pstride;
             const int r = pix / kHalo, c = pix - r * kHalo;
             const int sy = by0 - 1 + r, sx = bx0 - 1 + c, c8 = cfull0 + g * 8;
@@ -130,17 +140,24 @@ __global__ __launch_bounds__(256) void k_conv(ConvK K) {
                                               : K.b + (px0 + size_t(sy) * W + sx) * cb + (c8 - ca);
                 v = *reinterpret_cast<const uint4*>(src);
             }
-            tile[pix * pstride + g] = v;
+            tile[g * kPix + pix] = v;
         }
         if (K.a_up) {
-            for (int u = int(threadIdx.x); u < kHaloUp * kHaloUp * ustride; u += 256) {
+            for (int u = int(threadIdx.x); u < kPixUp * ustride; u += 256) {
                 const int pix = u / ustride, g = u - pix * ustride;
                 const int r = pix / kHaloUp, c = pix - r * kHaloUp;
                 const int uy = uy0 + r, ux = ux0 + c;
                 uint4 v = make_uint4(0u, 0u, 0u, 0u);
                 if (ux >= 0 && ux < (W >> 1) && uy >= 0 && uy < (H >> 1))
                     v = *reinterpret_cast<const uint4*>(K.a + (pxa0 + size_t(uy) * (W >> 1) + ux) * ca + g * 8);
-                tile[ubase + pix * ustride + g] = v;
+                tile[ubase + g * kPixUp + pix] = v;
+            }
+        }
+        if (MODE == 2) {
+            for (int u = int(threadIdx.x); u < MT * 16 * kq; u += 256) {
+                const int row = u / kq, col = u - row * kq;
+                tile[wbase + row * wst + col] =
+                    *reinterpret_cast<const uint4*>(K.w + size_t(co0 + row) * K.kpad + size_t(col) * 8);
             }
         }
         __syncthreads();
@@ -148,8 +165,14 @@ __global__ __launch_bounds__(256) void k_conv(ConvK K) {
     int tap = kg / cin, ci = kg - tap * cin;
     for (uint32_t kc = 0; kc < K.kpad; kc += 32) {
         bf16x8 af[MT];
+        if (MODE == 2) {
 #pragma unroll
-        for (int i = 0; i < MT; ++i) af[i] = ld8(wrow + size_t(i * 16) * K.kpad + kc);
+            for (int i = 0; i < MT; ++i)
+                af[i] = __builtin_bit_cast(bf16x8, tile[wbase + (i * 16 + (lane & 15)) * wst + int((kc + uint32_t(kg)) >> 3)]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < MT; ++i) af[i] = ld8(wrow + size_t(i * 16) * K.kpad + kc);
+        }
         bf16x8 bfr[4];
         const uint32_t kk = kc + uint32_t(kg);
         if (kk < K.ktap) {
@@ -162,13 +185,13 @@ __global__ __launch_bounds__(256) void k_conv(ConvK K) {
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int ucy = ((by0 + wv * 4 + j + dy) >> 1) - uy0;
-                        bfr[j] = __builtin_bit_cast(bf16x8, tile[ubase + (ucy * kHaloUp + ucx) * ustride + (ci >> 3)]);
+                        bfr[j] = __builtin_bit_cast(bf16x8, tile[ubase + (ci >> 3) * kPixUp + ucy * kHaloUp + ucx]);
                     }
                 } else {
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
                         bfr[j] = __builtin_bit_cast(
-                            bf16x8, tile[((wv * 4 + j + dy + 1) * kHalo + lx) * pstride + ((ci - cfull0) >> 3)]);
+                            bf16x8, tile[((ci - cfull0) >> 3) * kPix + (wv * 4 + j + dy + 1) * kHalo + lx]);
                 }
             } else {
                 const int sx = x + dx;
@@ -226,6 +249,236 @@ __global__ __launch_bounds__(256) void k_conv(ConvK K) {
                 st4(K.out + px * split + co, v0, v1, v2, v3);
             } else {
                 st4(K.out2 + px * (cout - split) + (co - split), v0, v1, v2, v3);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- forward convolution, fixed shape
+// k_conv with the layer's channel counts as template constants (the layers of the shipped
+// 32-channel network): the K loop is unrolled, so every LDS operand address is a per-lane
+// base plus an immediate and the weight loads are a per-lane pointer plus an immediate; the
+// runtime-shape kernel spends most of its issue slots on that index arithmetic. Same
+// operands, same MFMA sequence and the same RNE rounding to BF16 (the hardware conversion),
+// so the outputs equal k_conv's bit for bit. POOL: the epilogue also writes the 2 x 2
+// average of the rounded outputs at half resolution, summed in k_pool's order.
+struct ConvS {
+    const uint16_t* a;   // first input (half resolution when UP)
+    const uint16_t* b;   // skip input
+    const uint16_t* w;   // [COUT][KPAD]
+    uint16_t* out;       // NHWC, COUT channels
+    uint16_t* pool;      // NHWC at half resolution (POOL)
+    float* outf;         // float4 per pixel: channels 0..3 (OUTF)
+    uint32_t W, H, tiles_y, relu;
+};
+
+template <int CA, int CB, bool UP>
+struct ConvShape {
+    static constexpr int CIN = CA + CB;
+    static constexpr int KTAP = 9 * CIN;
+    static constexpr int KPAD = (KTAP + 1 + 31) / 32 * 32;
+    static constexpr int NSTEP = KPAD / 32;
+    static constexpr int NPF = (UP ? CB : CIN) / 8;  // full-resolution planes
+    static constexpr int NPU = UP ? CA / 8 : 0;      // half-resolution planes
+    static constexpr int UBASE = kPix * NPF;
+    static constexpr int UNITS = UBASE + kPixUp * NPU;
+    // operand of k-lane-group starting at k, row j of the wave: kind 0 full-resolution plane,
+    // 1 half-resolution plane, 2 bias one, 3 zero; off = the address's constant part in
+    // 16-byte units; dxi = dx + 1 (selects the lane's half-resolution column)
+    struct Op {
+        int kind, off, dxi;
+    };
+    static constexpr Op op(int k, int j) {
+        if (k >= KTAP) return Op{k == KTAP ? 2 : 3, 0, 0};
+        const int tap = k / CIN, ci = k - (k / CIN) * CIN;
+        const int dy = tap / 3 - 1, dx = tap - (tap / 3) * 3 - 1;
+        if (UP && ci < CA) {
+            // half-resolution row: floor((4 wv + j + dy) / 2) + 1 = 2 wv + floor((j + dy) / 2) + 1
+            const int t = j + dy;
+            const int fl = t >= 0 ? t / 2 : -1;
+            return Op{1, UBASE + (ci / 8) * kPixUp + (fl + 1) * kHaloUp, dx + 1};
+        }
+        const int plane = (UP ? ci - CA : ci) / 8;
+        return Op{0, plane * kPix + (j + dy + 1) * kHalo + (dx + 1), 0};
+    }
+};
+
+template <int CA, int CB, bool UP, int COUT, bool POOL, bool OUTF, bool WL>
+__global__ __launch_bounds__(256) void k_convs(ConvS K) {
+    using S = ConvShape<CA, CB, UP>;
+    constexpr int MT = COUT / 16;
+    constexpr int KQ = S::KPAD / 8, WST = KQ + 1;  // WL: weight rows in LDS, 16 bytes of padding each
+    extern __shared__ uint4 tile[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int gi = lane >> 4, x = lane & 15;
+    const int img = blockIdx.y / K.tiles_y, ty = blockIdx.y - img * K.tiles_y;
+    const int W = int(K.W), H = int(K.H);
+    const int bx0 = int(blockIdx.x) * 16, by0 = ty * 16;
+    const size_t px0 = size_t(img) * H * W;
+    // ---- stage the input tile (planes of 16-byte units, zero outside the image)
+    {
+        constexpr int NF = kPix * S::NPF;
+        for (int u = int(threadIdx.x); u < NF; u += 256) {
+            const int pix = u / S::NPF, g = u - (u / S::NPF) * S::NPF;
+            const int r = pix / kHalo, c = pix - (pix / kHalo) * kHalo;
+            const int sy = by0 - 1 + r, sx = bx0 - 1 + c;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (sx >= 0 && sx < W && sy >= 0 && sy < H) {
+                const size_t p = px0 + size_t(sy) * W + sx;
+                const int c8 = g * 8;
+                const uint16_t* src = UP ? K.b + p * CB + c8 : (c8 < CA ? K.a + p * CA + c8 : K.b + p * CB + (c8 - CA));
+                v = *reinterpret_cast<const uint4*>(src);
+            }
+            tile[g * kPix + pix] = v;
+        }
+        if constexpr (UP) {
+            const int Wh = W >> 1, Hh = H >> 1;
+            const int ux0 = (bx0 >> 1) - 1, uy0 = (by0 >> 1) - 1;
+            const size_t pxa0 = size_t(img) * Hh * Wh;
+            constexpr int NU = kPixUp * S::NPU;
+            for (int u = int(threadIdx.x); u < NU; u += 256) {
+                const int pix = u / S::NPU, g = u - (u / S::NPU) * S::NPU;
+                const int r = pix / kHaloUp, c = pix - (pix / kHaloUp) * kHaloUp;
+                const int uy = uy0 + r, ux = ux0 + c;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (ux >= 0 && ux < Wh && uy >= 0 && uy < Hh)
+                    v = *reinterpret_cast<const uint4*>(K.a + (pxa0 + size_t(uy) * Wh + ux) * CA + g * 8);
+                tile[S::UBASE + g * kPixUp + pix] = v;
+            }
+        }
+        if constexpr (WL) {
+            for (int u = int(threadIdx.x); u < COUT * KQ; u += 256) {
+                const int row = u / KQ, col = u - (u / KQ) * KQ;
+                tile[S::UNITS + row * WST + col] = *reinterpret_cast<const uint4*>(K.w + size_t(row) * S::KPAD + col * 8);
+            }
+        }
+        __syncthreads();
+    }
+    // ---- per-lane operand bases
+    const int fb0 = wv * 4 * kHalo + x;  // full-resolution pixel of row 0
+    // half-resolution column of the pixel for dx = -1, 0, 1: floor((x + dx) / 2) + 1, plus
+    // the wave's first half-resolution row (2 wv rows of 10)
+    const int u0 = 2 * wv * kHaloUp + ((x + 1) >> 1), u1 = 2 * wv * kHaloUp + ((x + 2) >> 1),
+              u2 = 2 * wv * kHaloUp + ((x + 3) >> 1);
+    auto ub0 = [&](int d) { return d == 0 ? u0 : (d == 1 ? u1 : u2); };
+    const int fbg = fb0 + gi * kPix;
+    auto ubg = [&](int d) { return ub0(d) + gi * kPixUp; };
+    const uint16_t* wl = K.w + size_t(x) * S::KPAD + gi * 8;
+    f32x4 acc[MT][4];
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < S::NSTEP; ++s) {
+        bf16x8 af[MT];
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+            af[i] = WL ? __builtin_bit_cast(bf16x8, tile[S::UNITS + (i * 16 + x) * WST + s * 4 + gi])
+                       : ld8(wl + i * 16 * S::KPAD + s * 32);
+        bf16x8 bfr[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const typename S::Op q0 = S::op(32 * s, j), q1 = S::op(32 * s + 8, j);
+            const typename S::Op q2 = S::op(32 * s + 16, j), q3 = S::op(32 * s + 24, j);
+            const int psz = q0.kind == 0 ? kPix : kPixUp;
+            const bool uniform = q0.kind <= 1 && q1.kind == q0.kind && q2.kind == q0.kind && q3.kind == q0.kind &&
+                                 q1.dxi == q0.dxi && q2.dxi == q0.dxi && q3.dxi == q0.dxi && q1.off == q0.off + psz &&
+                                 q2.off == q0.off + 2 * psz && q3.off == q0.off + 3 * psz;
+            if (uniform) {
+                const int off = q0.off + (q0.kind == 0 ? fbg : ubg(q0.dxi));
+                bfr[j] = __builtin_bit_cast(bf16x8, tile[off]);
+            } else {
+                auto lo = [&](const typename S::Op& q) {
+                    return q.kind == 0 ? q.off + fb0 : (q.kind == 1 ? q.off + ub0(q.dxi) : 0);
+                };
+                const int off = gi == 0 ? lo(q0) : gi == 1 ? lo(q1) : gi == 2 ? lo(q2) : lo(q3);
+                const int kind = gi == 0 ? q0.kind : gi == 1 ? q1.kind : gi == 2 ? q2.kind : q3.kind;
+                bf16x8 v = __builtin_bit_cast(bf16x8, tile[off]);
+                if (q0.kind >= 2 || q1.kind >= 2 || q2.kind >= 2 || q3.kind >= 2) {
+                    if (kind == 2) v = one8();
+                    else if (kind == 3) v = zero8();
+                }
+                bfr[j] = v;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+    // ---- epilogue: the output tile goes through LDS ([pixel][channel], the input tile's
+    // space) so that the global stores are whole 16-byte units of consecutive pixels (a
+    // lane's own 4 channels of one pixel per store left each cache line half written per
+    // instruction and took most of the kernel's time)
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2v __attribute__((ext_vector_type(2)));
+    __syncthreads();  // every wave is done reading the input tile
+    uint16_t* otile = reinterpret_cast<uint16_t*>(tile);
+    float4* ftile = reinterpret_cast<float4*>(tile);
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+        const int co = i * 16 + gi * 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float v0 = acc[i][j][0], v1 = acc[i][j][1], v2 = acc[i][j][2], v3 = acc[i][j][3];
+            if (K.relu) {
+                v0 = fmaxf(v0, 0.f);
+                v1 = fmaxf(v1, 0.f);
+                v2 = fmaxf(v2, 0.f);
+                v3 = fmaxf(v3, 0.f);
+            }
+            const int p = (wv * 4 + j) * 16 + x;
+            if (OUTF) {
+                if (co == 0) ftile[p] = make_float4(v0, v1, v2, v3);
+            } else {
+                const b2v p01 = __builtin_convertvector((f2v){v0, v1}, b2v);
+                const b2v p23 = __builtin_convertvector((f2v){v2, v3}, b2v);
+                *reinterpret_cast<uint2*>(otile + p * COUT + co) =
+                    make_uint2(__builtin_bit_cast(uint32_t, p01), __builtin_bit_cast(uint32_t, p23));
+            }
+        }
+    }
+    __syncthreads();
+    constexpr int CU = OUTF ? 1 : COUT / 8;  // 16-byte units per pixel
+    for (int c = int(threadIdx.x); c < 256 * CU; c += 256) {
+        const int p = c / CU, g = c - (c / CU) * CU;
+        const int gy = by0 + (p >> 4), gx = bx0 + (p & 15);
+        if (gx < W && gy < H) {
+            const size_t px = px0 + size_t(gy) * W + gx;
+            if (OUTF) *reinterpret_cast<uint4*>(K.outf + px * 4) = tile[p];
+            else *reinterpret_cast<uint4*>(K.out + px * COUT + g * 8) = tile[p * CU + g];
+        }
+    }
+    if (POOL) {
+        // 2 x 2 average of the rounded outputs in k_pool's order: (y, x), (y, x + 1),
+        // (y + 1, x), (y + 1, x + 1)
+        const int Wh = W >> 1, Hh = H >> 1;
+        for (int c = int(threadIdx.x); c < 64 * CU; c += 256) {
+            const int q = c / CU, g = c - (c / CU) * CU;
+            const int qy = q >> 3, qx = q & 7;
+            const int p00 = (2 * qy) * 16 + 2 * qx;
+            const uint4 t4[4] = {tile[p00 * CU + g], tile[(p00 + 1) * CU + g], tile[(p00 + 16) * CU + g],
+                                 tile[(p00 + 17) * CU + g]};
+            float o[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                const uint32_t w4[4] = {t4[d].x, t4[d].y, t4[d].z, t4[d].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    o[2 * e] += __uint_as_float(w4[e] << 16);
+                    o[2 * e + 1] += __uint_as_float(w4[e] & 0xffff0000u);
+                }
+            }
+            uint32_t r4[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                r4[e] = __builtin_bit_cast(uint32_t, __builtin_convertvector((f2v){0.25f * o[2 * e], 0.25f * o[2 * e + 1]}, b2v));
+            const int py = (by0 >> 1) + qy, pxh = (bx0 >> 1) + qx;
+            if (pxh < Wh && py < Hh) {
+                const size_t pp = size_t(img) * Hh * Wh + size_t(py) * Wh + pxh;
+                *reinterpret_cast<uint4*>(K.pool + pp * COUT + g * 8) = make_uint4(r4[0], r4[1], r4[2], r4[3]);
             }
         }
     }
@@ -655,6 +908,7 @@ struct gk_unet {
     DevBuf<float> feat;                // frame features (denoiser path)
     size_t ws_px = 0;                  // level-0 pixels the workspace holds
     bool ws_grad = false;
+    bool fixed_shapes = true;          // forward through k_convs where a shape is built
 };
 
 namespace {
@@ -747,16 +1001,22 @@ template <int MT>
 void launch_conv_t(const ConvK& K, uint32_t nimg, hipStream_t st) {
     const dim3 grid((K.W + 15) / 16, K.tiles_y * nimg, K.cout / (MT * 16));
     const uint32_t bytes = conv_tile_bytes(K.ca, K.cb, K.a_up);
+    const uint32_t wbytes = conv_weight_bytes(MT, K.kpad);
     bool stage = (K.W % 2u) == 0u && (K.H % 2u) == 0u && bytes <= max_lds_per_block();
+    // weights join the tile while both stay within 64 KB (two or more work-groups per CU)
+    if (stage && bytes + wbytes <= 65536u) {
+        k_conv<MT, 2><<<grid, 256, bytes + wbytes, st>>>(K);
+        return;
+    }
     if (stage && bytes > 65536u) {
-        const void* fn = reinterpret_cast<const void*>(&k_conv<MT, true>);
+        const void* fn = reinterpret_cast<const void*>(&k_conv<MT, 1>);
         if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)) != hipSuccess) {
             (void)hipGetLastError();
             stage = false;
         }
     }
-    if (stage) k_conv<MT, true><<<grid, 256, bytes, st>>>(K);
-    else k_conv<MT, false><<<grid, 256, 0, st>>>(K);
+    if (stage) k_conv<MT, 1><<<grid, 256, bytes, st>>>(K);
+    else k_conv<MT, 0><<<grid, 256, 0, st>>>(K);
 }
 
 int launch_conv(ConvK K, uint32_t nimg, hipStream_t st) {
@@ -780,6 +1040,58 @@ int launch_conv(ConvK K, uint32_t nimg, hipStream_t st) {
     }
     GK_HIP(hipGetLastError());
     return GK_OK;
+}
+
+// weights go to LDS (once per work-group instead of once per wave from L2) while the
+// work-group's LDS stays within 64 KB (two or more work-groups per CU; the level-0 decoder
+// convolution's 78 KB ran at 237 us against 174 us with weights from L2)
+constexpr uint32_t kConvWlBytes = 64u * 1024u;
+template <int CA, int CB, bool UP, int COUT, bool POOL, bool OUTF, bool WL>
+int launch_convs_w(const ConvS& K, uint32_t nimg, uint32_t bytes, hipStream_t st) {
+    if (bytes > max_lds_per_block()) return GK_ERR_LIMIT;
+    const void* fn = reinterpret_cast<const void*>(&k_convs<CA, CB, UP, COUT, POOL, OUTF, WL>);
+    if (bytes > 65536u) GK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+    const dim3 grid((K.W + 15) / 16, K.tiles_y * nimg, 1);
+    k_convs<CA, CB, UP, COUT, POOL, OUTF, WL><<<grid, 256, bytes, st>>>(K);
+    GK_HIP(hipGetLastError());
+    return GK_OK;
+}
+template <int CA, int CB, bool UP, int COUT, bool POOL, bool OUTF>
+int launch_convs_t(const ConvS& K, uint32_t nimg, hipStream_t st) {
+    using S = ConvShape<CA, CB, UP>;
+    const uint32_t in_bytes = uint32_t(S::UNITS) * 16u;
+    const uint32_t out_bytes = OUTF ? 256u * 16u : 256u * uint32_t(COUT) * 2u;
+    const uint32_t bytes = in_bytes > out_bytes ? in_bytes : out_bytes;
+    const uint32_t w_bytes = uint32_t(COUT) * uint32_t(S::KPAD / 8 + 1) * 16u;
+    if (in_bytes + w_bytes <= kConvWlBytes) {
+        const uint32_t b2 = in_bytes + w_bytes > out_bytes ? in_bytes + w_bytes : out_bytes;
+        return launch_convs_w<CA, CB, UP, COUT, POOL, OUTF, true>(K, nimg, b2, st);
+    }
+    return launch_convs_w<CA, CB, UP, COUT, POOL, OUTF, false>(K, nimg, bytes, st);
+}
+
+// the forward convolution of layer l through its fixed-shape kernel when one is built for
+// its channel counts (the 32-channel network), with the following 2 x 2 pool fused when
+// pool is set; kNoShape otherwise (the caller falls back to k_conv)
+constexpr int kNoShape = 1;
+int launch_convs(const gk_unet_layer& L, ConvS K, uint32_t nimg, bool pool, hipStream_t st) {
+    K.tiles_y = (K.H + 15) / 16;
+    const bool outf = K.outf != nullptr;
+    const uint32_t key[6] = {L.cin_a, L.cin_b, L.up, L.cout, pool ? 1u : 0u, outf ? 1u : 0u};
+    auto is = [&](uint32_t a, uint32_t b, uint32_t up, uint32_t co, uint32_t pl, uint32_t of) {
+        return key[0] == a && key[1] == b && key[2] == up && key[3] == co && key[4] == pl && key[5] == of;
+    };
+    // built where they beat k_conv (+ k_pool) at 1920x1088: the coarse levels' wide layers
+    // (level 3-4, and the decoder's 112- to 224-channel inputs) run faster through k_conv's
+    // rolled K loop, whose registers leave room for more waves
+    if (is(16, 0, 0, 32, 0, 0)) return launch_convs_t<16, 0, false, 32, false, false>(K, nimg, st);
+    if (is(32, 0, 0, 32, 1, 0)) return launch_convs_t<32, 0, false, 32, true, false>(K, nimg, st);
+    if (is(32, 0, 0, 48, 1, 0)) return launch_convs_t<32, 0, false, 48, true, false>(K, nimg, st);
+    if (is(48, 0, 0, 64, 1, 0)) return launch_convs_t<48, 0, false, 64, true, false>(K, nimg, st);
+    if (is(48, 32, 1, 32, 0, 0)) return launch_convs_t<48, 32, true, 32, false, false>(K, nimg, st);
+    if (is(32, 0, 0, 32, 0, 0)) return launch_convs_t<32, 0, false, 32, false, false>(K, nimg, st);
+    if (is(32, 0, 0, 16, 0, 1)) return launch_convs_t<32, 0, false, 16, false, true>(K, nimg, st);
+    return kNoShape;
 }
 
 template <int MT, int NT>
@@ -815,6 +1127,23 @@ int forward(gk_unet& n, uint32_t nimg, uint32_t h, uint32_t w) {
     for (uint32_t l = 0; l < kLayers; ++l) {
         const gk_unet_layer& L = n.L[l];
         const LayerSpec& S = kSpec[l];
+        const bool pool = l >= 1 && l <= 4;  // pool the skip output to the next level
+        const uint32_t lw = w >> L.level, lh = h >> L.level;
+        if (n.fixed_shapes && (!(pool || L.up) || ((lw % 2u) == 0u && (lh % 2u) == 0u))) {
+            ConvS Q{};
+            Q.a = tp(n, S.a);
+            Q.b = tp(n, S.b);
+            Q.w = n.w16.ptr + L.offset;
+            Q.W = lw;
+            Q.H = lh;
+            Q.relu = L.relu;
+            if (S.out == kNone) Q.outf = n.y32.ptr;
+            else Q.out = n.t[S.out].ptr;
+            if (pool) Q.pool = n.t[kSpec[l + 1].a].ptr;
+            const int s = launch_convs(L, Q, nimg, pool, st);
+            if (s == GK_OK) continue;
+            if (s != kNoShape) return s;
+        }
         ConvK K{};
         K.a = tp(n, S.a);
         K.b = tp(n, S.b);
@@ -835,9 +1164,8 @@ int forward(gk_unet& n, uint32_t nimg, uint32_t h, uint32_t w) {
         else K.out = n.t[S.out].ptr;
         int s = launch_conv(K, nimg, st);
         if (s != GK_OK) return s;
-        if (l >= 1 && l <= 4) {  // pool the skip output to the next level
+        if (pool) {
             const int src = S.out, dst = kSpec[l + 1].a;
-            const uint32_t lw = w >> L.level, lh = h >> L.level;
             const size_t cnt = size_t(nimg) * (lh / 2) * (lw / 2) * (n.tinfo[src].c / 8);
             k_pool<<<nblk(cnt), 256, 0, st>>>(n.t[src].ptr, n.t[dst].ptr, nimg, lh, lw, n.tinfo[src].c);
             GK_HIP(hipGetLastError());
@@ -990,6 +1318,8 @@ int gk_unet_create(gk_ctx* ctx, uint32_t base_channels, uint64_t seed, gk_unet**
     gk_unet* n = new gk_unet();
     n->ctx = &c;
     describe(*n, base_channels);
+    const char* generic = std::getenv("GKNX_UNET_GENERIC");
+    n->fixed_shapes = !(generic && generic[0] && generic[0] != '0');
     auto fail = [&](int s) {
         n->ctx = nullptr;
         delete n;

@@ -544,7 +544,10 @@ typedef struct gk_unet_layer {
 
 /* Creates a network on ctx's device and stream with He-normal weights drawn from
  * seed, and makes it ctx's GK_DENOISER_UNET. A context owns at most one network:
- * creating another destroys the previous one, and gk_destroy destroys it. */
+ * creating another destroys the previous one, and gk_destroy destroys it. The forward
+ * pass of base_channels 32 runs kernels built for its layer shapes; with
+ * GKNX_UNET_GENERIC=1 in the environment at creation it runs the runtime-shape kernels
+ * instead (same results, for comparison). */
 int gk_unet_create(gk_ctx* ctx, uint32_t base_channels, uint64_t seed, gk_unet** out);
 int gk_unet_destroy(gk_unet* net);
 int gk_unet_num_params(gk_unet* net, uint64_t* out);

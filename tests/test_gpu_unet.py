@@ -252,3 +252,30 @@ def test_trained_network_beats_bilateral_on_held_out_scenes():
             assert ru < rb and ru < rb_finite and ru < 0.5 * noisy, (which, noisy, rb, ru)
     finally:
         pr.close()
+
+
+def test_fixed_shape_forward_equals_runtime_shape_kernels():
+    """The forward pass of the 32-channel network through the kernels built for its layer
+    shapes (unrolled K loop, fused 2x2 pooling) equals the runtime-shape convolutions and
+    separate pooling bit for bit, on an image whose coarsest level is not a multiple of 16."""
+    import os
+
+    import torch
+    rng = np.random.default_rng(7)
+    h, w = 208, 176
+    x = features(rng, 1, h, w)
+    outs = []
+    for generic in ("0", "1"):
+        os.environ["GKNX_UNET_GENERIC"] = generic
+        try:
+            with gk.Context(16, 16) as ctx:
+                net = gk.UNet(ctx, 32, seed=3)
+                d_in = torch.from_numpy(x).cuda()
+                d_out = torch.empty(h * w * 3, device="cuda")
+                net.infer(d_in.data_ptr(), d_out.data_ptr(), h, w)
+                ctx.sync()
+                outs.append(d_out.cpu().numpy().copy())
+        finally:
+            os.environ.pop("GKNX_UNET_GENERIC", None)
+    assert np.isfinite(outs[0]).all() and np.abs(outs[0]).max() > 0
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
