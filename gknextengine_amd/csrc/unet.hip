@@ -26,6 +26,7 @@
 //   k_unpack / k_frame_input: elementwise and reductions
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -268,8 +269,19 @@ struct ConvS {
     const uint16_t* w;   // [COUT][KPAD]
     uint16_t* out;       // NHWC, COUT channels
     uint16_t* pool;      // NHWC at half resolution (POOL)
-    float* outf;         // float4 per pixel: channels 0..3 (OUTF)
+    float* outf;         // float4 per pixel: channels 0..3 (OUT 1)
+    // IN 1: FP32 features (vw x vh x 10); IN 2: the frame's targets (vw x vh), as k_frame_input
+    const float* feat;
+    const ushort4 *fd, *fs, *fa, *fn;
+    const float4* fh;
+    // OUT 2: linear radiance, 3 floats per pixel (vw x vh); OUT 3: the display image
+    float* orgb;
+    ushort4* odisp;
+    uint32_t vw, vh;     // valid (unpadded) image within the W x H tensor
+    float paper_white;
+    uint32_t hdr, linear;
     uint32_t W, H, tiles_y, relu;
+    uint32_t tiles;      // images x tile rows x tile columns (set by the launcher)
 };
 
 template <int CA, int CB, bool UP>
@@ -303,8 +315,481 @@ struct ConvShape {
     }
 };
 
-template <int CA, int CB, bool UP, int COUT, bool POOL, bool OUTF, bool WL>
+// Persistent: work-group b owns the contiguous run of tiles [b T / G, (b + 1) T / G) of the
+// T = images x tile rows x tile columns 16 x 16 tiles (G = gridDim.x), so one work-group's
+// consecutive tiles are neighbours and share their halo columns in its XCD's L2. WL: the
+// weight rows are copied to LDS once per work-group (16 bytes of padding per row, so that
+// the 16 rows of an A operand fall in distinct banks). DB: two input buffers; the next
+// tile's global loads are issued into registers before the current tile's K loop and
+// written to the other buffer after its epilogue, so their latency overlaps the MFMAs and
+// the stores. LDS: [weights][buffer 0][buffer 1], a buffer holding an input tile and, after
+// the K loop, the output tile.
+// IN: 0 the BF16 tensor a, 1 / 2 the network input computed from FP32 features / the frame's
+// targets (first layer); OUT: 0 BF16 tensor, 1 float4 per pixel, 2 linear radiance (k_unpack),
+// 3 the display image (k_unpack_display) (last layer)
+template <int CA, int CB, bool UP, int COUT, bool POOL, int OUT, bool WL, int IN, bool DB>
+struct ConvP {
+    using S = ConvShape<CA, CB, UP>;
+    static constexpr int MT = COUT / 16;
+    static constexpr int KQ = S::KPAD / 8, WST = KQ + 1;
+    static constexpr int WUNITS = WL ? COUT * WST : 0;
+    static constexpr int OUNITS = OUT != 0 ? 256 : 256 * COUT / 8;
+    static constexpr int BUFU = S::UNITS > OUNITS ? S::UNITS : OUNITS;
+    static constexpr int LDSU = WUNITS + (DB ? 2 : 1) * BUFU;
+    // registers per thread holding the next tile: 16-byte units, or (IN != 0) pixels
+    static constexpr int NPRE = IN != 0 ? (kPix + 255) / 256 : (S::UNITS + 255) / 256;
+    struct Raw {  // a computed-input pixel's sources
+        float v[10];
+        bool in;
+    };
+};
+
+template <int CA, int CB, bool UP, int COUT, bool POOL, int OUT, bool WL, int IN, bool DB>
 __global__ __launch_bounds__(256) void k_convs(ConvS K) {
+    using P = ConvP<CA, CB, UP, COUT, POOL, OUT, WL, IN, DB>;
+    using S = typename P::S;
+    constexpr int MT = P::MT, KQ = P::KQ, WST = P::WST;
+    constexpr bool OUTF = OUT != 0;
+    extern __shared__ uint4 lds[];
+    uint4* const wts = lds;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int gi = lane >> 4, x = lane & 15;
+    const int W = int(K.W), H = int(K.H);
+    const int ntx = (W + 15) >> 4, nty = int(K.tiles_y);
+    const int T = int(K.tiles);
+    const int t_begin = int((int64_t(blockIdx.x) * T) / gridDim.x);
+    const int t_end = int((int64_t(blockIdx.x + 1) * T) / gridDim.x);
+    if (t_begin >= t_end) return;
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2v __attribute__((ext_vector_type(2)));
+
+    // ---- next-tile loads into registers (zero outside the image)
+    uint4 pre[DB && IN == 0 ? P::NPRE : 1];           // DB: the next tile's units
+    typename P::Raw raw[IN != 0 && DB ? P::NPRE : 1];  // DB, computed input: sources, converted when stored
+    // a computed-input pixel's 16 channels as two 16-byte units of BF16
+    auto pack_input = [&](const typename P::Raw& r, uint4& lo, uint4& hi) {
+        uint32_t q[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+        if (r.in) {
+            float f[16];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) f[c] = log1pf(fmaxf(r.v[c], 0.f));
+#pragma unroll
+            for (int c = 3; c < 9; ++c) f[c] = r.v[c];
+            f[9] = IN == 2 ? log1pf(fminf(fmaxf(r.v[9], 0.f), 1e4f)) * 0.1f : r.v[9];
+#pragma unroll
+            for (int c = 10; c < 16; ++c) f[c] = 0.f;
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                q[e] = __builtin_bit_cast(uint32_t, __builtin_convertvector((f2v){f[2 * e], f[2 * e + 1]}, b2v));
+        }
+        lo = make_uint4(q[0], q[1], q[2], q[3]);
+        hi = make_uint4(q[4], q[5], q[6], q[7]);
+    };
+    auto tile_xy = [&](int t, int& img, int& bx0, int& by0) {
+        img = t / (ntx * nty);
+        const int r = t - img * ntx * nty;
+        by0 = (r / ntx) * 16;
+        bx0 = (r - (r / ntx) * ntx) * 16;
+    };
+    auto load_tile = [&](int t) {
+        int img, bx0, by0;
+        tile_xy(t, img, bx0, by0);
+        const size_t px0 = size_t(img) * H * W;
+        if constexpr (IN != 0) {
+#pragma unroll
+            for (int k = 0; k < P::NPRE; ++k) {
+                const int pix = int(threadIdx.x) + 256 * k;
+                const int r = pix / kHalo, c = pix - (pix / kHalo) * kHalo;
+                const int sy = by0 - 1 + r, sx = bx0 - 1 + c;
+                typename P::Raw& rk = raw[DB ? k : 0];
+                rk.in = pix < kPix && sx >= 0 && sx < W && sy >= 0 && sy < H && sx < int(K.vw) && sy < int(K.vh);
+                if (rk.in) {
+                    const size_t i = size_t(sy) * K.vw + sx;
+                    if (IN == 1) {
+                        const float2* f2 = reinterpret_cast<const float2*>(K.feat + i * 10);  // 8-byte aligned
+#pragma unroll
+                        for (int e = 0; e < 5; ++e) {
+                            const float2 q = f2[e];
+                            rk.v[2 * e] = q.x;
+                            rk.v[2 * e + 1] = q.y;
+                        }
+                    } else {
+                        const float4 d = from_h4(K.fd[i]), sp = from_h4(K.fs[i]), al = from_h4(K.fa[i]),
+                                     nm = from_h4(K.fn[i]);
+                        // as k_frame_input; log1p of the depth at write time
+                        rk.v[0] = d.x * al.x + sp.x;
+                        rk.v[1] = d.y * al.y + sp.y;
+                        rk.v[2] = d.z * al.z + sp.z;
+                        rk.v[3] = al.x;
+                        rk.v[4] = al.y;
+                        rk.v[5] = al.z;
+                        rk.v[6] = nm.x;
+                        rk.v[7] = nm.y;
+                        rk.v[8] = nm.z;
+                        rk.v[9] = K.fh[i].x;
+                    }
+                }
+            }
+        } else {
+            const int ux0 = (bx0 >> 1) - 1, uy0 = (by0 >> 1) - 1;
+            const int Wh = W >> 1, Hh = H >> 1;
+            const size_t pxa0 = size_t(img) * Hh * Wh;
+#pragma unroll
+            for (int k = 0; k < P::NPRE; ++k) {
+                const int u = int(threadIdx.x) + 256 * k;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (u < kPix * S::NPF) {
+                    const int pix = u / S::NPF, g = u - (u / S::NPF) * S::NPF;
+                    const int r = pix / kHalo, c = pix - (pix / kHalo) * kHalo;
+                    const int sy = by0 - 1 + r, sx = bx0 - 1 + c;
+                    if (sx >= 0 && sx < W && sy >= 0 && sy < H) {
+                        const size_t p = px0 + size_t(sy) * W + sx;
+                        const int c8 = g * 8;
+                        const uint16_t* src =
+                            UP ? K.b + p * CB + c8 : (c8 < CA ? K.a + p * CA + c8 : K.b + p * CB + (c8 - CA));
+                        v = *reinterpret_cast<const uint4*>(src);
+                    }
+                } else if (UP && u < S::UNITS) {
+                    constexpr int NPU = S::NPU > 0 ? S::NPU : 1;
+                    const int q = u - kPix * S::NPF;
+                    const int pix = q / NPU, g = q - (q / NPU) * NPU;
+                    const int r = pix / kHaloUp, c = pix - (pix / kHaloUp) * kHaloUp;
+                    const int uy = uy0 + r, ux = ux0 + c;
+                    if (ux >= 0 && ux < Wh && uy >= 0 && uy < Hh)
+                        v = *reinterpret_cast<const uint4*>(K.a + (pxa0 + size_t(uy) * Wh + ux) * CA + g * 8);
+                }
+                pre[DB ? k : 0] = v;
+            }
+        }
+    };
+    // ---- the loaded tile into an LDS buffer (planes of 16-byte units per 8-channel group)
+    auto store_tile = [&](uint4* buf) {
+        if constexpr (IN != 0) {
+#pragma unroll
+            for (int k = 0; k < P::NPRE; ++k) {
+                const int pix = int(threadIdx.x) + 256 * k;
+                if (pix < kPix) {
+                    uint4 lo, hi;
+                    pack_input(raw[DB ? k : 0], lo, hi);
+                    buf[pix] = lo;
+                    buf[kPix + pix] = hi;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < P::NPRE; ++k) {
+                const int u = int(threadIdx.x) + 256 * k;
+                if (u < kPix * S::NPF) {
+                    const int pix = u / S::NPF, g = u - (u / S::NPF) * S::NPF;
+                    buf[g * kPix + pix] = pre[DB ? k : 0];
+                } else if (UP && u < S::UNITS) {
+                    constexpr int NPU = S::NPU > 0 ? S::NPU : 1;
+                    const int q = u - kPix * S::NPF;
+                    const int pix = q / NPU, g = q - (q / NPU) * NPU;
+                    buf[S::UBASE + g * kPixUp + pix] = pre[DB ? k : 0];
+                }
+            }
+        }
+    };
+
+    // ---- !DB: a tile straight into an LDS buffer, unit by unit
+    auto stage_direct = [&](int t, uint4* buf) {
+        int img, bx0, by0;
+        tile_xy(t, img, bx0, by0);
+        const size_t px0 = size_t(img) * H * W;
+        if constexpr (IN != 0) {
+            for (int pix = int(threadIdx.x); pix < kPix; pix += 256) {
+                const int r = pix / kHalo, c = pix - (pix / kHalo) * kHalo;
+                const int sy = by0 - 1 + r, sx = bx0 - 1 + c;
+                typename P::Raw rk;
+                rk.in = sx >= 0 && sx < W && sy >= 0 && sy < H && sx < int(K.vw) && sy < int(K.vh);
+                if (rk.in) {
+                    const size_t i = size_t(sy) * K.vw + sx;
+                    if (IN == 1) {
+                        const float2* f2 = reinterpret_cast<const float2*>(K.feat + i * 10);
+#pragma unroll
+                        for (int e = 0; e < 5; ++e) {
+                            const float2 q = f2[e];
+                            rk.v[2 * e] = q.x;
+                            rk.v[2 * e + 1] = q.y;
+                        }
+                    } else {
+                        const float4 d = from_h4(K.fd[i]), sp = from_h4(K.fs[i]), al = from_h4(K.fa[i]),
+                                     nm = from_h4(K.fn[i]);
+                        rk.v[0] = d.x * al.x + sp.x;
+                        rk.v[1] = d.y * al.y + sp.y;
+                        rk.v[2] = d.z * al.z + sp.z;
+                        rk.v[3] = al.x;
+                        rk.v[4] = al.y;
+                        rk.v[5] = al.z;
+                        rk.v[6] = nm.x;
+                        rk.v[7] = nm.y;
+                        rk.v[8] = nm.z;
+                        rk.v[9] = K.fh[i].x;
+                    }
+                }
+                uint4 lo, hi;
+                pack_input(rk, lo, hi);
+                buf[pix] = lo;
+                buf[kPix + pix] = hi;
+            }
+        } else {
+            for (int u = int(threadIdx.x); u < kPix * S::NPF; u += 256) {
+                const int pix = u / S::NPF, g = u - (u / S::NPF) * S::NPF;
+                const int r = pix / kHalo, c = pix - (pix / kHalo) * kHalo;
+                const int sy = by0 - 1 + r, sx = bx0 - 1 + c;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (sx >= 0 && sx < W && sy >= 0 && sy < H) {
+                    const size_t p = px0 + size_t(sy) * W + sx;
+                    const int c8 = g * 8;
+                    const uint16_t* src = UP ? K.b + p * CB + c8 : (c8 < CA ? K.a + p * CA + c8 : K.b + p * CB + (c8 - CA));
+                    v = *reinterpret_cast<const uint4*>(src);
+                }
+                buf[g * kPix + pix] = v;
+            }
+            if constexpr (UP) {
+                const int Wh = W >> 1, Hh = H >> 1;
+                const int ux0 = (bx0 >> 1) - 1, uy0 = (by0 >> 1) - 1;
+                const size_t pxa0 = size_t(img) * Hh * Wh;
+                constexpr int NPU = S::NPU > 0 ? S::NPU : 1;
+                for (int u = int(threadIdx.x); u < kPixUp * S::NPU; u += 256) {
+                    const int pix = u / NPU, g = u - (u / NPU) * NPU;
+                    const int r = pix / kHaloUp, c = pix - (pix / kHaloUp) * kHaloUp;
+                    const int uy = uy0 + r, ux = ux0 + c;
+                    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                    if (ux >= 0 && ux < Wh && uy >= 0 && uy < Hh)
+                        v = *reinterpret_cast<const uint4*>(K.a + (pxa0 + size_t(uy) * Wh + ux) * CA + g * 8);
+                    buf[S::UBASE + g * kPixUp + pix] = v;
+                }
+            }
+        }
+    };
+
+    // ---- prologue: weights and the first tile
+    if constexpr (WL) {
+        for (int u = int(threadIdx.x); u < COUT * KQ; u += 256) {
+            const int row = u / KQ, col = u - (u / KQ) * KQ;
+            wts[row * WST + col] = *reinterpret_cast<const uint4*>(K.w + size_t(row) * S::KPAD + col * 8);
+        }
+    }
+    uint4* buf_a = lds + P::WUNITS;
+    uint4* buf_b = DB ? buf_a + P::BUFU : buf_a;
+    if (DB) {
+        load_tile(t_begin);
+        store_tile(buf_a);
+    } else {
+        stage_direct(t_begin, buf_a);
+    }
+    __syncthreads();
+
+    // ---- per-lane operand bases
+    const int fb0 = wv * 4 * kHalo + x;  // full-resolution pixel of row 0
+    // half-resolution column of the pixel for dx = -1, 0, 1: floor((x + dx) / 2) + 1, plus
+    // the wave's first half-resolution row (2 wv rows of 10)
+    const int u0 = 2 * wv * kHaloUp + ((x + 1) >> 1), u1 = 2 * wv * kHaloUp + ((x + 2) >> 1),
+              u2 = 2 * wv * kHaloUp + ((x + 3) >> 1);
+    auto ub0 = [&](int d) { return d == 0 ? u0 : (d == 1 ? u1 : u2); };
+    const int fbg = fb0 + gi * kPix;
+    auto ubg = [&](int d) { return ub0(d) + gi * kPixUp; };
+    const uint16_t* wl = K.w + size_t(x) * S::KPAD + gi * 8;
+
+    for (int t = t_begin; t < t_end; ++t) {
+        uint4* const tile = buf_a;
+        const bool more = t + 1 < t_end;
+        if (DB && more) load_tile(t + 1);
+        f32x4 acc[MT][4];
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < S::NSTEP; ++s) {
+            bf16x8 af[MT];
+#pragma unroll
+            for (int i = 0; i < MT; ++i)
+                af[i] = WL ? __builtin_bit_cast(bf16x8, wts[(i * 16 + x) * WST + s * 4 + gi])
+                           : ld8(wl + i * 16 * S::KPAD + s * 32);
+            bf16x8 bfr[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const typename S::Op q0 = S::op(32 * s, j), q1 = S::op(32 * s + 8, j);
+                const typename S::Op q2 = S::op(32 * s + 16, j), q3 = S::op(32 * s + 24, j);
+                const int psz = q0.kind == 0 ? kPix : kPixUp;
+                const bool uniform = q0.kind <= 1 && q1.kind == q0.kind && q2.kind == q0.kind && q3.kind == q0.kind &&
+                                     q1.dxi == q0.dxi && q2.dxi == q0.dxi && q3.dxi == q0.dxi &&
+                                     q1.off == q0.off + psz && q2.off == q0.off + 2 * psz && q3.off == q0.off + 3 * psz;
+                if (uniform) {
+                    const int off = q0.off + (q0.kind == 0 ? fbg : ubg(q0.dxi));
+                    bfr[j] = __builtin_bit_cast(bf16x8, tile[off]);
+                } else {
+                    auto lo = [&](const typename S::Op& q) {
+                        return q.kind == 0 ? q.off + fb0 : (q.kind == 1 ? q.off + ub0(q.dxi) : 0);
+                    };
+                    const int off = gi == 0 ? lo(q0) : gi == 1 ? lo(q1) : gi == 2 ? lo(q2) : lo(q3);
+                    const int kind = gi == 0 ? q0.kind : gi == 1 ? q1.kind : gi == 2 ? q2.kind : q3.kind;
+                    bf16x8 v = __builtin_bit_cast(bf16x8, tile[off]);
+                    if (q0.kind >= 2 || q1.kind >= 2 || q2.kind >= 2 || q3.kind >= 2) {
+                        if (kind == 2) v = one8();
+                        else if (kind == 3) v = zero8();
+                    }
+                    bfr[j] = v;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < MT; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        }
+        // ---- epilogue: the output tile goes through the buffer ([pixel][channel]) so that
+        // the global stores are whole 16-byte units of consecutive pixels (a lane's own 4
+        // channels of one pixel per store left each cache line half written per instruction)
+        int img, bx0, by0;
+        tile_xy(t, img, bx0, by0);
+        const size_t px0 = size_t(img) * H * W;
+        __syncthreads();  // every wave is done reading the input tile
+        uint16_t* otile = reinterpret_cast<uint16_t*>(tile);
+        float4* ftile = reinterpret_cast<float4*>(tile);
+#pragma unroll
+        for (int i = 0; i < MT; ++i) {
+            const int co = i * 16 + gi * 4;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float v0 = acc[i][j][0], v1 = acc[i][j][1], v2 = acc[i][j][2], v3 = acc[i][j][3];
+                if (K.relu) {
+                    v0 = fmaxf(v0, 0.f);
+                    v1 = fmaxf(v1, 0.f);
+                    v2 = fmaxf(v2, 0.f);
+                    v3 = fmaxf(v3, 0.f);
+                }
+                const int p = (wv * 4 + j) * 16 + x;
+                if (OUTF) {
+                    if (co == 0) ftile[p] = make_float4(v0, v1, v2, v3);
+                } else {
+                    const b2v p01 = __builtin_convertvector((f2v){v0, v1}, b2v);
+                    const b2v p23 = __builtin_convertvector((f2v){v2, v3}, b2v);
+                    *reinterpret_cast<uint2*>(otile + p * COUT + co) =
+                        make_uint2(__builtin_bit_cast(uint32_t, p01), __builtin_bit_cast(uint32_t, p23));
+                }
+            }
+        }
+        __syncthreads();
+        constexpr int CU = OUTF ? 1 : COUT / 8;  // 16-byte units per pixel
+        for (int c = int(threadIdx.x); c < 256 * CU; c += 256) {
+            const int p = c / CU, g = c - (c / CU) * CU;
+            const int gy = by0 + (p >> 4), gx = bx0 + (p & 15);
+            if (gx < W && gy < H) {
+                const size_t px = px0 + size_t(gy) * W + gx;
+                if (OUT == 0) {
+                    *reinterpret_cast<uint4*>(K.out + px * COUT + g * 8) = tile[p * CU + g];
+                } else if (OUT == 1) {
+                    *reinterpret_cast<uint4*>(K.outf + px * 4) = tile[p];
+                } else if (gx < int(K.vw) && gy < int(K.vh)) {
+                    const float4 z = ftile[p];
+                    const float r = fmaxf(expm1f(z.x), 0.f), gg = fmaxf(expm1f(z.y), 0.f), bb = fmaxf(expm1f(z.z), 0.f);
+                    const size_t tt = size_t(gy) * K.vw + gx;
+                    if (OUT == 2) {
+                        K.orgb[3 * tt] = r;
+                        K.orgb[3 * tt + 1] = gg;
+                        K.orgb[3 * tt + 2] = bb;
+                    } else {
+                        K.odisp[tt] = K.linear ? to_h4(r, gg, bb, 1.f)
+                                               : to_h4(display_encode(r, K.paper_white, K.hdr != 0u),
+                                                       display_encode(gg, K.paper_white, K.hdr != 0u),
+                                                       display_encode(bb, K.paper_white, K.hdr != 0u), 1.f);
+                    }
+                }
+            }
+        }
+        if (POOL) {
+            // 2 x 2 average of the rounded outputs in k_pool's order: (y, x), (y, x + 1),
+            // (y + 1, x), (y + 1, x + 1)
+            const int Wh = W >> 1, Hh = H >> 1;
+            for (int c = int(threadIdx.x); c < 64 * CU; c += 256) {
+                const int q = c / CU, g = c - (c / CU) * CU;
+                const int qy = q >> 3, qx = q & 7;
+                const int p00 = (2 * qy) * 16 + 2 * qx;
+                const uint4 t4[4] = {tile[p00 * CU + g], tile[(p00 + 1) * CU + g], tile[(p00 + 16) * CU + g],
+                                     tile[(p00 + 17) * CU + g]};
+                float o[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    const uint32_t w4[4] = {t4[d].x, t4[d].y, t4[d].z, t4[d].w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        o[2 * e] += __uint_as_float(w4[e] << 16);
+                        o[2 * e + 1] += __uint_as_float(w4[e] & 0xffff0000u);
+                    }
+                }
+                uint32_t r4[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    r4[e] = __builtin_bit_cast(uint32_t,
+                                               __builtin_convertvector((f2v){0.25f * o[2 * e], 0.25f * o[2 * e + 1]}, b2v));
+                const int py = (by0 >> 1) + qy, pxh = (bx0 >> 1) + qx;
+                if (pxh < Wh && py < Hh) {
+                    const size_t pp = size_t(img) * Hh * Wh + size_t(py) * Wh + pxh;
+                    *reinterpret_cast<uint4*>(K.pool + pp * COUT + g * 8) = make_uint4(r4[0], r4[1], r4[2], r4[3]);
+                }
+            }
+        }
+        if (!more) break;
+        if (DB) {
+            // the other buffer is free (its last use ended before this tile's K loop)
+            store_tile(buf_b);
+        } else {
+            __syncthreads();  // the output tile's reads are done before the buffer is refilled
+            stage_direct(t + 1, buf_a);
+        }
+        __syncthreads();
+        if (DB) {
+            uint4* const sw = buf_a;
+            buf_a = buf_b;
+            buf_b = sw;
+        }
+    }
+}
+
+// the network's 16 input channels of pixel (x, y) of the valid image, computed from FP32
+// features (IN 1) or the frame's targets (IN 2) as k_frame_input and k_pack do:
+// log(1 + max(radiance, 0)), albedo, normal, depth feature, zeros
+template <int IN>
+__device__ inline void conv_input(const ConvS& K, int x, int y, float (&f)[16]) {
+    const size_t i = size_t(y) * K.vw + x;
+    float v[10];
+    if (IN == 1) {
+        const float2* f2 = reinterpret_cast<const float2*>(K.feat + i * 10);  // 8-byte aligned
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const float2 t = f2[c];
+            v[2 * c] = t.x;
+            v[2 * c + 1] = t.y;
+        }
+    } else {
+        const float4 d = from_h4(K.fd[i]), s = from_h4(K.fs[i]), a = from_h4(K.fa[i]), nm = from_h4(K.fn[i]);
+        v[0] = d.x * a.x + s.x;
+        v[1] = d.y * a.y + s.y;
+        v[2] = d.z * a.z + s.z;
+        v[3] = a.x;
+        v[4] = a.y;
+        v[5] = a.z;
+        v[6] = nm.x;
+        v[7] = nm.y;
+        v[8] = nm.z;
+        v[9] = log1pf(fminf(fmaxf(K.fh[i].x, 0.f), 1e4f)) * 0.1f;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[c] = log1pf(fmaxf(v[c], 0.f));
+#pragma unroll
+    for (int c = 3; c < 10; ++c) f[c] = v[c];
+#pragma unroll
+    for (int c = 10; c < 16; ++c) f[c] = 0.f;
+}
+
+// One tile per work-group (the layers where persistence does not pay, see launch_convs_t):
+// the same K loop and epilogue as k_convs, the input staged straight into LDS.
+template <int CA, int CB, bool UP, int COUT, bool POOL, bool OUTF, bool WL, int IN>
+__global__ __launch_bounds__(256) void k_convs1(ConvS K) {
     using S = ConvShape<CA, CB, UP>;
     constexpr int MT = COUT / 16;
     constexpr int KQ = S::KPAD / 8, WST = KQ + 1;  // WL: weight rows in LDS, 16 bytes of padding each
@@ -316,7 +801,26 @@ __global__ __launch_bounds__(256) void k_convs(ConvS K) {
     const int bx0 = int(blockIdx.x) * 16, by0 = ty * 16;
     const size_t px0 = size_t(img) * H * W;
     // ---- stage the input tile (planes of 16-byte units, zero outside the image)
-    {
+    if constexpr (IN != 0) {
+        // the network input of each pixel computed once, both of its planes written
+        static_assert(CA == 16 && CB == 0 && !UP, "computed input: first layer only");
+        typedef float f2i __attribute__((ext_vector_type(2)));
+        typedef __bf16 b2i __attribute__((ext_vector_type(2)));
+        for (int pix = int(threadIdx.x); pix < kPix; pix += 256) {
+            const int r = pix / kHalo, c = pix - (pix / kHalo) * kHalo;
+            const int sy = by0 - 1 + r, sx = bx0 - 1 + c;
+            uint32_t q[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+            if (sx >= 0 && sx < W && sy >= 0 && sy < H && sx < int(K.vw) && sy < int(K.vh)) {
+                float f[16];
+                conv_input<IN>(K, sx, sy, f);
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    q[e] = __builtin_bit_cast(uint32_t, __builtin_convertvector((f2i){f[2 * e], f[2 * e + 1]}, b2i));
+            }
+            tile[pix] = make_uint4(q[0], q[1], q[2], q[3]);
+            tile[kPix + pix] = make_uint4(q[4], q[5], q[6], q[7]);
+        }
+    } else {
         constexpr int NF = kPix * S::NPF;
         for (int u = int(threadIdx.x); u < NF; u += 256) {
             const int pix = u / S::NPF, g = u - (u / S::NPF) * S::NPF;
@@ -346,14 +850,14 @@ __global__ __launch_bounds__(256) void k_convs(ConvS K) {
                 tile[S::UBASE + g * kPixUp + pix] = v;
             }
         }
-        if constexpr (WL) {
-            for (int u = int(threadIdx.x); u < COUT * KQ; u += 256) {
-                const int row = u / KQ, col = u - (u / KQ) * KQ;
-                tile[S::UNITS + row * WST + col] = *reinterpret_cast<const uint4*>(K.w + size_t(row) * S::KPAD + col * 8);
-            }
-        }
-        __syncthreads();
     }
+    if constexpr (WL) {
+        for (int u = int(threadIdx.x); u < COUT * KQ; u += 256) {
+            const int row = u / KQ, col = u - (u / KQ) * KQ;
+            tile[S::UNITS + row * WST + col] = *reinterpret_cast<const uint4*>(K.w + size_t(row) * S::KPAD + col * 8);
+        }
+    }
+    __syncthreads();
     // ---- per-lane operand bases
     const int fb0 = wv * 4 * kHalo + x;  // full-resolution pixel of row 0
     // half-resolution column of the pixel for dx = -1, 0, 1: floor((x + dx) / 2) + 1, plus
@@ -1042,40 +1546,74 @@ int launch_conv(ConvK K, uint32_t nimg, hipStream_t st) {
     return GK_OK;
 }
 
-// weights go to LDS (once per work-group instead of once per wave from L2) while the
-// work-group's LDS stays within 64 KB (two or more work-groups per CU; the level-0 decoder
-// convolution's 78 KB ran at 237 us against 174 us with weights from L2)
-constexpr uint32_t kConvWlBytes = 64u * 1024u;
-template <int CA, int CB, bool UP, int COUT, bool POOL, bool OUTF, bool WL>
-int launch_convs_w(const ConvS& K, uint32_t nimg, uint32_t bytes, hipStream_t st) {
+// weights in LDS and double buffering while the work-group's LDS stays within 80 KB (two
+// work-groups per CU); otherwise weights from L2, or a single buffer
+constexpr uint32_t kConvLdsBudget = 80u * 1024u;
+template <int CA, int CB, bool UP, int COUT, bool POOL, int OUT, bool WL, int IN, bool DB>
+int launch_convs_w(ConvS K, uint32_t nimg, hipStream_t st, bool persistent) {
+    using P = ConvP<CA, CB, UP, COUT, POOL, OUT, WL, IN, DB>;
+    const uint32_t bytes = uint32_t(P::LDSU) * 16u;
     if (bytes > max_lds_per_block()) return GK_ERR_LIMIT;
-    const void* fn = reinterpret_cast<const void*>(&k_convs<CA, CB, UP, COUT, POOL, OUTF, WL>);
+    const void* fn = reinterpret_cast<const void*>(&k_convs<CA, CB, UP, COUT, POOL, OUT, WL, IN, DB>);
     if (bytes > 65536u) GK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
-    const dim3 grid((K.W + 15) / 16, K.tiles_y * nimg, 1);
-    k_convs<CA, CB, UP, COUT, POOL, OUTF, WL><<<grid, 256, bytes, st>>>(K);
+    int dev = 0, ncu = 0, per_cu = 0;
+    GK_HIP(hipGetDevice(&dev));
+    GK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    GK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, bytes));
+    K.tiles = uint32_t(nimg) * K.tiles_y * ((K.W + 15) / 16);
+    uint32_t G = uint32_t(std::max(1, ncu) * std::max(1, per_cu));
+    if (!persistent || G > K.tiles) G = K.tiles;
+    k_convs<CA, CB, UP, COUT, POOL, OUT, WL, IN, DB><<<G, 256, bytes, st>>>(K);
     GK_HIP(hipGetLastError());
     return GK_OK;
 }
-template <int CA, int CB, bool UP, int COUT, bool POOL, bool OUTF>
-int launch_convs_t(const ConvS& K, uint32_t nimg, hipStream_t st) {
+// mode 0: persistent, weights in LDS and double buffering (80 KB at most); mode 1: one tile
+// per work-group, weights read from L2; mode 2: one tile per work-group, weights in LDS.
+// Chosen per layer by measurement at 1920x1088 (the level-0 decoder convolution's 47 KB of
+// weights leave no room for two buffers and two work-groups per CU; the computed-input first
+// layer and the level-2 encoder convolution gain nothing from persistence)
+template <int CA, int CB, bool UP, int COUT, bool POOL, bool OUTF, bool WL, int IN>
+int launch_convs1(ConvS K, uint32_t nimg, hipStream_t st) {
     using S = ConvShape<CA, CB, UP>;
-    const uint32_t in_bytes = uint32_t(S::UNITS) * 16u;
+    const uint32_t in_bytes = uint32_t(S::UNITS) * 16u + (WL ? uint32_t(COUT) * uint32_t(S::KPAD / 8 + 1) * 16u : 0u);
     const uint32_t out_bytes = OUTF ? 256u * 16u : 256u * uint32_t(COUT) * 2u;
     const uint32_t bytes = in_bytes > out_bytes ? in_bytes : out_bytes;
-    const uint32_t w_bytes = uint32_t(COUT) * uint32_t(S::KPAD / 8 + 1) * 16u;
-    if (in_bytes + w_bytes <= kConvWlBytes) {
-        const uint32_t b2 = in_bytes + w_bytes > out_bytes ? in_bytes + w_bytes : out_bytes;
-        return launch_convs_w<CA, CB, UP, COUT, POOL, OUTF, true>(K, nimg, b2, st);
-    }
-    return launch_convs_w<CA, CB, UP, COUT, POOL, OUTF, false>(K, nimg, bytes, st);
+    if (bytes > max_lds_per_block()) return GK_ERR_LIMIT;
+    const void* fn = reinterpret_cast<const void*>(&k_convs1<CA, CB, UP, COUT, POOL, OUTF, WL, IN>);
+    if (bytes > 65536u) GK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+    const dim3 grid((K.W + 15) / 16, K.tiles_y * nimg, 1);
+    k_convs1<CA, CB, UP, COUT, POOL, OUTF, WL, IN><<<grid, 256, bytes, st>>>(K);
+    GK_HIP(hipGetLastError());
+    return GK_OK;
+}
+template <int CA, int CB, bool UP, int COUT, bool POOL, int OUT, int IN = 0>
+int launch_convs_t(const ConvS& K, uint32_t nimg, hipStream_t st, int mode = 0) {
+    static_assert(OUT <= 1 || IN == 0, "");
+    if (mode == 1) return launch_convs1<CA, CB, UP, COUT, POOL, (OUT != 0), false, IN>(K, nimg, st);
+    if (mode == 2) return launch_convs1<CA, CB, UP, COUT, POOL, (OUT != 0), true, IN>(K, nimg, st);
+    if (ConvP<CA, CB, UP, COUT, POOL, OUT, true, IN, true>::LDSU * 16u <= kConvLdsBudget)
+        return launch_convs_w<CA, CB, UP, COUT, POOL, OUT, true, IN, true>(K, nimg, st, true);
+    return launch_convs_w<CA, CB, UP, COUT, POOL, OUT, false, IN, true>(K, nimg, st, true);
 }
 
 // the forward convolution of layer l through its fixed-shape kernel when one is built for
 // its channel counts (the 32-channel network), with the following 2 x 2 pool fused when
 // pool is set; kNoShape otherwise (the caller falls back to k_conv)
 constexpr int kNoShape = 1;
-int launch_convs(const gk_unet_layer& L, ConvS K, uint32_t nimg, bool pool, hipStream_t st) {
+int launch_convs(const gk_unet_layer& L, ConvS K, uint32_t nimg, bool pool, int in_mode, int out_mode,
+                 hipStream_t st) {
     K.tiles_y = (K.H + 15) / 16;
+    if (in_mode != 0 || out_mode >= 2) {
+        if (nimg != 1) return kNoShape;
+        if (in_mode != 0) {
+            if (!(L.cin_a == 16 && L.cin_b == 0 && !L.up && L.cout == 32 && !pool)) return kNoShape;
+            return in_mode == 1 ? launch_convs_t<16, 0, false, 32, false, 0, 1>(K, nimg, st, 2)
+                                : launch_convs_t<16, 0, false, 32, false, 0, 2>(K, nimg, st, 2);
+        }
+        if (!(L.cin_a == 32 && L.cin_b == 0 && !L.up && L.cout == 16 && !pool)) return kNoShape;
+        return out_mode == 2 ? launch_convs_t<32, 0, false, 16, false, 2>(K, nimg, st)
+                             : launch_convs_t<32, 0, false, 16, false, 3>(K, nimg, st);
+    }
     const bool outf = K.outf != nullptr;
     const uint32_t key[6] = {L.cin_a, L.cin_b, L.up, L.cout, pool ? 1u : 0u, outf ? 1u : 0u};
     auto is = [&](uint32_t a, uint32_t b, uint32_t up, uint32_t co, uint32_t pl, uint32_t of) {
@@ -1084,13 +1622,13 @@ int launch_convs(const gk_unet_layer& L, ConvS K, uint32_t nimg, bool pool, hipS
     // built where they beat k_conv (+ k_pool) at 1920x1088: the coarse levels' wide layers
     // (level 3-4, and the decoder's 112- to 224-channel inputs) run faster through k_conv's
     // rolled K loop, whose registers leave room for more waves
-    if (is(16, 0, 0, 32, 0, 0)) return launch_convs_t<16, 0, false, 32, false, false>(K, nimg, st);
-    if (is(32, 0, 0, 32, 1, 0)) return launch_convs_t<32, 0, false, 32, true, false>(K, nimg, st);
-    if (is(32, 0, 0, 48, 1, 0)) return launch_convs_t<32, 0, false, 48, true, false>(K, nimg, st);
-    if (is(48, 0, 0, 64, 1, 0)) return launch_convs_t<48, 0, false, 64, true, false>(K, nimg, st);
-    if (is(48, 32, 1, 32, 0, 0)) return launch_convs_t<48, 32, true, 32, false, false>(K, nimg, st);
-    if (is(32, 0, 0, 32, 0, 0)) return launch_convs_t<32, 0, false, 32, false, false>(K, nimg, st);
-    if (is(32, 0, 0, 16, 0, 1)) return launch_convs_t<32, 0, false, 16, false, true>(K, nimg, st);
+    if (is(16, 0, 0, 32, 0, 0)) return launch_convs_t<16, 0, false, 32, false, 0>(K, nimg, st, 2);
+    if (is(32, 0, 0, 32, 1, 0)) return launch_convs_t<32, 0, false, 32, true, 0>(K, nimg, st);
+    if (is(32, 0, 0, 48, 1, 0)) return launch_convs_t<32, 0, false, 48, true, 0>(K, nimg, st);
+    if (is(48, 0, 0, 64, 1, 0)) return launch_convs_t<48, 0, false, 64, true, 0>(K, nimg, st, 1);
+    if (is(48, 32, 1, 32, 0, 0)) return launch_convs_t<48, 32, true, 32, false, 0>(K, nimg, st, 1);
+    if (is(32, 0, 0, 32, 0, 0)) return launch_convs_t<32, 0, false, 32, false, 0>(K, nimg, st);
+    if (is(32, 0, 0, 16, 0, 1)) return launch_convs_t<32, 0, false, 16, false, 1>(K, nimg, st);
     return kNoShape;
 }
 
@@ -1122,13 +1660,64 @@ int launch_wgrad(WgradK K, hipStream_t st) {
 const uint16_t* tp(gk_unet& n, int i) { return i == kNone ? nullptr : n.t[i].ptr; }
 
 // forward pass over X (already packed) at n x h x w; output in y32
-int forward(gk_unet& n, uint32_t nimg, uint32_t h, uint32_t w) {
+// where the network's input comes from and its output goes: by default the packed tensor
+// T_X and y32; the inference paths instead compute the input inside the first convolution
+// (from FP32 features or the frame's targets) and write the radiance or the display image
+// from the last one (fused_io)
+struct FwdIO {
+    int in_mode = 0;                  // 0 T_X, 1 feat, 2 frame targets
+    int out_mode = 1;                 // 1 y32, 2 orgb, 3 odisp
+    const float* feat = nullptr;
+    const ushort4 *fd = nullptr, *fs = nullptr, *fa = nullptr, *fn = nullptr;
+    const float4* fh = nullptr;
+    float* orgb = nullptr;
+    ushort4* odisp = nullptr;
+    uint32_t vw = 0, vh = 0;
+    float paper_white = 0.f;
+    uint32_t hdr = 0, linear = 0;
+};
+
+// the fused first and last layers exist for this network (32 channels, one image)
+bool fused_io(const gk_unet& n, uint32_t nimg) {
+    const gk_unet_layer &a = n.L[0], &z = n.L[kLayers - 1];
+    return n.fixed_shapes && nimg == 1 && a.cin_a == 16 && a.cin_b == 0 && a.cout == 32 && z.cin_a == 32 &&
+           z.cin_b == 0 && z.cout == 16;
+}
+
+int forward(gk_unet& n, uint32_t nimg, uint32_t h, uint32_t w, const FwdIO& io = FwdIO{}) {
     hipStream_t st = n.ctx->stream;
     for (uint32_t l = 0; l < kLayers; ++l) {
         const gk_unet_layer& L = n.L[l];
         const LayerSpec& S = kSpec[l];
         const bool pool = l >= 1 && l <= 4;  // pool the skip output to the next level
         const uint32_t lw = w >> L.level, lh = h >> L.level;
+        const int in_mode = l == 0 ? io.in_mode : 0;
+        const int out_mode = l + 1 == kLayers ? io.out_mode : 0;
+        if (in_mode != 0 || out_mode >= 2) {  // fused_io() checked by the caller
+            ConvS Q{};
+            Q.a = tp(n, S.a);
+            Q.w = n.w16.ptr + L.offset;
+            Q.W = lw;
+            Q.H = lh;
+            Q.relu = L.relu;
+            Q.out = S.out == kNone ? nullptr : n.t[S.out].ptr;
+            Q.feat = io.feat;
+            Q.fd = io.fd;
+            Q.fs = io.fs;
+            Q.fa = io.fa;
+            Q.fn = io.fn;
+            Q.fh = io.fh;
+            Q.orgb = io.orgb;
+            Q.odisp = io.odisp;
+            Q.vw = io.vw;
+            Q.vh = io.vh;
+            Q.paper_white = io.paper_white;
+            Q.hdr = io.hdr;
+            Q.linear = io.linear;
+            const int s = launch_convs(L, Q, nimg, pool, in_mode, out_mode, st);
+            if (s != GK_OK) return s == kNoShape ? GK_ERR_STATE : s;
+            continue;
+        }
         if (n.fixed_shapes && (!(pool || L.up) || ((lw % 2u) == 0u && (lh % 2u) == 0u))) {
             ConvS Q{};
             Q.a = tp(n, S.a);
@@ -1140,7 +1729,7 @@ int forward(gk_unet& n, uint32_t nimg, uint32_t h, uint32_t w) {
             if (S.out == kNone) Q.outf = n.y32.ptr;
             else Q.out = n.t[S.out].ptr;
             if (pool) Q.pool = n.t[kSpec[l + 1].a].ptr;
-            const int s = launch_convs(L, Q, nimg, pool, st);
+            const int s = launch_convs(L, Q, nimg, pool, 0, 0, st);
             if (s == GK_OK) continue;
             if (s != kNoShape) return s;
         }
@@ -1290,6 +1879,23 @@ int unet_denoise_frame(Ctx& c) {
     const uint32_t wp = (W + 15u) & ~15u, hp = (H + 15u) & ~15u;
     int s = ensure_ws(n, 1, hp, wp, false);
     if (s != GK_OK) return s;
+    if (fused_io(n, 1)) {  // features computed in the first convolution, display from the last
+        FwdIO io;
+        io.in_mode = 2;
+        io.out_mode = 3;
+        io.fd = static_cast<const ushort4*>(c.target_ptr[GK_RT_ACCUM_DIFFUSE]);
+        io.fs = static_cast<const ushort4*>(c.target_ptr[GK_RT_ACCUM_SPECULAR]);
+        io.fa = static_cast<const ushort4*>(c.target_ptr[GK_RT_ACCUM_ALBEDO]);
+        io.fn = static_cast<const ushort4*>(c.target_ptr[GK_RT_NORMAL]);
+        io.fh = static_cast<const float4*>(c.target_ptr[GK_RT_PRIMARY_HIT]);
+        io.odisp = static_cast<ushort4*>(c.target_ptr[GK_RT_DENOISED]);
+        io.vw = W;
+        io.vh = H;
+        io.paper_white = c.params.paper_white_nit;
+        io.hdr = c.params.hdr ? 1u : 0u;
+        io.linear = (c.params.flags & GK_FRAME_LINEAR_OUTPUT) ? 1u : 0u;
+        return forward(n, 1, hp, wp, io);
+    }
     if (n.feat.count < size_t(n_px) * kIn) GK_HIP(n.feat.resize(size_t(n_px) * kIn));
     k_frame_input<<<nblk(n_px), 256, 0, c.stream>>>(
         static_cast<const ushort4*>(c.target_ptr[GK_RT_ACCUM_DIFFUSE]),
@@ -1480,6 +2086,16 @@ int gk_unet_infer(gk_unet* net, const float* d_input, float* d_output, uint32_t 
     const uint32_t hp = (h + 15u) & ~15u, wp = (w + 15u) & ~15u;
     int s = ensure_ws(*net, 1, hp, wp, false);
     if (s != GK_OK) return s;
+    if (fused_io(*net, 1)) {  // input computed in the first convolution, radiance from the last
+        FwdIO io;
+        io.in_mode = 1;
+        io.out_mode = 2;
+        io.feat = d_input;
+        io.orgb = d_output;
+        io.vw = w;
+        io.vh = h;
+        return forward(*net, 1, hp, wp, io);
+    }
     if ((s = pack_input(*net, d_input, 1, h, w, hp, wp)) != GK_OK) return s;
     if ((s = forward(*net, 1, hp, wp)) != GK_OK) return s;
     k_unpack<<<nblk(size_t(h) * w), 256, 0, net->ctx->stream>>>(net->y32.ptr, d_output, 1, h, w, hp, wp);
