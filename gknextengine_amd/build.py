@@ -36,13 +36,18 @@ COMMON = [
 ]
 
 
+# per-source additions: the U-Net's fixed-shape convolutions unroll K loops of up to 64
+# steps (each step's operand addresses are compile-time constants only when unrolled)
+EXTRA = {"unet.hip": ["-mllvm", "-pragma-unroll-threshold=262144"]}
+
+
 def sources() -> list[Path]:
     return sorted([*CSRC.glob("*.hip"), *CSRC.glob("*.cpp")])
 
 
 def _compile(src: Path) -> Path:
     obj = BUILD_DIR / (src.stem + ".o")
-    cmd = [HIPCC, *COMMON, "-x", "hip", "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *COMMON, *EXTRA.get(src.name, []), "-x", "hip", "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stdout}\n{res.stderr}")

@@ -1524,14 +1524,21 @@ void launch_conv_t(const ConvK& K, uint32_t nimg, hipStream_t st) {
 }
 
 int launch_conv(ConvK K, uint32_t nimg, hipStream_t st) {
+    // the most output channels per work-group that still give at least 256 work-groups (the
+    // coarse levels have 40 to 135 pixel tiles: one work-group per tile left most CUs idle),
+    // unless the staged input tile is above 64 KB (then every extra work-group stages it
+    // again: the level-3 decoder convolution's 86 KB tile ran at 72 us split three ways
+    // against 58 us whole)
     const uint32_t nm = K.cout / 16;
+    K.tiles_y = (K.H + 15) / 16;
+    const uint32_t tiles = ((K.W + 15) / 16) * K.tiles_y * nimg;
+    const bool split = conv_tile_bytes(K.ca, K.cb, K.a_up) <= 65536u;
     int mt = 1;
     for (int c = 8; c >= 1; --c)
-        if (nm % uint32_t(c) == 0u) {
+        if (nm % uint32_t(c) == 0u && (!split || tiles * (nm / uint32_t(c)) >= 256u)) {
             mt = c;
             break;
         }
-    K.tiles_y = (K.H + 15) / 16;
     switch (mt) {
         case 8: launch_conv_t<8>(K, nimg, st); break;
         case 7: launch_conv_t<7>(K, nimg, st); break;
@@ -1591,6 +1598,7 @@ int launch_convs_t(const ConvS& K, uint32_t nimg, hipStream_t st, int mode = 0) 
     static_assert(OUT <= 1 || IN == 0, "");
     if (mode == 1) return launch_convs1<CA, CB, UP, COUT, POOL, (OUT != 0), false, IN>(K, nimg, st);
     if (mode == 2) return launch_convs1<CA, CB, UP, COUT, POOL, (OUT != 0), true, IN>(K, nimg, st);
+    if (mode == 3) return launch_convs_w<CA, CB, UP, COUT, POOL, OUT, true, IN, false>(K, nimg, st, true);
     if (ConvP<CA, CB, UP, COUT, POOL, OUT, true, IN, true>::LDSU * 16u <= kConvLdsBudget)
         return launch_convs_w<CA, CB, UP, COUT, POOL, OUT, true, IN, true>(K, nimg, st, true);
     return launch_convs_w<CA, CB, UP, COUT, POOL, OUT, false, IN, true>(K, nimg, st, true);
@@ -1627,6 +1635,7 @@ int launch_convs(const gk_unet_layer& L, ConvS K, uint32_t nimg, bool pool, int 
     if (is(32, 0, 0, 48, 1, 0)) return launch_convs_t<32, 0, false, 48, true, 0>(K, nimg, st);
     if (is(48, 0, 0, 64, 1, 0)) return launch_convs_t<48, 0, false, 64, true, 0>(K, nimg, st, 1);
     if (is(48, 32, 1, 32, 0, 0)) return launch_convs_t<48, 32, true, 32, false, 0>(K, nimg, st, 1);
+    if (is(64, 48, 1, 48, 0, 0)) return launch_convs_t<64, 48, true, 48, false, 0>(K, nimg, st, 1);
     if (is(32, 0, 0, 32, 0, 0)) return launch_convs_t<32, 0, false, 32, false, 0>(K, nimg, st);
     if (is(32, 0, 0, 16, 0, 1)) return launch_convs_t<32, 0, false, 16, false, 1>(K, nimg, st);
     return kNoShape;
