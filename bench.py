@@ -747,7 +747,7 @@ def run_denoise(args):
                        "resolution": [W, H], **{f"{k}_{m}": v for k, d in res.items() for m, v in d.items()}},
             "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
-                         "kernel": "the denoiser pass (12 k_conv MFMA convolutions, pooling, feature gather)",
+                         "kernel": "the denoiser pass (12 MFMA convolutions: feature gather, pooling and display conversion fused)",
                          "flop_per_frame": c["unet_gflop_per_frame"] * 1e9}}
 
 
