@@ -912,6 +912,81 @@ def run_copy(args):
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None}}
 
 
+def run_diff_timing(args):
+    """Config 4's step alone: forward + adjoint of the L2 loss at 512x512, 4 spp, 4 bounces,
+    256 materials x 5 parameters (run_diff adds the gradient checks)."""
+    import numpy as np
+    import torch
+
+    import gknextengine_amd as gk
+    from gknextengine_amd import diff, scenes
+    scene, nmat = scenes.cornell_tiles()
+    base = list(scene.materials)
+    cam = scenes.CORNELL_CAM
+    W = H = 512
+    p = gk.make_frame_params(cam["eye"], cam["target"], W, H, fov=cam["fov"])
+    p.estimator = gk.ESTIMATOR_PT_MIS
+    p.samples = 4
+    p.bounces = p.max_bounces = 4
+    p.total_frames = 1
+    p.global_seed = 3
+    ctx = gk.Context(W, H)
+    ctx.upload(scene)
+    theta = diff.params_of(base).cuda()
+    target = torch.zeros((H, W, 3), device="cuda")
+
+    def step():
+        t = theta.clone().requires_grad_(True)
+        img = diff.render(t, ctx, base, p)
+        ((img - target) ** 2).mean().backward()
+        return t.grad
+
+    for _ in range(2):
+        step()
+    times = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    ctx.close()
+    ms = float(np.median(times) * 1e3)
+    return {"metric": "forward + adjoint ms", "value": round(ms, 3), "unit": "ms", "steps": args.steps,
+            "ms_per_step": round(ms, 3), "higher_is_better": False,
+            "config": {"workload": "differentiable PT_MIS (config 4) step: render + L2 loss + path-replay adjoint",
+                       "resolution": [W, H], "spp": 4, "bounces": 4, "materials": nmat, "params_per_material": 5,
+                       "paths_per_step": W * H * 4}}
+
+
+def run_side_configs(args):
+    """The other configurations, measured in the same process after the headline one (a
+    few seconds each; reduced reference spp for the relMSE figures)."""
+    import copy
+
+    import torch
+    side = {}
+    specs = (("cornell", run_cornell, {"relmse": 1024}), ("dynamic", run_dynamic, {}),
+             ("denoise", run_denoise, {"relmse": 256}), ("diff", run_diff_timing, {"steps": 10}))
+    for name, fn, over in specs:
+        a = copy.copy(args)
+        a.config = name
+        a.no_cpu_baseline = True
+        for k, v in over.items():
+            setattr(a, k, v)
+        t0 = time.perf_counter()
+        try:
+            r = fn(a)
+            r.pop("cpu_baseline", None)
+            r["wall_s"] = round(time.perf_counter() - t0, 1)
+            side[name] = r
+        except Exception as e:  # a side measurement never hides the headline line
+            side[name] = {"error": f"{type(e).__name__}: {e}"}
+        torch.cuda.empty_cache()
+        log("side config", name, "done")
+    return side
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -931,15 +1006,20 @@ def main():
                     help="dynamic config: per-frame TLAS update")
     ap.add_argument("--per-frame", action="store_true", help="dynamic config: per-frame trace ms and TLAS SAH")
     ap.add_argument("--frames", type=int, default=120, help="dynamic config: frames after the first (profiling runs)")
+    ap.add_argument("--no-side", action="store_true",
+                    help="city config: skip the other configurations' figures (side_configs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (DESIGN.md describes how it would "
-              "shard).", file=sys.stderr)
+        print("bench.py measures one GPU: the path tracer has no multi-GPU implementation in this library.",
+              file=sys.stderr)
         sys.exit(2)
     run = {"city": run_city, "dynamic": run_dynamic, "cornell": run_cornell, "denoise": run_denoise,
            "diff": run_diff, "probes": run_probes, "output": run_output,
            "copy": run_copy}[args.config]
-    print(json.dumps(run(args)), flush=True)
+    res = run(args)
+    if args.config == "city" and not args.no_side:
+        res["side_configs"] = run_side_configs(args)
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":
