@@ -284,7 +284,8 @@ struct ConvS {
     uint32_t tiles;      // images x tile rows x tile columns (set by the launcher)
 };
 
-template <int CA, int CB, bool UP>
+// R: 16-pixel rows per wave (the work-group's tile is 16 x 4R pixels)
+template <int CA, int CB, bool UP, int R = 4>
 struct ConvShape {
     static constexpr int CIN = CA + CB;
     static constexpr int KTAP = 9 * CIN;
@@ -292,8 +293,10 @@ struct ConvShape {
     static constexpr int NSTEP = KPAD / 32;
     static constexpr int NPF = (UP ? CB : CIN) / 8;  // full-resolution planes
     static constexpr int NPU = UP ? CA / 8 : 0;      // half-resolution planes
-    static constexpr int UBASE = kPix * NPF;
-    static constexpr int UNITS = UBASE + kPixUp * NPU;
+    static constexpr int PIX = kHalo * (4 * R + 2);  // full-resolution plane: 18 x (4R + 2)
+    static constexpr int PIXU = kHaloUp * (2 * R + 2);  // half-resolution plane: 10 x (2R + 2)
+    static constexpr int UBASE = PIX * NPF;
+    static constexpr int UNITS = UBASE + PIXU * NPU;
     // operand of k-lane-group starting at k, row j of the wave: kind 0 full-resolution plane,
     // 1 half-resolution plane, 2 bias one, 3 zero; off = the address's constant part in
     // 16-byte units; dxi = dx + 1 (selects the lane's half-resolution column)
@@ -305,13 +308,13 @@ struct ConvShape {
         const int tap = k / CIN, ci = k - (k / CIN) * CIN;
         const int dy = tap / 3 - 1, dx = tap - (tap / 3) * 3 - 1;
         if (UP && ci < CA) {
-            // half-resolution row: floor((4 wv + j + dy) / 2) + 1 = 2 wv + floor((j + dy) / 2) + 1
+            // half-resolution row: floor((R wv + j + dy) / 2) + 1 = (R / 2) wv + floor((j + dy) / 2) + 1
             const int t = j + dy;
             const int fl = t >= 0 ? t / 2 : -1;
-            return Op{1, UBASE + (ci / 8) * kPixUp + (fl + 1) * kHaloUp, dx + 1};
+            return Op{1, UBASE + (ci / 8) * PIXU + (fl + 1) * kHaloUp, dx + 1};
         }
         const int plane = (UP ? ci - CA : ci) / 8;
-        return Op{0, plane * kPix + (j + dy + 1) * kHalo + (dx + 1), 0};
+        return Op{0, plane * PIX + (j + dy + 1) * kHalo + (dx + 1), 0};
     }
 };
 
@@ -788,9 +791,9 @@ __device__ inline void conv_input(const ConvS& K, int x, int y, float (&f)[16]) 
 
 // One tile per work-group (the layers where persistence does not pay, see launch_convs_t):
 // the same K loop and epilogue as k_convs, the input staged straight into LDS.
-template <int CA, int CB, bool UP, int COUT, bool POOL, bool OUTF, bool WL, int IN>
+template <int CA, int CB, bool UP, int COUT, bool POOL, bool OUTF, bool WL, int IN, int R>
 __global__ __launch_bounds__(256) void k_convs1(ConvS K) {
-    using S = ConvShape<CA, CB, UP>;
+    using S = ConvShape<CA, CB, UP, R>;
     constexpr int MT = COUT / 16;
     constexpr int KQ = S::KPAD / 8, WST = KQ + 1;  // WL: weight rows in LDS, 16 bytes of padding each
     extern __shared__ uint4 tile[];
@@ -798,7 +801,7 @@ __global__ __launch_bounds__(256) void k_convs1(ConvS K) {
     const int gi = lane >> 4, x = lane & 15;
     const int img = blockIdx.y / K.tiles_y, ty = blockIdx.y - img * K.tiles_y;
     const int W = int(K.W), H = int(K.H);
-    const int bx0 = int(blockIdx.x) * 16, by0 = ty * 16;
+    const int bx0 = int(blockIdx.x) * 16, by0 = ty * 4 * R;
     const size_t px0 = size_t(img) * H * W;
     // ---- stage the input tile (planes of 16-byte units, zero outside the image)
     if constexpr (IN != 0) {
@@ -806,7 +809,7 @@ __global__ __launch_bounds__(256) void k_convs1(ConvS K) {
         static_assert(CA == 16 && CB == 0 && !UP, "computed input: first layer only");
         typedef float f2i __attribute__((ext_vector_type(2)));
         typedef __bf16 b2i __attribute__((ext_vector_type(2)));
-        for (int pix = int(threadIdx.x); pix < kPix; pix += 256) {
+        for (int pix = int(threadIdx.x); pix < S::PIX; pix += 256) {
             const int r = pix / kHalo, c = pix - (pix / kHalo) * kHalo;
             const int sy = by0 - 1 + r, sx = bx0 - 1 + c;
             uint32_t q[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
@@ -818,10 +821,10 @@ __global__ __launch_bounds__(256) void k_convs1(ConvS K) {
                     q[e] = __builtin_bit_cast(uint32_t, __builtin_convertvector((f2i){f[2 * e], f[2 * e + 1]}, b2i));
             }
             tile[pix] = make_uint4(q[0], q[1], q[2], q[3]);
-            tile[kPix + pix] = make_uint4(q[4], q[5], q[6], q[7]);
+            tile[S::PIX + pix] = make_uint4(q[4], q[5], q[6], q[7]);
         }
     } else {
-        constexpr int NF = kPix * S::NPF;
+        constexpr int NF = S::PIX * S::NPF;
         for (int u = int(threadIdx.x); u < NF; u += 256) {
             const int pix = u / S::NPF, g = u - (u / S::NPF) * S::NPF;
             const int r = pix / kHalo, c = pix - (pix / kHalo) * kHalo;
@@ -833,13 +836,13 @@ __global__ __launch_bounds__(256) void k_convs1(ConvS K) {
                 const uint16_t* src = UP ? K.b + p * CB + c8 : (c8 < CA ? K.a + p * CA + c8 : K.b + p * CB + (c8 - CA));
                 v = *reinterpret_cast<const uint4*>(src);
             }
-            tile[g * kPix + pix] = v;
+            tile[g * S::PIX + pix] = v;
         }
         if constexpr (UP) {
             const int Wh = W >> 1, Hh = H >> 1;
             const int ux0 = (bx0 >> 1) - 1, uy0 = (by0 >> 1) - 1;
             const size_t pxa0 = size_t(img) * Hh * Wh;
-            constexpr int NU = kPixUp * S::NPU;
+            constexpr int NU = S::PIXU * S::NPU;
             for (int u = int(threadIdx.x); u < NU; u += 256) {
                 const int pix = u / S::NPU, g = u - (u / S::NPU) * S::NPU;
                 const int r = pix / kHaloUp, c = pix - (pix / kHaloUp) * kHaloUp;
@@ -847,7 +850,7 @@ __global__ __launch_bounds__(256) void k_convs1(ConvS K) {
                 uint4 v = make_uint4(0u, 0u, 0u, 0u);
                 if (ux >= 0 && ux < Wh && uy >= 0 && uy < Hh)
                     v = *reinterpret_cast<const uint4*>(K.a + (pxa0 + size_t(uy) * Wh + ux) * CA + g * 8);
-                tile[S::UBASE + g * kPixUp + pix] = v;
+                tile[S::UBASE + g * S::PIXU + pix] = v;
             }
         }
     }
@@ -859,20 +862,20 @@ __global__ __launch_bounds__(256) void k_convs1(ConvS K) {
     }
     __syncthreads();
     // ---- per-lane operand bases
-    const int fb0 = wv * 4 * kHalo + x;  // full-resolution pixel of row 0
+    const int fb0 = wv * R * kHalo + x;  // full-resolution pixel of row 0
     // half-resolution column of the pixel for dx = -1, 0, 1: floor((x + dx) / 2) + 1, plus
     // the wave's first half-resolution row (2 wv rows of 10)
-    const int u0 = 2 * wv * kHaloUp + ((x + 1) >> 1), u1 = 2 * wv * kHaloUp + ((x + 2) >> 1),
-              u2 = 2 * wv * kHaloUp + ((x + 3) >> 1);
+    const int u0 = (R / 2) * wv * kHaloUp + ((x + 1) >> 1), u1 = (R / 2) * wv * kHaloUp + ((x + 2) >> 1),
+              u2 = (R / 2) * wv * kHaloUp + ((x + 3) >> 1);
     auto ub0 = [&](int d) { return d == 0 ? u0 : (d == 1 ? u1 : u2); };
-    const int fbg = fb0 + gi * kPix;
-    auto ubg = [&](int d) { return ub0(d) + gi * kPixUp; };
+    const int fbg = fb0 + gi * S::PIX;
+    auto ubg = [&](int d) { return ub0(d) + gi * S::PIXU; };
     const uint16_t* wl = K.w + size_t(x) * S::KPAD + gi * 8;
-    f32x4 acc[MT][4];
+    f32x4 acc[MT][R];
 #pragma unroll
     for (int i = 0; i < MT; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < R; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int s = 0; s < S::NSTEP; ++s) {
         bf16x8 af[MT];
@@ -880,12 +883,12 @@ __global__ __launch_bounds__(256) void k_convs1(ConvS K) {
         for (int i = 0; i < MT; ++i)
             af[i] = WL ? __builtin_bit_cast(bf16x8, tile[S::UNITS + (i * 16 + x) * WST + s * 4 + gi])
                        : ld8(wl + i * 16 * S::KPAD + s * 32);
-        bf16x8 bfr[4];
+        bf16x8 bfr[R];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < R; ++j) {
             const typename S::Op q0 = S::op(32 * s, j), q1 = S::op(32 * s + 8, j);
             const typename S::Op q2 = S::op(32 * s + 16, j), q3 = S::op(32 * s + 24, j);
-            const int psz = q0.kind == 0 ? kPix : kPixUp;
+            const int psz = q0.kind == 0 ? S::PIX : S::PIXU;
             const bool uniform = q0.kind <= 1 && q1.kind == q0.kind && q2.kind == q0.kind && q3.kind == q0.kind &&
                                  q1.dxi == q0.dxi && q2.dxi == q0.dxi && q3.dxi == q0.dxi && q1.off == q0.off + psz &&
                                  q2.off == q0.off + 2 * psz && q3.off == q0.off + 3 * psz;
@@ -909,7 +912,7 @@ __global__ __launch_bounds__(256) void k_convs1(ConvS K) {
 #pragma unroll
         for (int i = 0; i < MT; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < R; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     }
     // ---- epilogue: the output tile goes through LDS ([pixel][channel], the input tile's
@@ -925,7 +928,7 @@ __global__ __launch_bounds__(256) void k_convs1(ConvS K) {
     for (int i = 0; i < MT; ++i) {
         const int co = i * 16 + gi * 4;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < R; ++j) {
             float v0 = acc[i][j][0], v1 = acc[i][j][1], v2 = acc[i][j][2], v3 = acc[i][j][3];
             if (K.relu) {
                 v0 = fmaxf(v0, 0.f);
@@ -933,7 +936,7 @@ __global__ __launch_bounds__(256) void k_convs1(ConvS K) {
                 v2 = fmaxf(v2, 0.f);
                 v3 = fmaxf(v3, 0.f);
             }
-            const int p = (wv * 4 + j) * 16 + x;
+            const int p = (wv * R + j) * 16 + x;
             if (OUTF) {
                 if (co == 0) ftile[p] = make_float4(v0, v1, v2, v3);
             } else {
@@ -946,7 +949,7 @@ __global__ __launch_bounds__(256) void k_convs1(ConvS K) {
     }
     __syncthreads();
     constexpr int CU = OUTF ? 1 : COUT / 8;  // 16-byte units per pixel
-    for (int c = int(threadIdx.x); c < 256 * CU; c += 256) {
+    for (int c = int(threadIdx.x); c < 64 * R * CU; c += 256) {
         const int p = c / CU, g = c - (c / CU) * CU;
         const int gy = by0 + (p >> 4), gx = bx0 + (p & 15);
         if (gx < W && gy < H) {
@@ -959,7 +962,7 @@ __global__ __launch_bounds__(256) void k_convs1(ConvS K) {
         // 2 x 2 average of the rounded outputs in k_pool's order: (y, x), (y, x + 1),
         // (y + 1, x), (y + 1, x + 1)
         const int Wh = W >> 1, Hh = H >> 1;
-        for (int c = int(threadIdx.x); c < 64 * CU; c += 256) {
+        for (int c = int(threadIdx.x); c < 16 * R * CU; c += 256) {
             const int q = c / CU, g = c - (c / CU) * CU;
             const int qy = q >> 3, qx = q & 7;
             const int p00 = (2 * qy) * 16 + 2 * qx;
@@ -1575,21 +1578,23 @@ int launch_convs_w(ConvS K, uint32_t nimg, hipStream_t st, bool persistent) {
     return GK_OK;
 }
 // mode 0: persistent, weights in LDS and double buffering (80 KB at most); mode 1: one tile
-// per work-group, weights read from L2; mode 2: one tile per work-group, weights in LDS.
+// per work-group, weights read from L2; mode 2: one tile per work-group, weights in LDS;
+// mode 3: persistent, weights in LDS, one buffer.
 // Chosen per layer by measurement at 1920x1088 (the level-0 decoder convolution's 47 KB of
 // weights leave no room for two buffers and two work-groups per CU; the computed-input first
 // layer and the level-2 encoder convolution gain nothing from persistence)
-template <int CA, int CB, bool UP, int COUT, bool POOL, bool OUTF, bool WL, int IN>
+template <int CA, int CB, bool UP, int COUT, bool POOL, bool OUTF, bool WL, int IN, int R = 4>
 int launch_convs1(ConvS K, uint32_t nimg, hipStream_t st) {
-    using S = ConvShape<CA, CB, UP>;
+    using S = ConvShape<CA, CB, UP, R>;
     const uint32_t in_bytes = uint32_t(S::UNITS) * 16u + (WL ? uint32_t(COUT) * uint32_t(S::KPAD / 8 + 1) * 16u : 0u);
-    const uint32_t out_bytes = OUTF ? 256u * 16u : 256u * uint32_t(COUT) * 2u;
+    const uint32_t out_bytes = 64u * R * (OUTF ? 16u : uint32_t(COUT) * 2u);
     const uint32_t bytes = in_bytes > out_bytes ? in_bytes : out_bytes;
     if (bytes > max_lds_per_block()) return GK_ERR_LIMIT;
-    const void* fn = reinterpret_cast<const void*>(&k_convs1<CA, CB, UP, COUT, POOL, OUTF, WL, IN>);
+    const void* fn = reinterpret_cast<const void*>(&k_convs1<CA, CB, UP, COUT, POOL, OUTF, WL, IN, R>);
     if (bytes > 65536u) GK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+    K.tiles_y = (K.H + 4u * R - 1u) / (4u * R);
     const dim3 grid((K.W + 15) / 16, K.tiles_y * nimg, 1);
-    k_convs1<CA, CB, UP, COUT, POOL, OUTF, WL, IN><<<grid, 256, bytes, st>>>(K);
+    k_convs1<CA, CB, UP, COUT, POOL, OUTF, WL, IN, R><<<grid, 256, bytes, st>>>(K);
     GK_HIP(hipGetLastError());
     return GK_OK;
 }
