@@ -479,7 +479,9 @@ def run_cornell(args):
 def denoise_compare(ctx, net, p, ref_spp, reps=20):
     """One 1-spp frame of the current view (TemporalFrames = 1) through the bilateral and the
     learned denoisers, both writing linear radiance; relMSE of the raw frame and of each
-    denoised image against an FP64 ref_spp render; post-pass milliseconds of each."""
+    denoised image against an FP64 ref_spp render; post-pass milliseconds of each. Also
+    both denoisers after 16 frames of temporal accumulation of the static view (keys
+    *_temporal16), the renderer's usual mode."""
     import numpy as np
 
     import gknextengine_amd as gk
@@ -500,6 +502,20 @@ def denoise_compare(ctx, net, p, ref_spp, reps=20):
         out[name] = ctx.read("DENOISED")[..., :3].astype(np.float64)
         rad = ctx.read("RADIANCE")
         out["noisy"] = rad[..., :3] / np.maximum(rad[..., 3:4], 1.0)
+    # the renderer's own temporal path: 16 frames of the static view (other seeds each frame)
+    # accumulated by reprojection with TemporalFrames = 16, then each denoiser
+    for name, den in (("bilateral", gk.DENOISER_BILATERAL), ("unet", gk.DENOISER_UNET)):
+        q16 = gk.FrameParams.from_buffer_copy(p)
+        q16.temporal_frames = 16
+        q16.flags |= gk.FRAME_LINEAR_OUTPUT
+        q16.denoiser = den
+        ctx.reset()
+        for f in range(16):
+            q16.total_frames = p.total_frames + f
+            ctx.frame(q16)
+            ctx.render()
+        ctx.sync()
+        out[f"{name}_temporal16"] = ctx.read("DENOISED")[..., :3].astype(np.float64)
     ref = reference_frame(ctx, q, ref_spp)
     res = {}
     for k, v in out.items():
