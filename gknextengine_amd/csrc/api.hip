@@ -361,6 +361,7 @@ int gk_destroy(gk_ctx* ctx) {
     if (c.unet) (void)gk_unet_destroy(c.unet);  // detaches itself from c
     free_targets(c);
     for (auto& e : c.ev) if (e) (void)hipEventDestroy(e);
+    if (c.h_word) (void)hipHostFree(c.h_word);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     delete ctx;
     return GK_OK;

@@ -256,6 +256,7 @@ struct Ctx {
     uint64_t trace_launches = 0;
     bool stack_overflow = false;        // a traversal stack overflowed since gk_get_stats last reported
     bool trace_counter_zeroed = false;  // the next trace launch's counter is already 0
+    uint32_t* h_word = nullptr;         // pinned host word for counter read-backs
     gk_stats stats{};
 };
 
@@ -286,7 +287,7 @@ void project_sky_sh(const float* rgba, uint32_t w, uint32_t h, float out[27]);
 // trace.hip
 int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev,
                  uint32_t max_count, float4* hit, uint32_t* hit_inst, bool any_hit,
-                 uint32_t* occluded_flags);
+                 uint32_t* clear);
 // a closest-hit queue (o, d -> hit, hit_inst) and an occlusion queue (so, sd -> shit) in one launch
 int launch_trace_pair(Ctx& c, const float4* o, const float4* d, const uint32_t* count_dev, uint32_t max_count,
                       float4* hit, uint32_t* hit_inst, const float4* so, const float4* sd,

@@ -78,6 +78,8 @@ struct TraceJob {
     uint32_t max_count;
     float4* hit;                     // closest: (t, u, v, prim); any: w = 1 when occluded
     uint32_t* inst;                  // closest: instance of the hit (may be null)
+    uint32_t* clear;                 // a counter other than count_dev the launch sets to 0 (may be
+                                     // null): the queue the shading after the trace appends to
 };
 
 // MODE 0: one closest-hit job; 1: one any-hit job; 2: a closest-hit job then an any-hit
@@ -406,6 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(
     }
     const uint32_t count = c0 + c1;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (j0.clear) *j0.clear = 0u;
         atomicAdd(&ray_stats[MODE == 1 ? 1 : 0], (unsigned long long)c0);
         if (MODE == 2) atomicAdd(&ray_stats[1], (unsigned long long)c1);
     }
@@ -722,11 +725,12 @@ static int launch_jobs(Ctx& c, int mode, const TraceJob& j0, const TraceJob& j1)
 }
 
 // Trace a queue of rays. count_dev (device) caps the number of rays when non-null.
-// hit: float4 (t,u,v,prim) per ray (any-hit: w = 1 when occluded).
+// hit: float4 (t,u,v,prim) per ray (any-hit: w = 1 when occluded). clear (device, may be
+// null): a counter the launch zeroes, in place of a launch of its own.
 int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev,
-                 uint32_t max_count, float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t*) {
+                 uint32_t max_count, float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t* clear) {
     if (max_count == 0) return GK_OK;
-    const TraceJob j0{ray_o, ray_d, count_dev, max_count, hit, any_hit ? nullptr : hit_inst};
+    const TraceJob j0{ray_o, ray_d, count_dev, max_count, hit, any_hit ? nullptr : hit_inst, clear};
     return launch_jobs(c, any_hit ? 1 : 0, j0, TraceJob{});
 }
 
@@ -734,8 +738,8 @@ int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_
 int launch_trace_pair(Ctx& c, const float4* o, const float4* d, const uint32_t* count_dev, uint32_t max_count,
                       float4* hit, uint32_t* hit_inst, const float4* so, const float4* sd,
                       const uint32_t* scount_dev, uint32_t smax_count, float4* shit) {
-    const TraceJob j0{o, d, count_dev, max_count, hit, hit_inst};
-    const TraceJob j1{so, sd, scount_dev, smax_count, shit, nullptr};
+    const TraceJob j0{o, d, count_dev, max_count, hit, hit_inst, nullptr};
+    const TraceJob j1{so, sd, scount_dev, smax_count, shit, nullptr, nullptr};
     return launch_jobs(c, 2, j0, j1);
 }
 
@@ -872,7 +876,7 @@ extern "C" int gk_trace_static(gk_ctx* ctx, const gk_ray* d_rays, uint32_t n, gk
     auto* kern = any ? (two ? k_trace<1, true> : k_trace<1, false>) : (two ? k_trace<0, true> : k_trace<0, false>);
     uint32_t blocks = persistent_blocks(c.device, kern);
     if (blocks > kStaticBlocks) blocks = kStaticBlocks;
-    const TraceJob j0{so, sd, nullptr, n, reinterpret_cast<float4*>(d_hits), any ? nullptr : d_inst};
+    const TraceJob j0{so, sd, nullptr, n, reinterpret_cast<float4*>(d_hits), any ? nullptr : d_inst, nullptr};
     kern<<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, c.tlas_nodes.ptr, c.tlas_prims.ptr, c.inst.ptr,
                                          root, j0, TraceJob{}, head, spill, head + 1,
                                          reinterpret_cast<unsigned long long*>(head + 2));

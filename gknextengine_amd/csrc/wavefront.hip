@@ -36,7 +36,7 @@ using ptmis::vndf_sample;
 int post_process(Ctx& c);  // post.hip
 ProbeDev probe_dev(const Ctx& c);  // probe.hip
 int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev, uint32_t max_count,
-                 float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t* occluded_flags);
+                 float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t* clear);
 
 namespace {
 
@@ -410,8 +410,6 @@ __global__ void k_vis_primary(DScene s, FrameK K, Queues Q, const uint32_t* __re
     Q.hit_inst[p] = hinst;
 }
 
-__global__ void k_reset_counter(uint32_t* p) { *p = 0u; }
-
 // ================================================================ REF_V2
 // flags (misc.y): bit0 chanceReflect of the pending GetRayColor, bit1 BackFace,
 // bit2 origin material is dielectric, bit3 hitReflect (first call), bit4 hitMetal (first call)
@@ -510,8 +508,11 @@ __device__ inline void ref_end_sample(const DScene& s, const FrameK& K, const Ta
     emit = false;
 }
 
-__global__ __launch_bounds__(256) void k_ref_primary(DScene s, FrameK K, Targets T, Queues Q, PathState S) {
+// trace_next: the ray counter of the trace launch that follows, zeroed here
+__global__ __launch_bounds__(256) void k_ref_primary(DScene s, FrameK K, Targets T, Queues Q, PathState S,
+                                                     uint32_t* __restrict__ trace_next) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0u) *trace_next = 0u;
     const uint32_t n = Q.cnt[0];
     const bool live = i < n;
     bool emit = false;
@@ -581,9 +582,11 @@ __global__ __launch_bounds__(256) void k_ref_primary(DScene s, FrameK K, Targets
 }
 
 // GetRayColor after the trace (Shading.slang:1039-1081) and the Render loop body
-// (Shading.slang:1117-1171); cur = queue the hits belong to.
-__global__ __launch_bounds__(256) void k_ref_shade(DScene s, FrameK K, Targets T, Queues Q, PathState S, int cur) {
+// (Shading.slang:1117-1171); cur = queue the hits belong to; trace_next as k_ref_primary.
+__global__ __launch_bounds__(256) void k_ref_shade(DScene s, FrameK K, Targets T, Queues Q, PathState S, int cur,
+                                                   uint32_t* __restrict__ trace_next) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0u) *trace_next = 0u;
     const uint32_t n = Q.cnt[cur];
     const int nxt = cur ^ 1;
     const bool live = i < n;
@@ -1230,18 +1233,35 @@ int render_frame(Ctx& c) {
     }
     int cur = 0;
     if (ref) {
-        k_ref_primary<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P);
+        uint32_t* tnext = trace_counter(c);
+        if (!tnext) return GK_ERR_OOM;
+        k_ref_primary<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, tnext);
         GK_HIP(hipGetLastError());
         cur = 1;
         // worst case: every sample uses all bounces (samples * 4 for moving pixels); a sample
         // traces at least once, also with no bounces
         const uint32_t maxb = std::max(1u, std::max(K.bounces, K.max_bounces));
         const uint32_t iters = maxb * K.samples * 4u + 4u;
+        // The loop ends early once the queue about to be traced is empty: every sample has
+        // finished. Reading its counter back costs a stream synchronisation, so it is read
+        // after the iterations a pixel without motion can take, then every maxb iterations;
+        // a stream under capture runs the worst case.
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        GK_HIP(hipStreamIsCapturing(s, &cap));
+        if (cap == hipStreamCaptureStatusNone && !c.h_word) GK_HIP(hipHostMalloc(&c.h_word, sizeof(uint32_t)));
+        uint32_t check = cap == hipStreamCaptureStatusNone ? maxb * std::max(1u, K.samples) + 1u : iters;
         for (uint32_t it = 0; it < iters; ++it) {
-            st = launch_trace(c, Q.o[cur], Q.d[cur], Q.cnt + cur, n, Q.hit, Q.hit_inst, false, nullptr);
+            if (it == check) {
+                GK_HIP(hipMemcpyAsync(c.h_word, Q.cnt + cur, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                GK_HIP(hipStreamSynchronize(s));
+                if (*c.h_word == 0u) break;
+                check += maxb;
+            }
+            c.trace_counter_zeroed = true;
+            // the trace zeroes the counter of the queue k_ref_shade appends to
+            st = launch_trace(c, Q.o[cur], Q.d[cur], Q.cnt + cur, n, Q.hit, Q.hit_inst, false, Q.cnt + (cur ^ 1));
             if (st != GK_OK) return st;
-            k_reset_counter<<<1, 1, 0, s>>>(Q.cnt + (cur ^ 1));
-            k_ref_shade<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, cur);
+            k_ref_shade<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, cur, tnext);
             GK_HIP(hipGetLastError());
             cur ^= 1;
         }
