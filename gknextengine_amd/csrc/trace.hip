@@ -65,6 +65,8 @@ struct Lane {
     uint32_t level;                  // 0 = TLAS, 1 = BLAS
     uint32_t head, count, gsp;       // ring head, entries in ring, spilled entries
     uint32_t ray;
+    uint32_t sx, sy, sz;             // single level: byte offset (0 or 16) of the near plane
+                                     // of each axis within a node's lo/hi pair
     bool active, found, overflow;
     bool any;                        // occlusion ray: the first hit ends it
     bool job1;                       // ray of the second job (two-job launches)
@@ -171,6 +173,42 @@ __device__ inline void slab4(const Bvh4Node& n, const Lane& L, float limit, floa
     const float hz[4] = {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w};
 #pragma unroll
     for (int i = 0; i < 4; ++i) tl[i] = slab1(lx[i], hx[i], ly[i], hy[i], lz[i], hz[i], L.O, L.rD, L.tmin, limit);
+}
+
+// near_far_offsets: for a direction whose reciprocal component is negative the near plane
+// of that axis is the box's hi plane (offset 16 in the node's lo/hi float4 pair).
+__device__ inline void near_far_offsets(Lane& L) {
+    L.sx = (__float_as_uint(L.rD.x) >> 27) & 16u;
+    L.sy = (__float_as_uint(L.rD.y) >> 27) & 16u;
+    L.sz = (__float_as_uint(L.rD.z) >> 27) & 16u;
+}
+
+// slab4 with the near and far planes fetched by address: per axis, (near - O) * rD and
+// (far - O) * rD are the min and the max of the two plane distances, bit for bit (the
+// rounded difference and the product by a nonzero reciprocal are monotonic, and no term
+// is NaN), so the entry is max3 of the near terms and the exit min3 of the far terms and
+// the per-axis min/max pairs of slab1 go away. nb: the node array; node: its index.
+__device__ inline void slab4_ordered(const char* __restrict__ nb, uint32_t node, const Lane& L, float limit,
+                                     float tl[4], uint4& child, uint4& meta) {
+    const uint32_t base = node * uint32_t(sizeof(Bvh4Node));
+    const uint32_t ax = base | L.sx, ay = base | L.sy, az = base | L.sz;
+    auto ld = [&](uint32_t off) { return *reinterpret_cast<const float4*>(nb + off); };
+    const float4 nx = ld(ax), fx = ld(ax ^ 16u);
+    const float4 ny = ld(ay + 32u), fy = ld((ay ^ 16u) + 32u);
+    const float4 nz = ld(az + 64u), fz = ld((az ^ 16u) + 64u);
+    child = *reinterpret_cast<const uint4*>(nb + base + 96u);
+    meta = *reinterpret_cast<const uint4*>(nb + base + 112u);
+    const float nxs[4] = {nx.x, nx.y, nx.z, nx.w}, fxs[4] = {fx.x, fx.y, fx.z, fx.w};
+    const float nys[4] = {ny.x, ny.y, ny.z, ny.w}, fys[4] = {fy.x, fy.y, fy.z, fy.w};
+    const float nzs[4] = {nz.x, nz.y, nz.z, nz.w}, fzs[4] = {fz.x, fz.y, fz.z, fz.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float lo = widen_lo(fmaxf(fmaxf((nxs[i] - L.O.x) * L.rD.x, (nys[i] - L.O.y) * L.rD.y),
+                                        (nzs[i] - L.O.z) * L.rD.z));
+        const float hi = widen_hi(fminf(fminf((fxs[i] - L.O.x) * L.rD.x, (fys[i] - L.O.y) * L.rD.y),
+                                        (fzs[i] - L.O.z) * L.rD.z));
+        tl[i] = (hi >= lo && hi >= L.tmin && lo <= limit) ? lo : __builtin_inff();
+    }
 }
 
 // compare-exchange of two (distance, entry) pairs, nearer first
@@ -462,6 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(
                         L.tmin = o.w;
                         L.tmax = d.w;
                         L.rD = f3{safercp(L.D.x), safercp(L.D.y), safercp(L.D.z)};
+                        if (!TWO) near_far_offsets(L);
                         L.wO = L.O; L.wD = L.D; L.wrD = L.rD;
                         L.bt = L.tmax; L.bu = 0.f; L.bv = 0.f;
                         L.bprim = 0xffffffffu; L.binst = 0xffffffffu;
@@ -592,12 +631,19 @@ __global__ __launch_bounds__(kBlock) void k_trace(
             }
         }
         const bool tl_level = TWO && L.level == 0;
-        const Bvh4Node& N = tl_level ? tlas[L.node] : blas[L.node];
-        const Bvh4Node n = N;
         float tl[4];
-        slab4(n, L, L.bt, tl);
-        const uint32_t ch[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
-        const uint32_t mt4[4] = {n.meta.x, n.meta.y, n.meta.z, n.meta.w};
+        uint4 nch, nmeta;
+        if (TWO) {
+            const Bvh4Node& N = tl_level ? tlas[L.node] : blas[L.node];
+            const Bvh4Node n = N;
+            slab4(n, L, L.bt, tl);
+            nch = n.child;
+            nmeta = n.meta;
+        } else {
+            slab4_ordered(reinterpret_cast<const char*>(blas), L.node, L, L.bt, tl, nch, nmeta);
+        }
+        const uint32_t ch[4] = {nch.x, nch.y, nch.z, nch.w};
+        const uint32_t mt4[4] = {nmeta.x, nmeta.y, nmeta.z, nmeta.w};
         // TLAS leaves push their instances; BLAS leaves join the internal children as
         // entries (a BLAS node stores a leaf child as its stack entry, kBlasLeaf | count
         // << 27 | first triangle), all near to far. The
