@@ -65,8 +65,8 @@ struct Lane {
     uint32_t level;                  // 0 = TLAS, 1 = BLAS
     uint32_t head, count, gsp;       // ring head, entries in ring, spilled entries
     uint32_t ray;
-    uint32_t sx, sy, sz;             // single level: byte offset (0 or 16) of the near plane
-                                     // of each axis within a node's lo/hi pair
+    uint32_t sx, sy, sz;             // byte offset (0 or 16) of the near plane of each axis
+                                     // within a node's lo/hi pair, for the current rD
     bool active, found, overflow;
     bool any;                        // occlusion ray: the first hit ends it
     bool job1;                       // ray of the second job (two-job launches)
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(
                         L.tmin = o.w;
                         L.tmax = d.w;
                         L.rD = f3{safercp(L.D.x), safercp(L.D.y), safercp(L.D.z)};
-                        if (!TWO) near_far_offsets(L);
+                        near_far_offsets(L);
                         L.wO = L.O; L.wD = L.D; L.wrD = L.rD;
                         L.bt = L.tmax; L.bu = 0.f; L.bv = 0.f;
                         L.bprim = 0xffffffffu; L.binst = 0xffffffffu;
@@ -601,6 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(
                     if (!pop(L, ring, spill, gtid, stride, e)) { done = true; break; }
                     if (e == kStackMark) {  // leave the instance
                         L.O = L.wO; L.D = L.wD; L.rD = L.wrD;
+                        near_far_offsets(L);
                         L.level = 0;
                         continue;
                     }
@@ -610,6 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(
                         L.O = tpoint(L.wO, I.inv);
                         L.D = tvector(L.wD, I.inv);
                         L.rD = f3{safercp(L.D.x), safercp(L.D.y), safercp(L.D.z)};
+                        near_far_offsets(L);
                         L.cur_inst = ii;
                         L.level = 1;
                         push(L, ring, spill, gtid, stride, kStackMark);
@@ -633,15 +635,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(
         const bool tl_level = TWO && L.level == 0;
         float tl[4];
         uint4 nch, nmeta;
-        if (TWO) {
-            const Bvh4Node& N = tl_level ? tlas[L.node] : blas[L.node];
-            const Bvh4Node n = N;
-            slab4(n, L, L.bt, tl);
-            nch = n.child;
-            nmeta = n.meta;
-        } else {
-            slab4_ordered(reinterpret_cast<const char*>(blas), L.node, L, L.bt, tl, nch, nmeta);
-        }
+        slab4_ordered(reinterpret_cast<const char*>(tl_level ? tlas : blas), L.node, L, L.bt, tl, nch, nmeta);
         const uint32_t ch[4] = {nch.x, nch.y, nch.z, nch.w};
         const uint32_t mt4[4] = {nmeta.x, nmeta.y, nmeta.z, nmeta.w};
         // TLAS leaves push their instances; BLAS leaves join the internal children as
