@@ -29,6 +29,7 @@ FRAME_PRIMARY_FROM_VISBUFFER = 32
 FRAME_POST_ONLY = 64
 FRAME_TLAS_REBUILD = 128
 FRAME_FAST_GATHER = 256
+FRAME_SINGLE_PIPELINE = 512
 TRACE_CLOSEST, TRACE_ANY = 0, 1
 TRACE_STATIC_MAX_RAYS = 262144
 

@@ -135,6 +135,10 @@ struct BuildScratch {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream of a frame split into two independent path pipelines (created on first
+    // use; default priority), and the events that fork it from and join it to `stream`
+    hipStream_t stream2 = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     uint32_t width = 0, height = 0;
     bool scene_ready = false;
     bool frame_ready = false;
@@ -245,7 +249,8 @@ struct Ctx {
         DevBuf<float4> s_dir;                 // current ray direction
         DevBuf<float4> s_ppos, s_pnrm, s_palb;  // primary vertex, its (textured) albedo
         DevBuf<ushort4> hist_d, hist_s, hist_a;  // previous accumulation targets (RGBA16F)
-        DevBuf<uint32_t> spill;               // traversal stack overflow
+        DevBuf<uint32_t> spill;               // traversal stack overflow, then the launch counters
+        DevBuf<uint32_t> spill2;              // the same for launches on stream2
         uint32_t spill_threads = 0;
     } wf;
 
@@ -255,7 +260,7 @@ struct Ctx {
     uint32_t tev_used = 0;
     uint64_t trace_launches = 0;
     bool stack_overflow = false;        // a traversal stack overflowed since gk_get_stats last reported
-    bool trace_counter_zeroed = false;  // the next trace launch's counter is already 0
+    bool trace_counter_zeroed[2] = {false, false};  // per slot: the next launch's counter is already 0
     uint32_t* h_word = nullptr;         // pinned host word for counter read-backs
     gk_stats stats{};
 };
@@ -284,19 +289,21 @@ int update_lights(Ctx& c);  // world-space emissive triangles from the instance 
 // equirectangular RGBA32F image, out[channel * 9 + i]
 void project_sky_sh(const float* rgba, uint32_t w, uint32_t h, float out[27]);
 
-// trace.hip
+// trace.hip. A trace launch runs in a slot: 0 = `stream` with wf.spill, 1 = `stream2` with
+// wf.spill2; launches in different slots may run at the same time.
 int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev,
                  uint32_t max_count, float4* hit, uint32_t* hit_inst, bool any_hit,
-                 uint32_t* clear);
+                 uint32_t* clear, int slot = 0);
 // a closest-hit queue (o, d -> hit, hit_inst) and an occlusion queue (so, sd -> shit) in one launch
 int launch_trace_pair(Ctx& c, const float4* o, const float4* d, const uint32_t* count_dev, uint32_t max_count,
                       float4* hit, uint32_t* hit_inst, const float4* so, const float4* sd,
-                      const uint32_t* scount_dev, uint32_t smax_count, float4* shit);
+                      const uint32_t* scount_dev, uint32_t smax_count, float4* shit, int slot = 0);
 int launch_trace_user(Ctx& c, const gk_ray* rays, uint32_t n, gk_hit* hits, uint32_t* inst, uint32_t flags);
-// the ray counter a trace launch starts from (allocates the trace's spill area); a caller
-// whose kernel zeroed it sets Ctx::trace_counter_zeroed, and the next launch skips its reset
-uint32_t* trace_counter(Ctx& c);
-int ensure_spill(Ctx& c, uint32_t threads);
+// the ray counter a trace launch in `slot` starts from (allocates the slot's spill area); a
+// caller whose kernel zeroed it sets Ctx::trace_counter_zeroed[slot], and the next launch in
+// that slot skips its reset
+uint32_t* trace_counter(Ctx& c, int slot = 0);
+int ensure_spill(Ctx& c, uint32_t threads, int slot = 0);
 int collect_timing(Ctx& c);
 // gk_trace_static's device array: resolved per GPU by gk_create; the overflow flag of the
 // last call on c's GPU, read and cleared

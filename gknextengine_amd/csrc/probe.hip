@@ -27,11 +27,6 @@
 
 namespace gk {
 
-int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev, uint32_t max_count,
-                 float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t* clear);
-int launch_trace_pair(Ctx& c, const float4* o, const float4* d, const uint32_t* count_dev, uint32_t max_count,
-                      float4* hit, uint32_t* hit_inst, const float4* so, const float4* sd,
-                      const uint32_t* scount_dev, uint32_t smax_count, float4* shit);
 int collect_timing(Ctx& c);
 
 namespace {

@@ -712,11 +712,12 @@ uint32_t persistent_blocks(int device, K kern) {
 
 }  // namespace
 
-int ensure_spill(Ctx& c, uint32_t threads) {
+int ensure_spill(Ctx& c, uint32_t threads, int slot) {
     const size_t need = size_t(threads) * kSpill;
-    if (c.wf.spill.count < need + 8) {
-        GK_HIP(c.wf.spill.resize(need + 8));
-        GK_HIP(hipMemsetAsync(c.wf.spill.ptr + need, 0, 32, c.stream));
+    DevBuf<uint32_t>& sp = slot ? c.wf.spill2 : c.wf.spill;
+    if (sp.count < need + 8) {
+        GK_HIP(sp.resize(need + 8));
+        GK_HIP(hipMemsetAsync(sp.ptr + need, 0, 32, c.stream));
     }
     c.wf.spill_threads = threads;
     if (c.tev.empty()) {
@@ -729,24 +730,29 @@ int ensure_spill(Ctx& c, uint32_t threads) {
     return GK_OK;
 }
 
-// Launches one persistent trace over up to two jobs (mode as k_trace). Each launch is
-// bracketed by HIP events on the context stream (collected by collect_timing).
-uint32_t* trace_counter(Ctx& c) {
-    // the spill area is laid out for the largest grid, so its tail (counters) never moves
+uint32_t* trace_counter(Ctx& c, int slot) {
+    // the spill area is laid out for the largest grid, so its tail (counters) never moves.
+    // A slot's area is allocated (and its counters cleared on `stream`) on first use.
     const uint32_t cap_threads = cu_count(c.device) * kMaxBlocksPerCU * kBlock;
-    if (ensure_spill(c, cap_threads) != GK_OK) return nullptr;
-    return c.wf.spill.ptr + size_t(cap_threads) * kSpill;  // [next ray, overflow, stats u64 x2]
+    if (ensure_spill(c, cap_threads, slot) != GK_OK) return nullptr;
+    const DevBuf<uint32_t>& sp = slot ? c.wf.spill2 : c.wf.spill;
+    return sp.ptr + size_t(cap_threads) * kSpill;  // [next ray, overflow, stats u64 x2]
 }
 
-static int launch_jobs(Ctx& c, int mode, const TraceJob& j0, const TraceJob& j1) {
-    const bool zeroed = c.trace_counter_zeroed;  // applies to this launch only
-    c.trace_counter_zeroed = false;
-    uint32_t* tail = trace_counter(c);
+// Launches one persistent trace over up to two jobs (mode as k_trace) in a slot (its stream
+// and spill area). Each launch is bracketed by HIP events on that stream (collected by
+// collect_timing).
+static int launch_jobs(Ctx& c, int mode, const TraceJob& j0, const TraceJob& j1, int slot) {
+    const bool zeroed = c.trace_counter_zeroed[slot];  // applies to this launch only
+    c.trace_counter_zeroed[slot] = false;
+    uint32_t* tail = trace_counter(c, slot);
     if (!tail) return GK_ERR_OOM;
+    hipStream_t st = slot ? c.stream2 : c.stream;
+    uint32_t* spill = slot ? c.wf.spill2.ptr : c.wf.spill.ptr;
     uint32_t* next = tail;
     uint32_t* ovf = tail + 1;
     unsigned long long* rs = reinterpret_cast<unsigned long long*>(tail + 2);
-    if (!zeroed) k_reset_counter<<<1, 1, 0, c.stream>>>(next);
+    if (!zeroed) k_reset_counter<<<1, 1, 0, st>>>(next);
     const bool two = !c.blas_only;
     const uint32_t root = c.h_meshes.empty() ? 0u : c.h_meshes[0].root;
     const Bvh4Node* tl = c.tlas_nodes.ptr;
@@ -756,11 +762,11 @@ static int launch_jobs(Ctx& c, int mode, const TraceJob& j0, const TraceJob& j1)
     const uint32_t blocks = persistent_blocks(c.device, kern);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c.tev_used < c.tev.size()) { e0 = c.tev[c.tev_used].first; e1 = c.tev[c.tev_used].second; c.tev_used++; }
-    if (e0) GK_HIP(hipEventRecord(e0, c.stream));
-    kern<<<blocks, kBlock, 0, c.stream>>>(c.blas_nodes.ptr, c.tris.ptr, tl, c.tlas_prims.ptr, c.inst.ptr, root, j0,
-                                         j1, next, c.wf.spill.ptr, ovf, rs);
+    if (e0) GK_HIP(hipEventRecord(e0, st));
+    kern<<<blocks, kBlock, 0, st>>>(c.blas_nodes.ptr, c.tris.ptr, tl, c.tlas_prims.ptr, c.inst.ptr, root, j0, j1,
+                                   next, spill, ovf, rs);
     GK_HIP(hipGetLastError());
-    if (e1) GK_HIP(hipEventRecord(e1, c.stream));
+    if (e1) GK_HIP(hipEventRecord(e1, st));
     return GK_OK;
 }
 
@@ -768,19 +774,19 @@ static int launch_jobs(Ctx& c, int mode, const TraceJob& j0, const TraceJob& j1)
 // hit: float4 (t,u,v,prim) per ray (any-hit: w = 1 when occluded). clear (device, may be
 // null): a counter the launch zeroes, in place of a launch of its own.
 int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev,
-                 uint32_t max_count, float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t* clear) {
+                 uint32_t max_count, float4* hit, uint32_t* hit_inst, bool any_hit, uint32_t* clear, int slot) {
     if (max_count == 0) return GK_OK;
     const TraceJob j0{ray_o, ray_d, count_dev, max_count, hit, any_hit ? nullptr : hit_inst, clear};
-    return launch_jobs(c, any_hit ? 1 : 0, j0, TraceJob{});
+    return launch_jobs(c, any_hit ? 1 : 0, j0, TraceJob{}, slot);
 }
 
 // A closest-hit queue and an occlusion queue in one launch.
 int launch_trace_pair(Ctx& c, const float4* o, const float4* d, const uint32_t* count_dev, uint32_t max_count,
                       float4* hit, uint32_t* hit_inst, const float4* so, const float4* sd,
-                      const uint32_t* scount_dev, uint32_t smax_count, float4* shit) {
+                      const uint32_t* scount_dev, uint32_t smax_count, float4* shit, int slot) {
     const TraceJob j0{o, d, count_dev, max_count, hit, hit_inst, nullptr};
     const TraceJob j1{so, sd, scount_dev, smax_count, shit, nullptr, nullptr};
-    return launch_jobs(c, 2, j0, j1);
+    return launch_jobs(c, 2, j0, j1, slot);
 }
 
 // Sums the elapsed time of the trace launches recorded since the last call (the stream
@@ -794,17 +800,24 @@ int collect_timing(Ctx& c) {
     }
     c.tev_used = 0;
     if (c.wf.spill_threads) {
-        unsigned long long v[2] = {0, 0};
-        uint32_t ovf = 0;
-        const uint32_t* tail = c.wf.spill.ptr + size_t(c.wf.spill_threads) * kSpill;
-        GK_HIP(hipMemcpy(v, tail + 2, 16, hipMemcpyDeviceToHost));
-        GK_HIP(hipMemcpy(&ovf, tail + 1, 4, hipMemcpyDeviceToHost));
-        c.stats.rays_traced = v[0] + v[1];
-        c.stats.shadow_rays = v[1];
-        if (ovf) {  // kept until gk_get_stats reports it; the device flag starts over
-            c.stack_overflow = true;
-            const uint32_t zero = 0;
-            GK_HIP(hipMemcpy(const_cast<uint32_t*>(tail + 1), &zero, 4, hipMemcpyHostToDevice));
+        // both slots' counters: rays since the spill areas were allocated, overflow flags
+        c.stats.rays_traced = 0;
+        c.stats.shadow_rays = 0;
+        for (int slot = 0; slot < 2; ++slot) {
+            const DevBuf<uint32_t>& sp = slot ? c.wf.spill2 : c.wf.spill;
+            if (!sp.ptr) continue;
+            unsigned long long v[2] = {0, 0};
+            uint32_t ovf = 0;
+            const uint32_t* tail = sp.ptr + size_t(c.wf.spill_threads) * kSpill;
+            GK_HIP(hipMemcpy(v, tail + 2, 16, hipMemcpyDeviceToHost));
+            GK_HIP(hipMemcpy(&ovf, tail + 1, 4, hipMemcpyDeviceToHost));
+            c.stats.rays_traced += v[0] + v[1];
+            c.stats.shadow_rays += v[1];
+            if (ovf) {  // kept until gk_get_stats reports it; the device flag starts over
+                c.stack_overflow = true;
+                const uint32_t zero = 0;
+                GK_HIP(hipMemcpy(const_cast<uint32_t*>(tail + 1), &zero, 4, hipMemcpyHostToDevice));
+            }
         }
     }
     return GK_OK;

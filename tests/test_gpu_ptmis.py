@@ -207,3 +207,31 @@ def test_pt_mis_city_1080p_sorted_shading_gives_the_same_frame(city):
             ctx.sync()
             out.append(ctx.read("RADIANCE").copy())
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("W,H", [(520, 520), (515, 515), (1920, 1080)])
+def test_pt_mis_two_pipelines_give_the_single_pipeline_frame(city, W, H):
+    """Frames of 2^18 paths or more over a single-level scene run as two path pipelines on
+    two streams (the even and the odd 8x8 tiles; halves of the path range when the size is
+    not a multiple of 8). Two frames with temporal reuse, every target bit-identical to
+    GK_FRAME_SINGLE_PIPELINE's, with an odd tile count (65 x 65) too, and the same number
+    of rays traced."""
+    s, _ = city
+    names = [t for t in gk.TARGETS if t not in ("MINIGBUFFER", "RADIANCE64")]
+    outs = []
+    for flags in (gk.FRAME_SINGLE_PIPELINE, 0):
+        with gk.Context(W, H) as ctx:
+            ctx.upload(s)
+            for f in range(2):
+                p = params(W, H, scenes.CITY_CAM, frame=2 + f, bounces=8, sky=(0.6, 0.7, 0.9), seed=7)
+                p.denoiser = gk.DENOISER_BILATERAL
+                p.flags |= flags
+                ctx.frame(p)
+                ctx.render()
+            ctx.sync()
+            outs.append(({t: ctx.read(t).copy() for t in names}, ctx.stats().rays_traced))
+    (a, ra), (b, rb) = outs
+    assert ra == rb and ra > W * H, (ra, rb)
+    for t in names:
+        assert np.array_equal(a[t].view(np.uint8), b[t].view(np.uint8)), t
+    assert a["RADIANCE"][..., :3].max() > 0.1
