@@ -216,6 +216,8 @@ def run_city(args):
     elif args.sort == "on":
         p.flags |= gk.FRAME_MATERIAL_SORT
     sorted_shading = args.sort == "on"
+    if args.pipelines == 1:
+        p.flags |= gk.FRAME_SINGLE_PIPELINE
     frame = 0
 
     def step():
@@ -253,7 +255,8 @@ def run_city(args):
         step()
     ctx.sync()
     k1 = ctx.stats()
-    p.flags &= ~gk.FRAME_SINGLE_PIPELINE
+    if args.pipelines != 1:
+        p.flags &= ~gk.FRAME_SINGLE_PIPELINE
     krays = k1.rays_traced - k0.rays_traced
     ms_trace = (k1.ms_trace - k0.ms_trace) / kframes * args.steps
     achieved = BYTES_PER_RAY * krays / ((k1.ms_trace - k0.ms_trace) * 1e-3) / 1e9
@@ -291,7 +294,8 @@ def run_city(args):
                    "estimator": "PT_MIS", "material_sort": sorted_shading,
                    "material_models": len(models_used),
                    "rays_per_frame": round(rays / args.steps), "trace_ms_per_frame": round(ms_trace / args.steps, 3),
-                   "trace_launch_ms_per_frame_two_pipelines": round(ms_trace_split / args.steps, 3),
+                   "path_pipelines": args.pipelines,
+                   "trace_launch_ms_per_frame_timed": round(ms_trace_split / args.steps, 3),
                    "reproject_ms": round(s1.ms_reproject, 3), "denoise_ms": round(s1.ms_denoise, 3),
                    "blas_build_ms": round(bvh.build_ms, 2), "bvh_sah_cost": round(bvh.sah_cost, 2),
                    "blas_nodes": bvh.blas_nodes, **extra},
@@ -1040,6 +1044,8 @@ def main():
                     help="dynamic config: per-frame TLAS update")
     ap.add_argument("--per-frame", action="store_true", help="dynamic config: per-frame trace ms and TLAS SAH")
     ap.add_argument("--frames", type=int, default=120, help="dynamic config: frames after the first (profiling runs)")
+    ap.add_argument("--pipelines", type=int, default=2, choices=[1, 2],
+                    help="city: 1 renders the timed frames as one path pipeline (GK_FRAME_SINGLE_PIPELINE)")
     ap.add_argument("--no-side", action="store_true",
                     help="city config: skip the other configurations' figures (side_configs)")
     args = ap.parse_args()
