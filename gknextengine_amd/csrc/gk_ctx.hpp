@@ -261,6 +261,7 @@ struct Ctx {
     uint64_t trace_launches = 0;
     bool stack_overflow = false;        // a traversal stack overflowed since gk_get_stats last reported
     bool trace_counter_zeroed[2] = {false, false};  // per slot: the next launch's counter is already 0
+    bool trace_split = false;           // a frame's two path pipelines are being enqueued
     uint32_t* h_word = nullptr;         // pinned host word for counter read-backs
     gk_stats stats{};
 };

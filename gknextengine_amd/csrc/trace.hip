@@ -48,6 +48,11 @@ constexpr int kSpill = 64;           // global spill entries per thread
 // measured on the 1080p city (trace ms per frame): 6 -> 3.13, 5 -> 3.03, 4 -> 3.00,
 // 3 -> 3.09, 2 -> 3.57.
 constexpr int kMaxBlocksPerCU = 4;
+// The same while a frame's two path pipelines trace at once: a CU holds 5 of these
+// work-groups (LDS and registers), so grids of 3 per CU let both pipelines' launches be
+// resident together. 1080p city, ms per frame with two pipelines: 4 -> 3.18, 3 -> 2.96-2.99,
+// 2 -> 3.17, 3 and 2 -> 3.12-3.19; 4096 bricks: 4 -> 4.37, 3 -> 4.10, 2 -> 4.28.
+constexpr int kSplitBlocksPerCU = 3;
 constexpr uint32_t kRefill = 56;     // idle lanes that trigger a wave refill (<= 64)
 // postponed leaves that start a leaf phase (single level; measured on the 1080p city, trace
 // ms per frame: testing leaves as met 2.40, batches of 16 or 32 -> 2.36, 48 -> 2.37, 64 -> 2.41)
@@ -703,10 +708,10 @@ uint32_t cu_count(int device) {
 // Persistent grid: as many blocks as the kernel keeps resident per CU (its VGPR and LDS
 // use decide: 5 for the single-level kernels at 92 VGPRs, 4 for the two-level ones).
 template <class K>
-uint32_t persistent_blocks(int device, K kern) {
+uint32_t persistent_blocks(int device, K kern, int cap = kMaxBlocksPerCU) {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, 0) != hipSuccess || nb < 1) nb = 4;
-    if (nb > kMaxBlocksPerCU) nb = kMaxBlocksPerCU;
+    if (nb > cap) nb = cap;
     return cu_count(device) * uint32_t(nb);
 }
 
@@ -759,7 +764,7 @@ static int launch_jobs(Ctx& c, int mode, const TraceJob& j0, const TraceJob& j1,
     auto* kern = mode == 0 ? (two ? k_trace<0, true> : k_trace<0, false>)
                : mode == 1 ? (two ? k_trace<1, true> : k_trace<1, false>)
                            : (two ? k_trace<2, true> : k_trace<2, false>);
-    const uint32_t blocks = persistent_blocks(c.device, kern);
+    const uint32_t blocks = persistent_blocks(c.device, kern, c.trace_split ? kSplitBlocksPerCU : kMaxBlocksPerCU);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c.tev_used < c.tev.size()) { e0 = c.tev[c.tev_used].first; e1 = c.tev[c.tev_used].second; c.tev_used++; }
     if (e0) GK_HIP(hipEventRecord(e0, st));

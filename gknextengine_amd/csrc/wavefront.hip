@@ -1153,6 +1153,7 @@ static FrameK make_k(const Ctx& c) {
 }
 
 int render_frame(Ctx& c) {
+    c.trace_split = false;
     int st = ensure_wavefront(c);
     if (st != GK_OK) return st;
     auto& w = c.wf;
@@ -1226,14 +1227,13 @@ int render_frame(Ctx& c) {
     k_copy_objid<<<nblk(n), 256, 0, s>>>(T.objid, static_cast<uint32_t*>(c.target_ptr[GK_RT_OBJECTID_1]), n);
     GK_HIP(hipGetLastError());
     const bool ref = c.params.estimator == GK_ESTIMATOR_REF_V2;
-    // PT_MIS frames of many paths over a single-level scene run as two path pipelines, paths
-    // [0, n0) (even tiles) and [n0, n) (odd tiles), each with its own queue range, counters
-    // and trace slot, on two streams: a persistent trace launch spends a long tail finishing
-    // its last rays with most lanes idle, and the shading, resolve and small late-bounce
-    // launches of one pipeline overlap the other's traces (1080p city: 3.33 -> 3.18 ms per
-    // frame; two-level scenes measured slower, 4.26 -> 4.36 ms for the 4096 bricks).
-    const bool split = !ref && !sort && c.blas_only && n >= kSplitPaths &&
-                       !(c.params.flags & GK_FRAME_SINGLE_PIPELINE);
+    // PT_MIS frames of many paths run as two path pipelines, paths [0, n0) (even tiles) and
+    // [n0, n) (odd tiles), each with its own queue range, counters and trace slot, on two
+    // streams: a persistent trace launch spends a long tail finishing its last rays with most
+    // lanes idle, and one pipeline's launches overlap the other's. Each trace grid then holds
+    // 3 work-groups per CU (kSplitBlocksPerCU), so both fit at once (1080p city: 3.33 ->
+    // 2.97 ms per frame; 4096 bricks: 4.36 -> 4.10 ms).
+    const bool split = !ref && !sort && n >= kSplitPaths && !(c.params.flags & GK_FRAME_SINGLE_PIPELINE);
     const int np = split ? 2 : 1;
     const uint32_t n0 = !split ? n : (K.tiled ? std::min(n, K.tiles_even * 64u) : n / 2u);
     const uint32_t pbase[2] = {0u, n0}, pcount[2] = {n0, n - n0};
@@ -1266,6 +1266,7 @@ int render_frame(Ctx& c) {
         GK_HIP(hipEventRecord(c.fork_ev, s));
         GK_HIP(hipStreamWaitEvent(c.stream2, c.fork_ev, 0));
     }
+    c.trace_split = split;
     for (int h = 0; h < np; ++h) {
         k_raygen<<<nblk(pcount[h]), 256, 0, ss[h]>>>(K, QH[h], P, pcount[h], ref ? 0u : 1u, pbase[h]);
         GK_HIP(hipGetLastError());
@@ -1352,6 +1353,7 @@ int render_frame(Ctx& c) {
             }
             cur = nxt;
         }
+        c.trace_split = false;
         if (split) {
             GK_HIP(hipEventRecord(c.join_ev, c.stream2));
             GK_HIP(hipStreamWaitEvent(s, c.join_ev, 0));

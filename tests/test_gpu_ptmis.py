@@ -211,9 +211,8 @@ def test_pt_mis_city_1080p_sorted_shading_gives_the_same_frame(city):
 
 @pytest.mark.parametrize("W,H", [(520, 520), (515, 515), (1920, 1080)])
 def test_pt_mis_two_pipelines_give_the_single_pipeline_frame(city, W, H):
-    """Frames of 2^18 paths or more over a single-level scene run as two path pipelines on
-    two streams (the even and the odd 8x8 tiles; halves of the path range when the size is
-    not a multiple of 8). Two frames with temporal reuse, every target bit-identical to
+    """Frames of 2^18 paths or more run as two path pipelines on two streams (the even and
+    the odd 8x8 tiles; halves of the path range when the size is not a multiple of 8). Two frames with temporal reuse, every target bit-identical to
     GK_FRAME_SINGLE_PIPELINE's, with an odd tile count (65 x 65) too, and the same number
     of rays traced."""
     s, _ = city
@@ -235,3 +234,38 @@ def test_pt_mis_two_pipelines_give_the_single_pipeline_frame(city, W, H):
     for t in names:
         assert np.array_equal(a[t].view(np.uint8), b[t].view(np.uint8)), t
     assert a["RADIANCE"][..., :3].max() > 0.1
+
+
+def test_pt_mis_two_pipelines_on_moving_instances_and_visibility_buffer():
+    """The split on a two-level scene (4096 brick instances, 640x480 = 307,200 paths): a
+    frame, the instances moved and the TLAS rebuilt, a second frame, then a third whose
+    primary hits come from the visibility buffer the second wrote; every target bit-identical
+    to the single-pipeline frames."""
+    from gknextengine_amd import scenes as gs
+    b = scenes.Bricks()
+    W, H = 640, 480
+    names = [t for t in gk.TARGETS if t not in ("RADIANCE64",)]
+    outs = []
+    for flags in (gk.FRAME_SINGLE_PIPELINE, 0):
+        with gk.Context(W, H) as ctx:
+            ctx.upload(b.scene)
+            for f in range(3):
+                if f == 1:
+                    ctx.update_instances(b.worlds(1.0 / 60.0))
+                p = gk.make_frame_params(gs.BRICKS_CAM["eye"], gs.BRICKS_CAM["target"], W, H,
+                                         fov=gs.BRICKS_CAM["fov"], frame=f)
+                p.estimator = gk.ESTIMATOR_PT_MIS
+                p.bounces = p.max_bounces = 4
+                p.denoiser = gk.DENOISER_BILATERAL
+                p.flags |= flags | gk.FRAME_TLAS_REBUILD
+                if f == 2:
+                    p.flags |= gk.FRAME_PRIMARY_FROM_VISBUFFER
+                ctx.frame(p)
+                ctx.render()
+            ctx.sync()
+            outs.append(({t: ctx.read(t).copy() for t in names}, ctx.stats().rays_traced))
+    (a, ra), (c, rc) = outs
+    assert ra == rc, (ra, rc)
+    for t in names:
+        assert np.array_equal(a[t].view(np.uint8), c[t].view(np.uint8)), t
+    assert (a["MINIGBUFFER"] != 0).mean() > 0.3 and a["RADIANCE"][..., :3].max() > 0.1
