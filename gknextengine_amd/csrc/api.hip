@@ -358,14 +358,14 @@ int gk_destroy(gk_ctx* ctx) {
     Ctx& c = ctx->c;
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
-    if (c.stream2) (void)hipStreamSynchronize(c.stream2);
+    for (auto& ps : c.pstream) if (ps) (void)hipStreamSynchronize(ps);
     if (c.unet) (void)gk_unet_destroy(c.unet);  // detaches itself from c
     free_targets(c);
     for (auto& e : c.ev) if (e) (void)hipEventDestroy(e);
     if (c.h_word) (void)hipHostFree(c.h_word);
     if (c.fork_ev) (void)hipEventDestroy(c.fork_ev);
-    if (c.join_ev) (void)hipEventDestroy(c.join_ev);
-    if (c.stream2) (void)hipStreamDestroy(c.stream2);
+    for (auto& e : c.join_ev) if (e) (void)hipEventDestroy(e);
+    for (auto& ps : c.pstream) if (ps) (void)hipStreamDestroy(ps);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     delete ctx;
     return GK_OK;
@@ -592,9 +592,9 @@ int gk_render(gk_ctx* ctx) {
 int gk_sync(gk_ctx* ctx) {
     GK_CHECK_CTX(ctx);
     GK_HIP(hipStreamSynchronize(ctx->c.stream));
-    // a split frame joins its second stream into the first; this also covers a frame whose
+    // a split frame joins its other streams into the first; this also covers a frame whose
     // enqueue stopped on an error between the fork and the join
-    if (ctx->c.stream2) GK_HIP(hipStreamSynchronize(ctx->c.stream2));
+    for (auto& ps : ctx->c.pstream) if (ps) GK_HIP(hipStreamSynchronize(ps));
     return GK_OK;
 }
 

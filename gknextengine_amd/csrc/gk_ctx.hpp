@@ -135,10 +135,11 @@ struct BuildScratch {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // second stream of a frame split into two independent path pipelines (created on first
-    // use; default priority), and the events that fork it from and join it to `stream`
-    hipStream_t stream2 = nullptr;
-    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // streams of trace slots 1..3: a frame split into independent path pipelines runs
+    // pipeline h on slot h (created on first use; default priority); the events that fork
+    // them from and join them to `stream`
+    hipStream_t pstream[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};
     uint32_t width = 0, height = 0;
     bool scene_ready = false;
     bool frame_ready = false;
@@ -250,7 +251,7 @@ struct Ctx {
         DevBuf<float4> s_ppos, s_pnrm, s_palb;  // primary vertex, its (textured) albedo
         DevBuf<ushort4> hist_d, hist_s, hist_a;  // previous accumulation targets (RGBA16F)
         DevBuf<uint32_t> spill;               // traversal stack overflow, then the launch counters
-        DevBuf<uint32_t> spill2;              // the same for launches on stream2
+        DevBuf<uint32_t> pspill[3];           // the same for trace slots 1..3
         uint32_t spill_threads = 0;
     } wf;
 
@@ -260,7 +261,7 @@ struct Ctx {
     uint32_t tev_used = 0;
     uint64_t trace_launches = 0;
     bool stack_overflow = false;        // a traversal stack overflowed since gk_get_stats last reported
-    bool trace_counter_zeroed[2] = {false, false};  // per slot: the next launch's counter is already 0
+    bool trace_counter_zeroed[4] = {};  // per slot: the next launch's counter is already 0
     bool trace_split = false;           // a frame's two path pipelines are being enqueued
     uint32_t* h_word = nullptr;         // pinned host word for counter read-backs
     gk_stats stats{};
@@ -290,8 +291,8 @@ int update_lights(Ctx& c);  // world-space emissive triangles from the instance 
 // equirectangular RGBA32F image, out[channel * 9 + i]
 void project_sky_sh(const float* rgba, uint32_t w, uint32_t h, float out[27]);
 
-// trace.hip. A trace launch runs in a slot: 0 = `stream` with wf.spill, 1 = `stream2` with
-// wf.spill2; launches in different slots may run at the same time.
+// trace.hip. A trace launch runs in a slot (0..3): 0 = `stream` with wf.spill, h > 0 =
+// pstream[h - 1] with wf.pspill[h - 1]; launches in different slots may run at the same time.
 int launch_trace(Ctx& c, const float4* ray_o, const float4* ray_d, const uint32_t* count_dev,
                  uint32_t max_count, float4* hit, uint32_t* hit_inst, bool any_hit,
                  uint32_t* clear, int slot = 0);

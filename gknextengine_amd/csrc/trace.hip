@@ -719,7 +719,7 @@ uint32_t persistent_blocks(int device, K kern, int cap = kMaxBlocksPerCU) {
 
 int ensure_spill(Ctx& c, uint32_t threads, int slot) {
     const size_t need = size_t(threads) * kSpill;
-    DevBuf<uint32_t>& sp = slot ? c.wf.spill2 : c.wf.spill;
+    DevBuf<uint32_t>& sp = slot ? c.wf.pspill[slot - 1] : c.wf.spill;
     if (sp.count < need + 8) {
         GK_HIP(sp.resize(need + 8));
         GK_HIP(hipMemsetAsync(sp.ptr + need, 0, 32, c.stream));
@@ -740,7 +740,7 @@ uint32_t* trace_counter(Ctx& c, int slot) {
     // A slot's area is allocated (and its counters cleared on `stream`) on first use.
     const uint32_t cap_threads = cu_count(c.device) * kMaxBlocksPerCU * kBlock;
     if (ensure_spill(c, cap_threads, slot) != GK_OK) return nullptr;
-    const DevBuf<uint32_t>& sp = slot ? c.wf.spill2 : c.wf.spill;
+    const DevBuf<uint32_t>& sp = slot ? c.wf.pspill[slot - 1] : c.wf.spill;
     return sp.ptr + size_t(cap_threads) * kSpill;  // [next ray, overflow, stats u64 x2]
 }
 
@@ -752,8 +752,8 @@ static int launch_jobs(Ctx& c, int mode, const TraceJob& j0, const TraceJob& j1,
     c.trace_counter_zeroed[slot] = false;
     uint32_t* tail = trace_counter(c, slot);
     if (!tail) return GK_ERR_OOM;
-    hipStream_t st = slot ? c.stream2 : c.stream;
-    uint32_t* spill = slot ? c.wf.spill2.ptr : c.wf.spill.ptr;
+    hipStream_t st = slot ? c.pstream[slot - 1] : c.stream;
+    uint32_t* spill = slot ? c.wf.pspill[slot - 1].ptr : c.wf.spill.ptr;
     uint32_t* next = tail;
     uint32_t* ovf = tail + 1;
     unsigned long long* rs = reinterpret_cast<unsigned long long*>(tail + 2);
@@ -805,11 +805,11 @@ int collect_timing(Ctx& c) {
     }
     c.tev_used = 0;
     if (c.wf.spill_threads) {
-        // both slots' counters: rays since the spill areas were allocated, overflow flags
+        // every slot's counters: rays since the spill areas were allocated, overflow flags
         c.stats.rays_traced = 0;
         c.stats.shadow_rays = 0;
-        for (int slot = 0; slot < 2; ++slot) {
-            const DevBuf<uint32_t>& sp = slot ? c.wf.spill2 : c.wf.spill;
+        for (int slot = 0; slot < 4; ++slot) {
+            const DevBuf<uint32_t>& sp = slot ? c.wf.pspill[slot - 1] : c.wf.spill;
             if (!sp.ptr) continue;
             unsigned long long v[2] = {0, 0};
             uint32_t ovf = 0;

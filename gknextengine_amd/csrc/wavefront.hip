@@ -41,7 +41,10 @@ namespace {
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kEps = 1e-3f;                 // PreProcessor.slang:13
 constexpr float kMaxTrace = 1000.f;           // Shading.slang:15
-constexpr uint32_t kSplitPaths = 1u << 18;    // PT_MIS frames of at least this many paths run as two pipelines
+constexpr uint32_t kSplitPaths = 1u << 18;    // PT_MIS frames of at least this many paths are split
+// path pipelines of a split frame (trace slots allow 4; 1080p city, ms per frame: 2 -> 2.98,
+// 3 -> 3.08, 4 -> 4.43)
+constexpr int kPathParts = 2;
 constexpr uint32_t kMiss = 0xffffffffu;
 
 // per-frame constants
@@ -52,7 +55,8 @@ struct FrameK {
     float aperture, focus;  // thin lens (aperture 0: pinhole, no random numbers drawn)
     float sky_rot;          // SkyRotation
     uint32_t W, H, frame, seed, samples, bounces, max_bounces, temporal, has_sky, has_sun, tiled;
-    uint32_t tiles_even;  // tiled: 8x8 tiles with an even index (the first half of the path ids)
+    uint32_t tparts;      // tiled: the path ids fall into tparts parts, part h covering the
+    uint32_t tstart[4];   // 8x8 tiles t = h mod tparts from path tile tstart[h] on
     uint32_t vis_primary;  // primary hits come from GK_RT_MINIGBUFFER (left unwritten)
     uint32_t fast_gather;  // REF_V2 ExitAfterFirst
     ProbeDev probes;       // REF_V2 path termination
@@ -112,13 +116,15 @@ struct PathState {
     float4* palb;   // REF_V2 primary albedo (FetchGBufferV2: textured)
 };
 
-// Path p -> pixel. Tiled frames walk 8x8 tiles, 64 paths each; the first half of the path
-// ids covers the tiles of even row-major index and the second half the odd ones, so the two
-// halves of a split frame see the same mix of the image.
+// Path p -> pixel. Tiled frames walk 8x8 tiles, 64 paths each, in parts: part h of the path
+// ids covers the tiles whose row-major index is h modulo the part count, so the pipelines of
+// a split frame (one per part) see the same mix of the image.
 __device__ inline uint32_t pixel_of(const FrameK& K, uint32_t p) {
     if (!K.tiled) return p;
     const uint32_t t = p >> 6, within = p & 63u;
-    const uint32_t tile = t < K.tiles_even ? t * 2u : (t - K.tiles_even) * 2u + 1u;
+    uint32_t h = 0;
+    for (uint32_t k = 1; k < K.tparts; ++k) h = t >= K.tstart[k] ? k : h;
+    const uint32_t tile = (t - K.tstart[h]) * K.tparts + h;
     const uint32_t tx_n = K.W >> 3;
     const uint32_t tx = tile % tx_n, ty = tile / tx_n;
     const uint32_t x = (tx << 3) | (within & 7u), y = (ty << 3) | (within >> 3);
@@ -1145,7 +1151,15 @@ static FrameK make_k(const Ctx& c) {
     K.has_sky = P.has_sky;
     K.has_sun = P.has_sun;
     K.tiled = (c.width % 8 == 0 && c.height % 8 == 0) ? 1u : 0u;
-    K.tiles_even = ((c.width / 8u) * (c.height / 8u) + 1u) / 2u;
+    {
+        const uint32_t tiles = (c.width / 8u) * (c.height / 8u);
+        K.tparts = uint32_t(kPathParts);
+        uint32_t at = 0;
+        for (uint32_t h = 0; h < 4; ++h) {
+            K.tstart[h] = at;
+            if (h < K.tparts) at += (tiles + K.tparts - 1u - h) / K.tparts;  // tiles = h mod tparts
+        }
+    }
     K.vis_primary = (P.flags & GK_FRAME_PRIMARY_FROM_VISBUFFER) ? 1u : 0u;
     K.fast_gather = (P.flags & GK_FRAME_FAST_GATHER) ? 1u : 0u;
     K.probes = probe_dev(c);
@@ -1227,34 +1241,44 @@ int render_frame(Ctx& c) {
     k_copy_objid<<<nblk(n), 256, 0, s>>>(T.objid, static_cast<uint32_t*>(c.target_ptr[GK_RT_OBJECTID_1]), n);
     GK_HIP(hipGetLastError());
     const bool ref = c.params.estimator == GK_ESTIMATOR_REF_V2;
-    // PT_MIS frames of many paths run as two path pipelines, paths [0, n0) (even tiles) and
-    // [n0, n) (odd tiles), each with its own queue range, counters and trace slot, on two
-    // streams: a persistent trace launch spends a long tail finishing its last rays with most
+    // PT_MIS frames of many paths run as kPathParts path pipelines (part h: the 8x8 tiles
+    // h mod kPathParts), each with its own queue range, counters and trace slot, on its own
+    // stream: a persistent trace launch spends a long tail finishing its last rays with most
     // lanes idle, and one pipeline's launches overlap the other's. Each trace grid then holds
     // 3 work-groups per CU (kSplitBlocksPerCU), so both fit at once (1080p city: 3.33 ->
     // 2.97 ms per frame; 4096 bricks: 4.36 -> 4.10 ms).
     const bool split = !ref && !sort && n >= kSplitPaths && !(c.params.flags & GK_FRAME_SINGLE_PIPELINE);
-    const int np = split ? 2 : 1;
-    const uint32_t n0 = !split ? n : (K.tiled ? std::min(n, K.tiles_even * 64u) : n / 2u);
-    const uint32_t pbase[2] = {0u, n0}, pcount[2] = {n0, n - n0};
-    Queues QH[2] = {Q, Q};
-    hipStream_t ss[2] = {s, s};
-    uint32_t* tn[2] = {nullptr, nullptr};
+    const int np = split ? kPathParts : 1;
+    uint32_t pbase[4] = {0u, 0u, 0u, 0u}, pcount[4] = {n, 0u, 0u, 0u};
     if (split) {
-        if (!c.stream2) {
-            GK_HIP(hipStreamCreateWithFlags(&c.stream2, hipStreamNonBlocking));
-            GK_HIP(hipEventCreateWithFlags(&c.fork_ev, hipEventDisableTiming));
-            GK_HIP(hipEventCreateWithFlags(&c.join_ev, hipEventDisableTiming));
+        for (int h = 0; h < np; ++h) {
+            pbase[h] = K.tiled ? std::min(n, K.tstart[h] * 64u) : uint32_t(uint64_t(n) * uint32_t(h) / uint32_t(np));
+            const uint32_t end = h + 1 < np ? (K.tiled ? std::min(n, K.tstart[h + 1] * 64u)
+                                                      : uint32_t(uint64_t(n) * uint32_t(h + 1) / uint32_t(np)))
+                                            : n;
+            pcount[h] = end - pbase[h];
         }
-        ss[1] = c.stream2;
-        Queues& B = QH[1];
+    }
+    Queues QH[4] = {Q, Q, Q, Q};
+    hipStream_t ss[4] = {s, s, s, s};
+    uint32_t* tn[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int h = 1; h < np; ++h) {
+        if (!c.fork_ev) GK_HIP(hipEventCreateWithFlags(&c.fork_ev, hipEventDisableTiming));
+        if (!c.pstream[h - 1]) {
+            GK_HIP(hipStreamCreateWithFlags(&c.pstream[h - 1], hipStreamNonBlocking));
+            GK_HIP(hipEventCreateWithFlags(&c.join_ev[h - 1], hipEventDisableTiming));
+        }
+        ss[h] = c.pstream[h - 1];
+        // pipeline h: queue entries from its first path id on, counters from word 8 h
+        const uint32_t o = pbase[h];
+        Queues& B = QH[h];
         for (int k = 0; k < 2; ++k) {
-            B.o[k] += n0; B.d[k] += n0; B.path[k] += n0;
-            B.qthr[k] += n0; B.qrng[k] += n0; B.qmisc[k] += n0;
+            B.o[k] += o; B.d[k] += o; B.path[k] += o;
+            B.qthr[k] += o; B.qrng[k] += o; B.qmisc[k] += o;
         }
-        B.hit += n0; B.hit_inst += n0;
-        B.so += n0; B.sd += n0; B.sc_d += n0; B.sc_s += n0; B.spath += n0; B.shit += n0;
-        B.cnt += 8;
+        B.hit += o; B.hit_inst += o;
+        B.so += o; B.sd += o; B.sc_d += o; B.sc_s += o; B.spath += o; B.shit += o;
+        B.cnt += 8 * h;
     }
     // the slots' ray counters (a slot's first use allocates and clears them on `s`, so
     // before the fork)
@@ -1264,7 +1288,7 @@ int render_frame(Ctx& c) {
     }
     if (split) {
         GK_HIP(hipEventRecord(c.fork_ev, s));
-        GK_HIP(hipStreamWaitEvent(c.stream2, c.fork_ev, 0));
+        for (int h = 1; h < np; ++h) GK_HIP(hipStreamWaitEvent(ss[h], c.fork_ev, 0));
     }
     c.trace_split = split;
     for (int h = 0; h < np; ++h) {
@@ -1354,9 +1378,9 @@ int render_frame(Ctx& c) {
             cur = nxt;
         }
         c.trace_split = false;
-        if (split) {
-            GK_HIP(hipEventRecord(c.join_ev, c.stream2));
-            GK_HIP(hipStreamWaitEvent(s, c.join_ev, 0));
+        for (int h = 1; h < np; ++h) {
+            GK_HIP(hipEventRecord(c.join_ev[h - 1], ss[h]));
+            GK_HIP(hipStreamWaitEvent(s, c.join_ev[h - 1], 0));
         }
         k_pt_finalize<<<nblk(n), 256, 0, s>>>(K, T, P, n);
         GK_HIP(hipGetLastError());
