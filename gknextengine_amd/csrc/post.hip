@@ -185,7 +185,6 @@ constexpr int kR = 5, kJB = 16, kTile = kJB + 2 * kR;  // kJB x kJB pixels per w
 
 __global__ __launch_bounds__(kJB * kJB) void k_jbf(PostK K, PostT T) {
     __shared__ float3 sd[kTile][kTile];
-    __shared__ float3 ssp[kTile][kTile];
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int gx = blockIdx.x * kJB, gy = blockIdx.y * kJB;
     const int x = gx + tx, y = gy + ty;
@@ -194,16 +193,16 @@ __global__ __launch_bounds__(kJB * kJB) void k_jbf(PostK K, PostT T) {
         for (int idx = ty * kJB + tx; idx < kTile * kTile; idx += kJB * kJB) {
             const int row = idx / kTile, col = idx % kTile;
             const int cx = gx + col - kR, cy = gy + row - kR;
-            const float4 d = ld_h4(T.acc_d, cx, cy, K), s = ld_h4(T.acc_s, cx, cy, K);
+            const float4 d = ld_h4(T.acc_d, cx, cy, K);
             sd[row][col] = make_float3(d.x, d.y, d.z);
-            ssp[row][col] = make_float3(s.x, s.y, s.z);
         }
         __syncthreads();
         if (x >= int(K.W) || y >= int(K.H)) return;
         const float3 c3 = sd[ty + kR][tx + kR];
         const f3 cc = f3{c3.x + 0.001f, c3.y + 0.001f, c3.z + 0.001f};
-        const float3 s3 = ssp[ty + kR][tx + kR];
-        const f3 cspec = f3{s3.x + 0.001f, s3.y + 0.001f, s3.z + 0.001f};
+        // the specular term is read at the centre only
+        const float4 s4 = ld_h4(T.acc_s, x, y, K);
+        const f3 cspec = f3{s4.x + 0.001f, s4.y + 0.001f, s4.z + 0.001f};
         const f3 lc = f3{0.212671f, 0.715160f, 0.072169f};
         const float clum = dot(cc, lc);
         const float sigma = K.bf_sigma, sigma_l = K.bf_sigma_lum * 100.0f;
