@@ -1111,8 +1111,8 @@ int ensure_wavefront(Ctx& c) {
     GK_HIP(w.counters.resize(64));
     GK_HIP(w.sort_keys.resize(n));
     GK_HIP(w.sorted_idx.resize(n));
-    GK_HIP(w.sort_cnt.resize(32));
-    GK_HIP(hipMemsetAsync(w.sort_cnt.ptr, 0, 32 * 4, c.stream));
+    GK_HIP(w.sort_cnt.resize(32 * 4));  // 16 class counts and 16 cursors per path pipeline
+    GK_HIP(hipMemsetAsync(w.sort_cnt.ptr, 0, 32 * 4 * 4, c.stream));
     for (int k = 0; k < 2; ++k) {
         GK_HIP(w.q_thr[k].resize(n)); GK_HIP(w.q_rng[k].resize(n)); GK_HIP(w.q_misc[k].resize(n));
     }
@@ -1247,7 +1247,7 @@ int render_frame(Ctx& c) {
     // lanes idle, and one pipeline's launches overlap the other's. Each trace grid then holds
     // 3 work-groups per CU (kSplitBlocksPerCU), so both fit at once (1080p city: 3.33 ->
     // 2.97 ms per frame; 4096 bricks: 4.36 -> 4.10 ms).
-    const bool split = !ref && !sort && n >= kSplitPaths && !(c.params.flags & GK_FRAME_SINGLE_PIPELINE);
+    const bool split = !ref && n >= kSplitPaths && !(c.params.flags & GK_FRAME_SINGLE_PIPELINE);
     const int np = split ? kPathParts : 1;
     uint32_t pbase[4] = {0u, 0u, 0u, 0u}, pcount[4] = {n, 0u, 0u, 0u};
     if (split) {
@@ -1279,6 +1279,9 @@ int render_frame(Ctx& c) {
         B.hit += o; B.hit_inst += o;
         B.so += o; B.sd += o; B.sc_d += o; B.sc_s += o; B.spath += o; B.shit += o;
         B.cnt += 8 * h;
+        B.sort_keys += o;
+        if (B.sorted) B.sorted += o;
+        B.sort_cnt += 32 * h;
     }
     // the slots' ray counters (a slot's first use allocates and clears them on `s`, so
     // before the fork)
