@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the path-tracing hot path on one MI355X.
 
-Default workload (the metric's single-GPU configuration, config 2): the ~0.92 M-triangle
+Default workload (the metric's single-GPU configuration): the ~0.92 M-triangle
 procedural city at 1920x1080, PT_MIS estimator (GGX bounded-VNDF + Lambert, NEE to
 emissive triangles, MIS), 1 spp, 8 bounces, FP32, then reprojection and the bilateral
 denoiser. Scene and BVH are resident in HBM before the timed region; a step renders one
@@ -12,11 +12,11 @@ traversal kernel k_trace, timed with HIP events around every launch of a few mor
 rendered as one pipeline right after the timed region; ``cpu_baseline`` is the reference's
 tinybvh BVH::Intersect on host cores.
 
-Other configurations (``--config``): ``dynamic`` (config 3: 4096 moving instanced bricks,
+Other configurations (``--config``): ``dynamic`` (4096 moving instanced bricks,
 TLAS refit every frame, 4 bounces, reprojection + bilateral, 120 frames) and ``cornell``
-(config 1: 256x256, 16 frames x 1 spp, 4 bounces), ``diff`` (config 4: path-replay adjoint
+(256x256, 16 frames x 1 spp, 4 bounces), ``diff`` (path-replay adjoint
 at 512x512, 4 spp, with its finite-difference and 4096-spp reference-gradient checks) and
-``denoise`` (config 5: learned vs bilateral denoiser on 1-spp 1080p frames of the city and
+``denoise`` (learned vs bilateral denoiser on 1-spp 1080p frames of the city and
 the Cornell box), ``probes`` (the ambient-cube probe grid: voxelize and bake over the
 city), ``output`` (SuperResolution 1 render, HDR compose, linear and FSR EASU resolves), ``copy`` (a
 256 MiB device copy: the measured HBM ceiling and the counter calibration kernel). ``--relmse N``
@@ -289,7 +289,7 @@ def run_city(args):
         "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "procedural city (config 2), PT_MIS 1 spp 8 bounces + reproject + bilateral",
+        "config": {"workload": "procedural city, PT_MIS 1 spp 8 bounces + reproject + bilateral",
                    "resolution": [W, H], "triangles": int(ntris), "bounces": args.bounces, "spp": 1,
                    "estimator": "PT_MIS", "material_sort": sorted_shading,
                    "material_models": len(models_used),
@@ -346,7 +346,7 @@ def dynamic_cpu_baseline(ctx, b, p, W, H, worlds):
 
 
 def run_dynamic(args):
-    """Config 3: 120 frames of 4096 moving bricks at 1080p, 4 bounces, TLAS refit per
+    """120 frames of 4096 moving bricks at 1080p, 4 bounces, TLAS refit per
     frame from device-resident matrices, reprojection + bilateral."""
     import numpy as np
     import torch
@@ -416,7 +416,7 @@ def run_dynamic(args):
     return {"metric": METRIC, "value": round(rays / (ms.sum() * 1e-3) / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
             "steps": len(ms), "warmup": 9, "ms_per_step": round(float(np.median(ms)), 3), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": f"dynamic bricks (config 3): 4096 instances x 24 tris, TLAS "
+            "config": {"workload": f"dynamic bricks: 4096 instances x 24 tris, TLAS "
                                    f"{'rebuilt' if rebuild else 'refitted'} per frame, "
                                    "PT_MIS 1 spp 4 bounces, reproject + bilateral", "resolution": [W, H],
                        "tlas_update": args.tlas,
@@ -428,7 +428,7 @@ def run_dynamic(args):
 
 
 def run_cornell(args):
-    """Config 1: 256x256 Cornell box, 16 frames x 1 spp, 4 bounces."""
+    """The 256x256 Cornell box, 16 frames x 1 spp, 4 bounces."""
     import numpy as np
     import torch
 
@@ -494,7 +494,7 @@ def run_cornell(args):
     return {"metric": METRIC, "value": out["PT_MIS"]["mrays_per_s"], "unit": "Mrays/s", "n_gpus": 1, "steps": 16,
             "warmup": 3, "ms_per_step": out["PT_MIS"]["ms_per_frame"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "Cornell box (config 1), 256x256, 16 frames x 1 spp, 4 bounces", **out},
+            "config": {"workload": "Cornell box, 256x256, 16 frames x 1 spp, 4 bounces", **out},
             "cpu_baseline": cpu}
 
 
@@ -554,7 +554,7 @@ def denoise_compare(ctx, net, p, ref_spp, reps=20):
 
 
 def run_diff(args):
-    """Config 4: gradient of an L2 image loss with respect to 256 materials x (albedo rgb,
+    """Gradient of an L2 image loss with respect to 256 materials x (albedo rgb,
     roughness, metallic) of the tiled Cornell box at 512x512, 4 spp, 4 bounces, by the
     path-replay adjoint. The target is rendered (256 spp, other seeds) with the parameters
     perturbed by N(0, 0.05). Checks: central differences (h = 1e-3) of the same-seed 4-spp
@@ -719,7 +719,7 @@ def run_diff(args):
     return {"metric": METRIC, "value": round(ms, 3), "unit": "ms per forward+adjoint (512x512, 4 spp)",
             "n_gpus": 1, "steps": args.steps, "warmup": 2, "ms_per_step": round(ms, 3), "higher_is_better": False,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "differentiable render (config 4): tiled Cornell box, 256 materials x 5 params, "
+            "config": {"workload": "differentiable render: tiled Cornell box, 256 materials x 5 params, "
                                    "512x512, 4 spp, 4 bounces, L2 loss, path-replay adjoint",
                        "loss": loss, "paths_per_pass": W * H * spp,
                        "fd_albedo_h1e-3_median_rel_err": float(np.median(fd_err)),
@@ -752,7 +752,7 @@ def unet_flops(net, W, H, real_in=10):
 
 
 def run_denoise(args):
-    """Config 5: the learned denoiser against the bilateral filter on 1-spp frames at 1080p
+    """The learned denoiser against the bilateral filter on 1-spp frames at 1080p
     (the city at the benchmark camera, the Cornell box), relMSE against FP64 renders."""
     import gknextengine_amd as gk
     from gknextengine_amd import scenes
@@ -781,7 +781,7 @@ def run_denoise(args):
     return {"metric": METRIC, "value": c["relmse_unet"], "unit": "relMSE (linear, vs FP64 render)", "n_gpus": 1,
             "steps": 20, "warmup": 0, "ms_per_step": c["denoise_ms_unet"], "higher_is_better": False,
             "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
-            "config": {"workload": "learned denoiser (config 5) vs bilateral on 1-spp PT_MIS frames, 1920x1080",
+            "config": {"workload": "learned denoiser vs bilateral on 1-spp PT_MIS frames, 1920x1080",
                        "resolution": [W, H], **{f"{k}_{m}": v for k, d in res.items() for m, v in d.items()}},
             "roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
@@ -951,7 +951,7 @@ def run_copy(args):
 
 
 def run_diff_timing(args):
-    """Config 4's step alone: forward + adjoint of the L2 loss at 512x512, 4 spp, 4 bounces,
+    """The differentiable render's step alone: forward + adjoint of the L2 loss at 512x512, 4 spp, 4 bounces,
     256 materials x 5 parameters (run_diff adds the gradient checks)."""
     import numpy as np
     import torch
@@ -992,7 +992,7 @@ def run_diff_timing(args):
     ms = float(np.median(times) * 1e3)
     return {"metric": "forward + adjoint ms", "value": round(ms, 3), "unit": "ms", "steps": args.steps,
             "ms_per_step": round(ms, 3), "higher_is_better": False,
-            "config": {"workload": "differentiable PT_MIS (config 4) step: render + L2 loss + path-replay adjoint",
+            "config": {"workload": "differentiable PT_MIS step (512x512, 4 spp): render + L2 loss + path-replay adjoint",
                        "resolution": [W, H], "spp": 4, "bounces": 4, "materials": nmat, "params_per_material": 5,
                        "paths_per_step": W * H * 4}}
 

@@ -1,11 +1,11 @@
 """Benchmark scenes of the five measured configurations (host-side scene setup).
 
-    cornell   config 1: unit-cube Cornell box (gk_procgen_cornell), 256x256
-    city      config 2: ~0.92 M-triangle procedural city (gk_procgen_city), 1920x1080
-    bricks    config 3: 4096 instances of a 24-triangle brick (gk_procgen_brick) on a
+    cornell   unit-cube Cornell box (gk_procgen_cornell), 256x256
+    city      ~0.92 M-triangle procedural city (gk_procgen_city), 1920x1080
+    bricks    4096 instances of a 24-triangle brick (gk_procgen_brick) on a
               16^3 lattice with random rigid motion (angular velocity N(0,0.5) rad/s about
               a random axis, linear velocity N(0,1) m/s), 8 materials
-    tiles     config 4: the Cornell box with its walls split into 256 material tiles
+    tiles     the Cornell box with its walls split into 256 material tiles
 """
 from __future__ import annotations
 
@@ -50,7 +50,7 @@ def rotations(axes: np.ndarray, angles: np.ndarray) -> np.ndarray:
 
 
 class Bricks:
-    """Config 3: instanced bricks with per-instance rigid motion."""
+    """4096 instanced bricks with per-instance rigid motion."""
 
     def __init__(self, n_side=16, spacing=1.5, n_materials=8, seed=7):
         rng = np.random.default_rng(seed)
@@ -86,7 +86,7 @@ class Bricks:
 
 
 def cornell_tiles(tiles_per_wall=8, seed=3):
-    """Config 4 scene: the Cornell box whose walls and boxes are split into tiles with
+    """The differentiable-rendering scene: the Cornell box whose walls and boxes are split into tiles with
     their own Mixture material (256 materials = 5 walls x 8x8 tiles truncated, plus the
     boxes' faces), light unchanged. Returns (SceneData, material count)."""
     base = procgen_cornell()

@@ -31,7 +31,7 @@ def sha(a: np.ndarray) -> str:
 
 def main():
     meta = {}
-    # ---- config 1: Cornell box, 256x256 primary rays (every 4th pixel stored)
+    # ---- Cornell box, 256x256 primary rays (every 4th pixel stored)
     s = gk.procgen_cornell()
     tris = po.scene_triangles(s.meshes[0])
     W = H = 256
@@ -44,7 +44,7 @@ def main():
     np.savez_compressed(HERE / "cornell_tinybvh.npz", tris=tris, rays=rays, hits=hits,
                         occ_rays=scenes.random_rays(4096, 0.05, 0.95, seed=3, tmax=0.5), occ=occ)
     meta["cornell"] = {"tris_sha256": sha(tris), "nodes": nodes, "sah": sah, "triangles": int(tris.shape[0])}
-    # ---- config 2: city, BVH statistics and 4096 rays
+    # ---- 1080p procedural city, BVH statistics and 4096 rays
     c = gk.procgen_city()
     ctris = po.scene_triangles(c.meshes[0])
     cb = po.TinyBVH(ctris)
@@ -58,7 +58,7 @@ def main():
     chits, _ = cb.intersect(crays)
     np.savez_compressed(HERE / "city_tinybvh.npz", rays=crays, hits=chits)
     meta["city"] = {"tris_sha256": sha(ctris), "nodes": nodes, "sah": sah, "triangles": int(ctris.shape[0])}
-    # ---- config 3: 4096 instanced bricks (TLAS), frame 0
+    # ---- 4096 instanced bricks (TLAS), frame 0
     b = scenes.Bricks()
     btris = po.scene_triangles(b.scene.meshes[0])
     bb = po.TinyBVH(btris)

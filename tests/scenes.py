@@ -1,4 +1,4 @@
-"""Benchmark scenes shared by the tests (BASELINE configurations, scaled where noted)."""
+"""Benchmark scenes shared by the tests (the benchmark configurations, scaled where noted)."""
 from __future__ import annotations
 
 import numpy as np

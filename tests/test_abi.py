@@ -114,7 +114,7 @@ def test_procgen_counts():
 def test_city_full_size_matches_config():
     c = gk.procgen_city()
     nt = c.meshes[0]["indices"].shape[0]
-    assert 0.9e6 < nt < 0.95e6  # ~1M triangles (config 2)
+    assert 0.9e6 < nt < 0.95e6  # ~1M triangles (the procedural city)
     metal = np.array([m.metalness for m in c.materials[:4096]])
     rough = np.array([m.fuzziness for m in c.materials[:4096]])
     assert 0.25 < metal.mean() < 0.35 and set(np.unique(metal)) <= {0.0, 1.0}

@@ -1,4 +1,4 @@
-"""Config 4: path-replay adjoint of the differentiable PT_MIS.
+"""Path-replay adjoint of the differentiable PT_MIS (512x512 Cornell tiles, 256 materials).
 
 Checks:
   * the GPU primal equals the oracle's double-precision detached render;
@@ -176,7 +176,7 @@ def test_backward_replays_its_own_forward_after_interleaved_renders():
 
 
 def test_descent_recovers_perturbed_parameters_from_an_independent_target():
-    """Config 4's set-up at reduced size: the target is a 2048-spp render with parameters
+    """The differentiable render's set-up at reduced size: the target is a 2048-spp render with parameters
     perturbed by N(0, 0.05), drawn from seeds the optimisation never uses; plain
     loss.backward() (adjoint replayed with independent seeds) drives Adam back towards the
     target's albedos.

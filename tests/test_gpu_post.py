@@ -1,4 +1,4 @@
-"""Config 3 (dynamic instanced bricks): TLAS refit, motion vectors, reprojection and the
+"""Dynamic instanced bricks: TLAS refit, motion vectors, reprojection and the
 bilateral denoiser against the oracle restatements."""
 from __future__ import annotations
 

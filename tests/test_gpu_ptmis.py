@@ -92,7 +92,7 @@ CROPS = [(128, 96), (512, 400), (900, 500), (1400, 200), (1700, 900), (300, 800)
 
 @pytest.mark.parametrize("sky", [None, (0.6, 0.7, 0.9)])
 def test_pt_mis_city_1080p_8_bounces_matches_oracle_on_crops(city, sky):
-    """The benchmark frame (config 2: 0.92 M triangles, GGX materials with roughness
+    """The benchmark frame (the procedural city: 0.92 M triangles, GGX materials with roughness
     U(0.05, 1) and metallic Bernoulli(0.3), 64 emissive roofs, 1920x1080, 8 bounces) and
     the same frame under a sky, against the oracle on eight 64x64 crops."""
     s, osc = city

@@ -28,7 +28,7 @@ def relmse(x, ref):
 
 
 def test_ref_v2_cornell_frames_match_oracle():
-    """Config 1: 256x256, 16 frames x 1 spp, 4 bounces, REF_V2 estimator."""
+    """The Cornell box at 256x256, 16 frames x 1 spp, 4 bounces, REF_V2 estimator."""
     W = H = 256
     s = gk.procgen_cornell()
     osc = po.OracleScene(s)
