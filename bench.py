@@ -997,6 +997,51 @@ def run_diff_timing(args):
                        "paths_per_step": W * H * 4}}
 
 
+def run_city_ref_v2(args):
+    """The reference's own estimator (REF_V2: FPathTracingRendererV2's Render loop, early exit
+    0.5, no next-event estimation, probes off) on the 1080p procedural city, 1 spp, 8
+    bounces, with reprojection and the bilateral filter: 3 warm-up frames, then 20 timed."""
+    import torch
+
+    import gknextengine_amd as gk
+    from gknextengine_amd import scenes
+    W, H = 1920, 1080
+    scene = gk.procgen_city(64, 4, 256, 64, 2024)
+    ctx = gk.Context(W, H)
+    ctx.upload(scene)
+    cam = scenes.CITY_CAM
+    p = gk.make_frame_params(cam["eye"], cam["target"], W, H, fov=cam["fov"])
+    p.estimator = gk.ESTIMATOR_REF_V2
+    p.bounces = p.max_bounces = 8
+    p.samples = 1
+    p.global_seed = 1
+    p.denoiser = gk.DENOISER_BILATERAL
+    frames = 20
+    for f in range(3):
+        p.total_frames = f
+        ctx.frame(p)
+        ctx.render()
+    ctx.sync()
+    s0 = ctx.stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(frames):
+        p.total_frames = 3 + f
+        ctx.frame(p)
+        ctx.render()
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    s1 = ctx.stats()
+    ctx.close()
+    rays = s1.rays_traced - s0.rays_traced
+    return {"metric": METRIC, "value": round(rays / wall / 1e6, 2), "unit": "Mrays/s", "steps": frames,
+            "ms_per_step": round(wall / frames * 1e3, 3), "higher_is_better": True,
+            "config": {"workload": "procedural city, REF_V2 estimator 1 spp 8 bounces + reproject + bilateral",
+                       "resolution": [W, H], "rays_per_frame": round(rays / frames),
+                       "trace_ms_per_frame": round((s1.ms_trace - s0.ms_trace) / frames, 3)}}
+
+
 def run_side_configs(args):
     """The other configurations, measured in the same process after the headline one (a
     few seconds each; reduced reference spp for the relMSE figures)."""
@@ -1004,7 +1049,7 @@ def run_side_configs(args):
 
     import torch
     side = {}
-    specs = (("cornell", run_cornell, {"relmse": 1024}), ("dynamic", run_dynamic, {}),
+    specs = (("city_ref_v2", run_city_ref_v2, {}), ("cornell", run_cornell, {"relmse": 1024}), ("dynamic", run_dynamic, {}),
              ("denoise", run_denoise, {"relmse": 256}), ("diff", run_diff_timing, {"steps": 10}))
     for name, fn, over in specs:
         a = copy.copy(args)

@@ -263,7 +263,7 @@ struct Ctx {
     bool stack_overflow = false;        // a traversal stack overflowed since gk_get_stats last reported
     bool trace_counter_zeroed[4] = {};  // per slot: the next launch's counter is already 0
     bool trace_split = false;           // a frame's two path pipelines are being enqueued
-    uint32_t* h_word = nullptr;         // pinned host word for counter read-backs
+    uint32_t* h_word = nullptr;         // pinned host words (one per path pipeline) for counter read-backs
     gk_stats stats{};
 };
 

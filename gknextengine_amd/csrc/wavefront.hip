@@ -1241,13 +1241,13 @@ int render_frame(Ctx& c) {
     k_copy_objid<<<nblk(n), 256, 0, s>>>(T.objid, static_cast<uint32_t*>(c.target_ptr[GK_RT_OBJECTID_1]), n);
     GK_HIP(hipGetLastError());
     const bool ref = c.params.estimator == GK_ESTIMATOR_REF_V2;
-    // PT_MIS frames of many paths run as kPathParts path pipelines (part h: the 8x8 tiles
-    // h mod kPathParts), each with its own queue range, counters and trace slot, on its own
-    // stream: a persistent trace launch spends a long tail finishing its last rays with most
-    // lanes idle, and one pipeline's launches overlap the other's. Each trace grid then holds
-    // 3 work-groups per CU (kSplitBlocksPerCU), so both fit at once (1080p city: 3.33 ->
-    // 2.97 ms per frame; 4096 bricks: 4.36 -> 4.10 ms).
-    const bool split = !ref && n >= kSplitPaths && !(c.params.flags & GK_FRAME_SINGLE_PIPELINE);
+    // Frames of many paths run as kPathParts path pipelines (part h: the 8x8 tiles h mod
+    // kPathParts), each with its own queue range, counters and trace slot, on its own stream:
+    // a persistent trace launch spends a long tail finishing its last rays with most lanes
+    // idle, and one pipeline's launches overlap the other's. Each trace grid then holds 3
+    // work-groups per CU (kSplitBlocksPerCU), so both fit at once (1080p city, PT_MIS: 3.33
+    // -> 2.97 ms per frame; 4096 bricks: 4.36 -> 4.10 ms).
+    const bool split = n >= kSplitPaths && !(c.params.flags & GK_FRAME_SINGLE_PIPELINE);
     const int np = split ? kPathParts : 1;
     uint32_t pbase[4] = {0u, 0u, 0u, 0u}, pcount[4] = {n, 0u, 0u, 0u};
     if (split) {
@@ -1310,36 +1310,60 @@ int render_frame(Ctx& c) {
     }
     int cur = 0;
     if (ref) {
-        uint32_t* tnext = tn[0];
-        k_ref_primary<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, tnext);
-        GK_HIP(hipGetLastError());
+        for (int h = 0; h < np; ++h) {
+            k_ref_primary<<<nblk(pcount[h]), 256, 0, ss[h]>>>(S0, K, T, QH[h], P, tn[h]);
+            GK_HIP(hipGetLastError());
+        }
         cur = 1;
         // worst case: every sample uses all bounces (samples * 4 for moving pixels); a sample
         // traces at least once, also with no bounces
         const uint32_t maxb = std::max(1u, std::max(K.bounces, K.max_bounces));
         const uint32_t iters = maxb * K.samples * 4u + 4u;
-        // The loop ends early once the queue about to be traced is empty: every sample has
-        // finished. Reading its counter back costs a stream synchronisation, so it is read
-        // after the iterations a pixel without motion can take, then every maxb iterations;
-        // a stream under capture runs the worst case.
+        // A pipeline's loop ends early once the queue it is about to trace is empty: every
+        // sample has finished. Reading the counters back costs a synchronisation of the
+        // streams, so they are read after the iterations a pixel without motion can take,
+        // then every maxb iterations; a stream under capture runs the worst case.
         hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
         GK_HIP(hipStreamIsCapturing(s, &cap));
-        if (cap == hipStreamCaptureStatusNone && !c.h_word) GK_HIP(hipHostMalloc(&c.h_word, sizeof(uint32_t)));
+        if (cap == hipStreamCaptureStatusNone && !c.h_word) GK_HIP(hipHostMalloc(&c.h_word, 4 * sizeof(uint32_t)));
         uint32_t check = cap == hipStreamCaptureStatusNone ? maxb * std::max(1u, K.samples) + 1u : iters;
+        bool live[4] = {true, true, true, true};
         for (uint32_t it = 0; it < iters; ++it) {
             if (it == check) {
-                GK_HIP(hipMemcpyAsync(c.h_word, Q.cnt + cur, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-                GK_HIP(hipStreamSynchronize(s));
-                if (*c.h_word == 0u) break;
+                for (int h = 0; h < np; ++h)
+                    if (live[h])
+                        GK_HIP(hipMemcpyAsync(c.h_word + h, QH[h].cnt + cur, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                              ss[h]));
+                bool any = false;
+                for (int h = 0; h < np; ++h) {
+                    if (!live[h]) continue;
+                    GK_HIP(hipStreamSynchronize(ss[h]));
+                    live[h] = c.h_word[h] != 0u;
+                    any = any || live[h];
+                }
+                if (!any) break;
                 check += maxb;
             }
-            c.trace_counter_zeroed[0] = true;
-            // the trace zeroes the counter of the queue k_ref_shade appends to
-            st = launch_trace(c, Q.o[cur], Q.d[cur], Q.cnt + cur, n, Q.hit, Q.hit_inst, false, Q.cnt + (cur ^ 1));
-            if (st != GK_OK) return st;
-            k_ref_shade<<<nblk(n), 256, 0, s>>>(S0, K, T, Q, P, cur, tnext);
-            GK_HIP(hipGetLastError());
+            for (int h = 0; h < np; ++h) {
+                if (!live[h]) continue;
+                c.trace_counter_zeroed[h] = true;
+                // the trace zeroes the counter of the queue k_ref_shade appends to
+                const Queues& q = QH[h];
+                st = launch_trace(c, q.o[cur], q.d[cur], q.cnt + cur, pcount[h], q.hit, q.hit_inst, false,
+                                  q.cnt + (cur ^ 1), h);
+                if (st != GK_OK) return st;
+            }
+            for (int h = 0; h < np; ++h) {
+                if (!live[h]) continue;
+                k_ref_shade<<<nblk(pcount[h]), 256, 0, ss[h]>>>(S0, K, T, QH[h], P, cur, tn[h]);
+                GK_HIP(hipGetLastError());
+            }
             cur ^= 1;
+        }
+        c.trace_split = false;
+        for (int h = 1; h < np; ++h) {
+            GK_HIP(hipEventRecord(c.join_ev[h - 1], ss[h]));
+            GK_HIP(hipStreamWaitEvent(s, c.join_ev[h - 1], 0));
         }
     } else {
         // Each iteration shades the hits of queue cur, then one launch traces both the

@@ -260,8 +260,8 @@ typedef struct gk_frame_params {
  * neither ends on an emitter/sky nor starts at a dielectric primary ends there, multiplied
  * by the probe lookup at that bounce's vertex (Shading.slang:1118-1124). */
 #define GK_FRAME_FAST_GATHER 256u
-/* PT_MIS frames of at least 2^18 paths run as two independent path pipelines (even and
- * odd 8x8 pixel tiles) on two streams, so that one pipeline's launches overlap the other's;
+/* Frames of at least 2^18 paths run as two independent path pipelines (even and odd 8x8
+ * pixel tiles) on two streams, so that one pipeline's launches overlap the other's;
  * the image is the same either way. This flag keeps one pipeline. */
 #define GK_FRAME_SINGLE_PIPELINE 512u
 #define GK_FRAME_ACCUM_FP64 4u         /* also add the frame's radiance to GK_RT_RADIANCE64 */

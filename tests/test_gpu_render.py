@@ -305,3 +305,39 @@ def test_city_visibility_buffer_ids_and_hits_from_them():
     np.testing.assert_array_equal(rt, prim[sub])
     x = rays[sub, :3] + hit[sub, :1] * rays[sub, 4:7]
     np.testing.assert_allclose(pos, x, atol=2e-3, rtol=1e-4)
+
+
+def test_ref_v2_two_pipelines_give_the_single_pipeline_frame():
+    """REF_V2 frames of 2^18 paths or more run as two path pipelines (even and odd 8x8 tiles),
+    each ending its bounce loop when its own queue is empty. On 4096 moving bricks at
+    640x480 under a sky, 2 samples, three frames with instance motion: every target bit-identical to GK_FRAME_SINGLE_PIPELINE's, same ray count."""
+    from gknextengine_amd import scenes as gs
+    b = scenes.Bricks()
+    W, H = 640, 480
+    names = [t for t in gk.TARGETS if t not in ("RADIANCE64",)]
+    outs = []
+    for flags in (gk.FRAME_SINGLE_PIPELINE, 0):
+        with gk.Context(W, H) as ctx:
+            ctx.upload(b.scene)
+            for f in range(3):
+                if f:
+                    ctx.update_instances(b.worlds(f / 60.0))
+                p = gk.make_frame_params(gs.BRICKS_CAM["eye"], gs.BRICKS_CAM["target"], W, H,
+                                         fov=gs.BRICKS_CAM["fov"], frame=f)
+                p.estimator = gk.ESTIMATOR_REF_V2
+                p.bounces = p.max_bounces = 4
+                p.samples = 2
+                p.has_sky = 1
+                p.sky_color[:] = [0.5, 0.6, 0.8, 1.0]
+                p.denoiser = gk.DENOISER_BILATERAL
+                p.flags |= flags | gk.FRAME_TLAS_REBUILD
+                ctx.frame(p)
+                ctx.render()
+            ctx.sync()
+            outs.append(({t: ctx.read(t).copy() for t in names}, ctx.stats().rays_traced))
+    (a, ra), (c, rc) = outs
+    assert ra == rc, (ra, rc)
+    for t in names:
+        assert np.array_equal(a[t].view(np.uint8), c[t].view(np.uint8)), t
+    print(f"moving pixels {(a['MOTIONMOMENT'] != 0).mean():.3f}, rays {ra}")
+    assert ra > 2 * W * H and a["RADIANCE"][..., :3].max() > 0.1
