@@ -204,12 +204,15 @@ struct Ctx {
         DevBuf<gk_ambient_cube> cubes;
         DevBuf<gk_ambient_cube> next;   // bake results, applied when the pass ends
         DevBuf<gk_voxel> voxels;
-        DevBuf<uint32_t> list;          // probes of the chunk being baked
-        DevBuf<uint32_t> counters;      // [0] probes in list, [1] closest rays, [2] occlusion rays
-        DevBuf<float4> ro, rd, hit;
-        DevBuf<uint32_t> hit_inst;
-        DevBuf<float4> so, sd, shit;
-        DevBuf<uint32_t> face_sky;      // per (probe, face): sky byte, sun flag and bit
+        struct Work {                   // one chunk in flight; consecutive bake chunks alternate
+                                        // between two (trace slots 0 and 1, two streams)
+            DevBuf<uint32_t> list;      // probes of the chunk being baked
+            DevBuf<uint32_t> counters;  // [0] probes in list, [1] closest rays, [2] occlusion rays
+            DevBuf<float4> ro, rd, hit;
+            DevBuf<uint32_t> hit_inst;
+            DevBuf<float4> so, sd, shit;
+            DevBuf<uint32_t> face_sky;  // per (probe, face): sky byte, sun flag and bit
+        } w[2];
     } probes;
 
     // frame

@@ -420,7 +420,9 @@ GridK grid_k(const Ctx& c) {
 // keep room for the trace launches' timing events (folded into the frame statistics)
 int reserve_trace_events(Ctx& c) {
     if (c.tev_used + 64u > c.tev.size() && c.tev_used > 0) {
+        // the recorded launches may be on any slot's stream
         GK_HIP(hipStreamSynchronize(c.stream));
+        for (auto& ps : c.pstream) if (ps) GK_HIP(hipStreamSynchronize(ps));
         return collect_timing(c);
     }
     return GK_OK;
@@ -466,20 +468,21 @@ int probes_voxelize(Ctx& c) {
     int st = reserve_trace_events(c);
     if (st != GK_OK) return st;
     const uint32_t chunk = std::min(kChunk, P.count);
+    auto& W = P.w[0];
     const size_t nr = size_t(chunk) * kVoxRays;
-    GK_HIP(P.ro.resize(std::max(P.ro.count, nr)));
-    GK_HIP(P.rd.resize(std::max(P.rd.count, nr)));
-    GK_HIP(P.hit.resize(std::max(P.hit.count, nr)));
-    GK_HIP(P.hit_inst.resize(std::max(P.hit_inst.count, nr)));
+    GK_HIP(W.ro.resize(std::max(W.ro.count, nr)));
+    GK_HIP(W.rd.resize(std::max(W.rd.count, nr)));
+    GK_HIP(W.hit.resize(std::max(W.hit.count, nr)));
+    GK_HIP(W.hit_inst.resize(std::max(W.hit_inst.count, nr)));
     const GridK g = grid_k(c);
     const DScene s = make_dscene(c);
     for (uint32_t first = 0; first < P.count; first += chunk) {
         const uint32_t n = std::min(chunk, P.count - first);
-        k_vox_rays<<<nblk(size_t(n) * kVoxRays), 256, 0, c.stream>>>(g, first, n, P.ro.ptr, P.rd.ptr);
+        k_vox_rays<<<nblk(size_t(n) * kVoxRays), 256, 0, c.stream>>>(g, first, n, W.ro.ptr, W.rd.ptr);
         GK_HIP(hipGetLastError());
-        st = launch_trace(c, P.ro.ptr, P.rd.ptr, nullptr, n * kVoxRays, P.hit.ptr, P.hit_inst.ptr, false, nullptr);
+        st = launch_trace(c, W.ro.ptr, W.rd.ptr, nullptr, n * kVoxRays, W.hit.ptr, W.hit_inst.ptr, false, nullptr);
         if (st != GK_OK) return st;
-        k_vox_finish<<<nblk(n), 256, 0, c.stream>>>(s, g, first, n, P.rd.ptr, P.hit.ptr, P.hit_inst.ptr,
+        k_vox_finish<<<nblk(n), 256, 0, c.stream>>>(s, g, first, n, W.rd.ptr, W.hit.ptr, W.hit_inst.ptr,
                                                     c.tri_rec.ptr, P.voxels.ptr);
         GK_HIP(hipGetLastError());
         st = reserve_trace_events(c);
@@ -517,44 +520,74 @@ int probes_bake(Ctx& c, uint32_t first, uint32_t count) {
     }
     K.nany = K.has_light + K.has_sun;
     const uint32_t chunk = std::min(kChunk, count);
-    GK_HIP(P.list.resize(std::max<size_t>(P.list.count, chunk)));
-    GK_HIP(P.counters.resize(4));
-    GK_HIP(P.face_sky.resize(std::max<size_t>(P.face_sky.count, size_t(chunk) * 6)));
+    // Consecutive chunks alternate between two work sets, trace slots and streams, so one
+    // chunk's probe lookups and its trace's tail overlap the next chunk's trace. Chunks touch
+    // disjoint probes (their list, next and face results); k_bake_apply waits for both.
+    const bool two = count > chunk;
+    const int nw = two ? 2 : 1;
     const size_t nr = size_t(chunk) * kProbeRays;
-    GK_HIP(P.ro.resize(std::max(P.ro.count, nr)));
-    GK_HIP(P.rd.resize(std::max(P.rd.count, nr)));
-    GK_HIP(P.hit.resize(std::max(P.hit.count, nr)));
-    GK_HIP(P.hit_inst.resize(std::max(P.hit_inst.count, nr)));
     const size_t ns = size_t(chunk) * 6u * std::max(1u, K.nany);
-    GK_HIP(P.so.resize(std::max(P.so.count, ns)));
-    GK_HIP(P.sd.resize(std::max(P.sd.count, ns)));
-    GK_HIP(P.shit.resize(std::max(P.shit.count, ns)));
+    for (int h = 0; h < nw; ++h) {
+        auto& W = P.w[h];
+        GK_HIP(W.list.resize(std::max<size_t>(W.list.count, chunk)));
+        GK_HIP(W.counters.resize(4));
+        GK_HIP(W.face_sky.resize(std::max<size_t>(W.face_sky.count, size_t(chunk) * 6)));
+        GK_HIP(W.ro.resize(std::max(W.ro.count, nr)));
+        GK_HIP(W.rd.resize(std::max(W.rd.count, nr)));
+        GK_HIP(W.hit.resize(std::max(W.hit.count, nr)));
+        GK_HIP(W.hit_inst.resize(std::max(W.hit_inst.count, nr)));
+        GK_HIP(W.so.resize(std::max(W.so.count, ns)));
+        GK_HIP(W.sd.resize(std::max(W.sd.count, ns)));
+        GK_HIP(W.shit.resize(std::max(W.shit.count, ns)));
+        if (!trace_counter(c, h)) return GK_ERR_OOM;  // a slot's first use clears its counters on `stream`
+    }
+    hipStream_t ss[2] = {c.stream, c.stream};
+    if (two) {
+        if (!c.fork_ev) GK_HIP(hipEventCreateWithFlags(&c.fork_ev, hipEventDisableTiming));
+        if (!c.pstream[0]) {
+            GK_HIP(hipStreamCreateWithFlags(&c.pstream[0], hipStreamNonBlocking));
+            GK_HIP(hipEventCreateWithFlags(&c.join_ev[0], hipEventDisableTiming));
+        }
+        ss[1] = c.pstream[0];
+        GK_HIP(hipEventRecord(c.fork_ev, c.stream));
+        GK_HIP(hipStreamWaitEvent(ss[1], c.fork_ev, 0));
+    }
+    c.trace_split = two;
     const DScene s = make_dscene(c);
     const ProbeDev PD = probe_dev(c);
+    int h = 0;
     for (uint32_t c0 = first; c0 < first + count; c0 += chunk) {
         const uint32_t n = std::min(chunk, first + count - c0);
-        k_zero_u32<<<1, 64, 0, c.stream>>>(P.counters.ptr, 4);
-        k_bake_select<<<nblk(n), 256, 0, c.stream>>>(P.voxels.ptr, c0, n, P.list.ptr, P.counters.ptr);
-        k_bake_counts<<<1, 1, 0, c.stream>>>(P.counters.ptr, K.nany);
-        k_bake_rays<<<nblk(size_t(n) * kProbeRays), 256, 0, c.stream>>>(K, P.voxels.ptr, P.list.ptr, P.counters.ptr,
-                                                                         P.ro.ptr, P.rd.ptr, P.so.ptr, P.sd.ptr);
+        auto& W = P.w[h];
+        hipStream_t st_h = ss[h];
+        k_zero_u32<<<1, 64, 0, st_h>>>(W.counters.ptr, 4);
+        k_bake_select<<<nblk(n), 256, 0, st_h>>>(P.voxels.ptr, c0, n, W.list.ptr, W.counters.ptr);
+        k_bake_counts<<<1, 1, 0, st_h>>>(W.counters.ptr, K.nany);
+        k_bake_rays<<<nblk(size_t(n) * kProbeRays), 256, 0, st_h>>>(K, P.voxels.ptr, W.list.ptr, W.counters.ptr,
+                                                                     W.ro.ptr, W.rd.ptr, W.so.ptr, W.sd.ptr);
         GK_HIP(hipGetLastError());
         if (K.nany)
-            st = launch_trace_pair(c, P.ro.ptr, P.rd.ptr, P.counters.ptr + 1, n * kProbeRays, P.hit.ptr,
-                                   P.hit_inst.ptr, P.so.ptr, P.sd.ptr, P.counters.ptr + 2, n * 6u * K.nany,
-                                   P.shit.ptr);
+            st = launch_trace_pair(c, W.ro.ptr, W.rd.ptr, W.counters.ptr + 1, n * kProbeRays, W.hit.ptr,
+                                   W.hit_inst.ptr, W.so.ptr, W.sd.ptr, W.counters.ptr + 2, n * 6u * K.nany,
+                                   W.shit.ptr, h);
         else
-            st = launch_trace(c, P.ro.ptr, P.rd.ptr, P.counters.ptr + 1, n * kProbeRays, P.hit.ptr, P.hit_inst.ptr,
-                              false, nullptr);
+            st = launch_trace(c, W.ro.ptr, W.rd.ptr, W.counters.ptr + 1, n * kProbeRays, W.hit.ptr, W.hit_inst.ptr,
+                              false, nullptr, h);
         if (st != GK_OK) return st;
-        k_bake_face<<<nblk(size_t(n) * kProbeRays), 256, 0, c.stream>>>(s, K, PD, P.voxels.ptr, P.list.ptr, P.counters.ptr,
-                                                                 P.ro.ptr, P.rd.ptr, P.hit.ptr, P.hit_inst.ptr,
-                                                                 P.shit.ptr, P.next.ptr, P.face_sky.ptr);
-        k_bake_commit<<<nblk(n), 256, 0, c.stream>>>(P.list.ptr, P.counters.ptr, P.cubes.ptr, P.face_sky.ptr,
-                                                     P.next.ptr, P.voxels.ptr);
+        k_bake_face<<<nblk(size_t(n) * kProbeRays), 256, 0, st_h>>>(s, K, PD, P.voxels.ptr, W.list.ptr, W.counters.ptr,
+                                                             W.ro.ptr, W.rd.ptr, W.hit.ptr, W.hit_inst.ptr,
+                                                             W.shit.ptr, P.next.ptr, W.face_sky.ptr);
+        k_bake_commit<<<nblk(n), 256, 0, st_h>>>(W.list.ptr, W.counters.ptr, P.cubes.ptr, W.face_sky.ptr,
+                                                 P.next.ptr, P.voxels.ptr);
         GK_HIP(hipGetLastError());
         st = reserve_trace_events(c);
         if (st != GK_OK) return st;
+        h = two ? h ^ 1 : 0;
+    }
+    c.trace_split = false;
+    if (two) {
+        GK_HIP(hipEventRecord(c.join_ev[0], ss[1]));
+        GK_HIP(hipStreamWaitEvent(c.stream, c.join_ev[0], 0));
     }
     k_bake_apply<<<nblk(count), 256, 0, c.stream>>>(P.voxels.ptr, first, count, P.next.ptr, P.cubes.ptr);
     GK_HIP(hipGetLastError());
