@@ -343,10 +343,12 @@ __device__ inline void write_miss_gbuffer(const FrameK& K, const Targets& T, uin
 // ---------------------------------------------------------------- raygen
 // queue_state: PT_MIS keeps its per-path state with the queue entries (Queues::q*),
 // REF_V2 in PathState
-// camera rays of paths p_base .. p_base + n - 1 into queue entries 0 .. n - 1
-__global__ void k_raygen(FrameK K, Queues Q, PathState S, uint32_t n, uint32_t queue_state, uint32_t p_base) {
+// camera rays of paths p_base .. p_base + n - 1 into queue entries 0 .. n - 1; also zeroes the
+// queue counters and the ray counter of the trace launch that follows (trace_next)
+__global__ void k_raygen(FrameK K, Queues Q, PathState S, uint32_t n, uint32_t queue_state, uint32_t p_base,
+                         uint32_t* __restrict__ trace_next) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) { Q.cnt[0] = n; Q.cnt[1] = 0u; Q.cnt[2] = 0u; Q.cnt[3] = 0u; }
+    if (i == 0) { Q.cnt[0] = n; Q.cnt[1] = 0u; Q.cnt[2] = 0u; Q.cnt[3] = 0u; *trace_next = 0u; }
     if (i >= n) return;
     const uint32_t p = p_base + i;
     const uint32_t px = pixel_of(K, p);
@@ -1295,7 +1297,7 @@ int render_frame(Ctx& c) {
     }
     c.trace_split = split;
     for (int h = 0; h < np; ++h) {
-        k_raygen<<<nblk(pcount[h]), 256, 0, ss[h]>>>(K, QH[h], P, pcount[h], ref ? 0u : 1u, pbase[h]);
+        k_raygen<<<nblk(pcount[h]), 256, 0, ss[h]>>>(K, QH[h], P, pcount[h], ref ? 0u : 1u, pbase[h], tn[h]);
         GK_HIP(hipGetLastError());
         // primary rays: traced, or their hits taken from the caller's visibility buffer
         if (K.vis_primary) {
@@ -1303,6 +1305,7 @@ int render_frame(Ctx& c) {
                                                               c.proxy_inst.ptr, c.num_proxies, pcount[h], pbase[h]);
             GK_HIP(hipGetLastError());
         } else {
+            c.trace_counter_zeroed[h] = true;
             st = launch_trace(c, QH[h].o[0], QH[h].d[0], QH[h].cnt + 0, pcount[h], QH[h].hit, QH[h].hit_inst, false,
                               nullptr, h);
             if (st != GK_OK) return st;
