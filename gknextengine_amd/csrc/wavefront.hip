@@ -1126,6 +1126,11 @@ int ensure_wavefront(Ctx& c) {
     return GK_OK;
 }
 
+// whether a frame runs as kPathParts path pipelines (render_frame)
+static bool frame_splits(const Ctx& c) {
+    return c.width * c.height >= kSplitPaths && !(c.params.flags & GK_FRAME_SINGLE_PIPELINE);
+}
+
 static FrameK make_k(const Ctx& c) {
     const gk_frame_params& P = c.params;
     FrameK K;
@@ -1154,8 +1159,10 @@ static FrameK make_k(const Ctx& c) {
     K.has_sun = P.has_sun;
     K.tiled = (c.width % 8 == 0 && c.height % 8 == 0) ? 1u : 0u;
     {
+        // one part (plain row-major tile order, the best cache locality) unless the frame
+        // will be split into path pipelines
         const uint32_t tiles = (c.width / 8u) * (c.height / 8u);
-        K.tparts = uint32_t(kPathParts);
+        K.tparts = frame_splits(c) ? uint32_t(kPathParts) : 1u;
         uint32_t at = 0;
         for (uint32_t h = 0; h < 4; ++h) {
             K.tstart[h] = at;
@@ -1249,7 +1256,7 @@ int render_frame(Ctx& c) {
     // idle, and one pipeline's launches overlap the other's. Each trace grid then holds 3
     // work-groups per CU (kSplitBlocksPerCU), so both fit at once (1080p city, PT_MIS: 3.33
     // -> 2.97 ms per frame; 4096 bricks: 4.36 -> 4.10 ms).
-    const bool split = n >= kSplitPaths && !(c.params.flags & GK_FRAME_SINGLE_PIPELINE);
+    const bool split = frame_splits(c);
     const int np = split ? kPathParts : 1;
     uint32_t pbase[4] = {0u, 0u, 0u, 0u}, pcount[4] = {n, 0u, 0u, 0u};
     if (split) {
