@@ -1069,9 +1069,11 @@ __global__ void k_pt_shadow_resolve(Queues Q, PathState S, int cur, int sh) {
 }
 
 // PT_MIS targets: diffuse demodulated by the primary albedo, specular, FP32 radiance sum
-__global__ void k_pt_finalize(FrameK K, Targets T, PathState S, uint32_t n) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+// paths p_base .. p_base + n - 1
+__global__ void k_pt_finalize(FrameK K, Targets T, PathState S, uint32_t n, uint32_t p_base) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = p_base + i;
     const uint32_t px = pixel_of(K, p);
     const float inv = 1.f / float(K.samples > 0u ? K.samples : 1u);
     const float4 ld = S.ld[p], ls = S.ls[p];
@@ -1415,12 +1417,15 @@ int render_frame(Ctx& c) {
             cur = nxt;
         }
         c.trace_split = false;
+        // each pipeline writes its own pixels' targets before joining
+        for (int h = 0; h < np; ++h) {
+            k_pt_finalize<<<nblk(pcount[h]), 256, 0, ss[h]>>>(K, T, P, pcount[h], pbase[h]);
+            GK_HIP(hipGetLastError());
+        }
         for (int h = 1; h < np; ++h) {
             GK_HIP(hipEventRecord(c.join_ev[h - 1], ss[h]));
             GK_HIP(hipStreamWaitEvent(s, c.join_ev[h - 1], 0));
         }
-        k_pt_finalize<<<nblk(n), 256, 0, s>>>(K, T, P, n);
-        GK_HIP(hipGetLastError());
     }
     hipEventRecord(c.ev[2], s);
     if (!(c.params.flags & GK_FRAME_SKIP_POST)) {
