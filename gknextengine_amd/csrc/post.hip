@@ -44,17 +44,6 @@ struct PostT {
     ushort4* out;
 };
 
-// calculateWeight, Process.ReProject.comp.slang:32-47
-__device__ inline float calc_weight(float center_dist, bool same, bool center, f3 n, f3 cn) {
-    if (center) return 0.4f;
-    if (!same) return 0.f;
-    const float nd = fminf(fmaxf(dot(n, cn), 0.f), 1.f);
-    const float th = 0.98f;
-    if (nd < th) return 0.f;
-    const float nw = (nd - th) / (1.0f - th);
-    return nw * 2.0f / (center_dist * 1.5f + 4.0f);
-}
-
 // FilterHistoryColor, Process.ReProject.comp.slang:49-68
 __device__ inline f3 filter_history(const ushort4* prev, float sx, float sy, int px, int py, uint32_t cur, uint32_t p0,
                                     uint32_t p1, uint32_t p2, uint32_t p3, float4 curr, const PostK& K) {
@@ -79,9 +68,11 @@ __device__ inline f3 filter_history(const ushort4* prev, float sx, float sy, int
 constexpr int kRB = 16, kRP = 2, kTileP = kRB + 2 * kRP, kTileM = kRB + 4 * kRP;
 
 __global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
+    // the neighbourhood is converted to float once per tile element, and the motion samples
+    // are kept as their lengths, instead of once per tap
     __shared__ uint32_t s_obj[kTileP][kTileP];
-    __shared__ ushort4 s_nrm[kTileP][kTileP], s_d[kTileP][kTileP], s_s[kTileP][kTileP];
-    __shared__ float2 s_mv[kTileM][kTileM];
+    __shared__ float4 s_nrm[kTileP][kTileP], s_d[kTileP][kTileP], s_s[kTileP][kTileP];
+    __shared__ float s_ml[kTileM][kTileM];
     const int tx = int(threadIdx.x), ty = int(threadIdx.y);
     const int gx = int(blockIdx.x) * kRB, gy = int(blockIdx.y) * kRB;
     const int x = gx + tx, y = gy + ty;
@@ -113,14 +104,15 @@ __global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
             sv = T.src_s[j];
         }
         s_obj[r][cc] = ov;
-        s_nrm[r][cc] = nv;
-        s_d[r][cc] = dv;
-        s_s[r][cc] = sv;
+        s_nrm[r][cc] = from_h4(nv);
+        s_d[r][cc] = from_h4(dv);
+        s_s[r][cc] = from_h4(sv);
     }
     for (int k = ty * kRB + tx; k < kTileM * kTileM; k += kRB * kRB) {
         const int r = k / kTileM, cc = k % kTileM;
         const int mx = gx + cc - 2 * kRP, my = gy + r - 2 * kRP;
-        s_mv[r][cc] = inb(mx, my, K) ? T.motion[size_t(my) * K.W + mx] : make_float2(0.f, 0.f);
+        const float2 m2 = inb(mx, my, K) ? T.motion[size_t(my) * K.W + mx] : make_float2(0.f, 0.f);
+        s_ml[r][cc] = sqrtf(m2.x * m2.x + m2.y * m2.y);
     }
     __syncthreads();
     if (x >= int(K.W) || y >= int(K.H)) return;
@@ -146,24 +138,36 @@ __global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
     const f3 cn = ld3(from_h4(T.normal[i]));
     float4 sd = make_float4(0.f, 0.f, 0.f, 0.f), ss = make_float4(0.f, 0.f, 0.f, 0.f);
     float tw = 0.f, motion_in_range = 0.f;
-    for (int oy = -2; oy <= 2; ++oy)
+    // calculateWeight (Process.ReProject.comp.slang:32-47): 0.4 at the centre; else, for the
+    // same object and a clamped normal cosine nd >= 0.98,
+    // (nd - 0.98) / (1 - 0.98) * 2 / (1.5 |o| + 4), whose constant factor takes one of five
+    // values (|o|^2 = 1, 2, 4, 5, 8)
+    const float th = 0.98f;
+    auto kt_of = [&](int n) { return 2.0f / ((1.0f - th) * (sqrtf(float(n)) * 1.5f + 4.0f)); };
+    const float k1 = kt_of(1), k2 = kt_of(2), k4 = kt_of(4), k5 = kt_of(5), k8 = kt_of(8);
+    // rows one at a time, the five taps of a row unrolled: fully unrolled, all 25 taps' tile
+    // reads are hoisted ahead of the arithmetic and exhaust the registers
+#pragma unroll 1
+    for (int oy = -2; oy <= 2; ++oy) {
+        const int r = ty + kRP + oy;
+#pragma unroll
         for (int ox = -2; ox <= 2; ++ox) {
-            const int r = ty + kRP + oy, cc = tx + kRP + ox;
-            const uint32_t pid = s_obj[r][cc];
-            const f3 wn = ld3(from_h4(s_nrm[r][cc]));
-            const bool center = ox == 0 && oy == 0;
-            const float dist = sqrtf(float(ox * ox + oy * oy));
-            const float w = calc_weight(dist, pid == cur, center, wn, cn);
-            const float4 a = from_h4(s_d[r][cc]), b = from_h4(s_s[r][cc]);
-            sd.x += a.x * w; sd.y += a.y * w; sd.z += a.z * w; sd.w += a.w * w;
-            ss.x += b.x * w; ss.y += b.y * w; ss.z += b.z * w; ss.w += b.w * w;
+            const int cc = tx + kRP + ox;
+            const int n = ox * ox + oy * oy;
+            const float kt = n == 1 ? k1 : n == 2 ? k2 : n == 4 ? k4 : n == 5 ? k5 : k8;
+            const float4 n4 = s_nrm[r][cc];
+            const float nd = fminf(fmaxf(dot(f3{n4.x, n4.y, n4.z}, cn), 0.f), 1.f);
+            const float w = n == 0 ? 0.4f : (s_obj[r][cc] == cur && nd >= th) ? (nd - th) * kt : 0.f;
+            const float4 a = s_d[r][cc], b = s_s[r][cc];
+            sd.x += a.x * w; sd.y += a.y * w; sd.z += a.z * w;
+            ss.x += b.x * w; ss.y += b.y * w; ss.z += b.z * w;
             tw += w;
             // out-of-image samples are zero in the tile and add sqrt(0) = 0
-            const float2 m2 = s_mv[ty + 2 * kRP + 2 * oy][tx + 2 * kRP + 2 * ox];
-            motion_in_range += sqrtf(m2.x * m2.x + m2.y * m2.y);
+            motion_in_range += s_ml[ty + 2 * kRP + 2 * oy][tx + 2 * kRP + 2 * ox];
         }
-    sd.x /= tw; sd.y /= tw; sd.z /= tw; sd.w /= tw;
-    ss.x /= tw; ss.y /= tw; ss.z /= tw; ss.w /= tw;
+    }
+    sd.x /= tw; sd.y /= tw; sd.z /= tw;
+    ss.x /= tw; ss.y /= tw; ss.z /= tw;
     uint32_t p1 = ld_u32(T.obj1, ppx + 1, ppy, K);
     uint32_t p2 = ld_u32(T.obj1, ppx, ppy + 1, K);
     uint32_t p3 = ld_u32(T.obj1, ppx + 1, ppy + 1, K);
@@ -184,27 +188,28 @@ __global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
 constexpr int kR = 5, kJB = 16, kTile = kJB + 2 * kR;  // kJB x kJB pixels per work-group
 
 __global__ __launch_bounds__(kJB * kJB) void k_jbf(PostK K, PostT T) {
-    __shared__ float3 sd[kTile][kTile];
+    // each tile element holds c = colour + 0.001 and its luminance dot(c, lc): computed once
+    // per element here instead of once per tap below
+    __shared__ float4 sd[kTile][kTile];
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int gx = blockIdx.x * kJB, gy = blockIdx.y * kJB;
     const int x = gx + tx, y = gy + ty;
+    const f3 lc = f3{0.212671f, 0.715160f, 0.072169f};
     f3 total;
     if (K.bf_size > 0u) {
         for (int idx = ty * kJB + tx; idx < kTile * kTile; idx += kJB * kJB) {
             const int row = idx / kTile, col = idx % kTile;
             const int cx = gx + col - kR, cy = gy + row - kR;
             const float4 d = ld_h4(T.acc_d, cx, cy, K);
-            sd[row][col] = make_float3(d.x, d.y, d.z);
+            const f3 c = f3{d.x + 0.001f, d.y + 0.001f, d.z + 0.001f};
+            sd[row][col] = make_float4(c.x, c.y, c.z, dot(c, lc));
         }
         __syncthreads();
         if (x >= int(K.W) || y >= int(K.H)) return;
-        const float3 c3 = sd[ty + kR][tx + kR];
-        const f3 cc = f3{c3.x + 0.001f, c3.y + 0.001f, c3.z + 0.001f};
+        const float clum = sd[ty + kR][tx + kR].w;
         // the specular term is read at the centre only
         const float4 s4 = ld_h4(T.acc_s, x, y, K);
         const f3 cspec = f3{s4.x + 0.001f, s4.y + 0.001f, s4.z + 0.001f};
-        const f3 lc = f3{0.212671f, 0.715160f, 0.072169f};
-        const float clum = dot(cc, lc);
         const float sigma = K.bf_sigma, sigma_l = K.bf_sigma_lum * 100.0f;
         // the spatial weight depends on i*i + j*j only, which is 2, 10, 18 or (clamped to
         // distance 1) at least 26: four values, computed once with the per-tap expression
@@ -214,21 +219,23 @@ __global__ __launch_bounds__(kJB * kJB) void k_jbf(PostK K, PostT T) {
             return expf(-dist * dist / (2.0f * sigma * sigma));
         };
         const float F2 = spatial(2), F10 = spatial(10), F18 = spatial(18), Ffar = spatial(2 * kR * kR);
+        // exp(-dl^2 / (2 sL^2)) as the hardware exp2 of dl^2 * (-log2(e) / (2 sL^2))
+        const float lk = -1.44269504f / (2.0f * sigma_l * sigma_l);
         f3 acc = f3{0.f, 0.f, 0.f};
         float wsum = 0.f;
 #pragma unroll
         for (int i = -kR; i <= kR; i += 2)
 #pragma unroll
             for (int j = -kR; j <= kR; j += 2) {
-                const float3 q = sd[ty + kR + i][tx + kR + j];
-                const f3 ci = f3{q.x + 0.001f, q.y + 0.001f, q.z + 0.001f};
-                const float lumi = dot(ci, lc);
+                const float4 q = sd[ty + kR + i][tx + kR + j];
                 const int n = i * i + j * j;
-                const float dl = (clum - lumi) * (clum - lumi);
+                const float dl = (clum - q.w) * (clum - q.w);
                 const float Fi = n == 2 ? F2 : n == 10 ? F10 : n == 18 ? F18 : Ffar;
-                const float Li = __expf(-dl * dl / (2.0f * sigma_l * sigma_l));
-                acc = acc + ci * (Fi * Li);
-                wsum += Fi * Li;
+                const float w = Fi * __builtin_amdgcn_exp2f(dl * dl * lk);
+                acc.x = fmaf(q.x, w, acc.x);
+                acc.y = fmaf(q.y, w, acc.y);
+                acc.z = fmaf(q.z, w, acc.z);
+                wsum += w;
             }
         total = acc * (1.f / wsum);
         const f3 alb = ld3(from_h4(T.acc_a[size_t(y) * K.W + x]));
