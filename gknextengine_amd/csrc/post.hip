@@ -29,9 +29,6 @@ __device__ inline bool inb(int x, int y, const PostK& K) { return x >= 0 && y >=
 __device__ inline float4 ld_h4(const ushort4* p, int x, int y, const PostK& K) {
     return inb(x, y, K) ? from_h4(p[size_t(y) * K.W + x]) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
-__device__ inline uint32_t ld_u32(const uint32_t* p, int x, int y, const PostK& K) {
-    return inb(x, y, K) ? p[size_t(y) * K.W + x] : 0u;
-}
 __device__ inline float lerpf(float a, float b, float t) { return a * (1.f - t) + b * t; }
 
 struct PostT {
@@ -44,14 +41,41 @@ struct PostT {
     ushort4* out;
 };
 
-// FilterHistoryColor, Process.ReProject.comp.slang:49-68
-__device__ inline f3 filter_history(const ushort4* prev, float sx, float sy, int px, int py, uint32_t cur, uint32_t p0,
-                                    uint32_t p1, uint32_t p2, uint32_t p3, float4 curr, const PostK& K) {
+// The previous frame's 2x2 footprint at (px, py), with px, py inside the image: element
+// offsets of the right and lower neighbours (0 where they fall outside, so every address is
+// inside) and whether they are inside. Outside elements read as zero.
+struct Quad {
+    uint32_t b0, o1, oW;
+    bool x1, y1;
+};
+__device__ inline Quad quad_at(int px, int py, const PostK& K) {
+    Quad q;
+    q.x1 = uint32_t(px + 1) < K.W;
+    q.y1 = uint32_t(py + 1) < K.H;
+    q.b0 = uint32_t(py) * K.W + uint32_t(px);
+    q.o1 = q.x1 ? 1u : 0u;
+    q.oW = q.y1 ? K.W : 0u;
+    return q;
+}
+
+// FilterHistoryColor, Process.ReProject.comp.slang:49-68. The four loads are issued
+// unconditionally (from in-image addresses) and the selects pick history, zero or the
+// current colour.
+__device__ inline f3 filter_history(const ushort4* prev, float sx, float sy, const Quad& q, uint32_t cur,
+                                    uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3, float4 curr) {
     const f3 c = ld3(curr);
-    const f3 h0 = cur == p0 ? ld3(ld_h4(prev, px, py, K)) : c;
-    const f3 h1 = cur == p1 ? ld3(ld_h4(prev, px + 1, py, K)) : c;
-    const f3 h2 = cur == p2 ? ld3(ld_h4(prev, px, py + 1, K)) : c;
-    const f3 h3 = cur == p3 ? ld3(ld_h4(prev, px + 1, py + 1, K)) : c;
+    // outside elements are cleared bit by bit (+0 in half precision): every load's value is
+    // used, so none is moved under a branch of its own
+    const uint16_t m1 = q.x1 ? 0xFFFFu : 0u, m2 = q.y1 ? 0xFFFFu : 0u, m3 = m1 & m2;
+    const ushort4 r0 = prev[q.b0], r1 = prev[q.b0 + q.o1], r2 = prev[q.b0 + q.oW], r3 = prev[q.b0 + q.oW + q.o1];
+    const f3 t0 = ld3(from_h4(r0));
+    const f3 t1 = ld3(from_h4(make_ushort4(r1.x & m1, r1.y & m1, r1.z & m1, 0)));
+    const f3 t2 = ld3(from_h4(make_ushort4(r2.x & m2, r2.y & m2, r2.z & m2, 0)));
+    const f3 t3 = ld3(from_h4(make_ushort4(r3.x & m3, r3.y & m3, r3.z & m3, 0)));
+    const f3 h0 = cur == p0 ? t0 : c;
+    const f3 h1 = cur == p1 ? t1 : c;
+    const f3 h2 = cur == p2 ? t2 : c;
+    const f3 h3 = cur == p3 ? t3 : c;
     f3 a = f3{lerpf(h0.x, h1.x, sx), lerpf(h0.y, h1.y, sx), lerpf(h0.z, h1.z, sx)};
     f3 b = f3{lerpf(h2.x, h3.x, sx), lerpf(h2.y, h3.y, sx), lerpf(h2.z, h3.z, sx)};
     f3 h = f3{lerpf(a.x, b.x, sy), lerpf(a.y, b.y, sy), lerpf(a.z, b.z, sy)};
@@ -73,6 +97,7 @@ __global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
     __shared__ uint32_t s_obj[kTileP][kTileP];
     __shared__ float4 s_nrm[kTileP][kTileP], s_d[kTileP][kTileP], s_s[kTileP][kTileP];
     __shared__ float s_ml[kTileM][kTileM];
+    __shared__ float s_kt[25];
     const int tx = int(threadIdx.x), ty = int(threadIdx.y);
     const int gx = int(blockIdx.x) * kRB, gy = int(blockIdx.y) * kRB;
     const int x = gx + tx, y = gy + ty;
@@ -97,7 +122,7 @@ __global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
         uint32_t ov = 0u;
         ushort4 nv = z4, dv = z4, sv = z4;
         if (in) {
-            const size_t j = size_t(sy) * K.W + sx;
+            const uint32_t j = uint32_t(sy) * K.W + uint32_t(sx);
             ov = T.obj0[j];
             nv = T.normal[j];
             dv = T.src_d[j];
@@ -111,13 +136,20 @@ __global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
     for (int k = ty * kRB + tx; k < kTileM * kTileM; k += kRB * kRB) {
         const int r = k / kTileM, cc = k % kTileM;
         const int mx = gx + cc - 2 * kRP, my = gy + r - 2 * kRP;
-        const float2 m2 = inb(mx, my, K) ? T.motion[size_t(my) * K.W + mx] : make_float2(0.f, 0.f);
+        const float2 m2 = inb(mx, my, K) ? T.motion[uint32_t(my) * K.W + uint32_t(mx)] : make_float2(0.f, 0.f);
         s_ml[r][cc] = sqrtf(m2.x * m2.x + m2.y * m2.y);
+    }
+    // calculateWeight (Process.ReProject.comp.slang:32-47) off the centre: for the same
+    // object and a clamped normal cosine nd >= 0.98, (nd - 0.98) / (1 - 0.98) * 2 / (1.5 |o| + 4);
+    // the table holds its constant factor per tap, 0 at the centre (weighted apart, 0.4)
+    const float th = 0.98f;
+    if (const int t = ty * kRB + tx; t < 25) {
+        const int ox = t % 5 - 2, oy = t / 5 - 2, n = ox * ox + oy * oy;
+        s_kt[t] = n == 0 ? 0.f : 2.0f / ((1.0f - th) * (sqrtf(float(n)) * 1.5f + 4.0f));
     }
     __syncthreads();
     if (x >= int(K.W) || y >= int(K.H)) return;
-    const size_t i = size_t(y) * K.W + x;
-    const float4 cd = from_h4(T.src_d[i]), cs = from_h4(T.src_s[i]), ca = from_h4(T.src_a[i]);
+    const uint32_t i = uint32_t(y) * K.W + uint32_t(x);
     const float2 mv = T.motion[i];
     const float fx = float(x) + mv.x, fy = float(y) + mv.y;
     const int ppx = int(floorf(fx)), ppy = int(floorf(fy));
@@ -125,8 +157,8 @@ __global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
     // (Process.ReProject.comp.slang:109-110): the mixed-sign comparison converts previpos to
     // unsigned, so -1 fails it and history is used only for 0 <= previpos < size
     const bool inside = uint32_t(ppx) < K.W && uint32_t(ppy) < K.H;
-    const uint32_t cur = T.obj0[i];
-    uint32_t p0 = ld_u32(T.obj1, ppx, ppy, K);
+    // the centre of the tile is this pixel's own object id, normal and colours
+    const uint32_t cur = s_obj[ty + kRP][tx + kRP];
     const bool miss = cur == 65535u;
     const uint32_t moment = T.moment[i];
     if (miss || !inside || moment > 0u) {
@@ -135,16 +167,14 @@ __global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
         T.acc_a[i] = T.src_a[i];
         return;
     }
-    const f3 cn = ld3(from_h4(T.normal[i]));
+    const float4 cd = s_d[ty + kRP][tx + kRP], cs = s_s[ty + kRP][tx + kRP], ca = from_h4(T.src_a[i]);
+    const Quad q = quad_at(ppx, ppy, K);
+    const uint32_t o0 = T.obj1[q.b0], o1 = T.obj1[q.b0 + q.o1], o2 = T.obj1[q.b0 + q.oW],
+                   o3 = T.obj1[q.b0 + q.oW + q.o1];
+    const float4 cn4 = s_nrm[ty + kRP][tx + kRP];
+    const f3 cn = f3{cn4.x, cn4.y, cn4.z};
     float4 sd = make_float4(0.f, 0.f, 0.f, 0.f), ss = make_float4(0.f, 0.f, 0.f, 0.f);
     float tw = 0.f, motion_in_range = 0.f;
-    // calculateWeight (Process.ReProject.comp.slang:32-47): 0.4 at the centre; else, for the
-    // same object and a clamped normal cosine nd >= 0.98,
-    // (nd - 0.98) / (1 - 0.98) * 2 / (1.5 |o| + 4), whose constant factor takes one of five
-    // values (|o|^2 = 1, 2, 4, 5, 8)
-    const float th = 0.98f;
-    auto kt_of = [&](int n) { return 2.0f / ((1.0f - th) * (sqrtf(float(n)) * 1.5f + 4.0f)); };
-    const float k1 = kt_of(1), k2 = kt_of(2), k4 = kt_of(4), k5 = kt_of(5), k8 = kt_of(8);
     // rows one at a time, the five taps of a row unrolled: fully unrolled, all 25 taps' tile
     // reads are hoisted ahead of the arithmetic and exhaust the registers
 #pragma unroll 1
@@ -153,11 +183,10 @@ __global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
 #pragma unroll
         for (int ox = -2; ox <= 2; ++ox) {
             const int cc = tx + kRP + ox;
-            const int n = ox * ox + oy * oy;
-            const float kt = n == 1 ? k1 : n == 2 ? k2 : n == 4 ? k4 : n == 5 ? k5 : k8;
+            const float kt = s_kt[(oy + 2) * 5 + ox + 2];
             const float4 n4 = s_nrm[r][cc];
             const float nd = fminf(fmaxf(dot(f3{n4.x, n4.y, n4.z}, cn), 0.f), 1.f);
-            const float w = n == 0 ? 0.4f : (s_obj[r][cc] == cur && nd >= th) ? (nd - th) * kt : 0.f;
+            const float w = (s_obj[r][cc] == cur && nd >= th) ? (nd - th) * kt : 0.f;
             const float4 a = s_d[r][cc], b = s_s[r][cc];
             sd.x += a.x * w; sd.y += a.y * w; sd.z += a.z * w;
             ss.x += b.x * w; ss.y += b.y * w; ss.z += b.z * w;
@@ -166,16 +195,24 @@ __global__ __launch_bounds__(kRB * kRB) void k_reproject(PostK K, PostT T) {
             motion_in_range += s_ml[ty + 2 * kRP + 2 * oy][tx + 2 * kRP + 2 * ox];
         }
     }
+    // the centre, weight 0.4, last (the shader adds it in raster order; the sums differ by
+    // rounding only)
+    sd.x += cd.x * 0.4f; sd.y += cd.y * 0.4f; sd.z += cd.z * 0.4f;
+    ss.x += cs.x * 0.4f; ss.y += cs.y * 0.4f; ss.z += cs.z * 0.4f;
+    tw += 0.4f;
     sd.x /= tw; sd.y /= tw; sd.z /= tw;
     ss.x /= tw; ss.y /= tw; ss.z /= tw;
-    uint32_t p1 = ld_u32(T.obj1, ppx + 1, ppy, K);
-    uint32_t p2 = ld_u32(T.obj1, ppx, ppy + 1, K);
-    uint32_t p3 = ld_u32(T.obj1, ppx + 1, ppy + 1, K);
-    if (fabsf(motion_in_range) < 0.1f) { p0 = cur; p1 = cur; p2 = cur; p3 = cur; }
+    // object ids of the footprint, zero outside the image; all the current id when the
+    // neighbourhood barely moves
+    const bool still = fabsf(motion_in_range) < 0.1f;
+    const uint32_t p0 = still ? cur : o0;
+    const uint32_t p1 = still ? cur : (q.x1 ? o1 : 0u);
+    const uint32_t p2 = still ? cur : (q.y1 ? o2 : 0u);
+    const uint32_t p3 = still ? cur : (q.x1 && q.y1 ? o3 : 0u);
     const float subx = fx - floorf(fx), suby = fy - floorf(fy);
-    const f3 hd = filter_history(T.prev_d, subx, suby, ppx, ppy, cur, p0, p1, p2, p3, sd, K);
-    const f3 hs = filter_history(T.prev_s, subx, suby, ppx, ppy, cur, p0, p1, p2, p3, ss, K);
-    const f3 ha = filter_history(T.prev_a, subx, suby, ppx, ppy, cur, p0, p1, p2, p3, ca, K);
+    const f3 hd = filter_history(T.prev_d, subx, suby, q, cur, p0, p1, p2, p3, sd);
+    const f3 hs = filter_history(T.prev_s, subx, suby, q, cur, p0, p1, p2, p3, ss);
+    const f3 ha = filter_history(T.prev_a, subx, suby, q, cur, p0, p1, p2, p3, ca);
     float keep = 1.0f / float(K.temporal > 1u ? K.temporal : 1u);
     keep = fminf(fmaxf(keep, 0.f), 1.f);
     // .rgb written, alpha keeps the accumulation target's previous value
