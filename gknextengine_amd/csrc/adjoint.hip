@@ -358,6 +358,7 @@ extern "C" {
 
 int gk_render_radiance(gk_ctx* ctx, float* d_image) {
     if (!ctx || !d_image) return GK_ERR_ARG;
+    ctx->c.early_ok = false;  // queues work on the context's stream
     Ctx& c = ctx->c;
     if (!c.scene_ready || !c.frame_ready) return GK_ERR_STATE;
     if (c.params.aperture > 0.f) return GK_ERR_ARG;  // the differentiable camera is a pinhole
@@ -375,6 +376,7 @@ int gk_render_radiance(gk_ctx* ctx, float* d_image) {
 
 int gk_adjoint(gk_ctx* ctx, const float* d_dl_dimage, float* d_grad) {
     if (!ctx || !d_dl_dimage || !d_grad) return GK_ERR_ARG;
+    ctx->c.early_ok = false;  // queues work on the context's stream
     Ctx& c = ctx->c;
     if (!c.scene_ready || !c.frame_ready) return GK_ERR_STATE;
     if (c.params.aperture > 0.f) return GK_ERR_ARG;  // the differentiable camera is a pinhole

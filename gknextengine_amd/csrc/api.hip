@@ -544,7 +544,7 @@ int gk_instances_update_device(gk_ctx* ctx, const float* d_world16, uint32_t fir
 }
 
 int gk_get_bvh_stats(gk_ctx* ctx, gk_bvh_stats* out) {
-    GK_CHECK_CTX(ctx);
+    GK_CHECK_CTX_KEEP(ctx);
     if (!out) return GK_ERR_ARG;
     Ctx& c = ctx->c;
     if (c.scene_ready) {
@@ -557,7 +557,7 @@ int gk_get_bvh_stats(gk_ctx* ctx, gk_bvh_stats* out) {
 }
 
 int gk_frame(gk_ctx* ctx, const gk_frame_params* p) {
-    GK_CHECK_CTX(ctx);
+    GK_CHECK_CTX_KEEP(ctx);
     if (!p) return GK_ERR_ARG;
     if (p->estimator > GK_ESTIMATOR_PT_MIS || p->denoiser > GK_DENOISER_UNET) return GK_ERR_ARG;
     if (p->samples > 1024 || p->bounces > 64 || p->max_bounces > 64) return GK_ERR_LIMIT;
@@ -579,7 +579,7 @@ int gk_frame(gk_ctx* ctx, const gk_frame_params* p) {
 }
 
 int gk_render(gk_ctx* ctx) {
-    GK_CHECK_CTX(ctx);
+    GK_CHECK_CTX_KEEP(ctx);
     Ctx& c = ctx->c;
     if (!c.scene_ready || !c.frame_ready) return GK_ERR_STATE;
     GK_HIP(hipSetDevice(c.device));
@@ -590,7 +590,7 @@ int gk_render(gk_ctx* ctx) {
 }
 
 int gk_sync(gk_ctx* ctx) {
-    GK_CHECK_CTX(ctx);
+    GK_CHECK_CTX_KEEP(ctx);
     GK_HIP(hipStreamSynchronize(ctx->c.stream));
     // a split frame joins its other streams into the first; this also covers a frame whose
     // enqueue stopped on an error between the fork and the join
@@ -614,7 +614,7 @@ int gk_reset_accumulation(gk_ctx* ctx) {
 }
 
 int gk_read_target(gk_ctx* ctx, int target, void* host, size_t bytes) {
-    GK_CHECK_CTX(ctx);
+    GK_CHECK_CTX_KEEP(ctx);
     Ctx& c = ctx->c;
     if (target < 0 || target >= GK_RT_COUNT || !host) return GK_ERR_ARG;
     if (bytes != c.target_bytes[target]) return GK_ERR_ARG;
@@ -634,7 +634,7 @@ int gk_write_target(gk_ctx* ctx, int target, const void* host, size_t bytes) {
 }
 
 int gk_target_info(gk_ctx* ctx, int target, size_t* bytes, void** device_ptr) {
-    GK_CHECK_CTX(ctx);
+    GK_CHECK_CTX_KEEP(ctx);
     Ctx& c = ctx->c;
     if (target < 0 || target >= GK_RT_COUNT) return GK_ERR_ARG;
     if (bytes) *bytes = c.target_bytes[target];
@@ -646,7 +646,7 @@ int gk_target_info(gk_ctx* ctx, int target, size_t* bytes, void** device_ptr) {
 // time summed over the HIP events bracketing every trace launch, and the stage times of
 // the last frame (events ev[0..6] of render_frame / post_process).
 int gk_get_stats(gk_ctx* ctx, gk_stats* out) {
-    GK_CHECK_CTX(ctx);
+    GK_CHECK_CTX_KEEP(ctx);
     if (!out) return GK_ERR_ARG;
     Ctx& c = ctx->c;
     GK_HIP(hipStreamSynchronize(c.stream));

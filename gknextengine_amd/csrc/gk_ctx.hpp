@@ -136,10 +136,16 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     // streams of trace slots 1..3: a frame split into independent path pipelines runs
-    // pipeline h on slot h (created on first use; default priority); the events that fork
-    // them from and join them to `stream`
+    // pipeline h on pstream[h] and traces in slot h + 1 (created on first use; default
+    // priority); the events that fork them from and join them to `stream`
     hipStream_t pstream[3] = {nullptr, nullptr, nullptr};
     hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};
+    // The next split frame's pipelines may generate and trace their primary rays without
+    // waiting for the work queued on `stream` before the frame (the last frame's post
+    // passes): set by a split frame of early_w x early_h pixels; cleared by every entry point
+    // but frame set-up, render, sync and read-only queries (GK_CHECK_CTX)
+    bool early_ok = false;
+    uint32_t early_w = 0, early_h = 0;
     uint32_t width = 0, height = 0;
     bool scene_ready = false;
     bool frame_ready = false;
@@ -320,4 +326,8 @@ int static_scratch_overflow(Ctx& c, bool& overflow);
 struct gk_ctx {
     gk::Ctx c;
 };
-#define GK_CHECK_CTX(ctx) do { if (!(ctx)) return GK_ERR_ARG; } while (0)
+// Entry points that may queue work on the context's stream or change what a frame's path
+// pipelines read (scene, instances, queues, slots) end the pipelines' early start of the
+// next frame; frame set-up, render, sync and read-only queries use GK_CHECK_CTX_KEEP.
+#define GK_CHECK_CTX(ctx) do { if (!(ctx)) return GK_ERR_ARG; (ctx)->c.early_ok = false; } while (0)
+#define GK_CHECK_CTX_KEEP(ctx) do { if (!(ctx)) return GK_ERR_ARG; } while (0)

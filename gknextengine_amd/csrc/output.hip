@@ -195,7 +195,7 @@ int gk_render_extent(uint32_t out_w, uint32_t out_h, uint32_t super_resolution, 
 }
 
 int gk_upscale(gk_ctx* ctx, int mode, uint32_t out_w, uint32_t out_h, void* d_out) {
-    GK_CHECK_CTX(ctx);
+    GK_CHECK_CTX_KEEP(ctx);  // reads the display target only
     if (!d_out || out_w == 0 || out_h == 0 || (mode != GK_UPSCALE_LINEAR && mode != GK_UPSCALE_FSR_EASU))
         return GK_ERR_ARG;
     if (out_w > 16384 || out_h > 16384) return GK_ERR_LIMIT;
@@ -213,7 +213,7 @@ int gk_upscale(gk_ctx* ctx, int mode, uint32_t out_w, uint32_t out_h, void* d_ou
 }
 
 int gk_screenshot(gk_ctx* ctx, const void* d_rgba16f, uint32_t w, uint32_t h, const char* path) {
-    GK_CHECK_CTX(ctx);
+    GK_CHECK_CTX_KEEP(ctx);  // reads the display target only
     if (!path) return GK_ERR_ARG;
     gk::Ctx& c = ctx->c;
     if (!d_rgba16f) {

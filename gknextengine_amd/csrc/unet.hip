@@ -1932,6 +1932,7 @@ extern "C" {
 
 int gk_unet_create(gk_ctx* ctx, uint32_t base_channels, uint64_t seed, gk_unet** out) {
     if (!ctx || !out || base_channels < 8u || base_channels > 256u) return GK_ERR_ARG;
+    ctx->c.early_ok = false;  // queues work on the context's stream
     *out = nullptr;
     Ctx& c = ctx->c;
     GK_HIP(hipSetDevice(c.device));
@@ -2119,6 +2120,7 @@ int gk_unet_infer(gk_unet* net, const float* d_input, float* d_output, uint32_t 
 
 int gk_denoise_input(gk_ctx* ctx, float* d_input) {
     if (!ctx || !d_input) return GK_ERR_ARG;
+    ctx->c.early_ok = false;  // queues work on the context's stream
     Ctx& c = ctx->c;
     if (!c.frame_ready) return GK_ERR_STATE;
     GK_HIP(hipSetDevice(c.device));

@@ -1179,6 +1179,8 @@ static FrameK make_k(const Ctx& c) {
 
 int render_frame(Ctx& c) {
     c.trace_split = false;
+    const bool early_ok = c.early_ok && c.early_w == c.width && c.early_h == c.height;
+    c.early_ok = false;
     int st = ensure_wavefront(c);
     if (st != GK_OK) return st;
     auto& w = c.wf;
@@ -1199,6 +1201,7 @@ int render_frame(Ctx& c) {
         hipEventRecord(c.ev[3], s);
         return GK_OK;
     }
+    const bool tlas_work = c.tlas_dirty;
     if (c.tlas_dirty) {
         st = (c.params.flags & GK_FRAME_TLAS_REBUILD) ? rebuild_tlas(c) : refit_tlas(c);
         if (st != GK_OK) return st;
@@ -1271,15 +1274,22 @@ int render_frame(Ctx& c) {
         }
     }
     Queues QH[4] = {Q, Q, Q, Q};
+    // pipeline h of a split frame runs on pstream[h] and traces in slot h + 1; an unsplit
+    // frame runs on `s` and traces in slot 0
     hipStream_t ss[4] = {s, s, s, s};
+    int sl[4] = {0, 0, 0, 0};
     uint32_t* tn[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (int h = 1; h < np; ++h) {
+    static_assert(kPathParts < 4, "a split frame's pipelines use trace slots 1..kPathParts");
+    for (int h = 0; split && h < np; ++h) {
         if (!c.fork_ev) GK_HIP(hipEventCreateWithFlags(&c.fork_ev, hipEventDisableTiming));
-        if (!c.pstream[h - 1]) {
-            GK_HIP(hipStreamCreateWithFlags(&c.pstream[h - 1], hipStreamNonBlocking));
-            GK_HIP(hipEventCreateWithFlags(&c.join_ev[h - 1], hipEventDisableTiming));
+        if (!c.pstream[h]) {
+            GK_HIP(hipStreamCreateWithFlags(&c.pstream[h], hipStreamNonBlocking));
+            GK_HIP(hipEventCreateWithFlags(&c.join_ev[h], hipEventDisableTiming));
         }
-        ss[h] = c.pstream[h - 1];
+        ss[h] = c.pstream[h];
+        sl[h] = h + 1;
+    }
+    for (int h = 1; h < np; ++h) {
         // pipeline h: queue entries from its first path id on, counters from word 8 h
         const uint32_t o = pbase[h];
         Queues& B = QH[h];
@@ -1297,12 +1307,24 @@ int render_frame(Ctx& c) {
     // the slots' ray counters (a slot's first use allocates and clears them on `s`, so
     // before the fork)
     for (int h = 0; h < np; ++h) {
-        tn[h] = trace_counter(c, h);
+        tn[h] = trace_counter(c, sl[h]);
         if (!tn[h]) return GK_ERR_OOM;
     }
+    // Early start: ray generation and the primary trace read the queues, path state and
+    // slots of their own pipeline (each stream's previous frame is behind them in stream
+    // order) and the scene's BVH, and write no target. When no work that they depend on was
+    // queued on `s` since the last split frame of this size (no TLAS update, no other entry
+    // point), they start without waiting for `s`, so the last frame's post passes run beside
+    // them; the pipelines wait for `s` (this frame's object-id copy after those passes)
+    // before their first kernel that writes a target. Under stream capture every stream
+    // forks from the captured one.
+    hipStreamCaptureStatus cap0 = hipStreamCaptureStatusNone;
+    GK_HIP(hipStreamIsCapturing(s, &cap0));
+    const bool early = split && early_ok && !tlas_work && !K.vis_primary && cap0 == hipStreamCaptureStatusNone;
     if (split) {
         GK_HIP(hipEventRecord(c.fork_ev, s));
-        for (int h = 1; h < np; ++h) GK_HIP(hipStreamWaitEvent(ss[h], c.fork_ev, 0));
+        if (!early)
+            for (int h = 0; h < np; ++h) GK_HIP(hipStreamWaitEvent(ss[h], c.fork_ev, 0));
     }
     c.trace_split = split;
     for (int h = 0; h < np; ++h) {
@@ -1314,12 +1336,14 @@ int render_frame(Ctx& c) {
                                                               c.proxy_inst.ptr, c.num_proxies, pcount[h], pbase[h]);
             GK_HIP(hipGetLastError());
         } else {
-            c.trace_counter_zeroed[h] = true;
+            c.trace_counter_zeroed[sl[h]] = true;
             st = launch_trace(c, QH[h].o[0], QH[h].d[0], QH[h].cnt + 0, pcount[h], QH[h].hit, QH[h].hit_inst, false,
-                              nullptr, h);
+                              nullptr, sl[h]);
             if (st != GK_OK) return st;
         }
     }
+    if (early)
+        for (int h = 0; h < np; ++h) GK_HIP(hipStreamWaitEvent(ss[h], c.fork_ev, 0));
     int cur = 0;
     if (ref) {
         for (int h = 0; h < np; ++h) {
@@ -1358,11 +1382,11 @@ int render_frame(Ctx& c) {
             }
             for (int h = 0; h < np; ++h) {
                 if (!live[h]) continue;
-                c.trace_counter_zeroed[h] = true;
+                c.trace_counter_zeroed[sl[h]] = true;
                 // the trace zeroes the counter of the queue k_ref_shade appends to
                 const Queues& q = QH[h];
                 st = launch_trace(c, q.o[cur], q.d[cur], q.cnt + cur, pcount[h], q.hit, q.hit_inst, false,
-                                  q.cnt + (cur ^ 1), h);
+                                  q.cnt + (cur ^ 1), sl[h]);
                 if (st != GK_OK) return st;
             }
             for (int h = 0; h < np; ++h) {
@@ -1373,9 +1397,9 @@ int render_frame(Ctx& c) {
             cur ^= 1;
         }
         c.trace_split = false;
-        for (int h = 1; h < np; ++h) {
-            GK_HIP(hipEventRecord(c.join_ev[h - 1], ss[h]));
-            GK_HIP(hipStreamWaitEvent(s, c.join_ev[h - 1], 0));
+        for (int h = 0; split && h < np; ++h) {
+            GK_HIP(hipEventRecord(c.join_ev[h], ss[h]));
+            GK_HIP(hipStreamWaitEvent(s, c.join_ev[h], 0));
         }
     } else {
         // Each iteration shades the hits of queue cur, then one launch traces both the
@@ -1398,16 +1422,16 @@ int render_frame(Ctx& c) {
                 }
                 k_pt_shade<<<nblk(m), 256, 0, ss[h]>>>(S0, K, T, q, P, cur, sh, tn[h]);
                 GK_HIP(hipGetLastError());
-                c.trace_counter_zeroed[h] = true;
+                c.trace_counter_zeroed[sl[h]] = true;
             }
             for (int h = 0; h < np; ++h) {
                 const Queues& q = QH[h];
                 const uint32_t m = pcount[h];
                 if (it + 1 < iters)
                     st = launch_trace_pair(c, q.o[nxt], q.d[nxt], q.cnt + nxt, m, q.hit, q.hit_inst, q.so, q.sd,
-                                           q.cnt + sh, m, q.shit, h);
+                                           q.cnt + sh, m, q.shit, sl[h]);
                 else
-                    st = launch_trace(c, q.so, q.sd, q.cnt + sh, m, q.shit, nullptr, true, nullptr, h);
+                    st = launch_trace(c, q.so, q.sd, q.cnt + sh, m, q.shit, nullptr, true, nullptr, sl[h]);
                 if (st != GK_OK) return st;
             }
             for (int h = 0; h < np; ++h) {
@@ -1422,9 +1446,9 @@ int render_frame(Ctx& c) {
             k_pt_finalize<<<nblk(pcount[h]), 256, 0, ss[h]>>>(K, T, P, pcount[h], pbase[h]);
             GK_HIP(hipGetLastError());
         }
-        for (int h = 1; h < np; ++h) {
-            GK_HIP(hipEventRecord(c.join_ev[h - 1], ss[h]));
-            GK_HIP(hipStreamWaitEvent(s, c.join_ev[h - 1], 0));
+        for (int h = 0; split && h < np; ++h) {
+            GK_HIP(hipEventRecord(c.join_ev[h], ss[h]));
+            GK_HIP(hipStreamWaitEvent(s, c.join_ev[h], 0));
         }
     }
     hipEventRecord(c.ev[2], s);
@@ -1433,6 +1457,9 @@ int render_frame(Ctx& c) {
         if (st != GK_OK) return st;
     }
     hipEventRecord(c.ev[3], s);
+    c.early_ok = split && cap0 == hipStreamCaptureStatusNone;
+    c.early_w = c.width;
+    c.early_h = c.height;
     return GK_OK;
 }
 
