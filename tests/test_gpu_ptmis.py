@@ -238,6 +238,62 @@ def test_pt_mis_two_pipelines_give_the_single_pipeline_frame(city, W, H):
     assert a["RADIANCE"][..., :3].max() > 0.1
 
 
+def test_split_frames_queued_with_user_traces(city, device):
+    """gk_trace uses the context's ray queues, which a split frame's ray generation also
+    writes. Three split frames (520 x 520) queued back to back, each followed by a gk_trace
+    of 2^16 random rays queued without a synchronisation: the frames' targets equal those of
+    the same frames without the traces, and every trace returns the hits of its rays traced
+    alone (a frame after another entry point waits for the context's stream before its
+    ray generation)."""
+    import torch
+    s, _ = city
+    W = H = 520
+    n = 1 << 16
+    rng = np.random.default_rng(11)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = rng.uniform([-30.0, 1.0, -30.0], [30.0, 30.0, 30.0], (n, 3))
+    d = rng.normal(size=(n, 3))
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 7] = 1e30
+    r = torch.from_numpy(rays).to(device)
+    names = [t for t in gk.TARGETS if t not in ("MINIGBUFFER", "RADIANCE64")]
+
+    def frame_params(f):
+        p = params(W, H, scenes.CITY_CAM, frame=f, bounces=8, sky=(0.6, 0.7, 0.9), seed=3)
+        p.denoiser = gk.DENOISER_BILATERAL
+        return p
+
+    with gk.Context(W, H) as ctx:          # the rays alone
+        ctx.upload(s)
+        h0 = torch.zeros((n, 4), dtype=torch.float32, device=device)
+        i0 = torch.zeros(n, dtype=torch.int32, device=device)
+        torch.cuda.synchronize()
+        ctx.trace(r.data_ptr(), n, h0.data_ptr(), i0.data_ptr())
+        ctx.sync()
+        alone = h0.cpu().numpy()
+    outs = []
+    for with_traces in (False, True):
+        with gk.Context(W, H) as ctx:
+            ctx.upload(s)
+            hs = [torch.zeros((n, 4), dtype=torch.float32, device=device) for _ in range(3)]
+            ins = [torch.zeros(n, dtype=torch.int32, device=device) for _ in range(3)]
+            torch.cuda.synchronize()
+            for f in range(3):
+                ctx.frame(frame_params(f))
+                ctx.render()
+                if with_traces:
+                    ctx.trace(r.data_ptr(), n, hs[f].data_ptr(), ins[f].data_ptr())
+            ctx.sync()
+            outs.append({t: ctx.read(t).copy() for t in names})
+            if with_traces:
+                for h in hs:
+                    assert np.array_equal(h.cpu().numpy().view(np.uint32), alone.view(np.uint32))
+    hits = int((alone[:, 0] < 1e30).sum())
+    assert hits > n // 4, hits
+    for t in names:
+        assert np.array_equal(outs[0][t].view(np.uint8), outs[1][t].view(np.uint8)), t
+
+
 def test_pt_mis_two_pipelines_on_moving_instances_and_visibility_buffer():
     """The split on a two-level scene (4096 brick instances, 640x480 = 307,200 paths): a
     frame, the instances moved and the TLAS rebuilt, a second frame, then a third whose
