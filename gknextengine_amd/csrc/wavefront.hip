@@ -91,7 +91,9 @@ struct Queues {
     float4* sc_s;   // shadow contribution to Ls (rgb)
     uint32_t* spath;
     float4* shit;   // occlusion result per shadow entry (w = 1 when occluded)
-    uint32_t* cnt;  // [0],[1] extension queue sizes, [2] shadow queue size
+    // queue sizes: extension queue k at word 2k, the shadow queue of the bounce that fills
+    // extension queue k at 2k + 1, so a bounce's two appends share one 8-byte word
+    uint32_t* cnt;
     uint32_t* sort_keys;    // material class of each queue entry
     uint32_t* sorted;       // queue entries in material order (nullptr: path order)
     uint32_t* sort_cnt;     // 16 class counts, then 16 cursors
@@ -176,7 +178,10 @@ __device__ inline void camera_ray_lens(const FrameK& K, uint32_t x, uint32_t y, 
 // take consecutive ranges. Every thread of the group must call these.
 //
 // Two appends (extension and shadow queue) at once:
-__device__ inline void block_append2(uint32_t* ca, bool wa, uint32_t& sa, uint32_t* cb, bool wb, uint32_t& sb) {
+// Appends to two queues from a 256-thread work-group with one 64-bit atomic on the pair of
+// 32-bit sizes at pair[0] (a) and pair[1] (b) (8-byte aligned; a's size stays below 2^32,
+// so the low half never carries): sa, sb = this thread's slots. Every thread must call it.
+__device__ inline void block_append_pair(uint32_t* pair, bool wa, uint32_t& sa, bool wb, uint32_t& sb) {
     __shared__ uint32_t cnt[10];
     const uint64_t ma = __ballot(wa), mb = __ballot(wb);
     const uint32_t wv = threadIdx.x >> 6;
@@ -184,8 +189,12 @@ __device__ inline void block_append2(uint32_t* ca, bool wa, uint32_t& sa, uint32
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t ta = cnt[0] + cnt[1] + cnt[2] + cnt[3], tb = cnt[4] + cnt[5] + cnt[6] + cnt[7];
-        cnt[8] = ta ? atomicAdd(ca, ta) : 0u;
-        cnt[9] = tb ? atomicAdd(cb, tb) : 0u;
+        uint64_t old = 0u;
+        if (ta | tb)
+            old = atomicAdd(reinterpret_cast<unsigned long long*>(pair),
+                            (static_cast<unsigned long long>(tb) << 32) | ta);
+        cnt[8] = uint32_t(old);
+        cnt[9] = uint32_t(old >> 32);
     }
     __syncthreads();
     uint32_t ba = cnt[8], bb = cnt[9];
@@ -348,6 +357,7 @@ __device__ inline void write_miss_gbuffer(const FrameK& K, const Targets& T, uin
 __global__ void k_raygen(FrameK K, Queues Q, PathState S, uint32_t n, uint32_t queue_state, uint32_t p_base,
                          uint32_t* __restrict__ trace_next) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // queue 0 (word 0) holds the n primary rays; queue 1 (word 2) and both shadow sizes are 0
     if (i == 0) { Q.cnt[0] = n; Q.cnt[1] = 0u; Q.cnt[2] = 0u; Q.cnt[3] = 0u; *trace_next = 0u; }
     if (i >= n) return;
     const uint32_t p = p_base + i;
@@ -592,7 +602,7 @@ __global__ __launch_bounds__(256) void k_ref_primary(DScene s, FrameK K, Targets
         }
         S.rng[p] = make_uint4(seed.x, seed.y, seed.z, seed.w);
     }
-    const uint32_t slot = block_append(&Q.cnt[1], emit);
+    const uint32_t slot = block_append(&Q.cnt[2], emit);
     if (emit) { Q.o[1][slot] = ro; Q.d[1][slot] = rd; Q.path[1][slot] = p; }
 }
 
@@ -602,7 +612,7 @@ __global__ __launch_bounds__(256) void k_ref_shade(DScene s, FrameK K, Targets T
                                                    uint32_t* __restrict__ trace_next) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0u) *trace_next = 0u;
-    const uint32_t n = Q.cnt[cur];
+    const uint32_t n = Q.cnt[2 * cur];
     const int nxt = cur ^ 1;
     const bool live = i < n;
     bool emit = false;
@@ -712,19 +722,19 @@ __global__ __launch_bounds__(256) void k_ref_shade(DScene s, FrameK K, Targets T
         S.misc[p] = make_uint4(px, flags, counters, mi);
         S.rng[p] = make_uint4(seed.x, seed.y, seed.z, seed.w);
     }
-    const uint32_t slot = block_append(&Q.cnt[nxt], emit);
+    const uint32_t slot = block_append(&Q.cnt[2 * nxt], emit);
     if (emit) { Q.o[nxt][slot] = ro; Q.d[nxt][slot] = rd; Q.path[nxt][slot] = p; }
 }
 
 
 // PT_MIS flags (misc.y): bits 0-7 depth, bit8 lobe0 decided, bit9 lobe0 = diffuse,
 // bit10 last vertex was delta (or camera), bits 16-27 sample index
-// sh: the shadow queue's counter (2 and 3 alternate between bounces)
+// sh: the shadow queue's counter word, 2 nxt + 1 (3 and 1 alternate between bounces)
 __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T, Queues Q, PathState S, int cur,
                                                   int sh, uint32_t* __restrict__ trace_next) {
     const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (i0 == 0u) *trace_next = 0u;  // the ray counter of the trace launch that follows
-    const uint32_t n = Q.cnt[cur];
+    const uint32_t n = Q.cnt[2 * cur];
     const int nxt = cur ^ 1;
     bool emit = false, emit_sh = false;
     float4 ro{}, rd{}, so{}, sd{}, scd{}, scs{}, nthr{};
@@ -983,7 +993,7 @@ __global__ __launch_bounds__(256) void k_pt_shade(DScene s, FrameK K, Targets T,
         nrng = make_uint4(seed.x, seed.y, seed.z, seed.w);
     }
     uint32_t slot, ss;
-    block_append2(&Q.cnt[nxt], emit, slot, &Q.cnt[sh], emit_sh, ss);
+    block_append_pair(&Q.cnt[2 * nxt], emit, slot, emit_sh, ss);  // sh == 2 * nxt + 1
     if (emit) {
         Q.o[nxt][slot] = ro;
         Q.d[nxt][slot] = rd;
@@ -1012,7 +1022,7 @@ __global__ __launch_bounds__(256) void k_sort_count(DScene s, Queues Q, int cur)
     if (threadIdx.x < 16) hist[threadIdx.x] = 0u;
     __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < Q.cnt[cur]) {
+    if (i < Q.cnt[2 * cur]) {
         const uint32_t k = sort_key(s, Q.hit[i], Q.hit_inst[i]);
         Q.sort_keys[i] = k;
         atomicAdd(&hist[k], 1u);
@@ -1036,7 +1046,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(Queues Q, int cur) {
     if (threadIdx.x < 16) hist[threadIdx.x] = 0u;
     __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < Q.cnt[cur];
+    const bool live = i < Q.cnt[2 * cur];
     uint32_t k = 0, rank = 0;
     if (live) {
         k = Q.sort_keys[i];
@@ -1054,8 +1064,8 @@ __global__ __launch_bounds__(256) void k_sort_scatter(Queues Q, int cur) {
 __global__ void k_pt_shadow_resolve(Queues Q, PathState S, int cur, int sh) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
-        Q.cnt[cur] = 0u;
-        Q.cnt[sh ^ 1] = 0u;
+        Q.cnt[2 * cur] = 0u;
+        Q.cnt[sh ^ 2] = 0u;
     }
     if (i >= Q.cnt[sh]) return;
     if (__float_as_uint(Q.shit[i].w) != 0u) return;  // occluded
@@ -1368,7 +1378,7 @@ int render_frame(Ctx& c) {
             if (it == check) {
                 for (int h = 0; h < np; ++h)
                     if (live[h])
-                        GK_HIP(hipMemcpyAsync(c.h_word + h, QH[h].cnt + cur, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                        GK_HIP(hipMemcpyAsync(c.h_word + h, QH[h].cnt + 2 * cur, sizeof(uint32_t), hipMemcpyDeviceToHost,
                                               ss[h]));
                 bool any = false;
                 for (int h = 0; h < np; ++h) {
@@ -1385,8 +1395,8 @@ int render_frame(Ctx& c) {
                 c.trace_counter_zeroed[sl[h]] = true;
                 // the trace zeroes the counter of the queue k_ref_shade appends to
                 const Queues& q = QH[h];
-                st = launch_trace(c, q.o[cur], q.d[cur], q.cnt + cur, pcount[h], q.hit, q.hit_inst, false,
-                                  q.cnt + (cur ^ 1), sl[h]);
+                st = launch_trace(c, q.o[cur], q.d[cur], q.cnt + 2 * cur, pcount[h], q.hit, q.hit_inst, false,
+                                  q.cnt + 2 * (cur ^ 1), sl[h]);
                 if (st != GK_OK) return st;
             }
             for (int h = 0; h < np; ++h) {
@@ -1408,10 +1418,12 @@ int render_frame(Ctx& c) {
         // pipelines' launches are issued alternately.
         const uint32_t iters = (K.bounces + 1u) * K.samples;
         for (uint32_t it = 0; it < iters; ++it) {
-            // queue counters: cur (size of this bounce's hits), cur ^ 1 (next queue, zero),
-            // sh (this bounce's shadow rays, zero); k_raygen and the previous resolve cleared them
-            const int sh = 2 + int(it & 1u);
+            // queue counters: 2 cur (size of this bounce's hits), 2 nxt (next queue, zero),
+            // sh = 2 nxt + 1 (this bounce's shadow rays, zero; next to the next queue's size,
+            // so k_pt_shade appends to both with one atomic); k_raygen and the previous
+            // resolve cleared them
             const int nxt = cur ^ 1;
+            const int sh = 2 * nxt + 1;
             for (int h = 0; h < np; ++h) {
                 const Queues& q = QH[h];
                 const uint32_t m = pcount[h];
@@ -1428,7 +1440,7 @@ int render_frame(Ctx& c) {
                 const Queues& q = QH[h];
                 const uint32_t m = pcount[h];
                 if (it + 1 < iters)
-                    st = launch_trace_pair(c, q.o[nxt], q.d[nxt], q.cnt + nxt, m, q.hit, q.hit_inst, q.so, q.sd,
+                    st = launch_trace_pair(c, q.o[nxt], q.d[nxt], q.cnt + 2 * nxt, m, q.hit, q.hit_inst, q.so, q.sd,
                                            q.cnt + sh, m, q.shit, sl[h]);
                 else
                     st = launch_trace(c, q.so, q.sd, q.cnt + sh, m, q.shit, nullptr, true, nullptr, sl[h]);
