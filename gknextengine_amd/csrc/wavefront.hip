@@ -1122,7 +1122,7 @@ int ensure_wavefront(Ctx& c) {
     GK_HIP(w.hit.resize(n)); GK_HIP(w.hit_inst.resize(n));
     GK_HIP(w.shadow_o.resize(n)); GK_HIP(w.shadow_d.resize(n));
     GK_HIP(w.shadow_c.resize(2 * size_t(n))); GK_HIP(w.shadow_path.resize(n)); GK_HIP(w.shadow_hit.resize(n));
-    GK_HIP(w.counters.resize(64));
+    GK_HIP(w.counters.resize(32 * kPathParts));  // 32 words per path pipeline
     GK_HIP(w.sort_keys.resize(n));
     GK_HIP(w.sorted_idx.resize(n));
     GK_HIP(w.sort_cnt.resize(32 * 4));  // 16 class counts and 16 cursors per path pipeline
@@ -1300,7 +1300,7 @@ int render_frame(Ctx& c) {
         sl[h] = h + 1;
     }
     for (int h = 1; h < np; ++h) {
-        // pipeline h: queue entries from its first path id on, counters from word 8 h
+        // pipeline h: queue entries from its first path id on, counters from word 32 h
         const uint32_t o = pbase[h];
         Queues& B = QH[h];
         for (int k = 0; k < 2; ++k) {
@@ -1309,7 +1309,7 @@ int render_frame(Ctx& c) {
         }
         B.hit += o; B.hit_inst += o;
         B.so += o; B.sd += o; B.sc_d += o; B.sc_s += o; B.spath += o; B.shit += o;
-        B.cnt += 8 * h;
+        B.cnt += 32 * h;  // a 128-byte line of its own: the pipelines' appends do not share one
         B.sort_keys += o;
         if (B.sorted) B.sorted += o;
         B.sort_cnt += 32 * h;
