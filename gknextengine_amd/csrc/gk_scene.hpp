@@ -319,8 +319,9 @@ __device__ inline bool trace_closest_single(const DScene& s, f3 O, f3 D, float t
             lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
             a = (lz[i] - O.z) * rD.z; b = (hz[i] - O.z) * rD.z;
             lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
-            lo = lo > 0.f ? lo * 0.99999904632568359375f : lo * 1.00000095367431640625f;
-            hi = hi > 0.f ? hi * 1.00000095367431640625f : hi * 0.99999904632568359375f;
+            // widened by 2^-16 relative, as k_trace's box test
+            lo = lo > 0.f ? lo * 0.9999847412109375f : lo * 1.0000152587890625f;
+            hi = hi > 0.f ? hi * 1.0000152587890625f : hi * 0.9999847412109375f;
             if (!(hi >= lo && hi >= tmin && lo <= best.t)) continue;
             if (me[i] == 0u) { kt[i] = lo; kc[i] = ch[i]; continue; }
             if (level == 0) {
@@ -413,8 +414,9 @@ __device__ inline bool occluded_single(const DScene& s, f3 O, f3 D, float tmin, 
             lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
             a = (lz[i] - O.z) * rD.z; b = (hz[i] - O.z) * rD.z;
             lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
-            lo = lo > 0.f ? lo * 0.99999904632568359375f : lo * 1.00000095367431640625f;
-            hi = hi > 0.f ? hi * 1.00000095367431640625f : hi * 0.99999904632568359375f;
+            // widened by 2^-16 relative, as k_trace's box test
+            lo = lo > 0.f ? lo * 0.9999847412109375f : lo * 1.0000152587890625f;
+            hi = hi > 0.f ? hi * 1.0000152587890625f : hi * 0.9999847412109375f;
             if (!(hi >= lo && hi >= tmin && lo <= tmax)) continue;
             if (me[i] == 0u) { if (sp < 96) stack[sp++] = ch[i]; continue; }
             if (level == 0) {

@@ -29,7 +29,9 @@
 //   * Triangle test: Moeller-Trumbore with exactly tinybvh's operations and rejects
 //     (tiny_bvh.h:6815-6844). Closest hit keeps the smallest t, ties broken towards the
 //     lower (instance, primitive), so results do not depend on traversal order. Box
-//     tests are widened by 2^-20 relative so they never cull an accepted triangle.
+//     tests are widened by 2^-16 relative so they do not cull an accepted triangle (a
+//     triangle test accepts points a few tens of ulps outside the triangle, near its
+//     edges; 2^-20 missed one such shared-edge hit, and the result depended on ray order).
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -143,12 +145,12 @@ __device__ inline f3 tvector(f3 v, const float* M) {
               M[8] * v.x + M[9] * v.y + M[10] * v.z};
 }
 
-// widen an interval end by 2^-20 relative: x +- |x| * 2^-20 is exactly x * (1 +- 2^-20), and
+// widen an interval end by 2^-16 relative: x +- |x| * 2^-16 is exactly x * (1 +- 2^-16), and
 // the fma rounds it once, so this equals the rounded product; one instruction (|x| is a
 // source modifier). An end at +-infinity becomes NaN and the box is culled: a box whose
 // entry or exit lies at infinity holds no hit at a finite distance.
-__device__ inline float widen_hi(float x) { return __builtin_fmaf(__builtin_fabsf(x), 0x1p-20f, x); }
-__device__ inline float widen_lo(float x) { return __builtin_fmaf(-__builtin_fabsf(x), 0x1p-20f, x); }
+__device__ inline float widen_hi(float x) { return __builtin_fmaf(__builtin_fabsf(x), 0x1p-16f, x); }
+__device__ inline float widen_lo(float x) { return __builtin_fmaf(-__builtin_fabsf(x), 0x1p-16f, x); }
 
 // slab test of one child box; returns its entry distance, or +inf when missed
 __device__ inline float slab1(float lx, float hx, float ly, float hy, float lz, float hz, f3 O, f3 rD,

@@ -8,7 +8,7 @@
 //       tiny_bvh.h:6718-6731; BVH::Build(BLASInstance*), tiny_bvh.h:1565-1603).
 // Traversal returns the closest hit with ties broken towards the lower (inst, prim):
 //       the answer equals a brute-force loop over all triangles, independent of the
-//       tree. Box tests are widened by a relative 2^-20 so they never cull a triangle
+//       tree. Box tests are widened by a relative 2^-16 so they do not cull a triangle
 //       that the Moeller-Trumbore test accepts.
 #include <algorithm>
 #include <limits>
@@ -185,8 +185,8 @@ T slab(const V3<T>& O, const V3<T>& rD, const F3& bmin, const F3& bmax, T tmin, 
     lo = std::max(lo, std::min(t1, t2)); hi = std::min(hi, std::max(t1, t2));
     t1 = (T(bmin.z) - O.z) * rD.z; t2 = (T(bmax.z) - O.z) * rD.z;
     lo = std::max(lo, std::min(t1, t2)); hi = std::min(hi, std::max(t1, t2));
-    hi = hi + std::fabs(hi) * T(9.5367431640625e-07);  // widen by 2^-20 relative
-    lo = lo - std::fabs(lo) * T(9.5367431640625e-07);
+    hi = hi + std::fabs(hi) * T(1.52587890625e-05);  // widen by 2^-16 relative
+    lo = lo - std::fabs(lo) * T(1.52587890625e-05);
     if (hi >= lo && lo <= tmax && hi >= tmin) return lo;
     return std::numeric_limits<T>::infinity();
 }

@@ -213,10 +213,12 @@ def test_pt_mis_city_1080p_sorted_shading_gives_the_same_frame(city):
 def test_pt_mis_two_pipelines_give_the_single_pipeline_frame(city, W, H):
     """Frames of 2^18 paths or more run as two path pipelines on two streams (the even and
     the odd 8x8 tiles; halves of the path range when the size is not a multiple of 8). Three
-    frames with temporal reuse, queued back to back: from the second on, the pipelines
-    generate and trace their primary rays while the previous frame's reprojection and
-    bilateral filter run. Every target bit-identical to GK_FRAME_SINGLE_PIPELINE's, with an
-    odd tile count (65 x 65) too, and the same number of rays traced."""
+    frames with temporal reuse and a moving camera, queued back to back: from the second on,
+    the pipelines generate and trace their primary rays while the previous frame's
+    reprojection and bilateral filter run. Every target bit-identical to
+    GK_FRAME_SINGLE_PIPELINE's, with an odd tile count (65 x 65) too, and the same number of
+    rays traced (at 1080p the second frame holds the shared-edge tie of
+    test_closest_hit_does_not_depend_on_ray_order)."""
     s, _ = city
     names = [t for t in gk.TARGETS if t not in ("MINIGBUFFER", "RADIANCE64")]
     outs = []
@@ -224,7 +226,9 @@ def test_pt_mis_two_pipelines_give_the_single_pipeline_frame(city, W, H):
         with gk.Context(W, H) as ctx:
             ctx.upload(s)
             for f in range(3):
-                p = params(W, H, scenes.CITY_CAM, frame=2 + f, bounces=8, sky=(0.6, 0.7, 0.9), seed=7)
+                eye = tuple(e + 0.15 * f * d for e, d in zip(scenes.CITY_CAM["eye"], (1.0, 0.0, 0.5)))
+                cam = dict(scenes.CITY_CAM, eye=eye)
+                p = params(W, H, cam, frame=2 + f, bounces=8, sky=(0.6, 0.7, 0.9), seed=7)
                 p.denoiser = gk.DENOISER_BILATERAL
                 p.flags |= flags
                 ctx.frame(p)

@@ -243,3 +243,39 @@ def test_tlas_rebuild_and_refit_alternate_bit_exact(device):
     assert np.array_equal(_u32(h), _u32(o))
     print({str(k): round(v, 3) for k, v in sah.items()})
     assert sah["refit from t=0"] > 1.3 * sah[(2.0, True)]
+
+
+def test_closest_hit_does_not_depend_on_ray_order(device):
+    """The city camera moved by (0.15, 0, 0.075): pixel (417, 551) hits the shared edge of
+    two coplanar wall triangles, 168450 and 168459, at the same t, and the triangle test
+    accepts both (float barycentrics overlap by about 7e-6). Triangle 168450's box entry lies
+    16 ulps beyond that t, so a 2^-20 box widening culled its leaf whenever 168459 was found
+    first, and the answer depended on which rays shared a wave. With 2^-16 every order
+    returns the lower primitive: the frame's 2 M primary rays traced in order and in two
+    permutations give the same hits bit for bit."""
+    import torch
+    s = gk.procgen_city(64, 4, 256, 64, 2024)
+    W, H = 1920, 1080
+    eye = tuple(e + 0.15 * d for e, d in zip(scenes.CITY_CAM["eye"], (1.0, 0.0, 0.5)))
+    p = gk.make_frame_params(eye, scenes.CITY_CAM["target"], W, H, fov=scenes.CITY_CAM["fov"], frame=3)
+    rays = po.primary_rays(p, W, H).astype(np.float32)
+    n = rays.shape[0]
+    rng = np.random.default_rng(5)
+    res = []
+    with gk.Context(W, H) as ctx:
+        ctx.upload(s)
+        for k in range(3):
+            perm = np.arange(n) if k == 0 else rng.permutation(n)
+            r = torch.from_numpy(np.ascontiguousarray(rays[perm])).to(device)
+            h = torch.zeros((n, 4), dtype=torch.float32, device=device)
+            inst = torch.zeros(n, dtype=torch.int32, device=device)
+            torch.cuda.synchronize()
+            ctx.trace(r.data_ptr(), n, h.data_ptr(), inst.data_ptr())
+            ctx.sync()
+            out = np.empty((n, 4), np.float32)
+            out[perm] = h.cpu().numpy()
+            res.append(out)
+    i = 551 * W + 417
+    assert _u32(res[0][i:i + 1, 3])[0] == 168450, res[0][i]
+    for k in (1, 2):
+        assert np.array_equal(_u32(res[k]), _u32(res[0])), int((_u32(res[k]) != _u32(res[0])).any(1).sum())
